@@ -1,0 +1,207 @@
+/*
+ * trv_gpu.h -- C ABI of the MI355X (gfx950) three-point clustering engine.
+ *
+ * Every entry point replaces one function of Triumvirate's three-point path
+ * (cited below as reference file:line).  Plain pointers and sizes only: no
+ * C++ or torch types cross this boundary, so a Cython `extern`, a ctypes
+ * stub or a C++ program can bind it unchanged.
+ *
+ * Conventions
+ *   - Caller owns every input and output array (as with the reference's
+ *     Cython layer, src/triumvirate/_threept.pyx:137-161).  The library owns
+ *     all device memory it allocates and frees it before returning, except
+ *     where an entry point documents a persistent workspace.
+ *   - Complex outputs are interleaved (re, im) doubles, i.e. the memory layout
+ *     of std::complex<double>[dim].
+ *   - Mesh index order is row-major with the last (z) index fastest, as
+ *     MeshField::ret_grid_index (src/triumvirate/src/field.cpp:533-538).
+ *   - Return value: TRVG_OK (0) or an error code; the message of the last
+ *     error on the calling thread is available from trvg_last_error().
+ *     Parameter errors map to the reference's trv::sys::InvalidParameterError,
+ *     data errors to trv::sys::InvalidDataError (include/monitor.hpp:731-779);
+ *     device errors are reported, never silently ignored.
+ */
+#ifndef TRV_GPU_H_INCLUDED
+#define TRV_GPU_H_INCLUDED
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRVG_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+  TRVG_OK = 0,
+  TRVG_ERR_PARAM = 1,     /* invalid parameter (InvalidParameterError) */
+  TRVG_ERR_DATA = 2,      /* invalid data (InvalidDataError) */
+  TRVG_ERR_DEVICE = 3,    /* HIP runtime / kernel launch failure */
+  TRVG_ERR_ALLOC = 4,     /* device or host allocation failure */
+  TRVG_ERR_INTERNAL = 5
+};
+
+/* Binning shapes, i.e. ParameterSet::shape after validation
+ * (src/triumvirate/src/parameters.cpp:830-850: form "full" with
+ * ell1 == ell2 becomes shape "triu"). */
+enum {
+  TRVG_SHAPE_DIAG = 0,
+  TRVG_SHAPE_OFFDIAG = 1,
+  TRVG_SHAPE_ROW = 2,
+  TRVG_SHAPE_FULL = 3,
+  TRVG_SHAPE_TRIU = 4
+};
+
+/* Field precision of assignment, FFTs and Y_lm products.  Bin sums,
+ * normalisation, shot-noise amplitudes and Wigner-3j coupling are always
+ * accumulated in FP64. */
+enum {
+  TRVG_PREC_FP64 = 64,
+  TRVG_PREC_FP32 = 32
+};
+
+/* A particle catalogue viewed in place.  Element i of a column is
+ * col[i * stride].  stride = 1 for separate (SoA) arrays; stride = 7 for
+ * the reference's 56-byte ParticleData {pos[3], nz, ws, wc, w}
+ * (include/particles.hpp:63-69), with x = &pdata[0].pos[0], etc.
+ * The total weight is w = ws * wc as in ParticleCatalogue::load_particle_data
+ * (src/particles.cpp:553).  nz may be NULL where unused. */
+typedef struct trvg_catalogue {
+  long long ntotal;
+  long long stride;
+  const double* x;
+  const double* y;
+  const double* z;
+  const double* nz;
+  const double* ws;
+  const double* wc;
+} trvg_catalogue;
+
+/* The numerical subset of trv::ParameterSet (include/parameters.hpp:58-240)
+ * that the three-point path reads, after ParameterSet::validate. */
+typedef struct trvg_params {
+  double boxsize[3];
+  int ngrid[3];
+  int assignment_order;   /* 1 ngp, 2 cic, 3 tsc, 4 pcs */
+  int interlace;          /* forced to 0 for three-point statistics */
+  int ell1, ell2, ELL;
+  int shape;              /* TRVG_SHAPE_* */
+  int idx_bin;
+  int precision;          /* TRVG_PREC_FP64 (drop-in) or TRVG_PREC_FP32 */
+  int verbose;
+  int device;             /* HIP device ordinal; -1 = current device */
+} trvg_params;
+
+/* trv::Binning (include/dataobjs.hpp:58): edges has num_bins + 1 entries,
+ * centres has num_bins entries. */
+typedef struct trvg_binning {
+  int num_bins;
+  const double* bin_edges;
+  const double* bin_centres;
+} trvg_binning;
+
+/* trv::BispecMeasurements (include/dataobjs.hpp:249-265).  The caller sets
+ * `dim` to the capacity of each array (see trvg_result_dim); the library
+ * sets it to the filled length.  bk_raw/bk_shot hold 2*dim doubles. */
+typedef struct trvg_bispec_result {
+  int dim;
+  double* k1_bin;
+  double* k2_bin;
+  double* k1_eff;
+  double* k2_eff;
+  int* nmodes_1;
+  int* nmodes_2;
+  double* bk_raw;
+  double* bk_shot;
+} trvg_bispec_result;
+
+/* trv::ThreePCFMeasurements (include/dataobjs.hpp:267-283). */
+typedef struct trvg_3pcf_result {
+  int dim;
+  double* r1_bin;
+  double* r2_bin;
+  double* r1_eff;
+  double* r2_eff;
+  int* npairs_1;
+  int* npairs_2;
+  double* zeta_raw;
+  double* zeta_shot;
+} trvg_3pcf_result;
+
+/* ---------------------------------------------------------------------- */
+/* Library information and errors                                          */
+/* ---------------------------------------------------------------------- */
+
+int trvg_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* trvg_last_error(void);
+
+/* Number of visible HIP devices (0 when none); never initialises a context
+ * beyond what hipGetDeviceCount does. */
+int trvg_device_count(int* count);
+
+/* Data-vector dimension for a shape and bin count: the dv_dim switch of
+ * compute_bispec (src/threept.cpp:275-298). */
+int trvg_result_dim(int shape, int num_bins, int idx_bin, int* dim);
+
+/* ---------------------------------------------------------------------- */
+/* Normalisation                                                           */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces trv::calc_bispec_normalisation_from_particles
+ * (include/threept.hpp:138-140; src/threept.cpp:96-136):
+ * 1 / (alpha * sum_p ws nz^2 wc^3).  Fails with TRVG_ERR_DATA if the sum
+ * is zero (all nz zero). */
+int trvg_calc_bispec_normalisation_from_particles(
+  const trvg_catalogue* particles, double alpha, double* norm_factor);
+
+/* Replaces trv::calc_bispec_normalisation_from_mesh
+ * (include/threept.hpp:150-152; src/threept.cpp:138-149, field.cpp:2017-2065):
+ * 1 / (V_cell sum_x n(x)^3) / alpha^3 with n assigned on the GPU. */
+int trvg_calc_bispec_normalisation_from_mesh(
+  const trvg_catalogue* particles, const trvg_params* params, double alpha,
+  double* norm_factor);
+
+/* ---------------------------------------------------------------------- */
+/* Estimators                                                              */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces trv::compute_bispec (include/threept.hpp:237-242;
+ * src/threept.cpp:248-1012): survey-type data + random catalogues, local
+ * plane-parallel lines of sight.  los_data / los_rand are ntotal x 3
+ * doubles, the memory layout of trv::LineOfSight[] (dataobjs.hpp:186). */
+int trvg_compute_bispec(
+  const trvg_catalogue* data, const trvg_catalogue* rand,
+  const double* los_data, const double* los_rand,
+  const trvg_params* params, const trvg_binning* kbinning,
+  double norm_factor, trvg_bispec_result* out);
+
+/* Replaces trv::compute_3pcf (include/threept.hpp:257-262;
+ * src/threept.cpp:1014-1471). */
+int trvg_compute_3pcf(
+  const trvg_catalogue* data, const trvg_catalogue* rand,
+  const double* los_data, const double* los_rand,
+  const trvg_params* params, const trvg_binning* rbinning,
+  double norm_factor, trvg_3pcf_result* out);
+
+/* Replaces trv::compute_bispec_in_gpp_box (include/threept.hpp:274-278;
+ * src/threept.cpp:1473-2188): periodic box, global plane-parallel. */
+int trvg_compute_bispec_in_gpp_box(
+  const trvg_catalogue* data,
+  const trvg_params* params, const trvg_binning* kbinning,
+  double norm_factor, trvg_bispec_result* out);
+
+/* Replaces trv::compute_3pcf_in_gpp_box (include/threept.hpp:290-294;
+ * src/threept.cpp:2190-2619). */
+int trvg_compute_3pcf_in_gpp_box(
+  const trvg_catalogue* data,
+  const trvg_params* params, const trvg_binning* rbinning,
+  double norm_factor, trvg_3pcf_result* out);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif  /* TRV_GPU_H_INCLUDED */

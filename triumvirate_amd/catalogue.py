@@ -1,0 +1,177 @@
+"""Particle catalogue and box geometry
+(reference: ``triumvirate.catalogue.ParticleCatalogue``).
+
+Columns are kept as float64 numpy arrays (x, y, z, nz, ws, wc).  The
+geometry operations restate the reference's arithmetic:
+
+* ``compute_los``: unit vectors, zero norm replaced by 1
+  (catalogue.py:437-458);
+* ``centre``: shift by mean(ref bounds) - L/2 (catalogue.py:490-545);
+* ``pad``: shift so the reference minimum sits at the pad
+  (catalogue.py:547-646);
+* ``periodise``: ``(x + L/2 - (max + min)/2) % L`` (catalogue.py:648-676);
+* ``offset_coords``: subtract the origin (catalogue.py:678-690).
+"""
+
+import warnings
+
+import numpy as np
+
+
+class ParticleCatalogue:
+    """Particle catalogue.
+
+    Parameters
+    ----------
+    x, y, z : array of float
+        Cartesian coordinates.
+    nz : float or array of float, optional
+        Redshift-dependent background number density (default 0, with a
+        warning, as the reference).
+    ws, wc : float or array of float, optional
+        Sample and clustering weights (default 1).
+    """
+
+    def __init__(self, x, y, z, nz=None, ws=1., wc=1., logger=None):
+        self._logger = logger
+        x = np.array(x, dtype=np.float64, copy=True).ravel()
+        n = len(x)
+        if nz is None:
+            warnings.warn(
+                "Catalogue 'nz' field is None and thus set to zero, "
+                "which may raise errors in some computations.")
+            nz = 0.
+
+        def col(v):
+            a = np.array(v, dtype=np.float64)
+            if a.ndim == 0:
+                return np.full(n, float(a))
+            a = a.ravel().copy()
+            if len(a) != n:
+                raise ValueError(
+                    "Inconsistent particle data dimensions (source=extdata).")
+            return a
+
+        self._pdata = {
+            'x': x, 'y': col(y), 'z': col(z),
+            'nz': col(nz), 'ws': col(ws), 'wc': col(wc),
+        }
+        self._source = f'extdata:{id(self._pdata)}'
+        self._calc_bounds()
+        self.ntotal = n
+        self.wtotal = (self._pdata['ws'] * self._pdata['wc']).sum()
+        self.wstotal = self._pdata['ws'].sum()
+
+    @classmethod
+    def read_from_file(cls, filepath, names=('x', 'y', 'z', 'nz'),
+                       logger=None):
+        """Read a whitespace-separated plain-text catalogue."""
+        data = np.loadtxt(filepath, ndmin=2)
+        cols = {name: data[:, i] for i, name in enumerate(names)}
+        return cls(cols['x'], cols['y'], cols['z'], nz=cols.get('nz'),
+                   ws=cols.get('ws', 1.), wc=cols.get('wc', 1.),
+                   logger=logger)
+
+    def __len__(self):
+        return self.ntotal
+
+    def __getitem__(self, key):
+        return self._pdata[key]
+
+    def __setitem__(self, key, value):
+        self._pdata[key] = np.broadcast_to(
+            np.asarray(value, dtype=np.float64), (self.ntotal,)).copy()
+        if key in ('x', 'y', 'z'):
+            self._calc_bounds()
+        if key in ('ws', 'wc'):
+            self.wtotal = (self._pdata['ws'] * self._pdata['wc']).sum()
+            self.wstotal = self._pdata['ws'].sum()
+
+    def __str__(self):
+        return f"ParticleCatalogue(source={self._source})"
+
+    def _calc_bounds(self):
+        self.bounds = {
+            ax: (self._pdata[ax].min(), self._pdata[ax].max())
+            for ax in ('x', 'y', 'z')
+        }
+
+    def _check_bounds_in_boxsize(self, boxsize):
+        return [ax for i, ax in enumerate('xyz')
+                if abs(self.bounds[ax][1] - self.bounds[ax][0]) > boxsize[i]]
+
+    def compute_los(self):
+        """Unit line-of-sight vectors, shape (N, 3)."""
+        norm = np.sqrt(self._pdata['x']**2 + self._pdata['y']**2
+                       + self._pdata['z']**2)
+        norm[norm == 0.] = 1.
+        return np.ascontiguousarray(np.transpose([
+            self._pdata['x'] / norm,
+            self._pdata['y'] / norm,
+            self._pdata['z'] / norm,
+        ]))
+
+    def compute_mean_density(self, volume=None, boxsize=None):
+        if np.isscalar(boxsize):
+            boxsize = [boxsize, boxsize, boxsize]
+        volume = volume or np.prod(boxsize)
+        self._pdata['nz'] = np.full(self.ntotal, self.ntotal / volume)
+
+    def offset_coords(self, origin):
+        for ax, c in zip('xyz', origin):
+            self._pdata[ax] = self._pdata[ax] - c
+        self._calc_bounds()
+
+    def centre(self, boxsize, catalogue_ref=None):
+        if np.isscalar(boxsize):
+            boxsize = [boxsize, boxsize, boxsize]
+        ref = self if catalogue_ref is None else catalogue_ref
+        if self._check_bounds_in_boxsize(boxsize):
+            warnings.warn(
+                "Catalogue extent exceeds the box size. Some particles may "
+                "lie outside the box after centring.")
+        origin = np.array([
+            np.mean(ref.bounds[ax]) - boxsize[i] / 2.
+            for i, ax in enumerate('xyz')
+        ])
+        self.offset_coords(origin)
+        if catalogue_ref is not None:
+            catalogue_ref.offset_coords(origin)
+
+    def pad(self, boxsize, ngrid=None, boxsize_pad=None, ngrid_pad=None,
+            catalogue_ref=None):
+        if boxsize_pad is None and ngrid_pad is None:
+            warnings.warn(
+                "`boxsize_pad` and `ngrid_pad` are both None. "
+                "No padding is applied to the catalogue.")
+            return
+        if boxsize_pad is not None and ngrid_pad is not None:
+            raise ValueError(
+                "Conflicting padding as `boxsize_pad` and `ngrid_pad` "
+                "are both set (not None).")
+        if np.isscalar(boxsize):
+            boxsize = [boxsize, boxsize, boxsize]
+        ref = self if catalogue_ref is None else catalogue_ref
+        origin = np.array([ref.bounds[ax][0] for ax in 'xyz'])
+        if boxsize_pad:
+            origin -= np.multiply(boxsize_pad, boxsize)
+        if ngrid_pad:
+            origin -= np.multiply(ngrid_pad, np.divide(boxsize, ngrid))
+        self.offset_coords(origin)
+        if catalogue_ref is not None:
+            catalogue_ref.offset_coords(origin)
+
+    def periodise(self, boxsize):
+        if np.isscalar(boxsize):
+            boxsize = [boxsize, boxsize, boxsize]
+        for i, ax in enumerate('xyz'):
+            self._pdata[ax] = (
+                self._pdata[ax] + boxsize[i] / 2.
+                - (self.bounds[ax][1] + self.bounds[ax][0]) / 2.
+            ) % boxsize[i]
+        self._calc_bounds()
+
+    def columns(self):
+        """(x, y, z, nz, ws, wc) float64 arrays."""
+        p = self._pdata
+        return p['x'], p['y'], p['z'], p['nz'], p['ws'], p['wc']
