@@ -200,6 +200,56 @@ int trvg_compute_3pcf_in_gpp_box(
   const trvg_params* params, const trvg_binning* rbinning,
   double norm_factor, trvg_3pcf_result* out);
 
+
+/* ---------------------------------------------------------------------- */
+/* Engine extensions (no counterpart in the reference)                     */
+/* ---------------------------------------------------------------------- */
+
+/* Device-resident catalogue: uploaded once, reused by any number of
+ * estimator calls (for callers that keep particles in HBM, e.g. repeated
+ * measurements and benchmarks).  `los` (ntotal x 3) is required for
+ * survey-type (local plane-parallel) use and may be NULL otherwise.
+ * device = -1 uses the current device. */
+typedef struct trvg_device_catalogue trvg_device_catalogue;
+
+int trvg_catalogue_upload(const trvg_catalogue* cat, const double* los,
+                          int device, trvg_device_catalogue** out);
+int trvg_catalogue_free(trvg_device_catalogue* cat);
+
+/* As trvg_compute_bispec / trvg_compute_bispec_in_gpp_box (rand == NULL)
+ * on device-resident catalogues. */
+int trvg_compute_bispec_dev(const trvg_device_catalogue* data,
+                            const trvg_device_catalogue* rand,
+                            const trvg_params* params,
+                            const trvg_binning* kbinning, double norm_factor,
+                            trvg_bispec_result* out);
+
+/* As trvg_compute_3pcf / trvg_compute_3pcf_in_gpp_box (rand == NULL). */
+int trvg_compute_3pcf_dev(const trvg_device_catalogue* data,
+                          const trvg_device_catalogue* rand,
+                          const trvg_params* params,
+                          const trvg_binning* rbinning, double norm_factor,
+                          trvg_3pcf_result* out);
+
+/* Frees the cached device work buffers (they are otherwise kept for reuse
+ * by later calls). */
+int trvg_release_workspace(void);
+
+/* Component entry points (diagnostics and parity tests). */
+
+/* Mesh assignment of a catalogue with per-particle weights
+ * (mode 0: 1; 1: w; 2: w Y_lm(los); 3: w^2 conj(Y_lm(los))), field.cpp
+ * MeshField::assign_weighted_field_to_mesh (src/field.cpp:569-1112).
+ * mesh_out: 2 * nmesh doubles (interleaved complex). */
+int trvg_mesh_assign(const trvg_catalogue* cat, const double* los,
+                     const trvg_params* params, int mode, int ell, int m,
+                     double* mesh_out);
+
+/* In-place unnormalised 3-D FFT of an interleaved complex host mesh
+ * (sign -1 forward, +1 backward), computed on the device in
+ * params->precision. */
+int trvg_fft3d(const trvg_params* params, double* mesh, int sign);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

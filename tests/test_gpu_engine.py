@@ -1,0 +1,263 @@
+"""GPU parity tests of the HIP engine (run on an MI355X).
+
+Every estimator call here goes through the engine's C ABI
+(``libtrvgpu.so``); the CPU FP64 oracle and the reference's own fixture
+files are the checkers.
+
+Tolerances (stated per test):
+  * mode / pair counts: bit-exact;
+  * FP64 estimates vs the oracle: 1e-10 relative to max |B| of the multipole;
+  * FP64 estimates vs reference fixtures: 2e-9 (the files print 10 digits);
+  * FP32 estimates vs FP64: 1e-4 relative to max |B| (see test docstring).
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import conftest as cf
+import parity as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def eng():
+    from triumvirate_amd import _lib
+    lib = _lib.load()
+    n = C.c_int(0)
+    assert lib.trvg_device_count(C.byref(n)) == 0
+    if n.value == 0:
+        pytest.fail("No HIP device visible to the engine.")
+    return lib
+
+
+@pytest.fixture(scope='module')
+def gpu_backend(eng):
+    from triumvirate_amd.threept import CLibBackend
+    return CLibBackend(eng, 'trvg_')
+
+
+@pytest.fixture(scope='module')
+def oracle_backend():
+    import oracle
+    return oracle.backend(schedule=1)
+
+
+def _params(ngrid, L, order=3, prec=64):
+    from triumvirate_amd import _abi
+    p = _abi.Params()
+    for i in range(3):
+        p.ngrid[i] = ngrid[i]
+        p.boxsize[i] = L[i]
+    p.assignment_order = order
+    p.precision = prec
+    p.device = -1
+    return p
+
+
+# ---------------------------------------------------------------------------
+# Components
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize('shape', [(32, 32, 32), (16, 32, 64), (64, 16, 8)])
+@pytest.mark.parametrize('prec', [64, 32])
+def test_fft3d_matches_numpy(eng, shape, prec):
+    """Forward/backward unnormalised FFT vs numpy.fft (FFTW convention).
+    Tolerance: 1e-12 (FP64) / 2e-6 (FP32) relative to max |X|."""
+    from triumvirate_amd import _abi
+    print(f"fft3d shape={shape} prec={prec}", flush=True)
+    g = np.random.default_rng(7)
+    a = g.normal(size=shape) + 1j * g.normal(size=shape)
+    p = _params(shape, (1., 1., 1.), prec=prec)
+    tol = 1e-12 if prec == 64 else 2e-6
+    for sign, ref in ((-1, np.fft.fftn(a)), (+1, np.fft.ifftn(a) * a.size)):
+        buf = np.ascontiguousarray(a.astype(np.complex128)).view(np.float64).copy()
+        assert eng.trvg_fft3d(C.byref(p), _abi.dptr(buf), sign) == 0, \
+            eng.trvg_last_error()
+        out = buf.view(np.complex128).reshape(shape)
+        err = np.max(np.abs(out - ref)) / np.max(np.abs(ref))
+        assert err < tol, (sign, err)
+
+
+@pytest.mark.parametrize('order', [1, 2, 3, 4])
+@pytest.mark.parametrize('mode,ell,m', [(0, 0, 0), (1, 0, 0), (2, 2, 1),
+                                        (3, 2, -2)])
+def test_assignment_matches_oracle(eng, oracle_lib, order, mode, ell, m):
+    """Mesh assignment vs the oracle's restatement of field.cpp:569-1112,
+    including wrap-around at both box faces.  Tolerance 1e-12 relative to
+    the largest cell value (FP64 atomics reorder the sums)."""
+    from triumvirate_amd import _abi
+    print(f"assign order={order} mode={mode} l={ell} m={m}", flush=True)
+    ng, L = (16, 32, 16), (100., 200., 160.)
+    g = np.random.default_rng(11 + order)
+    n = 4000
+    x = g.uniform(0, L[0], n)
+    y = g.uniform(0, L[1], n)
+    z = g.uniform(0, L[2], n)
+    # Put some particles right at the faces and cell edges.
+    x[:20] = 0.
+    y[20:40] = np.nextafter(L[1], 0)
+    z[40:60] = L[2] / ng[2] * 3
+    ws = g.uniform(0.5, 1.5, n)
+    wc = g.uniform(0.5, 1.5, n)
+    los = g.normal(size=(n, 3))
+    los /= np.linalg.norm(los, axis=1)[:, None]
+    los = np.ascontiguousarray(los)
+    cat = _abi.CatalogueView(x, y, z, np.zeros(n), ws, wc)
+    p = _params(ng, L, order=order)
+    nm = int(np.prod(ng))
+    out = np.zeros(2 * nm)
+    assert eng.trvg_mesh_assign(C.byref(cat.struct), _abi.dptr(los),
+                                C.byref(p), mode, ell, m, _abi.dptr(out)) == 0, \
+        eng.trvg_last_error()
+    # Oracle: explicit complex weights.
+    w = ws * wc
+    if mode == 0:
+        wt = np.ones(n, complex)
+    elif mode == 1:
+        wt = w.astype(complex)
+    else:
+        yl = np.zeros((n, 2))
+        for i in range(n):
+            oracle_lib.trvo_ylm(ell, m, _abi.dptr(los[i].copy()),
+                                _abi.dptr(yl[i]))
+        yv = yl[:, 0] + 1j * yl[:, 1]
+        wt = yv * w if mode == 2 else np.conj(yv) * w**2
+    wbuf = np.ascontiguousarray(wt).view(np.float64).copy()
+    ref = np.zeros(2 * nm)
+    assert oracle_lib.trvo_assign(C.byref(cat.struct), _abi.dptr(wbuf),
+                                  C.byref(p), 0, _abi.dptr(ref)) == 0
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(out - ref)) / scale < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# End to end
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize('case', P.FIXTURE_CASES,
+                         ids=[P.fixture_name(*c) for c in P.FIXTURE_CASES])
+def test_reference_fixtures_public_api(eng, case):
+    """The public API (compute_bispec, compute_3pcf, ..._in_gpp_box) on the
+    reference's test catalogues reproduces the reference fixture files."""
+    import triumvirate_amd as trv
+    kind, deg, form, idx, geo = case
+    name = P.fixture_name(*case)
+    print(f"fixture {name}", flush=True)
+    ps = cf.make_test_paramset()
+    b = cf.binning_fourier() if kind == 'bispec' else cf.binning_config()
+    data = cf.make_test_data_catalogue()
+    if geo == 'gpp':
+        fn = (trv.compute_bispec_in_gpp_box if kind == 'bispec'
+              else trv.compute_3pcf_in_gpp_box)
+        res = fn(data, degrees=deg, binning=b, form=form, idx_bin=idx,
+                 paramset=ps)
+    else:
+        fn = trv.compute_bispec if kind == 'bispec' else trv.compute_3pcf
+        res = fn(data, cf.make_test_rand_catalogue(), degrees=deg, binning=b,
+                 form=form, idx_bin=idx, paramset=ps)
+    P.check_against_fixture(res, kind, name)
+
+
+@pytest.mark.parametrize('case', P.FIXTURE_CASES,
+                         ids=[P.fixture_name(*c) for c in P.FIXTURE_CASES])
+def test_fixture_configs_match_oracle(gpu_backend, oracle_backend, case):
+    """Same configurations, GPU vs oracle at FP64 tolerance 1e-10."""
+    name = P.fixture_name(*case)
+    print(f"oracle-compare {name}", flush=True)
+    kind = case[0]
+    a = P.run_fixture_case(gpu_backend, *case)
+    b = P.run_fixture_case(oracle_backend, *case)
+    P.compare_results(a, b, kind, rtol=1e-10)
+
+
+FORM_CASES = [
+    ('bispec', (0, 0, 0), 'full', None, 'gpp', 4),
+    ('bispec', (2, 0, 2), 'full', None, 'gpp', 2),
+    ('bispec', (0, 0, 0), 'off-diag', 1, 'lpp', 3),
+    ('bispec', (0, 0, 0), 'off-diag', -2, 'gpp', 1),
+    ('bispec', (2, 0, 2), 'row', 1, 'lpp', 2),
+    ('bispec', (1, 1, 0), 'full', None, 'lpp', 3),
+    ('bispec', (0, 2, 2), 'full', None, 'lpp', 4),
+    ('3pcf', (1, 1, 0), 'full', None, 'lpp', 3),
+    ('3pcf', (0, 0, 0), 'full', None, 'gpp', 2),
+    ('3pcf', (2, 0, 2), 'off-diag', 0, 'lpp', 4),
+]
+
+
+@pytest.mark.parametrize('case', FORM_CASES,
+                         ids=['-'.join(map(str, c)) for c in FORM_CASES])
+def test_forms_and_schemes_match_oracle(gpu_backend, oracle_backend, case):
+    """Forms beyond the fixtures (full/triu, off-diag, row), other
+    assignment schemes, non-unit weights, 32^3 mesh: GPU vs oracle, 1e-10."""
+    from triumvirate_amd import threept as T
+    from triumvirate_amd.dataobjs import Binning
+    kind, deg, form, idx, geo, order = case
+    print(f"form-case {case}", flush=True)
+    scheme = {1: 'ngp', 2: 'cic', 3: 'tsc', 4: 'pcs'}[order]
+    ps = cf.make_test_paramset(
+        assignment=scheme, ngrid={'x': 32, 'y': 32, 'z': 32},
+        boxsize={'x': 800., 'y': 800., 'z': 800.})
+    if kind == 'bispec':
+        b = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.11, num_bins=5)
+    else:
+        b = Binning('config', 'lin', bin_min=30., bin_max=130., num_bins=5)
+    out = []
+    for be in (gpu_backend, oracle_backend):
+        if geo == 'gpp':
+            d = P.uniform_box_catalogue(3000, 800., seed=5, weights=False)
+            out.append(T._compute_3pt_stats_sim_like(
+                be, kind, d, paramset=ps, degrees=deg, binning=b, form=form,
+                idx_bin=idx))
+        else:
+            d, r = P.survey_pair(2000, 8000, 700., seed=9, weights=True)
+            out.append(T._compute_3pt_stats_survey_like(
+                be, kind, d, r, paramset=ps, degrees=deg, binning=b,
+                form=form, idx_bin=idx))
+    P.compare_results(out[0], out[1], kind, rtol=1e-10)
+
+
+def test_fp32_error_bound(gpu_backend):
+    """FP32 fields (assignment, FFT, Y_lm products) with FP64 sums vs the
+    FP64 engine path: max |B32 - B64| / max |B64| per multipole < 1e-4 on a
+    64^3 periodic box of 2e4 particles (measured values are reported by the
+    benchmark; this bound guards regressions)."""
+    from triumvirate_amd import threept as T
+    from triumvirate_amd.dataobjs import Binning
+    b = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.21, num_bins=8)
+    res = {}
+    for prec in ('fp64', 'fp32'):
+        ps = cf.make_test_paramset(ngrid={'x': 64, 'y': 64, 'z': 64},
+                                   precision=prec)
+        d = P.uniform_box_catalogue(20000, 1000., seed=3)
+        res[prec] = T._compute_3pt_stats_sim_like(
+            gpu_backend, 'bispec', d, paramset=ps, degrees=(0, 0, 0),
+            binning=b, form='full')
+    for key in ('bk_raw', 'bk_shot'):
+        a, r = res['fp32'][key], res['fp64'][key]
+        err = np.max(np.abs(a - r)) / np.max(np.abs(r))
+        print(f"fp32 {key} rel err {err:.3e}", flush=True)
+        assert err < 1e-4
+    assert np.array_equal(res['fp32']['nmodes_1'], res['fp64']['nmodes_1'])
+
+
+def test_errors_map_to_reference_exceptions(gpu_backend):
+    """Vanishing 3-j couplings raise InvalidParameterError; all-zero nz
+    raises InvalidDataError (threept.cpp:73-89, 120-131)."""
+    from triumvirate_amd import threept as T
+    from triumvirate_amd.parameters import InvalidParameterError
+    ps = cf.make_test_paramset()
+    with pytest.raises(InvalidParameterError):
+        T._compute_3pt_stats_sim_like(
+            gpu_backend, 'bispec', cf.make_test_data_catalogue(), paramset=ps,
+            degrees=(1, 0, 0), binning=cf.binning_fourier(), form='diag')
+    d = cf.make_test_data_catalogue()
+    d['nz'] = 0.
+    d2 = P.uniform_box_catalogue(100, 1000., seed=1)
+    x, y, z, nz, ws, wc = d2.columns()
+    from triumvirate_amd import _abi
+    view = _abi.CatalogueView(x, y, z, np.zeros_like(nz), ws, wc)
+    with pytest.raises(T.InvalidDataError):
+        gpu_backend.norm_particles(view, 1.)

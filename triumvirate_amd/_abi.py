@@ -190,4 +190,23 @@ def declare_common(lib, prefix):
 def declare_extensions(lib):
     """Attach argtypes for engine entry points beyond the reference's
     interface (present only in the HIP library)."""
+    P = C.POINTER
+    vp = C.c_void_p
+    lib.trvg_catalogue_upload.argtypes = [P(Catalogue), _dp, C.c_int, P(vp)]
+    lib.trvg_catalogue_upload.restype = C.c_int
+    lib.trvg_catalogue_free.argtypes = [vp]
+    lib.trvg_catalogue_free.restype = C.c_int
+    lib.trvg_compute_bispec_dev.argtypes = [vp, vp, P(Params), P(Binning),
+                                            C.c_double, P(BispecResult)]
+    lib.trvg_compute_bispec_dev.restype = C.c_int
+    lib.trvg_compute_3pcf_dev.argtypes = [vp, vp, P(Params), P(Binning),
+                                          C.c_double, P(ThreePCFResult)]
+    lib.trvg_compute_3pcf_dev.restype = C.c_int
+    lib.trvg_release_workspace.argtypes = []
+    lib.trvg_release_workspace.restype = C.c_int
+    lib.trvg_mesh_assign.argtypes = [P(Catalogue), _dp, P(Params), C.c_int,
+                                     C.c_int, C.c_int, _dp]
+    lib.trvg_mesh_assign.restype = C.c_int
+    lib.trvg_fft3d.argtypes = [P(Params), _dp, C.c_int]
+    lib.trvg_fft3d.restype = C.c_int
     return lib

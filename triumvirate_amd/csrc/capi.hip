@@ -1,0 +1,407 @@
+// capi.hip -- extern "C" entry points declared in include/trv_gpu.h.
+//
+// Every entry point converts C++ exceptions into a status code plus a
+// thread-local message (trvg_last_error); HIP failures are never ignored.
+#include <cstring>
+#include <string>
+
+#include "engine.hpp"
+
+using namespace trvg;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& fn) {
+  try {
+    g_err.clear();
+    fn();
+    return TRVG_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "Host allocation failed.";
+    return TRVG_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return TRVG_ERR_INTERNAL;
+  } catch (...) {
+    g_err = "Unknown error.";
+    return TRVG_ERR_INTERNAL;
+  }
+}
+
+void require_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    throw Error(TRVG_ERR_DEVICE, "No HIP device is available.");
+  }
+}
+
+void check_binning(const trvg_binning* b) {
+  if (b == nullptr || b->bin_edges == nullptr || b->bin_centres == nullptr ||
+      b->num_bins <= 0)
+    throw Error(TRVG_ERR_PARAM, "Invalid binning.");
+  for (int i = 0; i < b->num_bins; i++)
+    if (!(b->bin_edges[i] <= b->bin_edges[i + 1]))
+      throw Error(TRVG_ERR_PARAM, "Bin edges must be non-decreasing.");
+}
+
+trvg_params three_point_params(const trvg_params* p, int num_bins) {
+  if (p == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
+  trvg_params q = *p;
+  q.interlace = 0;   // parameters.cpp:1240-1249
+  validate_params(q, num_bins);
+  return q;
+}
+
+void write_bispec(const Result& r, trvg_bispec_result* out) {
+  if (out == nullptr) throw Error(TRVG_ERR_PARAM, "Null result.");
+  if (out->dim < r.dim)
+    throw Error(TRVG_ERR_PARAM, "Result capacity " + std::to_string(out->dim) +
+                                    " < data-vector dimension " +
+                                    std::to_string(r.dim) + ".");
+  for (int i = 0; i < r.dim; i++) {
+    out->k1_bin[i] = r.b1[i]; out->k2_bin[i] = r.b2[i];
+    out->k1_eff[i] = r.e1[i]; out->k2_eff[i] = r.e2[i];
+    out->nmodes_1[i] = r.n1[i]; out->nmodes_2[i] = r.n2[i];
+    out->bk_raw[2 * i] = r.raw[i].real(); out->bk_raw[2 * i + 1] = r.raw[i].imag();
+    out->bk_shot[2 * i] = r.shot[i].real(); out->bk_shot[2 * i + 1] = r.shot[i].imag();
+  }
+  out->dim = r.dim;
+}
+
+void write_3pcf(const Result& r, trvg_3pcf_result* out) {
+  if (out == nullptr) throw Error(TRVG_ERR_PARAM, "Null result.");
+  if (out->dim < r.dim)
+    throw Error(TRVG_ERR_PARAM, "Result capacity " + std::to_string(out->dim) +
+                                    " < data-vector dimension " +
+                                    std::to_string(r.dim) + ".");
+  for (int i = 0; i < r.dim; i++) {
+    out->r1_bin[i] = r.b1[i]; out->r2_bin[i] = r.b2[i];
+    out->r1_eff[i] = r.e1[i]; out->r2_eff[i] = r.e2[i];
+    out->npairs_1[i] = r.n1[i]; out->npairs_2[i] = r.n2[i];
+    out->zeta_raw[2 * i] = r.raw[i].real(); out->zeta_raw[2 * i + 1] = r.raw[i].imag();
+    out->zeta_shot[2 * i] = r.shot[i].real(); out->zeta_shot[2 * i + 1] = r.shot[i].imag();
+  }
+  out->dim = r.dim;
+}
+
+Result dispatch_bispec(Context& ctx, const DeviceCatalogue& d,
+                       const DeviceCatalogue* r, bool survey,
+                       const trvg_binning& b, double norm) {
+  if (ctx.p.precision == TRVG_PREC_FP32)
+    return run_bispec<float>(ctx, d, r, survey, b, norm);
+  return run_bispec<double>(ctx, d, r, survey, b, norm);
+}
+
+Result dispatch_3pcf(Context& ctx, const DeviceCatalogue& d,
+                     const DeviceCatalogue* r, bool survey,
+                     const trvg_binning& b, double norm) {
+  if (ctx.p.precision == TRVG_PREC_FP32)
+    return run_3pcf<float>(ctx, d, r, survey, b, norm);
+  return run_3pcf<double>(ctx, d, r, survey, b, norm);
+}
+
+void require_los(const double* los, const char* which) {
+  if (los == nullptr)
+    throw Error(TRVG_ERR_DATA, std::string("Lines of sight for the ") + which +
+                                   " catalogue are required.");
+}
+
+}  // namespace
+
+// Opaque device-resident catalogue.
+struct trvg_device_catalogue {
+  int device = 0;
+  DeviceCatalogue cat;
+};
+
+extern "C" {
+
+int trvg_abi_version(void) { return TRVG_ABI_VERSION; }
+
+const char* trvg_last_error(void) { return g_err.c_str(); }
+
+int trvg_device_count(int* count) {
+  return guarded([&] {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      n = 0;
+    }
+    *count = n;
+  });
+}
+
+int trvg_result_dim(int shape, int num_bins, int idx_bin, int* dim) {
+  return guarded([&] { *dim = result_dim(shape, num_bins, idx_bin); });
+}
+
+int trvg_calc_bispec_normalisation_from_particles(
+    const trvg_catalogue* c, double alpha, double* norm_factor) {
+  return guarded([&] {
+    if (c == nullptr || c->x == nullptr)
+      throw Error(TRVG_ERR_DATA, "Particle data are uninitialised.");
+    const long long st = c->stride > 0 ? c->stride : 1;
+    double s = 0.;
+    for (long long i = 0; i < c->ntotal; i++) {
+      double ws = c->ws ? c->ws[i * st] : 1.;
+      double wc = c->wc ? c->wc[i * st] : 1.;
+      double nz = c->nz ? c->nz[i * st] : 0.;
+      s += ws * std::pow(nz, 2) * std::pow(wc, 3);
+    }
+    if (s == 0.)
+      throw Error(TRVG_ERR_DATA,
+                  "Particle 'nz' values appear to be all zeros. "
+                  "Check the input catalogue contains valid 'nz' field.");
+    *norm_factor = 1. / (alpha * s);
+  });
+}
+
+int trvg_calc_bispec_normalisation_from_mesh(const trvg_catalogue* c,
+                                             const trvg_params* params,
+                                             double alpha, double* norm_factor) {
+  return guarded([&] {
+    require_device();
+    if (params == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
+    trvg_params q = *params;
+    q.interlace = 0;
+    validate_params(q, 1);
+    Context ctx(q);
+    DeviceCatalogue dc;
+    upload_catalogue(c, nullptr, dc, ctx.stream);
+    double nf = (q.precision == TRVG_PREC_FP32) ? run_mesh_norm<float>(ctx, dc)
+                                                : run_mesh_norm<double>(ctx, dc);
+    *norm_factor = nf / std::pow(alpha, 3);
+  });
+}
+
+int trvg_compute_bispec(const trvg_catalogue* data, const trvg_catalogue* rand,
+                        const double* los_data, const double* los_rand,
+                        const trvg_params* params, const trvg_binning* kb,
+                        double norm_factor, trvg_bispec_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(kb);
+    trvg_params q = three_point_params(params, kb->num_bins);
+    require_los(los_data, "data");
+    require_los(los_rand, "random");
+    Context ctx(q);
+    DeviceCatalogue d, r;
+    upload_catalogue(data, los_data, d, ctx.stream);
+    upload_catalogue(rand, los_rand, r, ctx.stream);
+    write_bispec(dispatch_bispec(ctx, d, &r, true, *kb, norm_factor), out);
+  });
+}
+
+int trvg_compute_3pcf(const trvg_catalogue* data, const trvg_catalogue* rand,
+                      const double* los_data, const double* los_rand,
+                      const trvg_params* params, const trvg_binning* rb,
+                      double norm_factor, trvg_3pcf_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(rb);
+    trvg_params q = three_point_params(params, rb->num_bins);
+    require_los(los_data, "data");
+    require_los(los_rand, "random");
+    Context ctx(q);
+    DeviceCatalogue d, r;
+    upload_catalogue(data, los_data, d, ctx.stream);
+    upload_catalogue(rand, los_rand, r, ctx.stream);
+    write_3pcf(dispatch_3pcf(ctx, d, &r, true, *rb, norm_factor), out);
+  });
+}
+
+int trvg_compute_bispec_in_gpp_box(const trvg_catalogue* data,
+                                   const trvg_params* params,
+                                   const trvg_binning* kb, double norm_factor,
+                                   trvg_bispec_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(kb);
+    trvg_params q = three_point_params(params, kb->num_bins);
+    Context ctx(q);
+    DeviceCatalogue d;
+    upload_catalogue(data, nullptr, d, ctx.stream);
+    write_bispec(dispatch_bispec(ctx, d, nullptr, false, *kb, norm_factor), out);
+  });
+}
+
+int trvg_compute_3pcf_in_gpp_box(const trvg_catalogue* data,
+                                 const trvg_params* params,
+                                 const trvg_binning* rb, double norm_factor,
+                                 trvg_3pcf_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(rb);
+    trvg_params q = three_point_params(params, rb->num_bins);
+    Context ctx(q);
+    DeviceCatalogue d;
+    upload_catalogue(data, nullptr, d, ctx.stream);
+    write_3pcf(dispatch_3pcf(ctx, d, nullptr, false, *rb, norm_factor), out);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident catalogues (engine extension).
+// ---------------------------------------------------------------------------
+
+int trvg_catalogue_upload(const trvg_catalogue* c, const double* los,
+                          int device, trvg_device_catalogue** out) {
+  return guarded([&] {
+    require_device();
+    if (out == nullptr) throw Error(TRVG_ERR_PARAM, "Null output handle.");
+    if (device >= 0) TRVG_HIP(hipSetDevice(device));
+    auto h = new trvg_device_catalogue();
+    try {
+      TRVG_HIP(hipGetDevice(&h->device));
+      upload_catalogue(c, los, h->cat, nullptr);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int trvg_catalogue_free(trvg_device_catalogue* h) {
+  return guarded([&] { delete h; });
+}
+
+int trvg_compute_bispec_dev(const trvg_device_catalogue* data,
+                            const trvg_device_catalogue* rand,
+                            const trvg_params* params, const trvg_binning* kb,
+                            double norm_factor, trvg_bispec_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(kb);
+    if (data == nullptr) throw Error(TRVG_ERR_DATA, "Null data catalogue.");
+    trvg_params q = three_point_params(params, kb->num_bins);
+    if (q.device < 0) q.device = data->device;
+    Context ctx(q);
+    const bool survey = (rand != nullptr);
+    if (survey && (data->cat.los.get() == nullptr || rand->cat.los.get() == nullptr))
+      throw Error(TRVG_ERR_DATA, "Survey catalogues need lines of sight.");
+    write_bispec(dispatch_bispec(ctx, data->cat, survey ? &rand->cat : nullptr,
+                                 survey, *kb, norm_factor), out);
+  });
+}
+
+int trvg_compute_3pcf_dev(const trvg_device_catalogue* data,
+                          const trvg_device_catalogue* rand,
+                          const trvg_params* params, const trvg_binning* rb,
+                          double norm_factor, trvg_3pcf_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(rb);
+    if (data == nullptr) throw Error(TRVG_ERR_DATA, "Null data catalogue.");
+    trvg_params q = three_point_params(params, rb->num_bins);
+    if (q.device < 0) q.device = data->device;
+    Context ctx(q);
+    const bool survey = (rand != nullptr);
+    if (survey && (data->cat.los.get() == nullptr || rand->cat.los.get() == nullptr))
+      throw Error(TRVG_ERR_DATA, "Survey catalogues need lines of sight.");
+    write_3pcf(dispatch_3pcf(ctx, data->cat, survey ? &rand->cat : nullptr,
+                             survey, *rb, norm_factor), out);
+  });
+}
+
+int trvg_release_workspace(void) {
+  return guarded([&] { pool().release_all(); });
+}
+
+// ---------------------------------------------------------------------------
+// Component entry points (engine diagnostics; used by the parity tests).
+// ---------------------------------------------------------------------------
+
+// Assign `c` with per-particle weight mode (0 unit, 1 w, 2 w*Y(los),
+// 3 w^2*conj(Y(los))) into an FP64 complex mesh returned in `mesh_out`
+// (2*nmesh doubles).  Uses params->precision for the device mesh.
+int trvg_mesh_assign(const trvg_catalogue* c, const double* los,
+                     const trvg_params* params, int mode, int ell, int m,
+                     double* mesh_out) {
+  return guarded([&] {
+    require_device();
+    if (params == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
+    trvg_params q = *params;
+    q.interlace = 0;
+    validate_params(q, 1);
+    Context ctx(q);
+    DeviceCatalogue dc;
+    upload_catalogue(c, los, dc, ctx.stream);
+    const long long nm = ctx.g.nmesh;
+    std::vector<double> h(2 * nm);
+    if (q.precision == TRVG_PREC_FP32) {
+      DevArray<float2> f(nm);
+      TRVG_HIP(hipMemsetAsync(f.get(), 0, nm * sizeof(float2), ctx.stream));
+      DevParticles pv = dc.view();
+      int cw = (mode >= 2 && !(ell == 0 && m == 0));
+      unsigned grid = 1024;
+      switch (q.assignment_order) {
+        case 1: hipLaunchKernelGGL((k_assign<float, 1>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
+        case 2: hipLaunchKernelGGL((k_assign<float, 2>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
+        case 3: hipLaunchKernelGGL((k_assign<float, 3>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
+        default: hipLaunchKernelGGL((k_assign<float, 4>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
+      }
+      TRVG_LAUNCH_CHECK();
+      std::vector<float> hf(2 * nm);
+      TRVG_HIP(hipMemcpyAsync(hf.data(), f.get(), nm * sizeof(float2), hipMemcpyDeviceToHost, ctx.stream));
+      TRVG_HIP(hipStreamSynchronize(ctx.stream));
+      for (long long i = 0; i < 2 * nm; i++) h[i] = hf[i];
+    } else {
+      DevArray<double2> f(nm);
+      TRVG_HIP(hipMemsetAsync(f.get(), 0, nm * sizeof(double2), ctx.stream));
+      DevParticles pv = dc.view();
+      int cw = (mode >= 2 && !(ell == 0 && m == 0));
+      unsigned grid = 1024;
+      switch (q.assignment_order) {
+        case 1: hipLaunchKernelGGL((k_assign<double, 1>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
+        case 2: hipLaunchKernelGGL((k_assign<double, 2>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
+        case 3: hipLaunchKernelGGL((k_assign<double, 3>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
+        default: hipLaunchKernelGGL((k_assign<double, 4>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
+      }
+      TRVG_LAUNCH_CHECK();
+      TRVG_HIP(hipMemcpyAsync(h.data(), f.get(), nm * sizeof(double2), hipMemcpyDeviceToHost, ctx.stream));
+      TRVG_HIP(hipStreamSynchronize(ctx.stream));
+    }
+    std::memcpy(mesh_out, h.data(), 2 * nm * sizeof(double));
+  });
+}
+
+// In-place 3-D FFT of an interleaved complex host mesh (sign -1 forward,
+// +1 backward, unnormalised), run on the device in params->precision.
+int trvg_fft3d(const trvg_params* params, double* mesh, int sign) {
+  return guarded([&] {
+    require_device();
+    if (params == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
+    trvg_params q = *params;
+    validate_params(q, 1);
+    Context ctx(q);
+    const long long nm = ctx.g.nmesh;
+    if (q.precision == TRVG_PREC_FP32) {
+      std::vector<float> hf(2 * nm);
+      for (long long i = 0; i < 2 * nm; i++) hf[i] = float(mesh[i]);
+      DevArray<float2> f(nm);
+      TRVG_HIP(hipMemcpyAsync(f.get(), hf.data(), nm * sizeof(float2), hipMemcpyHostToDevice, ctx.stream));
+      fft3d<float>(ctx, f.get(), LoadSame<float>{f.get()}, sign < 0 ? -1 : 1);
+      TRVG_HIP(hipMemcpyAsync(hf.data(), f.get(), nm * sizeof(float2), hipMemcpyDeviceToHost, ctx.stream));
+      TRVG_HIP(hipStreamSynchronize(ctx.stream));
+      for (long long i = 0; i < 2 * nm; i++) mesh[i] = hf[i];
+    } else {
+      DevArray<double2> f(nm);
+      TRVG_HIP(hipMemcpyAsync(f.get(), mesh, nm * sizeof(double2), hipMemcpyHostToDevice, ctx.stream));
+      fft3d<double>(ctx, f.get(), LoadSame<double>{f.get()}, sign < 0 ? -1 : 1);
+      TRVG_HIP(hipMemcpyAsync(mesh, f.get(), nm * sizeof(double2), hipMemcpyDeviceToHost, ctx.stream));
+      TRVG_HIP(hipStreamSynchronize(ctx.stream));
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,279 @@
+// common.hpp -- shared host/device definitions of the gfx950 engine.
+//
+// Layout: every mesh field is an interleaved complex array of
+// ngrid[0]*ngrid[1]*ngrid[2] cells, row-major with the z index fastest
+// (the index order of MeshField::ret_grid_index,
+// src/triumvirate/src/field.cpp:533-538).  T2 is double2 (FP64 path) or
+// float2 (FP32 path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/trv_gpu.h"
+
+namespace trvg {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define TRVG_HIP(expr)                                                      \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      throw ::trvg::Error(TRVG_ERR_DEVICE, std::string(#expr) + " failed: " \
+                                               + hipGetErrorString(e_));    \
+    }                                                                       \
+  } while (0)
+
+#define TRVG_LAUNCH_CHECK() TRVG_HIP(hipGetLastError())
+
+template <typename T> struct C2;
+template <> struct C2<double> { using type = double2; };
+template <> struct C2<float> { using type = float2; };
+template <typename T> using c2 = typename C2<T>::type;
+
+template <typename T>
+__host__ __device__ __forceinline__ c2<T> mk(T a, T b) {
+  c2<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+__host__ __device__ __forceinline__ c2<T> cadd(c2<T> a, c2<T> b) {
+  return mk<T>(a.x + b.x, a.y + b.y);
+}
+template <typename T>
+__host__ __device__ __forceinline__ c2<T> csub(c2<T> a, c2<T> b) {
+  return mk<T>(a.x - b.x, a.y - b.y);
+}
+template <typename T>
+__host__ __device__ __forceinline__ c2<T> cmul(c2<T> a, c2<T> b) {
+  return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__host__ __device__ __forceinline__ c2<T> cscale(c2<T> a, T s) {
+  return mk<T>(a.x * s, a.y * s);
+}
+
+__host__ __device__ __forceinline__ double2 dmk(double a, double b) {
+  return make_double2(a, b);
+}
+__host__ __device__ __forceinline__ double2 dmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ __forceinline__ double2 dconj(double2 a) {
+  return make_double2(a.x, -a.y);
+}
+
+// ---------------------------------------------------------------------------
+// Mesh geometry (uniform over every kernel)
+// ---------------------------------------------------------------------------
+
+struct Geom {
+  int n[3];
+  long long nmesh;
+  double L[3];
+  double dk[3];       // 2 pi / L
+  double dr[3];       // L / n
+  double vol, vol_cell;
+};
+
+inline Geom make_geom(const trvg_params& p) {
+  Geom g;
+  for (int a = 0; a < 3; a++) {
+    g.n[a] = p.ngrid[a];
+    g.L[a] = p.boxsize[a];
+    g.dk[a] = 2. * M_PI / p.boxsize[a];
+    g.dr[a] = p.boxsize[a] / p.ngrid[a];
+  }
+  g.nmesh = (long long)g.n[0] * g.n[1] * g.n[2];
+  g.vol = g.L[0] * g.L[1] * g.L[2];
+  g.vol_cell = g.vol / double(g.nmesh);
+  return g;
+}
+
+// Signed FFT frequency / position index (field.cpp:540-562).
+__host__ __device__ __forceinline__ int shifted(int i, int n) {
+  return (i < n / 2) ? i : i - n;
+}
+
+// |v| = sqrt(x*x + y*y + z*z) with every operation IEEE round-to-nearest
+// and no contraction, so that bin membership tests agree bit-for-bit with
+// the reference's plain FP64 arithmetic (maths.cpp:57-59).
+__device__ __forceinline__ double mag3_exact(double x, double y, double z) {
+  double s = __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)),
+                       __dmul_rn(z, z));
+  return __dsqrt_rn(s);
+}
+
+// Wavevector / separation component: (i or i - n) * step, exactly rounded.
+__device__ __forceinline__ double grid_coord(int i, int n, double step) {
+  return __dmul_rn(double(shifted(i, n)), step);
+}
+
+// ---------------------------------------------------------------------------
+// Reduced spherical harmonic  sqrt(4 pi/(2l+1)) conj(Y_lm(v^))
+// (maths.cpp:171-220).  Special cases as the reference: 1 for (0,0),
+// 0 for |v| < 1e-9, azimuth 0 for r_xy < 1e-9.  The azimuthal factor
+// e^{i m phi} is formed as ((x + i y)/r_xy)^m instead of through acos.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline double sph_plm_dev(int l, int m, double x) {
+  double pmm = 1.;
+  double somx2 = sqrt((1. - x) * (1. + x));
+  for (int i = 1; i <= m; i++) pmm *= -somx2 * sqrt((2. * i - 1.) / (2. * i));
+  pmm *= sqrt((2. * m + 1.) / (4. * M_PI));
+  if (l == m) return pmm;
+  double pmmp1 = x * sqrt(2. * m + 3.) * pmm;
+  if (l == m + 1) return pmmp1;
+  double pll = 0.;
+  for (int ll = m + 2; ll <= l; ll++) {
+    double a = sqrt((4. * ll * ll - 1.) / (double(ll * ll) - double(m * m)));
+    double b = sqrt(((2. * ll + 1.) * (double((ll - 1) * (ll - 1)) - double(m * m)))
+                    / ((2. * ll - 3.) * (double(ll * ll) - double(m * m))));
+    pll = a * x * pmmp1 - b * pmm;
+    pmm = pmmp1;
+    pmmp1 = pll;
+  }
+  return pll;
+}
+
+__host__ __device__ inline double2 ylm_reduced_dev(int ell, int m, double x,
+                                                   double y, double z) {
+  if (ell == 0 && m == 0) return dmk(1., 0.);
+  double r = sqrt(x * x + y * y + z * z);
+  if (fabs(r) < 1.e-9) return dmk(0., 0.);
+  double mu = z / r;
+  double rxy = sqrt(x * x + y * y);
+  double c = 1., s = 0.;
+  if (fabs(rxy) >= 1.e-9) {
+    c = x / rxy;
+    s = y / rxy;
+  }
+  int am = m < 0 ? -m : m;
+  double er = 1., ei = 0.;  // (c + i s)^|m|
+  for (int q = 0; q < am; q++) {
+    double t = er * c - ei * s;
+    ei = er * s + ei * c;
+    er = t;
+  }
+  if (m < 0) ei = -ei;  // e^{i m phi}
+  double p = sph_plm_dev(ell, am, mu);
+  double parity = (m < 0 && (am & 1)) ? -1. : 1.;
+  double norm = sqrt(4. * M_PI / (2. * ell + 1.));
+  // parity * conj(e^{i m phi} P) * norm
+  return dmk(parity * er * p * norm, -parity * ei * p * norm);
+}
+
+// ---------------------------------------------------------------------------
+// Spherical Bessel j_l (FP64), same algorithm as the host node generator:
+// ascending series for x < 1 + l, upward recurrence otherwise.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline double sph_bessel_dev(int l, double x) {
+  if (x == 0.) return (l == 0) ? 1. : 0.;
+  if (x < 1. + l) {
+    double pref = 1.;
+    for (int i = 1; i <= l; i++) pref *= x / (2. * i + 1.);
+    double term = 1., sum = 1., h = -0.5 * x * x;
+    for (int k = 1; k < 200; k++) {
+      term *= h / (k * (2. * l + 2. * k + 1.));
+      sum += term;
+      if (fabs(term) < 1.e-18 * fabs(sum)) break;
+    }
+    return pref * sum;
+  }
+  double sx = sin(x), cx = cos(x);
+  double j0 = sx / x;
+  if (l == 0) return j0;
+  double j1 = sx / (x * x) - cx / x;
+  for (int n = 1; n < l; n++) {
+    double j2 = (2. * n + 1.) / x * j1 - j0;
+    j0 = j1;
+    j1 = j2;
+  }
+  return j1;
+}
+
+// Natural cubic spline of j_l on x_i = 0.05 i (maths.cpp:309-375).
+// y: node values; c: second derivative / 2 at the nodes.
+struct SplineDev {
+  const double* y;
+  const double* c;
+  int n;          // number of nodes
+  int ell;
+  double step;    // 0.05
+  double split;   // max(1000, l^2)
+
+  __device__ double eval(double x) const {
+    if (x >= split) return sph_bessel_dev(ell, x);
+    // Interval i with x_i <= x < x_{i+1}, x_i = 0 + step * i.
+    int i = int(x / step);
+    if (i > n - 2) i = n - 2;
+    if (i < 0) i = 0;
+    if (x < __dmul_rn(step, double(i)) && i > 0) i--;
+    else if (i + 1 < n - 1 && x >= __dmul_rn(step, double(i + 1))) i++;
+    double xlo = __dmul_rn(step, double(i));
+    double xhi = __dmul_rn(step, double(i + 1));
+    double dx = xhi - xlo;
+    double dy = y[i + 1] - y[i];
+    double delx = x - xlo;
+    double ci = c[i], cip1 = c[i + 1];
+    double b = dy / dx - dx * (cip1 + 2. * ci) / 3.;
+    double d = (cip1 - ci) / (3. * dx);
+    return y[i] + delx * (b + delx * (ci + delx * d));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Per-axis tables for the assignment window W(k) and the aliasing C1(k)
+// (field.cpp:1114-1201, 3403-3564), filled on the host in FP64.
+// ---------------------------------------------------------------------------
+
+struct AxisTables {
+  const double* sinc[3];   // sin(u)/u (1 at u = 0), u = pi * shifted(i) / n
+  const double* sin2[3];   // sin(u)^2 (0 at u = 0)
+  const double* u[3];      // u itself (for the interlaced iso-approximation)
+  int order;
+  int interlace;
+};
+
+__device__ __forceinline__ double window_at(const AxisTables& t, int i, int j,
+                                            int k) {
+  double wk = t.sinc[0][i] * t.sinc[1][j] * t.sinc[2][k];
+  double w = wk;
+  for (int q = 1; q < t.order; q++) w *= wk;
+  return w;
+}
+
+__device__ __forceinline__ double c1_axis(double s2, int order) {
+  if (order == 1) return 1.;
+  if (order == 2) return 1. - 2. / 3. * s2;
+  if (order == 3) return 1. - s2 + 2. / 15. * s2 * s2;
+  return 1. - 4. / 3. * s2 + 2. / 5. * s2 * s2 - 4. / 315. * s2 * s2 * s2;
+}
+
+__device__ __forceinline__ double alias_at(const AxisTables& t, int i, int j,
+                                           int k) {
+  if (!t.interlace) {
+    return c1_axis(t.sin2[0][i], t.order) * c1_axis(t.sin2[1][j], t.order)
+           * c1_axis(t.sin2[2][k], t.order);
+  }
+  double ux = t.u[0][i], uy = t.u[1][j], uz = t.u[2][k];
+  double uh = sqrt(ux * ux + uy * uy + uz * uz) / 2.;
+  double s2h = sin(uh) * sin(uh);
+  double c2h = cos(uh) * cos(uh);
+  double cc = c2h;
+  for (int q = 1; q < t.order; q++) cc *= c2h;
+  return cc * (t.order == 1 ? 1. : c1_axis(s2h, t.order));
+}
+
+}  // namespace trvg

@@ -1,0 +1,164 @@
+// engine.hpp -- host-side runtime of the gfx950 three-point engine:
+// device memory pool, FFT plans (twiddle tables), per-axis window tables,
+// spherical-Bessel splines and the templated estimator drivers.
+#pragma once
+
+#include <complex>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "fft.hpp"
+#include "kernels.hpp"
+
+namespace trvg {
+
+using cplx = std::complex<double>;
+
+// ---------------------------------------------------------------------------
+// Device memory: a size-keyed cache of hipMalloc'ed blocks, reused across
+// calls so that steady-state calls do not allocate.
+// ---------------------------------------------------------------------------
+
+class Pool {
+ public:
+  void* get(size_t bytes);
+  void put(void* p, size_t bytes);
+  void release_all();
+  ~Pool() { release_all(); }
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+};
+
+Pool& pool();
+
+template <typename E>
+class DevArray {
+ public:
+  DevArray() = default;
+  explicit DevArray(size_t n) { reset(n); }
+  ~DevArray() { free_mem(); }
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  DevArray(DevArray&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevArray& operator=(DevArray&& o) noexcept {
+    if (this != &o) { free_mem(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  void reset(size_t n) {
+    free_mem();
+    n_ = n;
+    p_ = n ? static_cast<E*>(pool().get(n * sizeof(E))) : nullptr;
+  }
+  E* get() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  void free_mem() {
+    if (p_) pool().put(p_, n_ * sizeof(E));
+    p_ = nullptr;
+    n_ = 0;
+  }
+  E* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Device catalogue (owned copy of the columns the kernels read).
+// ---------------------------------------------------------------------------
+
+struct DeviceCatalogue {
+  long long n = 0;
+  DevArray<double> x, y, z, w, los;
+  double wstotal = 0., wtotal = 0.;
+  DevParticles view() const {
+    DevParticles p;
+    p.x = x.get(); p.y = y.get(); p.z = z.get(); p.w = w.get();
+    p.los = los.get();
+    p.n = n;
+    return p;
+  }
+};
+
+// Upload a (possibly strided) host catalogue; `los` optional (n x 3).
+void upload_catalogue(const trvg_catalogue* c, const double* los,
+                      DeviceCatalogue& out, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Spherical Bessel spline (host build, device tables).
+// ---------------------------------------------------------------------------
+
+struct Spline {
+  int ell = 0;
+  double split = 1000.;
+  double step = 0.05;
+  int n = 0;
+  DevArray<double> y, c;
+  SplineDev dev() const {
+    SplineDev s;
+    s.y = y.get(); s.c = c.get(); s.n = n; s.ell = ell; s.step = step;
+    s.split = split;
+    return s;
+  }
+};
+
+void build_spline(int ell, Spline& sp, hipStream_t s);
+
+// Host natural-spline node values and coefficients (for tests).
+void spline_nodes(int ell, std::vector<double>& y, std::vector<double>& c,
+                  double& split);
+
+// ---------------------------------------------------------------------------
+// Per-call context.
+// ---------------------------------------------------------------------------
+
+struct Context {
+  trvg_params p;
+  Geom g;
+  hipStream_t stream = nullptr;
+  DevArray<double> tab_data;   // per-axis sinc, sin^2, u
+  AxisTables tab;
+  int sm_count = 256;
+
+  explicit Context(const trvg_params& params);
+  ~Context();
+};
+
+// Maths shared with the drivers.
+double wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3);
+double coupling_3pt(int l1, int l2, int L, int m1, int m2, int M);
+int result_dim(int shape, int nb, int idx_bin);
+void pair_bins(int shape, int nb, int idx_bin, std::vector<int>& ra,
+               std::vector<int>& cb);
+void validate_params(const trvg_params& p, int num_bins);
+
+// FFT of a mesh: first pass (z) loads through `Loader`, then y and x passes.
+template <typename T, class Loader>
+void fft3d(Context& ctx, c2<T>* data, const Loader& load, int sign);
+
+// Estimators.
+struct Result {
+  int dim = 0;
+  std::vector<double> b1, b2, e1, e2;
+  std::vector<int> n1, n2;
+  std::vector<cplx> raw, shot;
+};
+
+template <typename T>
+Result run_bispec(Context& ctx, const DeviceCatalogue& d,
+                  const DeviceCatalogue* r, bool survey,
+                  const trvg_binning& kb, double norm_factor);
+
+template <typename T>
+Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
+                const DeviceCatalogue* r, bool survey,
+                const trvg_binning& rb, double norm_factor);
+
+template <typename T>
+double run_mesh_norm(Context& ctx, const DeviceCatalogue& c);
+
+}  // namespace trvg

@@ -1,0 +1,310 @@
+// engine_core.hip -- host runtime pieces of the engine: memory pool,
+// catalogue upload, spline construction, window tables, Wigner 3-j and the
+// binning-shape index maps.
+#include <algorithm>
+#include <cstring>
+
+#include "engine.hpp"
+
+namespace trvg {
+
+// ---------------------------------------------------------------------------
+// Pool
+// ---------------------------------------------------------------------------
+
+Pool& pool() {
+  static Pool* p = new Pool();  // process lifetime; device memory is
+                                // reclaimed with the context at exit
+  return *p;
+}
+
+void* Pool::get(size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = free_.find(bytes);
+    if (it != free_.end()) {
+      void* p = it->second;
+      free_.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    release_all();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(TRVG_ERR_ALLOC,
+                  "Device allocation of " + std::to_string(bytes) +
+                      " bytes failed: " + hipGetErrorString(e));
+    }
+  }
+  return p;
+}
+
+void Pool::put(void* p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(mu_);
+  free_.emplace(bytes, p);
+}
+
+void Pool::release_all() {
+  std::lock_guard<std::mutex> lk(mu_);
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Catalogue upload
+// ---------------------------------------------------------------------------
+
+void upload_catalogue(const trvg_catalogue* c, const double* los,
+                      DeviceCatalogue& out, hipStream_t s) {
+  if (c == nullptr || c->x == nullptr || c->y == nullptr || c->z == nullptr)
+    throw Error(TRVG_ERR_DATA, "Particle data are uninitialised.");
+  if (c->ntotal <= 0)
+    throw Error(TRVG_ERR_DATA, "Particle catalogue is empty.");
+  const long long n = c->ntotal;
+  const long long st = c->stride > 0 ? c->stride : 1;
+  std::vector<double> hx(n), hy(n), hz(n), hw(n);
+  double wst = 0., wt = 0.;
+  for (long long i = 0; i < n; i++) {
+    hx[i] = c->x[i * st];
+    hy[i] = c->y[i * st];
+    hz[i] = c->z[i * st];
+    double ws = c->ws ? c->ws[i * st] : 1.;
+    double wc = c->wc ? c->wc[i * st] : 1.;
+    hw[i] = ws * wc;   // particles.cpp:553
+    wst += ws;
+    wt += hw[i];
+  }
+  out.n = n;
+  out.wstotal = wst;
+  out.wtotal = wt;
+  out.x.reset(n); out.y.reset(n); out.z.reset(n); out.w.reset(n);
+  TRVG_HIP(hipMemcpyAsync(out.x.get(), hx.data(), n * 8, hipMemcpyHostToDevice, s));
+  TRVG_HIP(hipMemcpyAsync(out.y.get(), hy.data(), n * 8, hipMemcpyHostToDevice, s));
+  TRVG_HIP(hipMemcpyAsync(out.z.get(), hz.data(), n * 8, hipMemcpyHostToDevice, s));
+  TRVG_HIP(hipMemcpyAsync(out.w.get(), hw.data(), n * 8, hipMemcpyHostToDevice, s));
+  if (los) {
+    out.los.reset(3 * n);
+    TRVG_HIP(hipMemcpyAsync(out.los.get(), los, 3 * n * 8,
+                            hipMemcpyHostToDevice, s));
+  }
+  TRVG_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Spherical Bessel spline (maths.cpp:309-375): nodes x_i = 0.05 i up to
+// split = max(1000, l^2); natural boundary; second-derivative/2
+// coefficients from the tridiagonal system (Thomas algorithm).
+// ---------------------------------------------------------------------------
+
+void spline_nodes(int ell, std::vector<double>& y, std::vector<double>& c,
+                  double& split) {
+  const double step = 0.05;
+  split = 1000.;
+  split = (split >= double(ell * ell)) ? split : double(ell * ell);
+  int n = int((split - 0.) / step) + 1;
+  std::vector<double> x(n);
+  y.resize(n);
+  for (int i = 0; i < n; i++) {
+    x[i] = 0. + step * i;
+    y[i] = sph_bessel_dev(ell, x[i]);
+  }
+  c.assign(n, 0.);
+  int m = n - 2;
+  std::vector<double> diag(m), off(m), rhs(m), sol(m);
+  for (int i = 1; i <= m; i++) {
+    double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+    diag[i - 1] = 2. * (h0 + h1);
+    off[i - 1] = h1;
+    rhs[i - 1] = 3. * ((y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0);
+  }
+  for (int i = 1; i < m; i++) {
+    double f = off[i - 1] / diag[i - 1];
+    diag[i] -= f * off[i - 1];
+    rhs[i] -= f * rhs[i - 1];
+  }
+  sol[m - 1] = rhs[m - 1] / diag[m - 1];
+  for (int i = m - 2; i >= 0; i--) sol[i] = (rhs[i] - off[i] * sol[i + 1]) / diag[i];
+  for (int i = 0; i < m; i++) c[i + 1] = sol[i];
+}
+
+void build_spline(int ell, Spline& sp, hipStream_t s) {
+  std::vector<double> y, c;
+  double split;
+  spline_nodes(ell, y, c, split);
+  sp.ell = ell;
+  sp.split = split;
+  sp.n = int(y.size());
+  sp.y.reset(sp.n);
+  sp.c.reset(sp.n);
+  TRVG_HIP(hipMemcpyAsync(sp.y.get(), y.data(), sp.n * 8, hipMemcpyHostToDevice, s));
+  TRVG_HIP(hipMemcpyAsync(sp.c.get(), c.data(), sp.n * 8, hipMemcpyHostToDevice, s));
+  TRVG_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+Context::Context(const trvg_params& params) : p(params) {
+  if (p.device >= 0) TRVG_HIP(hipSetDevice(p.device));
+  g = make_geom(p);
+  TRVG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  int dev = 0;
+  TRVG_HIP(hipGetDevice(&dev));
+  TRVG_HIP(hipDeviceGetAttribute(&sm_count, hipDeviceAttributeMultiprocessorCount, dev));
+  // Per-axis tables: sinc, sin^2, u for each axis (field.cpp:1138-1149,
+  // 3441-3465).
+  long long tot = 3LL * (g.n[0] + g.n[1] + g.n[2]);
+  std::vector<double> h(tot);
+  long long off = 0;
+  long long offs[3][3];
+  for (int kind = 0; kind < 3; kind++) {
+    for (int a = 0; a < 3; a++) {
+      offs[kind][a] = off;
+      for (int i = 0; i < g.n[a]; i++) {
+        int ii = shifted(i, g.n[a]);
+        double u = M_PI * ii / double(g.n[a]);
+        double v;
+        if (kind == 0) v = (ii != 0) ? std::sin(u) / u : 1.;
+        else if (kind == 1) v = (ii != 0) ? std::sin(u) * std::sin(u) : 0.;
+        else v = u;
+        h[off + i] = v;
+      }
+      off += g.n[a];
+    }
+  }
+  tab_data.reset(tot);
+  TRVG_HIP(hipMemcpyAsync(tab_data.get(), h.data(), tot * 8, hipMemcpyHostToDevice, stream));
+  TRVG_HIP(hipStreamSynchronize(stream));
+  for (int a = 0; a < 3; a++) {
+    tab.sinc[a] = tab_data.get() + offs[0][a];
+    tab.sin2[a] = tab_data.get() + offs[1][a];
+    tab.u[a] = tab_data.get() + offs[2][a];
+  }
+  tab.order = p.assignment_order;
+  tab.interlace = p.interlace;
+}
+
+Context::~Context() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wigner 3-j (Racah's formula) and coupling (threept.cpp:65-71).
+// ---------------------------------------------------------------------------
+
+static double lfact(int n) { return std::lgamma(double(n) + 1.); }
+
+double wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+  if (m1 + m2 + m3 != 0) return 0.;
+  if (j3 < std::abs(j1 - j2) || j3 > j1 + j2) return 0.;
+  if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3) return 0.;
+  double pre = 0.5 * (lfact(j1 + j2 - j3) + lfact(j1 - j2 + j3)
+                      + lfact(-j1 + j2 + j3) - lfact(j1 + j2 + j3 + 1)
+                      + lfact(j1 + m1) + lfact(j1 - m1) + lfact(j2 + m2)
+                      + lfact(j2 - m2) + lfact(j3 + m3) + lfact(j3 - m3));
+  int kmin = std::max(0, std::max(j2 - j3 - m1, j1 - j3 + m2));
+  int kmax = std::min(j1 + j2 - j3, std::min(j1 - m1, j2 + m2));
+  double sum = 0.;
+  for (int k = kmin; k <= kmax; k++) {
+    double t = -(lfact(k) + lfact(j1 + j2 - j3 - k) + lfact(j1 - m1 - k)
+                 + lfact(j2 + m2 - k) + lfact(j3 - j2 + m1 + k)
+                 + lfact(j3 - j1 - m2 + k));
+    sum += ((k % 2 == 0) ? 1. : -1.) * std::exp(pre + t);
+  }
+  int ph = j1 - j2 - m3;
+  return ((((ph % 2) + 2) % 2 == 0) ? 1. : -1.) * sum;
+}
+
+double coupling_3pt(int l1, int l2, int L, int m1, int m2, int M) {
+  return double(2 * l1 + 1) * double(2 * l2 + 1) * double(2 * L + 1)
+         * wigner_3j(l1, l2, L, 0, 0, 0) * wigner_3j(l1, l2, L, m1, m2, M);
+}
+
+int result_dim(int shape, int nb, int idx_bin) {
+  switch (shape) {
+    case TRVG_SHAPE_DIAG: case TRVG_SHAPE_ROW: return nb;
+    case TRVG_SHAPE_OFFDIAG: return nb - std::abs(idx_bin);
+    case TRVG_SHAPE_FULL: return nb * nb;
+    case TRVG_SHAPE_TRIU: return nb * (nb + 1) / 2;
+    default:
+      throw Error(TRVG_ERR_PARAM, "Three-point statistic form is not recognised.");
+  }
+}
+
+// (row bin, column bin) per data-vector entry (threept.cpp:465-770).
+void pair_bins(int shape, int nb, int idx_bin, std::vector<int>& ra,
+               std::vector<int>& cb) {
+  int dim = result_dim(shape, nb, idx_bin);
+  ra.assign(dim, 0);
+  cb.assign(dim, 0);
+  if (shape == TRVG_SHAPE_DIAG) {
+    for (int d = 0; d < dim; d++) ra[d] = cb[d] = d;
+  } else if (shape == TRVG_SHAPE_OFFDIAG) {
+    for (int d = 0; d < dim; d++) {
+      if (idx_bin >= 0) { ra[d] = d; cb[d] = d + std::abs(idx_bin); }
+      else { ra[d] = d + std::abs(idx_bin); cb[d] = d; }
+    }
+  } else if (shape == TRVG_SHAPE_ROW) {
+    for (int d = 0; d < dim; d++) { ra[d] = idx_bin; cb[d] = d; }
+  } else if (shape == TRVG_SHAPE_FULL) {
+    for (int r = 0; r < nb; r++)
+      for (int c = 0; c < nb; c++) { ra[r * nb + c] = r; cb[r * nb + c] = c; }
+  } else {
+    for (int r = 0; r < nb; r++)
+      for (int c = r; c < nb; c++) {
+        int d = (2 * nb - r + 1) * r / 2 + (c - r);
+        ra[d] = r;
+        cb[d] = c;
+      }
+  }
+}
+
+static bool pow2_in_range(int n) {
+  return n >= 8 && n <= 1024 && (n & (n - 1)) == 0;
+}
+
+void validate_params(const trvg_params& p, int num_bins) {
+  for (int a = 0; a < 3; a++) {
+    if (!(p.boxsize[a] > 0.))
+      throw Error(TRVG_ERR_PARAM, "Box size must be positive.");
+    if (!pow2_in_range(p.ngrid[a]))
+      throw Error(TRVG_ERR_PARAM,
+                  "Mesh number per dimension must be a power of two in "
+                  "[8, 1024]: ngrid[" + std::to_string(a) + "] = " +
+                      std::to_string(p.ngrid[a]) + ".");
+  }
+  long long nmesh = (long long)p.ngrid[0] * p.ngrid[1] * p.ngrid[2];
+  if (nmesh < 2048)
+    throw Error(TRVG_ERR_PARAM, "Mesh must have at least 2048 cells.");
+  if (p.assignment_order < 1 || p.assignment_order > 4)
+    throw Error(TRVG_ERR_PARAM,
+                "Mesh assignment scheme must be 'ngp', 'cic', 'tsc' or 'pcs'.");
+  if (p.precision != TRVG_PREC_FP64 && p.precision != TRVG_PREC_FP32)
+    throw Error(TRVG_ERR_PARAM, "Precision must be 64 or 32.");
+  if (p.ell1 < 0 || p.ell2 < 0 || p.ELL < 0)
+    throw Error(TRVG_ERR_PARAM, "Multipole degrees must be non-negative.");
+  if (num_bins <= 0)
+    throw Error(TRVG_ERR_PARAM, "Number of bins must be positive.");
+  if ((p.shape == TRVG_SHAPE_ROW || p.shape == TRVG_SHAPE_OFFDIAG) &&
+      std::abs(p.idx_bin) >= num_bins)
+    throw Error(TRVG_ERR_PARAM,
+                "Bin index `idx_bin` must be < `num_bins` in absolute value.");
+  if (p.shape == TRVG_SHAPE_ROW && p.idx_bin < 0)
+    throw Error(TRVG_ERR_PARAM, "Row index `idx_bin` must be non-negative.");
+  (void)result_dim(p.shape, num_bins, p.idx_bin);
+  if (std::fabs(wigner_3j(p.ell1, p.ell2, p.ELL, 0, 0, 0)) < 1.e-9)
+    throw Error(TRVG_ERR_PARAM,
+                "Specified three-point correlator multipole vanishes "
+                "identically owing to zero-valued Wigner 3-j symbol.");
+}
+
+}  // namespace trvg

@@ -1,0 +1,208 @@
+// fft.hpp -- 3-D complex FFT for gfx950: three batched 1-D passes, each a
+// radix-4/radix-2 Stockham autosort transform held in LDS (the
+// formulation of Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors", SC 2008).
+//
+// Sizes: powers of two from 8 to 1024 per axis.  Sign convention as FFTW
+// (forward e^{-ikx}, backward e^{+ikx}), unnormalised.
+//
+// Each pass is in place: a workgroup loads E = 256*EPT complex elements
+// (L = E/n lines of one axis), transforms them in LDS and writes them back
+// to the same addresses, so no second buffer is needed.  The first pass
+// (along z, contiguous lines) takes a "loader" functor that produces each
+// input element from one or more source fields; shell filters, window
+// deconvolution, scalings and two-point products are fused there instead
+// of costing a separate read+write sweep of the mesh.
+#pragma once
+
+#include "common.hpp"
+
+namespace trvg {
+
+constexpr int FFT_THREADS = 256;
+
+// In-LDS Stockham transform of L lines of length N (N = 2^LOGN) stored as
+// buf[line * (N + 1) + n] (one pad element per line against bank conflicts).  W[m] = e^{sign 2 pi i m / N} (m < N) in LDS.
+template <typename T, int LOGN, int EPT>
+__device__ __forceinline__ void lds_fft_lines(c2<T>* buf, const c2<T>* W,
+                                              int sign) {
+  constexpr int N = 1 << LOGN;
+  constexpr int LS = N + 1;  // padded LDS line stride
+  constexpr int E = FFT_THREADS * EPT;
+  constexpr int NLINES = E / N;
+  const int t = threadIdx.x;
+  int Ns = 1;
+  int rem = LOGN;
+#pragma unroll
+  for (int stage = 0; stage < (LOGN + 1) / 2; stage++) {
+    if (rem >= 2) {
+      // radix-4
+      constexpr int NB = E / 4;               // butterflies per workgroup
+      constexpr int PER = NB / FFT_THREADS;   // = EPT / 4
+      c2<T> v[PER > 0 ? PER : 1][4];
+      int jj[PER > 0 ? PER : 1], ll[PER > 0 ? PER : 1];
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        int b = t + u * FFT_THREADS;
+        int line = b / (N / 4);
+        int j = b % (N / 4);
+        jj[u] = j;
+        ll[u] = line;
+        int k = j & (Ns - 1);
+        const c2<T>* src = buf + line * LS;
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[u][q] = src[j + q * (N / 4)];
+        int tstep = k * (N / (Ns * 4));
+        v[u][1] = cmul<T>(v[u][1], W[tstep]);
+        v[u][2] = cmul<T>(v[u][2], W[2 * tstep]);
+        v[u][3] = cmul<T>(v[u][3], W[3 * tstep]);
+        c2<T> t0 = cadd<T>(v[u][0], v[u][2]);
+        c2<T> t1 = csub<T>(v[u][0], v[u][2]);
+        c2<T> t2 = cadd<T>(v[u][1], v[u][3]);
+        c2<T> d = csub<T>(v[u][1], v[u][3]);
+        // t3 = sign * i * d
+        c2<T> t3 = (sign < 0) ? mk<T>(d.y, -d.x) : mk<T>(-d.y, d.x);
+        v[u][0] = cadd<T>(t0, t2);
+        v[u][2] = csub<T>(t0, t2);
+        v[u][1] = cadd<T>(t1, t3);
+        v[u][3] = csub<T>(t1, t3);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        int j = jj[u];
+        int k = j & (Ns - 1);
+        int idxD = (j / Ns) * Ns * 4 + k;
+        c2<T>* dst = buf + ll[u] * LS;
+#pragma unroll
+        for (int q = 0; q < 4; q++) dst[idxD + q * Ns] = v[u][q];
+      }
+      __syncthreads();
+      Ns *= 4;
+      rem -= 2;
+    } else if (rem == 1) {
+      constexpr int NB = E / 2;
+      constexpr int PER = NB / FFT_THREADS;   // = EPT / 2
+      c2<T> v[PER][2];
+      int jj[PER], ll[PER];
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        int b = t + u * FFT_THREADS;
+        int line = b / (N / 2);
+        int j = b % (N / 2);
+        jj[u] = j;
+        ll[u] = line;
+        int k = j & (Ns - 1);
+        const c2<T>* src = buf + line * LS;
+        v[u][0] = src[j];
+        v[u][1] = cmul<T>(src[j + N / 2], W[k * (N / (Ns * 2))]);
+        c2<T> a = v[u][0], bb = v[u][1];
+        v[u][0] = cadd<T>(a, bb);
+        v[u][1] = csub<T>(a, bb);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        int j = jj[u];
+        int k = j & (Ns - 1);
+        int idxD = (j / Ns) * Ns * 2 + k;
+        c2<T>* dst = buf + ll[u] * LS;
+        dst[idxD] = v[u][0];
+        dst[idxD + Ns] = v[u][1];
+      }
+      __syncthreads();
+      Ns *= 2;
+      rem -= 1;
+    }
+  }
+  (void)NLINES;
+}
+
+// First pass along z (contiguous lines), with a fused loader.
+// Loader: c2<T> operator()(long long g, int i, int j, int k) const.
+template <typename T, int LOGN, int EPT, class Loader>
+__global__ void __launch_bounds__(FFT_THREADS)
+fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
+           int n0, int n1, int sign) {
+  constexpr int N = 1 << LOGN;
+  constexpr int E = FFT_THREADS * EPT;
+  constexpr int NLINES = E / N;
+  extern __shared__ char smem_raw[];
+  constexpr int LS = N + 1;
+  c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
+  c2<T>* W = buf + NLINES * LS;
+  const int t = threadIdx.x;
+  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
+  const long long line0 = (long long)blockIdx.x * NLINES;
+  const long long base = line0 * N;
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    int idx = t + e * FFT_THREADS;
+    long long g = base + idx;
+    long long line = line0 + idx / N;
+    int k = idx % N;
+    int i = int(line / n1), j = int(line % n1);
+    buf[(idx / N) * LS + k] = load(g, i, j, k);
+  }
+  __syncthreads();
+  lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    int idx = t + e * FFT_THREADS;
+    data[base + idx] = buf[(idx / N) * LS + idx % N];
+  }
+  (void)n0;
+}
+
+// Strided pass: lines of length N with element stride S.  Line `id` starts
+// at (id / inner) * outer_stride + id % inner, so consecutive lines are
+// adjacent in memory within a slice:
+//   y pass: S = n2, inner = n2, outer_stride = n1*n2;
+//   x pass: S = n1*n2, inner = n1*n2, outer_stride = 0.
+// Each workgroup takes L = E/N consecutive lines.
+template <typename T, int LOGN, int EPT>
+__global__ void __launch_bounds__(FFT_THREADS)
+fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
+                 long long S, long long outer_stride, long long inner,
+                 int sign) {
+  constexpr int N = 1 << LOGN;
+  constexpr int E = FFT_THREADS * EPT;
+  constexpr int NLINES = E / N;
+  extern __shared__ char smem_raw[];
+  constexpr int LS = N + 1;
+  c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
+  c2<T>* W = buf + NLINES * LS;
+  const int t = threadIdx.x;
+  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
+  const long long id0 = (long long)blockIdx.x * NLINES;
+  // Global element (n, l): line id0 + l, position n; LDS line-major [l][n].
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    int idx = t + e * FFT_THREADS;
+    int n = idx / NLINES, l = idx % NLINES;
+    long long id = id0 + l;
+    long long base = (id / inner) * outer_stride + id % inner;
+    buf[l * LS + n] = data[base + (long long)n * S];
+  }
+  __syncthreads();
+  lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    int idx = t + e * FFT_THREADS;
+    int n = idx / NLINES, l = idx % NLINES;
+    long long id = id0 + l;
+    long long base = (id / inner) * outer_stride + id % inner;
+    data[base + (long long)n * S] = buf[l * LS + n];
+  }
+}
+
+// Identity loader (plain in-place transform).
+template <typename T>
+struct LoadSame {
+  const c2<T>* src;
+  __device__ c2<T> operator()(long long g, int, int, int) const {
+    return src[g];
+  }
+};
+
+}  // namespace trvg

@@ -1,0 +1,682 @@
+// kernels.hpp -- mesh kernels of the three-point engine (gfx950).
+//
+// All bin sums, normalisation sums and shot-noise amplitudes accumulate in
+// FP64; T (double or float) is only the storage/arithmetic type of mesh
+// values.  Integer counts (mode / pair counts) are exact.
+#pragma once
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace trvg {
+
+constexpr int RED_THREADS = 256;
+
+// ---------------------------------------------------------------------------
+// Particles
+// ---------------------------------------------------------------------------
+
+// Device catalogue: SoA FP64 columns.  w = ws * wc.  los may be null.
+struct DevParticles {
+  const double* x;
+  const double* y;
+  const double* z;
+  const double* w;
+  const double* los;   // 3 per particle (observer-frame unit vectors)
+  long long n;
+};
+
+enum WeightMode {
+  WM_UNIT = 0,       // 1
+  WM_W = 1,          // w
+  WM_YLM_W = 2,      // w * Y(los)                    (field.cpp:1246-1323)
+  WM_YLMC_W2 = 3,    // w^2 * conj(Y(los))            (field.cpp:1364-1447)
+};
+
+__device__ __forceinline__ double2 particle_weight(const DevParticles& p,
+                                                   long long i, int mode,
+                                                   int ell, int m) {
+  if (mode == WM_UNIT) return dmk(1., 0.);
+  double w = p.w[i];
+  if (mode == WM_W) return dmk(w, 0.);
+  double2 y = ylm_reduced_dev(ell, m, p.los[3 * i], p.los[3 * i + 1],
+                              p.los[3 * i + 2]);
+  if (mode == WM_YLM_W) return dmk(y.x * w, y.y * w);
+  double w2 = w * w;
+  return dmk(y.x * w2, -y.y * w2);
+}
+
+// Mesh assignment (field.cpp:569-1112): window weights and periodic index
+// rules per scheme; cell value += w * W_x W_y W_z / V_cell (times `scale`,
+// e.g. -alpha for randoms).  `shadow` selects the half-cell-shifted pass
+// used for interlacing.  Hardware FP64/FP32 atomics on the coarse-grained
+// mesh.
+template <typename T, int ORDER>
+__global__ void __launch_bounds__(256)
+k_assign(c2<T>* __restrict__ mesh, DevParticles p, Geom g, int mode, int ell,
+         int m, double scale, int cplx_w, int shadow) {
+  const double inv_vol_cell = 1 / g.vol_cell;
+  for (long long pid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       pid < p.n; pid += (long long)gridDim.x * blockDim.x) {
+    const double pos[3] = {p.x[pid], p.y[pid], p.z[pid]};
+    int ijk[ORDER][3];
+    double win[ORDER][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const int ng = g.n[a];
+      double loc = __ddiv_rn(__dmul_rn(double(ng), pos[a]), g.L[a]);
+      if (shadow) {
+        loc = __dadd_rn(loc, 0.5);
+        if (loc > ng) loc = __dadd_rn(loc, -double(ng));
+      }
+      int idx = int(loc);
+      double s = __dadd_rn(loc, -double(idx));
+      if (ORDER == 1) {
+        if (s >= 0.5) idx = (idx == ng - 1) ? 0 : idx + 1;
+        ijk[0][a] = idx;
+        win[0][a] = 1.;
+      } else if (ORDER == 2) {
+        ijk[0][a] = idx;
+        ijk[1][a] = (idx == ng - 1) ? 0 : idx + 1;
+        win[0][a] = 1. - s;
+        win[1][a] = s;
+      } else if (ORDER == 3) {
+        if (s < 0.5) {
+          ijk[0][a] = (idx == 0) ? ng - 1 : idx - 1;
+          ijk[1][a] = idx;
+          ijk[2][a] = (idx == ng - 1) ? 0 : idx + 1;
+          win[0][a] = 0.5 * (0.5 - s) * (0.5 - s);
+          win[1][a] = 0.75 - s * s;
+          win[2][a] = 0.5 * (0.5 + s) * (0.5 + s);
+        } else {
+          ijk[0][a] = idx;
+          ijk[1][a] = (idx == ng - 1) ? 0 : idx + 1;
+          ijk[2][a] = (ijk[1][a] == ng - 1) ? 0 : ijk[1][a] + 1;
+          double sp = 1. - s;
+          win[0][a] = 0.5 * (0.5 + sp) * (0.5 + sp);
+          win[1][a] = 0.75 - sp * sp;
+          win[2][a] = 0.5 * (0.5 - sp) * (0.5 - sp);
+        }
+      } else {
+        ijk[0][a] = (idx == 0) ? ng - 1 : idx - 1;
+        ijk[1][a] = idx;
+        ijk[2][a] = (idx == ng - 1) ? 0 : idx + 1;
+        ijk[3][a] = (ijk[2][a] == ng - 1) ? 0 : ijk[2][a] + 1;
+        double sm = 1. - s;
+        win[0][a] = 1. / 6 * sm * sm * sm;
+        win[1][a] = 1. / 6 * (4. - 6. * s * s + 3. * s * s * s);
+        win[2][a] = 1. / 6 * (4. - 6. * sm * sm + 3. * sm * sm * sm);
+        win[3][a] = 1. / 6 * s * s * s;
+      }
+    }
+    double2 wt = particle_weight(p, pid, mode, ell, m);
+    double wre = inv_vol_cell * wt.x * scale;
+    double wim = inv_vol_cell * wt.y * scale;
+#pragma unroll
+    for (int il = 0; il < ORDER; il++) {
+#pragma unroll
+      for (int jl = 0; jl < ORDER; jl++) {
+        const long long row =
+            ((long long)ijk[il][0] * g.n[1] + ijk[jl][1]) * g.n[2];
+        const double wij = win[il][0] * win[jl][1];
+#pragma unroll
+        for (int kl = 0; kl < ORDER; kl++) {
+          const long long gid = row + ijk[kl][2];
+          // Same guard as the reference: only the flattened index is
+          // checked (a particle at x = L lands in a neighbouring row).
+          if (gid < 0 || gid >= g.nmesh) continue;
+          const double wv = wij * win[kl][2];
+          T* cell = reinterpret_cast<T*>(mesh + gid);
+          unsafeAtomicAdd(cell, T(wre * wv));
+          if (cplx_w) unsafeAtomicAdd(cell + 1, T(wim * wv));
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused first-pass loaders (see fft.hpp)
+// ---------------------------------------------------------------------------
+
+// (src + shift) * scale, shift real (MeshField::fourier_transform with the
+// vol_cell factor, field.cpp:1496-1509; mean subtraction of
+// compute_unweighted_field_fluctuations_insitu, field.cpp:1225-1244).
+template <typename T>
+struct LoadScaleShift {
+  const c2<T>* src;
+  T shift, scale;
+  __device__ c2<T> operator()(long long g, int, int, int) const {
+    c2<T> v = src[g];
+    return mk<T>((v.x + shift) * scale, v.y * scale);
+  }
+};
+
+// src / W(k) * scale (apply_assignment_compensation then the 1/V of
+// inv_fourier_transform, field.cpp:1657-1720, 1764-1785).
+template <typename T>
+struct LoadDeconv {
+  const c2<T>* src;
+  AxisTables tab;
+  double scale;
+  __device__ c2<T> operator()(long long g, int i, int j, int k) const {
+    c2<T> v = src[g];
+    double f = scale / window_at(tab, i, j, k);
+    return mk<T>(T(double(v.x) * f), T(double(v.y) * f));
+  }
+};
+
+// Shell filter: y_lm(k^) src(k) / W(k) for k_lo <= |k| < k_hi, else 0
+// (inv_fourier_transform_ylm_wgtd_field_band_limited, field.cpp:1792-1906;
+// the 1/n_modes factor is applied to the pair sums instead).
+template <typename T>
+struct LoadShell {
+  const c2<T>* src;
+  AxisTables tab;
+  Geom geo;
+  double k_lo, k_hi;
+  int ell, m;
+  __device__ c2<T> operator()(long long g, int i, int j, int k) const {
+    double kx = grid_coord(i, geo.n[0], geo.dk[0]);
+    double ky = grid_coord(j, geo.n[1], geo.dk[1]);
+    double kz = grid_coord(k, geo.n[2], geo.dk[2]);
+    double km = mag3_exact(kx, ky, kz);
+    if (!(k_lo <= km && km < k_hi)) return mk<T>(T(0), T(0));
+    c2<T> v = src[g];
+    double iw = 1. / window_at(tab, i, j, k);
+    double2 fk = dmk(double(v.x) * iw, double(v.y) * iw);
+    double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
+    double2 r = dmul(y, fk);
+    return mk<T>(T(r.x), T(r.y));
+  }
+};
+
+// j_l(|k| r) y_lm(k^) src(k) / W(k) / V
+// (inv_fourier_transform_sjl_ylm_wgtd_field, field.cpp:1908-2010).
+template <typename T>
+struct LoadSjl {
+  const c2<T>* src;
+  AxisTables tab;
+  Geom geo;
+  SplineDev sj;
+  double r;
+  int ell, m;
+  __device__ c2<T> operator()(long long g, int i, int j, int k) const {
+    double kx = grid_coord(i, geo.n[0], geo.dk[0]);
+    double ky = grid_coord(j, geo.n[1], geo.dk[1]);
+    double kz = grid_coord(k, geo.n[2], geo.dk[2]);
+    double km = mag3_exact(kx, ky, kz);
+    c2<T> v = src[g];
+    double iw = 1. / window_at(tab, i, j, k);
+    double2 fk = dmk(double(v.x) * iw, double(v.y) * iw);
+    double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
+    double2 yf = dmul(y, fk);
+    double jv = sj.eval(km * r);
+    return mk<T>(T(jv * yf.x / geo.vol), T(jv * yf.y / geo.vol));
+  }
+};
+
+// (f_a f_b^* / C1 - S C1 / C1) / V  (field.cpp:3007-3025, 3271-3289).
+template <typename T>
+struct LoadTwopt {
+  const c2<T>* fa;
+  const c2<T>* fb;
+  AxisTables tab;
+  double2 sbar;
+  double inv_vol;
+  __device__ c2<T> operator()(long long g, int i, int j, int k) const {
+    c2<T> a = fa[g], b = fb[g];
+    double2 pk = dmul(dmk(a.x, a.y), dmk(b.x, -double(b.y)));
+    double c1 = alias_at(tab, i, j, k);
+    double w = tab.interlace ? window_at(tab, i, j, k) * window_at(tab, i, j, k)
+                             : c1;
+    double2 sn = dmk(sbar.x * c1 / w, sbar.y * c1 / w);
+    pk = dmk(pk.x / w - sn.x, pk.y / w - sn.y);
+    return mk<T>(T(pk.x * inv_vol), T(pk.y * inv_vol));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Bin location
+// ---------------------------------------------------------------------------
+
+// Bin b with edges[b] <= v < edges[b+1], or -1.  Edges ascending.
+__device__ __forceinline__ int find_bin(const double* edges, int nb, double v) {
+  if (!(edges[0] <= v && v < edges[nb])) return -1;
+  int lo = 0, hi = nb;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (edges[mid] <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Shell statistics: n_modes and sum |k| per bin under the exact rule
+// k_lo <= |k| < k_hi (field.cpp:1815-1830).  Per-block partial sums.
+// ---------------------------------------------------------------------------
+
+static __global__ void __launch_bounds__(RED_THREADS)
+k_shell_stats(Geom g, const double* __restrict__ edges, int nb,
+              unsigned long long* __restrict__ counts,
+              double* __restrict__ ksum_part) {
+  extern __shared__ double sh_ks[];
+  unsigned int* sh_cnt = reinterpret_cast<unsigned int*>(sh_ks + nb);
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    sh_ks[b] = 0.;
+    sh_cnt[b] = 0u;
+  }
+  __syncthreads();
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       c < g.nmesh; c += (long long)gridDim.x * blockDim.x) {
+    int k = int(c % g.n[2]);
+    long long ij = c / g.n[2];
+    int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
+    double km = mag3_exact(grid_coord(i, g.n[0], g.dk[0]),
+                           grid_coord(j, g.n[1], g.dk[1]),
+                           grid_coord(k, g.n[2], g.dk[2]));
+    int b = find_bin(edges, nb, km);
+    if (b >= 0) {
+      atomicAdd(&sh_cnt[b], 1u);
+      atomicAdd(&sh_ks[b], km);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    atomicAdd(&counts[b], (unsigned long long)sh_cnt[b]);
+    ksum_part[(long long)blockIdx.x * nb + b] = sh_ks[b];
+  }
+}
+
+// Sum per-block partials in a fixed order (deterministic).
+static __global__ void k_sum_partials(const double* __restrict__ part, int nblocks,
+                               int width, double* __restrict__ out) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= width) return;
+  double s = 0.;
+  for (int b = 0; b < nblocks; b++) s += part[(long long)b * width + col];
+  out[col] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Pair product-reduction  S_p = sum_x FA[ia_p](x) FB[ib_p](x) G(x)
+// (threept.cpp:493-517).  A chunk of PR_CELLS cells of every needed field is
+// staged in LDS (cell-major), H = FB * G is formed once per chunk, and each
+// thread accumulates up to PR_MAXP pairs in FP64 registers across all chunks
+// the block visits.  Output: per-block partials [gridDim.x][P] (double2).
+// ---------------------------------------------------------------------------
+
+constexpr int PR_CELLS = 32;
+constexpr int PR_MAXP = 8;
+
+template <typename T>
+static __global__ void __launch_bounds__(RED_THREADS)
+k_pair_reduce(const c2<T>* __restrict__ FA, int na,
+              const c2<T>* __restrict__ FB, int nbf,
+              const c2<T>* __restrict__ G, long long nmesh,
+              const int2* __restrict__ pairs, int P,
+              double2* __restrict__ part) {
+  extern __shared__ double2 sh[];
+  double2* sa = sh;                          // [PR_CELLS][na]
+  double2* sb = sh + PR_CELLS * na;          // [PR_CELLS][nbf]
+  double2 acc[PR_MAXP];
+  int ia[PR_MAXP], ib[PR_MAXP];
+#pragma unroll
+  for (int s = 0; s < PR_MAXP; s++) {
+    acc[s] = dmk(0., 0.);
+    int p = threadIdx.x + s * RED_THREADS;
+    ia[s] = p < P ? pairs[p].x : 0;
+    ib[s] = p < P ? pairs[p].y : 0;
+  }
+  const long long nchunks = (nmesh + PR_CELLS - 1) / PR_CELLS;
+  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const long long c0 = ch * PR_CELLS;
+    __syncthreads();
+    for (int e = threadIdx.x; e < PR_CELLS * na; e += RED_THREADS) {
+      int f = e / PR_CELLS, c = e % PR_CELLS;
+      long long gidx = c0 + c;
+      c2<T> v = gidx < nmesh ? FA[(long long)f * nmesh + gidx] : mk<T>(T(0), T(0));
+      sa[c * na + f] = dmk(v.x, v.y);
+    }
+    for (int e = threadIdx.x; e < PR_CELLS * nbf; e += RED_THREADS) {
+      int f = e / PR_CELLS, c = e % PR_CELLS;
+      long long gidx = c0 + c;
+      double2 h = dmk(0., 0.);
+      if (gidx < nmesh) {
+        c2<T> v = FB[(long long)f * nmesh + gidx];
+        c2<T> gg = G[gidx];
+        h = dmul(dmk(v.x, v.y), dmk(gg.x, gg.y));
+      }
+      sb[c * nbf + f] = h;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PR_MAXP; s++) {
+      if (threadIdx.x + s * RED_THREADS >= P) break;
+      double2 a = acc[s];
+      for (int c = 0; c < PR_CELLS; c++) {
+        double2 x = sa[c * na + ia[s]];
+        double2 y = sb[c * nbf + ib[s]];
+        a.x += x.x * y.x - x.y * y.y;
+        a.y += x.x * y.y + x.y * y.x;
+      }
+      acc[s] = a;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < PR_MAXP; s++) {
+    int p = threadIdx.x + s * RED_THREADS;
+    if (p < P) part[(long long)blockIdx.x * P + p] = acc[s];
+  }
+}
+
+static __global__ void k_sum_partials2(const double2* __restrict__ part, int nblocks,
+                                int width, double2* __restrict__ out) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= width) return;
+  double sx = 0., sy = 0.;
+  for (int b = 0; b < nblocks; b++) {
+    double2 v = part[(long long)b * width + col];
+    sx += v.x;
+    sy += v.y;
+  }
+  out[col] = dmk(sx, sy);
+}
+
+// ---------------------------------------------------------------------------
+// Two-point statistics in Fourier space with the reference's two-level
+// binning (field.cpp:2511-2703): fine bin ik = int(|k| / 1e-5) (< 1e6);
+// the fine bin enters coarse bin b when edges[b] <= ik * 1e-5 < edges[b+1].
+// Accumulates n_modes, sum(f_a f_b^*/C1 * y) and sum(S C1/C1 * y) per bin.
+// Output per block: [nb] counts (global atomics) and [nb][4] doubles.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static __global__ void __launch_bounds__(RED_THREADS)
+k_twopt_fourier(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
+                Geom g, AxisTables tab, double2 sbar, int ell, int m,
+                const double* __restrict__ edges, int nb,
+                unsigned long long* __restrict__ counts,
+                double* __restrict__ part) {
+  extern __shared__ double sh2[];   // [nb][4] then counts
+  unsigned int* sh_cnt = reinterpret_cast<unsigned int*>(sh2 + 4 * nb);
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    sh2[4 * b] = sh2[4 * b + 1] = sh2[4 * b + 2] = sh2[4 * b + 3] = 0.;
+    sh_cnt[b] = 0u;
+  }
+  __syncthreads();
+  const int n_sample = 1000000;
+  const double dk_sample = 1.e-5;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       c < g.nmesh; c += (long long)gridDim.x * blockDim.x) {
+    int k = int(c % g.n[2]);
+    long long ij = c / g.n[2];
+    int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
+    double kx = grid_coord(i, g.n[0], g.dk[0]);
+    double ky = grid_coord(j, g.n[1], g.dk[1]);
+    double kz = grid_coord(k, g.n[2], g.dk[2]);
+    double km = mag3_exact(kx, ky, kz);
+    int ik = int(__ddiv_rn(km, dk_sample));
+    if (ik < 0 || ik >= n_sample) continue;
+    int b = find_bin(edges, nb, __dmul_rn(double(ik), dk_sample));
+    if (b < 0) continue;
+    c2<T> a = fa[c], bb = fb[c];
+    double2 pk = dmul(dmk(a.x, a.y), dmk(bb.x, -double(bb.y)));
+    double c1 = alias_at(tab, i, j, k);
+    double w = tab.interlace ? window_at(tab, i, j, k) * window_at(tab, i, j, k)
+                             : c1;
+    double2 sn = dmk(sbar.x * c1 / w, sbar.y * c1 / w);
+    pk = dmk(pk.x / w, pk.y / w);
+    double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
+    pk = dmul(pk, y);
+    sn = dmul(sn, y);
+    atomicAdd(&sh_cnt[b], 1u);
+    atomicAdd(&sh2[4 * b], pk.x);
+    atomicAdd(&sh2[4 * b + 1], pk.y);
+    atomicAdd(&sh2[4 * b + 2], sn.x);
+    atomicAdd(&sh2[4 * b + 3], sn.y);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    atomicAdd(&counts[b], (unsigned long long)sh_cnt[b]);
+    for (int q = 0; q < 4; q++)
+      part[((long long)blockIdx.x * nb + b) * 4 + q] = sh2[4 * b + q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Three-point correlation shot noise: fine bins of width 1 (< 1e5)
+// (field.cpp:3097-3170): n_pairs, sum r, sum t(x) y_a(x^) y_b(x^).
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static __global__ void __launch_bounds__(RED_THREADS)
+k_3pcf_shot_bins(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
+                 int mb, const double* __restrict__ edges, int nb,
+                 unsigned long long* __restrict__ counts,
+                 double* __restrict__ part) {
+  extern __shared__ double sh3[];   // [nb][3] then counts
+  unsigned int* sh_cnt = reinterpret_cast<unsigned int*>(sh3 + 3 * nb);
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    sh3[3 * b] = sh3[3 * b + 1] = sh3[3 * b + 2] = 0.;
+    sh_cnt[b] = 0u;
+  }
+  __syncthreads();
+  const int n_sample = 100000;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       c < g.nmesh; c += (long long)gridDim.x * blockDim.x) {
+    int k = int(c % g.n[2]);
+    long long ij = c / g.n[2];
+    int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
+    double rx = grid_coord(i, g.n[0], g.dr[0]);
+    double ry = grid_coord(j, g.n[1], g.dr[1]);
+    double rz = grid_coord(k, g.n[2], g.dr[2]);
+    double rm = mag3_exact(rx, ry, rz);
+    int ir = int(rm);   // r / dr_sample with dr_sample = 1
+    if (ir < 0 || ir >= n_sample) continue;
+    int b = find_bin(edges, nb, double(ir));
+    if (b < 0) continue;
+    c2<T> tv = t[c];
+    double2 ya = ylm_reduced_dev(la, ma, rx, ry, rz);
+    double2 yb = ylm_reduced_dev(lb, mb, rx, ry, rz);
+    double2 v = dmul(dmk(tv.x, tv.y), dmul(ya, yb));
+    atomicAdd(&sh_cnt[b], 1u);
+    atomicAdd(&sh3[3 * b], rm);
+    atomicAdd(&sh3[3 * b + 1], v.x);
+    atomicAdd(&sh3[3 * b + 2], v.y);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    atomicAdd(&counts[b], (unsigned long long)sh_cnt[b]);
+    for (int q = 0; q < 3; q++)
+      part[((long long)blockIdx.x * nb + b) * 3 + q] = sh3[3 * b + q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bispectrum shot noise S_ij_k (field.cpp:3198-3396)
+//   S(k_a, k_b) = V_cell sum_x j_a(k_a |x|) j_b(k_b |x|) y_a(x^) y_b(x^) t(x).
+// With equal cell sizes on the three axes |x| depends on the cell only
+// through n = i^2 + j^2 + k^2, so the angular-weighted field is first
+// histogrammed into Q[n] and each pair then costs one pass over n.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static __global__ void __launch_bounds__(RED_THREADS)
+k_radial_hist(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
+              int mb, double* __restrict__ Q) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       c < g.nmesh; c += (long long)gridDim.x * blockDim.x) {
+    int k = int(c % g.n[2]);
+    long long ij = c / g.n[2];
+    int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
+    int si = shifted(i, g.n[0]), sj = shifted(j, g.n[1]), sk = shifted(k, g.n[2]);
+    long long nsq = (long long)si * si + (long long)sj * sj + (long long)sk * sk;
+    double rx = si * g.dr[0], ry = sj * g.dr[1], rz = sk * g.dr[2];
+    c2<T> tv = t[c];
+    double2 v = dmk(tv.x, tv.y);
+    if (!(la == 0 && ma == 0 && lb == 0 && mb == 0)) {
+      double2 ya = ylm_reduced_dev(la, ma, rx, ry, rz);
+      double2 yb = ylm_reduced_dev(lb, mb, rx, ry, rz);
+      v = dmul(v, dmul(ya, yb));
+    }
+    unsafeAtomicAdd(&Q[2 * nsq], v.x);
+    unsafeAtomicAdd(&Q[2 * nsq + 1], v.y);
+  }
+}
+
+// One block per pair: S_p = V_cell sum_n j_a(ka_p r_n) j_b(kb_p r_n) Q[n].
+static __global__ void __launch_bounds__(RED_THREADS)
+k_shot_pairs_hist(const double* __restrict__ Q, long long nq, double dr,
+                  SplineDev sa, SplineDev sb, const double* __restrict__ ka,
+                  const double* __restrict__ kb, double vol_cell,
+                  double2* __restrict__ out) {
+  __shared__ double rs[RED_THREADS], is[RED_THREADS];
+  const int p = blockIdx.x;
+  const double kap = ka[p], kbp = kb[p];
+  double sx = 0., sy = 0.;
+  for (long long n = threadIdx.x; n < nq; n += blockDim.x) {
+    double qx = Q[2 * n], qy = Q[2 * n + 1];
+    if (qx == 0. && qy == 0.) continue;
+    double r = sqrt(double(n)) * dr;
+    double jj = sa.eval(kap * r) * sb.eval(kbp * r);
+    sx += jj * qx;
+    sy += jj * qy;
+  }
+  rs[threadIdx.x] = sx;
+  is[threadIdx.x] = sy;
+  __syncthreads();
+  for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      rs[threadIdx.x] += rs[threadIdx.x + s];
+      is[threadIdx.x] += is[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[p] = dmk(rs[0] * vol_cell, is[0] * vol_cell);
+}
+
+// General cells: direct sum over the mesh for a group of pairs.
+template <typename T>
+static __global__ void __launch_bounds__(RED_THREADS)
+k_shot_pairs_direct(const c2<T>* __restrict__ t, Geom g, int la, int ma,
+                    int lb, int mb, SplineDev sa, SplineDev sb,
+                    const double* __restrict__ ka, const double* __restrict__ kb,
+                    int P, double2* __restrict__ part) {
+  constexpr int GP = 8;
+  double ax[GP], ay[GP];
+  for (int q = 0; q < GP; q++) ax[q] = ay[q] = 0.;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       c < g.nmesh; c += (long long)gridDim.x * blockDim.x) {
+    int k = int(c % g.n[2]);
+    long long ij = c / g.n[2];
+    int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
+    double rx = grid_coord(i, g.n[0], g.dr[0]);
+    double ry = grid_coord(j, g.n[1], g.dr[1]);
+    double rz = grid_coord(k, g.n[2], g.dr[2]);
+    double rm = mag3_exact(rx, ry, rz);
+    c2<T> tv = t[c];
+    double2 ya = ylm_reduced_dev(la, ma, rx, ry, rz);
+    double2 yb = ylm_reduced_dev(lb, mb, rx, ry, rz);
+    double2 v = dmul(dmk(tv.x, tv.y), dmul(ya, yb));
+    for (int q = 0; q < GP && q < P; q++) {
+      double jj = sa.eval(ka[q] * rm) * sb.eval(kb[q] * rm);
+      ax[q] += jj * v.x;
+      ay[q] += jj * v.y;
+    }
+  }
+  __shared__ double red[RED_THREADS];
+  for (int q = 0; q < GP && q < P; q++) {
+    for (int comp = 0; comp < 2; comp++) {
+      red[threadIdx.x] = comp ? ay[q] : ax[q];
+      __syncthreads();
+      for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        if (comp) part[(long long)blockIdx.x * P + q].y = red[0];
+        else part[(long long)blockIdx.x * P + q].x = red[0];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scalar reductions
+// ---------------------------------------------------------------------------
+
+// Per-block partial of sum_x Re(f)^3 (calc_grid_based_powlaw_norm,
+// field.cpp:2017-2065).
+template <typename T>
+static __global__ void __launch_bounds__(RED_THREADS)
+k_cube_sum(const c2<T>* __restrict__ f, long long n, double* __restrict__ part) {
+  __shared__ double red[RED_THREADS];
+  double s = 0.;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x) {
+    double v = f[c].x;
+    s += v * v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int q = RED_THREADS / 2; q > 0; q >>= 1) {
+    if (threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Per-block partial of sum_p y_lm(los_p) w_p^3 (threept.cpp:156-236).
+static __global__ void __launch_bounds__(RED_THREADS)
+k_sbar(DevParticles p, int ell, int m, double2* __restrict__ part) {
+  __shared__ double rr[RED_THREADS], ri[RED_THREADS];
+  double sx = 0., sy = 0.;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < p.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double w = p.w[i];
+    double w3 = w * w * w;
+    double2 y = (ell == 0 && m == 0)
+                    ? dmk(1., 0.)
+                    : ylm_reduced_dev(ell, m, p.los[3 * i], p.los[3 * i + 1],
+                                      p.los[3 * i + 2]);
+    sx += y.x * w3;
+    sy += y.y * w3;
+  }
+  rr[threadIdx.x] = sx;
+  ri[threadIdx.x] = sy;
+  __syncthreads();
+  for (int q = RED_THREADS / 2; q > 0; q >>= 1) {
+    if (threadIdx.x < q) {
+      rr[threadIdx.x] += rr[threadIdx.x + q];
+      ri[threadIdx.x] += ri[threadIdx.x + q];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = dmk(rr[0], ri[0]);
+}
+
+// Elementwise helpers.
+template <typename T>
+__global__ void k_zero(c2<T>* __restrict__ f, long long n) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x)
+    f[c] = mk<T>(T(0), T(0));
+}
+
+static __global__ void k_zero_d(double* __restrict__ f, long long n) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x)
+    f[c] = 0.;
+}
+
+template <typename T>
+__global__ void k_copy(c2<T>* __restrict__ dst, const c2<T>* __restrict__ src,
+                       long long n) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x)
+    dst[c] = src[c];
+}
+
+}  // namespace trvg
