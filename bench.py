@@ -1,0 +1,271 @@
+#!/usr/bin/env python
+"""Benchmark: bispectrum bin-pair evaluations per second at a 256^3 mesh.
+
+Workload (one step): survey-like, local plane-parallel B_000 (form full ->
+upper triangle, 210 pairs) + B_202 (full, 400 pairs) over 20 linear k-bins
+in [0.005, 0.205] h/Mpc; 1e6 lognormal data points and 5e6 uniform randoms
+in a quarter-sky shell r in [500, 1500] Mpc/h; box L = 3200 Mpc/h centred on
+the randoms; 256^3 mesh, PCS assignment (interlacing off, as the reference
+forces for three-point statistics).  Fields in FP32, sums in FP64.  The
+catalogues are uploaded to HBM once before the timed region.
+
+Prints ONE JSON line with the metric, throughput, per-kernel roofline and a
+CPU baseline (the oracle's reference-schedule per-pair loop on host cores).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ngrid 256]
+"""
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_inputs(nd, nr, ngen):
+    from triumvirate_amd import mocks
+    from triumvirate_amd.catalogue import ParticleCatalogue
+    t = time.time()
+    xd, yd, zd, nbar = mocks.lognormal_survey(nd, ngrid=ngen, seed=2)
+    xr, yr, zr = mocks.uniform_shell_randoms(nr, seed=3)
+    data = ParticleCatalogue(xd, yd, zd, nz=nbar)
+    rand = ParticleCatalogue(xr, yr, zr, nz=nbar)
+    los_d, los_r = data.compute_los(), rand.compute_los()
+    data.centre([3200.] * 3, catalogue_ref=rand)
+    log(f"inputs: {data.ntotal} data, {rand.ntotal} randoms "
+        f"({time.time() - t:.1f} s)")
+    return data, rand, los_d, los_r
+
+
+def params_struct(ngrid, degrees, shape, prec):
+    from triumvirate_amd import _abi
+    p = _abi.Params()
+    for i in range(3):
+        p.ngrid[i] = ngrid
+        p.boxsize[i] = 3200.
+    p.assignment_order = 4
+    p.interlace = 0
+    p.ell1, p.ell2, p.ELL = degrees
+    p.shape = _abi.SHAPE_CODES[shape]
+    p.idx_bin = 0
+    p.precision = prec
+    p.verbose = 20
+    p.device = 0
+    return p
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--ngrid', type=int, default=256)
+    ap.add_argument('--ndata', type=int, default=1_000_000)
+    ap.add_argument('--nrand', type=int, default=5_000_000)
+    ap.add_argument('--cpu-pairs', type=int, default=6,
+                    help='bin pairs in the CPU-baseline sample (0: skip)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU figures are "
+              "not produced (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    from triumvirate_amd import _abi, _lib
+    from triumvirate_amd.dataobjs import Binning
+    lib = _lib.load()
+    ndev = C.c_int(0)
+    lib.trvg_device_count(C.byref(ndev))
+    if ndev.value < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    data, rand, los_d, los_r = make_inputs(args.ndata, args.nrand, args.ngrid)
+    kb = Binning('fourier', 'lin', bin_min=0.005, bin_max=0.205, num_bins=20)
+    kbv = _abi.BinningView(kb.bin_edges, kb.bin_centres)
+
+    def check(st):
+        if st != 0:
+            raise RuntimeError(lib.trvg_last_error().decode())
+
+    # Device-resident catalogues (upload excluded from the timed region).
+    t_up = time.perf_counter()
+    handles = []
+    for cat, los in ((data, los_d), (rand, los_r)):
+        x, y, z, nz, ws, wc = cat.columns()
+        view = _abi.CatalogueView(x, y, z, nz, ws, wc)
+        h = C.c_void_p()
+        check(lib.trvg_catalogue_upload(C.byref(view.struct), _abi.dptr(los),
+                                        0, C.byref(h)))
+        handles.append(h)
+    t_up = time.perf_counter() - t_up
+    hd, hr = handles
+
+    # Particle normalisation (reference default convention).
+    x, y, z, nz, ws, wc = rand.columns()
+    alpha = data.wstotal / rand.wstotal
+    norm = C.c_double()
+    rv = _abi.CatalogueView(x, y, z, nz, ws, wc)
+    check(lib.trvg_calc_bispec_normalisation_from_particles(
+        C.byref(rv.struct), alpha, C.byref(norm)))
+
+    multipoles = [((0, 0, 0), 'triu'), ((2, 0, 2), 'full')]
+
+    def step(prec):
+        out = {}
+        for deg, shape in multipoles:
+            p = params_struct(args.ngrid, deg, shape, prec)
+            dim = _abi.result_dim(p.shape, 20, 0)
+            res = _abi.ResultBuffers(dim, 'bispec')
+            check(lib.trvg_compute_bispec_dev(hd, hr, C.byref(p),
+                                              C.byref(kbv.struct), norm.value,
+                                              C.byref(res.struct)))
+            out[deg] = res.as_dict()
+        return out
+
+    pairs_per_step = 210 + 400
+
+    # FP64 reference run (drop-in precision) and its timing.
+    log("FP64 step ...")
+    t = time.perf_counter()
+    r64 = step(64)
+    t64 = time.perf_counter() - t
+    log(f"FP64 step {t64 * 1e3:.1f} ms")
+
+    for _ in range(args.warmup):
+        step(32)
+    log(f"timing {args.steps} FP32 steps ...")
+    ncls = lib.trvg_profile_class_count()
+    check(lib.trvg_profile_enable(1))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r32 = step(32)
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    kernels = {}
+    for kc in range(ncls):
+        ms, n, by = C.c_double(), C.c_longlong(), C.c_double()
+        check(lib.trvg_profile_read(kc, C.byref(ms), C.byref(n), C.byref(by)))
+        if n.value:
+            name = lib.trvg_profile_class_name(kc).decode()
+            kernels[name] = {
+                'ms_per_step': ms.value / args.steps,
+                'launches_per_step': n.value / args.steps,
+                'avg_us': 1e3 * ms.value / n.value,
+                'bytes_per_launch': by.value / n.value,
+                'achieved_GBs': (by.value / n.value) / (ms.value / n.value * 1e-3) / 1e9
+                if by.value > 0 else None,
+            }
+    check(lib.trvg_profile_enable(0))
+    log(f"FP32 step {ms_per_step:.1f} ms (profiled)")
+
+    # Untimed-by-profiler check of the profiling overhead.
+    t = time.perf_counter()
+    step(32)
+    ms_unprof = (time.perf_counter() - t) * 1e3
+
+    # FP32 error against FP64, per multipole.
+    errs = {}
+    for deg, _ in multipoles:
+        tag = 'B' + ''.join(map(str, deg))
+        e = {}
+        for key in ('bk_raw', 'bk_shot'):
+            a, b = r32[deg][key], r64[deg][key]
+            e[key] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        est32 = r32[deg]['bk_raw'] - r32[deg]['bk_shot']
+        est64 = r64[deg]['bk_raw'] - r64[deg]['bk_shot']
+        e['bk'] = float(np.max(np.abs(est32 - est64)) / np.max(np.abs(est64)))
+        e['nmodes_equal'] = bool(np.array_equal(r32[deg]['nmodes_1'],
+                                                r64[deg]['nmodes_1']))
+        errs[tag] = e
+
+    # Roofline: the dominant HBM-bound kernel class by time.
+    dom = max((k for k in kernels if kernels[k]['achieved_GBs']),
+              key=lambda k: kernels[k]['ms_per_step'])
+    kd = kernels[dom]
+    roofline = {
+        'bound': 'hbm', 'kernel': dom,
+        'achieved': round(kd['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS,
+        'unit': 'GB/s', 'frac': round(kd['achieved_GBs'] / HBM_PEAK_GBS, 4),
+        'traffic': None,
+        'bytes_per_launch': kd['bytes_per_launch'],
+        'avg_launch_us': round(kd['avg_us'], 2),
+    }
+
+    # CPU baseline: oracle reference-schedule per-pair loop on host cores.
+    cpu = None
+    if args.cpu_pairs > 0:
+        import oracle
+        olib = oracle.load()
+        threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+        olib.trvo_set_threads(threads)
+        p = params_struct(args.ngrid, (0, 0, 0), 'full', 64)
+        sec = C.c_double()
+        log(f"CPU baseline sample: {args.cpu_pairs} pairs on {threads} threads")
+        check_o = olib.trvo_bench_pairs(C.byref(p), C.byref(kbv.struct),
+                                        args.cpu_pairs, C.byref(sec))
+        if check_o == 0:
+            per_pair = sec.value / args.cpu_pairs
+            # Pair-loop work per step: 210 pairs x 1 term (B_000) + 400 pairs
+            # x 3 terms (B_202); fixed per-term costs are not counted.
+            work = 210 * 1 + 400 * 3
+            cpu = {
+                'value': round(pairs_per_step / (per_pair * work), 5),
+                'unit': 'bin-pair evals/s', 'cores': threads, 'kind': 'port',
+                'sample': (f"reference-schedule per-pair loop body (2 shell "
+                           f"IFFTs + triple sum + shot-noise IFFT + spline "
+                           f"pass), FP64, {args.ngrid}^3, {args.cpu_pairs} "
+                           f"pairs, {sec.value:.1f} s; per-term assignment/"
+                           f"FFT costs excluded (favours the CPU)"),
+                'seconds_per_pair': round(per_pair, 3),
+            }
+
+    value = pairs_per_step / (ms_per_step / 1e3)
+    line = {
+        'metric': 'B_ℓ bin-pair evals/s at 256³ mesh, FP32 vs FP64 at stated '
+                  'err; fwd+bwd time',
+        'value': round(value, 2),
+        'unit': 'bin-pair evals/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(ms_per_step, 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic (seeded lognormal mock + uniform randoms)',
+        'config': {
+            'workload': 'survey-like lpp B_000 (full->triu, 210 pairs) + '
+                        'B_202 (full, 400 pairs), 20 lin k-bins '
+                        '[0.005,0.205], 1e6 lognormal data + 5e6 randoms, '
+                        'L=3200, PCS, interlace off',
+            'ngrid': args.ngrid, 'pairs_per_step': pairs_per_step,
+            'n_data': int(data.ntotal), 'n_rand': int(rand.ntotal),
+            'precision': 'fp32 fields, fp64 sums',
+        },
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+        'fp64': {'value': round(pairs_per_step / t64, 2),
+                 'ms_per_step': round(t64 * 1e3, 3)},
+        'fp32_rel_err': errs,
+        'ms_per_step_unprofiled': round(ms_unprof, 3),
+        'upload_ms': round(t_up * 1e3, 1),
+        'pcie_inclusive_value': round(
+            pairs_per_step / (ms_per_step / 1e3 + t_up), 2),
+        'kernels': kernels,
+    }
+    for h in handles:
+        lib.trvg_catalogue_free(h)
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

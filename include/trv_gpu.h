@@ -235,6 +235,17 @@ int trvg_compute_3pcf_dev(const trvg_device_catalogue* data,
  * by later calls). */
 int trvg_release_workspace(void);
 
+/* Kernel-class profiler.  When enabled (which also clears the totals),
+ * every kernel launch is bracketed by HIP events on the engine's stream and
+ * the elapsed time, launch count and algorithmic bytes (the compulsory HBM
+ * traffic of the launch) are accumulated per kernel class.  Classes are
+ * numbered 0 .. trvg_profile_class_count() - 1. */
+int trvg_profile_enable(int on);
+int trvg_profile_read(int kernel_class, double* total_ms, long long* launches,
+                      double* algorithmic_bytes);
+const char* trvg_profile_class_name(int kernel_class);
+int trvg_profile_class_count(void);
+
 /* Component entry points (diagnostics and parity tests). */
 
 /* Mesh assignment of a catalogue with per-particle weights

@@ -45,6 +45,9 @@ def load():
     lib.trvo_assign.restype = C.c_int
     lib.trvo_fourier_transform.argtypes = [C.POINTER(_abi.Params), dp, dp]
     lib.trvo_fourier_transform.restype = C.c_int
+    lib.trvo_bench_pairs.argtypes = [C.POINTER(_abi.Params),
+                                     C.POINTER(_abi.Binning), C.c_int, dp]
+    lib.trvo_bench_pairs.restype = C.c_int
     _lib = lib
     return lib
 

@@ -1581,7 +1581,7 @@ int trvo_assign(const trvg_catalogue* cat, const double* weights,
     Mesh m(p);
     std::vector<cplx> f(p.nmesh, 0.);
     m.assign_pass(c, w, f, shadow != 0);
-    std::memcpy(out, f.data(), sizeof(cplx) * p.nmesh);
+    std::memcpy(out, static_cast<const void*>(f.data()), sizeof(cplx) * p.nmesh);
   });
 }
 
@@ -1593,10 +1593,11 @@ int trvo_fourier_transform(const trvg_params* params, double* field,
     Params p = make_params(params);
     p.interlace = (shadow != nullptr);
     Mesh m(p);
-    std::memcpy(m.f.data(), field, sizeof(cplx) * p.nmesh);
-    if (shadow) std::memcpy(m.fs.data(), shadow, sizeof(cplx) * p.nmesh);
+    std::memcpy(static_cast<void*>(m.f.data()), field, sizeof(cplx) * p.nmesh);
+    if (shadow)
+      std::memcpy(static_cast<void*>(m.fs.data()), shadow, sizeof(cplx) * p.nmesh);
     m.fourier_transform();
-    std::memcpy(field, m.f.data(), sizeof(cplx) * p.nmesh);
+    std::memcpy(field, static_cast<const void*>(m.f.data()), sizeof(cplx) * p.nmesh);
   });
 }
 
@@ -1624,6 +1625,57 @@ int trvo_calc_bispec_normalisation_from_mesh(const trvg_catalogue* cat,
     Mesh m(p);
     double nf = m.grid_powlaw_norm(c, 3);
     *norm = nf / std::pow(alpha, 3);
+  });
+}
+
+// Bounded CPU-baseline sample: the per-pair loop body of the reference
+// schedule (threept.cpp:465-517 and 950-964 with field.cpp:1792-1906,
+// 3198-3396) run `npairs` times on the mesh of `params` (random dense
+// fields; the cost does not depend on field values), cycling over the
+// bins of `kb`.  Returns the wall time of the loop in seconds.
+int trvo_bench_pairs(const trvg_params* params, const trvg_binning* kb,
+                     int npairs, double* seconds) {
+  return guarded([&] {
+    Params p = make_params(params);
+    p.interlace = false;
+    std::vector<cplx> rnd(p.nmesh);
+    unsigned long long st = 88172645463325252ULL;
+    for (long long g = 0; g < p.nmesh; g++) {
+      st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+      double a = double(st % 1000003) / 1000003. - 0.5;
+      st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+      double b = double(st % 1000003) / 1000003. - 0.5;
+      rnd[g] = cplx(a, b);
+    }
+    Mesh dn00(p), N00(p), G(p), Fa(p), Fb(p);
+    dn00.f = rnd;
+    N00.f = rnd;
+    G.f = rnd;
+    Stats stats(p);
+    std::vector<cplx> yka = ylm_table(dn00, p.ell1, 0, true);
+    std::vector<cplx> ykb = ylm_table(dn00, p.ell2, 0, true);
+    std::vector<cplx> yra = ylm_table(dn00, p.ell1, 0, false);
+    std::vector<cplx> yrb = ylm_table(dn00, p.ell2, 0, false);
+    BesselSpline sja(p.ell1), sjb(p.ell2);
+    stats.compute_alias();
+    dn00.compute_window(p.order);
+    Fa.compute_window(p.order);
+    Fb.compute_window(p.order);
+    double t0 = omp_get_wtime();
+    double sink = 0.;
+    for (int q = 0; q < npairs; q++) {
+      int b = q % kb->num_bins;
+      double ea, eb;
+      int na, nbm;
+      Fa.ifft_ylm_band_limited(dn00, yka, kb->bin_edges[b], kb->bin_edges[b + 1], ea, na);
+      Fb.ifft_ylm_band_limited(dn00, ykb, kb->bin_edges[b], kb->bin_edges[b + 1], eb, nbm);
+      cplx s = triple_sum(Fa, Fb, G);
+      std::vector<cplx> t = stats.twopt_config_field(dn00, N00, cplx(1., 0.));
+      cplx sn = stats.shotnoise_bispec_from_t(t, yra, yrb, sja, sjb, ea, eb);
+      sink += s.real() + sn.real();
+    }
+    *seconds = omp_get_wtime() - t0;
+    if (sink == 12345.6789) *seconds += 0.;  // keep the work observable
   });
 }
 

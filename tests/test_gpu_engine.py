@@ -92,10 +92,14 @@ def test_assignment_matches_oracle(eng, oracle_lib, order, mode, ell, m):
     print(f"assign order={order} mode={mode} l={ell} m={m}", flush=True)
     ng, L = (16, 32, 16), (100., 200., 160.)
     g = np.random.default_rng(11 + order)
-    n = 4000
+    n = 9000
     x = g.uniform(0, L[0], n)
     y = g.uniform(0, L[1], n)
     z = g.uniform(0, L[2], n)
+    # A dense clump inside one 8^3-cell tile: several work chunks per tile.
+    x[4000:] = g.uniform(10., 40., 5000)
+    y[4000:] = g.uniform(10., 40., 5000)
+    z[4000:] = g.uniform(10., 40., 5000)
     # Put some particles right at the faces and cell edges.
     x[:20] = 0.
     y[20:40] = np.nextafter(L[1], 0)

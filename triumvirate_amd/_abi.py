@@ -209,4 +209,12 @@ def declare_extensions(lib):
     lib.trvg_mesh_assign.restype = C.c_int
     lib.trvg_fft3d.argtypes = [P(Params), _dp, C.c_int]
     lib.trvg_fft3d.restype = C.c_int
+    lib.trvg_profile_enable.argtypes = [C.c_int]
+    lib.trvg_profile_enable.restype = C.c_int
+    lib.trvg_profile_read.argtypes = [C.c_int, _dp, P(C.c_longlong), _dp]
+    lib.trvg_profile_read.restype = C.c_int
+    lib.trvg_profile_class_name.argtypes = [C.c_int]
+    lib.trvg_profile_class_name.restype = C.c_char_p
+    lib.trvg_profile_class_count.argtypes = []
+    lib.trvg_profile_class_count.restype = C.c_int
     return lib

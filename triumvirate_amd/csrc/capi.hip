@@ -108,6 +108,19 @@ Result dispatch_3pcf(Context& ctx, const DeviceCatalogue& d,
   return run_3pcf<double>(ctx, d, r, survey, b, norm);
 }
 
+// Parameters of a mesh-only operation: the statistic's degrees and binning
+// shape play no part, so they are not validated.
+trvg_params mesh_params(const trvg_params* p) {
+  if (p == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
+  trvg_params q = *p;
+  q.interlace = 0;
+  q.shape = TRVG_SHAPE_DIAG;
+  q.idx_bin = 0;
+  q.ell1 = q.ell2 = q.ELL = 0;
+  validate_params(q, 1);
+  return q;
+}
+
 void require_los(const double* los, const char* which) {
   if (los == nullptr)
     throw Error(TRVG_ERR_DATA, std::string("Lines of sight for the ") + which +
@@ -170,10 +183,7 @@ int trvg_calc_bispec_normalisation_from_mesh(const trvg_catalogue* c,
                                              double alpha, double* norm_factor) {
   return guarded([&] {
     require_device();
-    if (params == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
-    trvg_params q = *params;
-    q.interlace = 0;
-    validate_params(q, 1);
+    trvg_params q = mesh_params(params);
     Context ctx(q);
     DeviceCatalogue dc;
     upload_catalogue(c, nullptr, dc, ctx.stream);
@@ -313,6 +323,25 @@ int trvg_compute_3pcf_dev(const trvg_device_catalogue* data,
   });
 }
 
+int trvg_profile_enable(int on) {
+  return guarded([&] { Profiler::get().enable(on != 0); });
+}
+
+int trvg_profile_read(int kernel_class, double* total_ms, long long* launches,
+                      double* algorithmic_bytes) {
+  return guarded([&] {
+    if (kernel_class < 0 || kernel_class >= KC_COUNT)
+      throw Error(TRVG_ERR_PARAM, "Unknown kernel class.");
+    Profiler::get().read(kernel_class, total_ms, launches, algorithmic_bytes);
+  });
+}
+
+const char* trvg_profile_class_name(int kernel_class) {
+  return kernel_class_name(kernel_class);
+}
+
+int trvg_profile_class_count(void) { return KC_COUNT; }
+
 int trvg_release_workspace(void) {
   return guarded([&] { pool().release_all(); });
 }
@@ -329,45 +358,23 @@ int trvg_mesh_assign(const trvg_catalogue* c, const double* los,
                      double* mesh_out) {
   return guarded([&] {
     require_device();
-    if (params == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
-    trvg_params q = *params;
-    q.interlace = 0;
-    validate_params(q, 1);
+    trvg_params q = mesh_params(params);
     Context ctx(q);
     DeviceCatalogue dc;
     upload_catalogue(c, los, dc, ctx.stream);
     const long long nm = ctx.g.nmesh;
     std::vector<double> h(2 * nm);
+    if (mode < 0 || mode > 3) throw Error(TRVG_ERR_PARAM, "Unknown weight mode.");
     if (q.precision == TRVG_PREC_FP32) {
       DevArray<float2> f(nm);
-      TRVG_HIP(hipMemsetAsync(f.get(), 0, nm * sizeof(float2), ctx.stream));
-      DevParticles pv = dc.view();
-      int cw = (mode >= 2 && !(ell == 0 && m == 0));
-      unsigned grid = 1024;
-      switch (q.assignment_order) {
-        case 1: hipLaunchKernelGGL((k_assign<float, 1>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
-        case 2: hipLaunchKernelGGL((k_assign<float, 2>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
-        case 3: hipLaunchKernelGGL((k_assign<float, 3>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
-        default: hipLaunchKernelGGL((k_assign<float, 4>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
-      }
-      TRVG_LAUNCH_CHECK();
+      assign_field<float>(ctx, f.get(), dc, mode, ell, m, 1.);
       std::vector<float> hf(2 * nm);
       TRVG_HIP(hipMemcpyAsync(hf.data(), f.get(), nm * sizeof(float2), hipMemcpyDeviceToHost, ctx.stream));
       TRVG_HIP(hipStreamSynchronize(ctx.stream));
       for (long long i = 0; i < 2 * nm; i++) h[i] = hf[i];
     } else {
       DevArray<double2> f(nm);
-      TRVG_HIP(hipMemsetAsync(f.get(), 0, nm * sizeof(double2), ctx.stream));
-      DevParticles pv = dc.view();
-      int cw = (mode >= 2 && !(ell == 0 && m == 0));
-      unsigned grid = 1024;
-      switch (q.assignment_order) {
-        case 1: hipLaunchKernelGGL((k_assign<double, 1>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
-        case 2: hipLaunchKernelGGL((k_assign<double, 2>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
-        case 3: hipLaunchKernelGGL((k_assign<double, 3>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
-        default: hipLaunchKernelGGL((k_assign<double, 4>), dim3(grid), dim3(256), 0, ctx.stream, f.get(), pv, ctx.g, mode, ell, m, 1., cw, 0); break;
-      }
-      TRVG_LAUNCH_CHECK();
+      assign_field<double>(ctx, f.get(), dc, mode, ell, m, 1.);
       TRVG_HIP(hipMemcpyAsync(h.data(), f.get(), nm * sizeof(double2), hipMemcpyDeviceToHost, ctx.stream));
       TRVG_HIP(hipStreamSynchronize(ctx.stream));
     }
@@ -380,9 +387,7 @@ int trvg_mesh_assign(const trvg_catalogue* c, const double* los,
 int trvg_fft3d(const trvg_params* params, double* mesh, int sign) {
   return guarded([&] {
     require_device();
-    if (params == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
-    trvg_params q = *params;
-    validate_params(q, 1);
+    trvg_params q = mesh_params(params);
     Context ctx(q);
     const long long nm = ctx.g.nmesh;
     if (q.precision == TRVG_PREC_FP32) {

@@ -71,10 +71,29 @@ class DevArray {
 // Device catalogue (owned copy of the columns the kernels read).
 // ---------------------------------------------------------------------------
 
+// Tile-sorted copy of a catalogue for one mesh geometry (see
+// k_assign_tiled): columns in tile order, the chunked work list, and the
+// trailing range of particles outside the mesh.
+struct TiledView {
+  int n[3] = {0, 0, 0};
+  double L[3] = {0., 0., 0.};
+  TileGeom tg;
+  DevArray<double> x, y, z, w, los;
+  DevArray<TileWork> work;
+  int nwork = 0;
+  long long nin = 0;        // particles inside the mesh (tile order)
+  long long nstraggle = 0;  // particles [nin, n) outside it
+  bool matches(const Geom& g) const {
+    return n[0] == g.n[0] && n[1] == g.n[1] && n[2] == g.n[2] &&
+           L[0] == g.L[0] && L[1] == g.L[1] && L[2] == g.L[2];
+  }
+};
+
 struct DeviceCatalogue {
   long long n = 0;
   DevArray<double> x, y, z, w, los;
   double wstotal = 0., wtotal = 0.;
+  mutable std::unique_ptr<TiledView> tiled;  // built on first use
   DevParticles view() const {
     DevParticles p;
     p.x = x.get(); p.y = y.get(); p.z = z.get(); p.w = w.get();
@@ -113,6 +132,49 @@ void spline_nodes(int ell, std::vector<double>& y, std::vector<double>& c,
                   double& split);
 
 // ---------------------------------------------------------------------------
+// Kernel-class profiler: when enabled, every launch is bracketed by HIP
+// events on the engine stream; elapsed times, launch counts and the
+// algorithmic bytes of each launch are accumulated per class.
+// ---------------------------------------------------------------------------
+
+enum KernelClass {
+  KC_FFT_Z = 0,        // first FFT pass (with fused loader)
+  KC_FFT_STRIDED,      // y / x FFT passes
+  KC_ASSIGN,           // particle -> mesh assignment
+  KC_PAIR_REDUCE,      // sum_x F_a F_b G over bin pairs
+  KC_TWOPT,            // two-point binned statistics
+  KC_SHELL_STATS,      // mode counts / k_eff
+  KC_RADIAL,           // shot-noise radial histogram
+  KC_SHOT_PAIRS,       // shot-noise per-pair sums
+  KC_MISC,             // zero fills, partial sums, particle sums
+  KC_COUNT
+};
+
+const char* kernel_class_name(int kc);
+
+struct ProfRecord {
+  int kc;
+  double bytes;
+  hipEvent_t a, b;
+};
+
+class Profiler {
+ public:
+  static Profiler& get();
+  bool enabled() const { return on_; }
+  void enable(bool on);
+  void add(int kc, double ms, double bytes);
+  void read(int kc, double* ms, long long* n, double* bytes);
+
+ private:
+  std::mutex mu_;
+  bool on_ = false;
+  double ms_[KC_COUNT] = {};
+  long long n_[KC_COUNT] = {};
+  double bytes_[KC_COUNT] = {};
+};
+
+// ---------------------------------------------------------------------------
 // Per-call context.
 // ---------------------------------------------------------------------------
 
@@ -123,9 +185,25 @@ struct Context {
   DevArray<double> tab_data;   // per-axis sinc, sin^2, u
   AxisTables tab;
   int sm_count = 256;
+  std::vector<ProfRecord> prof;
 
   explicit Context(const trvg_params& params);
   ~Context();
+  // Resolve pending profiler records (synchronises the stream).
+  void flush_profile();
+};
+
+// Brackets one kernel launch with profiler events (no-op when disabled).
+class ProfScope {
+ public:
+  ProfScope(Context& ctx, int kc, double bytes);
+  ~ProfScope();
+
+ private:
+  Context& ctx_;
+  int kc_;
+  double bytes_;
+  hipEvent_t a_ = nullptr;
 };
 
 // Maths shared with the drivers.
@@ -137,8 +215,21 @@ void pair_bins(int shape, int nb, int idx_bin, std::vector<int>& ra,
 void validate_params(const trvg_params& p, int num_bins);
 
 // FFT of a mesh: first pass (z) loads through `Loader`, then y and x passes.
+// read_bytes: compulsory bytes the loader reads (profiler accounting; a
+// negative value means one in-place read of the mesh).
 template <typename T, class Loader>
-void fft3d(Context& ctx, c2<T>* data, const Loader& load, int sign);
+void fft3d(Context& ctx, c2<T>* data, const Loader& load, int sign,
+           double read_bytes = -1., const Prune* prune = nullptr);
+
+// Pruning descriptor for an inverse transform whose input vanishes for
+// |k| >= k_hi (disabled when the band covers the whole kx range).
+Prune make_prune(const Geom& g, double k_hi);
+
+// Zero `mesh` and assign catalogue `c` with weight mode (WeightMode) and
+// scale through the production (tile-binned) path.
+template <typename T>
+void assign_field(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c, int mode,
+                  int ell, int m, double scale);
 
 // Estimators.
 struct Result {

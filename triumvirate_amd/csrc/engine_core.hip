@@ -193,7 +193,88 @@ Context::Context(const trvg_params& params) : p(params) {
 Context::~Context() {
   if (stream) {
     (void)hipStreamSynchronize(stream);
+    try {
+      flush_profile();
+    } catch (...) {
+    }
     (void)hipStreamDestroy(stream);
+  }
+}
+
+void Context::flush_profile() {
+  if (prof.empty()) return;
+  TRVG_HIP(hipStreamSynchronize(stream));
+  for (auto& r : prof) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess)
+      Profiler::get().add(r.kc, ms, r.bytes);
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  prof.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Profiler
+// ---------------------------------------------------------------------------
+
+const char* kernel_class_name(int kc) {
+  static const char* names[KC_COUNT] = {
+      "fft_pass_z", "fft_pass_strided", "assign", "pair_reduce", "twopt_bins",
+      "shell_stats", "shot_radial_hist", "shot_pairs", "misc"};
+  return (kc >= 0 && kc < KC_COUNT) ? names[kc] : "";
+}
+
+Profiler& Profiler::get() {
+  static Profiler* p = new Profiler();
+  return *p;
+}
+
+void Profiler::enable(bool on) {
+  std::lock_guard<std::mutex> lk(mu_);
+  on_ = on;
+  for (int i = 0; i < KC_COUNT; i++) {
+    ms_[i] = 0.;
+    n_[i] = 0;
+    bytes_[i] = 0.;
+  }
+}
+
+void Profiler::add(int kc, double ms, double bytes) {
+  std::lock_guard<std::mutex> lk(mu_);
+  ms_[kc] += ms;
+  n_[kc] += 1;
+  bytes_[kc] += bytes;
+}
+
+void Profiler::read(int kc, double* ms, long long* n, double* bytes) {
+  std::lock_guard<std::mutex> lk(mu_);
+  *ms = ms_[kc];
+  *n = n_[kc];
+  *bytes = bytes_[kc];
+}
+
+ProfScope::ProfScope(Context& ctx, int kc, double bytes)
+    : ctx_(ctx), kc_(kc), bytes_(bytes) {
+  if (!Profiler::get().enabled()) return;
+  TRVG_HIP(hipEventCreate(&a_));
+  TRVG_HIP(hipEventRecord(a_, ctx_.stream));
+}
+
+ProfScope::~ProfScope() {
+  if (a_ == nullptr) return;
+  hipEvent_t b = nullptr;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a_);
+    return;
+  }
+  (void)hipEventRecord(b, ctx_.stream);
+  ctx_.prof.push_back(ProfRecord{kc_, bytes_, a_, b});
+  if (ctx_.prof.size() > 4096) {
+    try {
+      ctx_.flush_profile();
+    } catch (...) {
+    }
   }
 }
 

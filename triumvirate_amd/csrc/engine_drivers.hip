@@ -50,31 +50,36 @@ template <typename T> struct FftEpt { static constexpr int v = 8; };
 template <> struct FftEpt<float> { static constexpr int v = 16; };
 
 template <typename T, int LOGN, class Loader>
-static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign) {
+static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign,
+                     double read_bytes, double write_bytes, const Prune& pr) {
   constexpr int EPT = FftEpt<T>::v;
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   constexpr int NL = E / N;
   size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>);
   long long nblocks = ctx.g.nmesh / E;
+  ProfScope ps(ctx, KC_FFT_Z, read_bytes + write_bytes);
   hipLaunchKernelGGL((fft_pass_z<T, LOGN, EPT, Loader>), dim3(unsigned(nblocks)),
                      dim3(FFT_THREADS), smem, ctx.stream, data, load,
-                     twiddles<T>(N, sign), ctx.g.n[0], ctx.g.n[1], sign);
+                     twiddles<T>(N, sign), ctx.g.n[0], ctx.g.n[1], sign, pr);
   TRVG_LAUNCH_CHECK();
 }
 
 template <typename T, int LOGN>
 static void launch_strided(Context& ctx, c2<T>* data, long long S,
-                           long long outer_stride, long long inner, int sign) {
+                           long long outer_stride, long long inner, int sign,
+                           const Prune& pr, int prune_mode, double bytes) {
   constexpr int EPT = FftEpt<T>::v;
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   constexpr int NL = E / N;
   size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>);
   long long nblocks = ctx.g.nmesh / E;
+  ProfScope ps(ctx, KC_FFT_STRIDED, bytes);
   hipLaunchKernelGGL((fft_pass_strided<T, LOGN, EPT>), dim3(unsigned(nblocks)),
                      dim3(FFT_THREADS), smem, ctx.stream, data,
-                     twiddles<T>(N, sign), S, outer_stride, inner, sign);
+                     twiddles<T>(N, sign), S, outer_stride, inner, sign, pr,
+                     prune_mode);
   TRVG_LAUNCH_CHECK();
 }
 
@@ -97,14 +102,56 @@ static int ilog2(int n) {
     default: throw Error(TRVG_ERR_PARAM, "Unsupported FFT size."); \
   }
 
+Prune make_prune(const Geom& g, double k_hi) {
+  Prune pr{};
+  pr.on = 0;
+  const double kcut2 = k_hi * k_hi * (1. + 1.e-9);
+  // Worth it only if the band leaves part of the (kx, ky) plane empty.
+  const double kx_max = (g.n[0] / 2) * g.dk[0];
+  const double ky_max = (g.n[1] / 2) * g.dk[1];
+  if (kx_max * kx_max + ky_max * ky_max < kcut2) return pr;
+  pr.on = 1;
+  pr.kcut2 = kcut2;
+  pr.dk0 = g.dk[0];
+  pr.dk1 = g.dk[1];
+  pr.n0 = g.n[0];
+  pr.n1 = g.n[1];
+  return pr;
+}
+
 template <typename T, class Loader>
-void fft3d(Context& ctx, c2<T>* data, const Loader& load, int sign) {
+void fft3d(Context& ctx, c2<T>* data, const Loader& load, int sign,
+           double read_bytes, const Prune* prune) {
   const Geom& g = ctx.g;
-  TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z<T, LG, Loader>(ctx, data, load, sign)));
+  Prune pr{};
+  pr.on = 0;
+  if (prune) pr = *prune;
+  const double cb = sizeof(c2<T>);
+  // Lines / slabs each pass touches (for the profiler's algorithmic bytes).
+  double d_lines = double(g.n[0]) * g.n[1], slabs = g.n[0];
+  if (pr.on) {
+    d_lines = 0.;
+    slabs = 0.;
+    for (int i = 0; i < g.n[0]; i++) {
+      double kx = shifted(i, g.n[0]) * g.dk[0];
+      if (kx * kx < pr.kcut2) slabs += 1.;
+      for (int j = 0; j < g.n[1]; j++) {
+        double ky = shifted(j, g.n[1]) * g.dk[1];
+        if (kx * kx + ky * ky < pr.kcut2) d_lines += 1.;
+      }
+    }
+  }
+  if (read_bytes < 0.) read_bytes = double(g.nmesh) * cb;
+  const double zw = d_lines * g.n[2] * cb;
+  const double slab_vals = slabs * g.n[1] * g.n[2];
+  TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z<T, LG, Loader>(
+      ctx, data, load, sign, read_bytes, zw, pr)));
   TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided<T, LG>(
-      ctx, data, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], sign)));
+      ctx, data, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], sign, pr, 1,
+      zw + slab_vals * cb)));
   TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_strided<T, LG>(
-      ctx, data, (long long)g.n[1] * g.n[2], 0, (long long)g.n[1] * g.n[2], sign)));
+      ctx, data, (long long)g.n[1] * g.n[2], 0, (long long)g.n[1] * g.n[2], sign,
+      pr, 2, slab_vals * cb + double(g.nmesh) * cb)));
 }
 
 // ---------------------------------------------------------------------------
@@ -121,31 +168,130 @@ static unsigned grid_for(const Context& ctx, long long work, int per_cu = 8) {
 
 template <typename T>
 static void zero_field(Context& ctx, c2<T>* f) {
+  ProfScope ps(ctx, KC_MISC, double(ctx.g.nmesh) * sizeof(c2<T>));
   hipLaunchKernelGGL((k_zero<T>), dim3(grid_for(ctx, ctx.g.nmesh)),
                      dim3(RED_THREADS), 0, ctx.stream, f, ctx.g.nmesh);
   TRVG_LAUNCH_CHECK();
+}
+
+// Tile-sorted view of `c` for the context's mesh (built once, cached on the
+// catalogue).
+static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
+  if (c.tiled && c.tiled->matches(ctx.g)) return *c.tiled;
+  auto tv = std::make_unique<TiledView>();
+  const Geom& g = ctx.g;
+  for (int a = 0; a < 3; a++) {
+    tv->n[a] = g.n[a];
+    tv->L[a] = g.L[a];
+    tv->tg.nt[a] = (g.n[a] + TILE - 1) / TILE;
+  }
+  tv->tg.ntiles = (long long)tv->tg.nt[0] * tv->tg.nt[1] * tv->tg.nt[2];
+  const long long nt = tv->tg.ntiles;
+  DevArray<int> tile_of(c.n);
+  DevArray<unsigned int> counts(nt + 1), offs(nt + 2);
+  TRVG_HIP(hipMemsetAsync(counts.get(), 0, (nt + 1) * 4, ctx.stream));
+  DevParticles pv = c.view();
+  unsigned grid = grid_for(ctx, c.n, 16);
+  {
+    ProfScope ps(ctx, KC_ASSIGN, c.n * 28.);
+    hipLaunchKernelGGL(k_tile_count, dim3(grid), dim3(256), 0, ctx.stream, pv,
+                       g, tv->tg, tile_of.get(), counts.get());
+    TRVG_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx.stream,
+                     counts.get(), nt + 1, offs.get());
+  TRVG_LAUNCH_CHECK();
+  std::vector<unsigned int> hoff(nt + 2);
+  TRVG_HIP(hipMemcpyAsync(hoff.data(), offs.get(), (nt + 2) * 4,
+                          hipMemcpyDeviceToHost, ctx.stream));
+  tv->x.reset(c.n); tv->y.reset(c.n); tv->z.reset(c.n); tv->w.reset(c.n);
+  if (c.los.get()) tv->los.reset(3 * c.n);
+  {
+    ProfScope ps(ctx, KC_ASSIGN, c.n * (c.los.get() ? 116. : 68.));
+    hipLaunchKernelGGL(k_tile_scatter, dim3(grid), dim3(256), 0, ctx.stream, pv,
+                       tile_of.get(), offs.get(), tv->x.get(), tv->y.get(),
+                       tv->z.get(), tv->w.get(), tv->los.get());
+    TRVG_LAUNCH_CHECK();
+  }
+  TRVG_HIP(hipStreamSynchronize(ctx.stream));
+  // Chunked work list (host), one entry per <= CH particles of a tile.
+  const int CH = 2048;
+  std::vector<TileWork> work;
+  for (long long t = 0; t < nt; t++) {
+    long long b = hoff[t], e = hoff[t + 1];
+    for (long long s0 = b; s0 < e; s0 += CH) {
+      work.push_back(TileWork{int(t), int(std::min<long long>(CH, e - s0)), s0});
+    }
+  }
+  tv->nin = hoff[nt];
+  tv->nstraggle = c.n - tv->nin;
+  tv->nwork = int(work.size());
+  if (!work.empty()) {
+    tv->work.reset(work.size());
+    TRVG_HIP(hipMemcpyAsync(tv->work.get(), work.data(),
+                            work.size() * sizeof(TileWork),
+                            hipMemcpyHostToDevice, ctx.stream));
+    TRVG_HIP(hipStreamSynchronize(ctx.stream));
+  }
+  c.tiled = std::move(tv);
+  return *c.tiled;
+}
+
+template <typename T, int ORDER>
+static void assign_order(Context& ctx, c2<T>* mesh, const TiledView& tv,
+                         int mode, int ell, int m, double scale, int cplx_w) {
+  DevParticles sv;
+  sv.x = tv.x.get(); sv.y = tv.y.get(); sv.z = tv.z.get(); sv.w = tv.w.get();
+  sv.los = tv.los.get();
+  sv.n = tv.nin + tv.nstraggle;
+  if (tv.nwork > 0) {
+    hipLaunchKernelGGL((k_assign_tiled<T, ORDER>), dim3(tv.nwork), dim3(256), 0,
+                       ctx.stream, mesh, sv, ctx.g, tv.tg, tv.work.get(), mode,
+                       ell, m, scale, cplx_w);
+    TRVG_LAUNCH_CHECK();
+  }
+  if (tv.nstraggle > 0) {
+    DevParticles st = sv;
+    st.x += tv.nin; st.y += tv.nin; st.z += tv.nin; st.w += tv.nin;
+    if (st.los) st.los += 3 * tv.nin;
+    st.n = tv.nstraggle;
+    hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(grid_for(ctx, st.n, 16)),
+                       dim3(256), 0, ctx.stream, mesh, st, ctx.g, mode, ell, m,
+                       scale, cplx_w, 0);
+    TRVG_LAUNCH_CHECK();
+  }
 }
 
 template <typename T>
 static void assign(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c,
                    int mode, int ell, int m, double scale) {
   int cplx_w = ((mode == WM_YLM_W || mode == WM_YLMC_W2) && !(ell == 0 && m == 0));
-  unsigned grid = grid_for(ctx, c.n, 16);
-  DevParticles pv = c.view();
-  if ((mode == WM_YLM_W || mode == WM_YLMC_W2) && pv.los == nullptr)
+  if ((mode == WM_YLM_W || mode == WM_YLMC_W2) && c.los.get() == nullptr)
     throw Error(TRVG_ERR_DATA, "Lines of sight are required for survey-type fields.");
+  const TiledView& tv = tiled_view(ctx, c);
+  // Algorithmic bytes: particle columns read once (position, weight,
+  // line of sight when used) plus one write of the mesh.
+  double pbytes = 24. + (mode == WM_UNIT ? 0. : 8.) +
+                  ((mode == WM_YLM_W || mode == WM_YLMC_W2) ? 24. : 0.);
+  ProfScope ps(ctx, KC_ASSIGN, c.n * pbytes + double(ctx.g.nmesh) * sizeof(c2<T>));
   switch (ctx.p.assignment_order) {
-    case 1: hipLaunchKernelGGL((k_assign<T, 1>), dim3(grid), dim3(256), 0, ctx.stream,
-                               mesh, pv, ctx.g, mode, ell, m, scale, cplx_w, 0); break;
-    case 2: hipLaunchKernelGGL((k_assign<T, 2>), dim3(grid), dim3(256), 0, ctx.stream,
-                               mesh, pv, ctx.g, mode, ell, m, scale, cplx_w, 0); break;
-    case 3: hipLaunchKernelGGL((k_assign<T, 3>), dim3(grid), dim3(256), 0, ctx.stream,
-                               mesh, pv, ctx.g, mode, ell, m, scale, cplx_w, 0); break;
-    default: hipLaunchKernelGGL((k_assign<T, 4>), dim3(grid), dim3(256), 0, ctx.stream,
-                                mesh, pv, ctx.g, mode, ell, m, scale, cplx_w, 0); break;
+    case 1: assign_order<T, 1>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
+    case 2: assign_order<T, 2>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
+    case 3: assign_order<T, 3>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
+    default: assign_order<T, 4>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
   }
-  TRVG_LAUNCH_CHECK();
 }
+
+template <typename T>
+void assign_field(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c, int mode,
+                  int ell, int m, double scale) {
+  zero_field<T>(ctx, mesh);
+  assign<T>(ctx, mesh, c, mode, ell, m, scale);
+}
+template void assign_field<double>(Context&, double2*, const DeviceCatalogue&,
+                                   int, int, int, double);
+template void assign_field<float>(Context&, float2*, const DeviceCatalogue&,
+                                  int, int, int, double);
 
 // Survey field  sum_d y w - alpha sum_r y w  (quad: conj(y) w^2 with
 // +alpha^2), or box field (unit weights, optional mean subtraction), then
@@ -187,6 +333,7 @@ static ShellStats shell_stats(Context& ctx, const trvg_binning& kb) {
   DevArray<double> part((size_t)grid * nb), ks(nb);
   TRVG_HIP(hipMemsetAsync(cnt.get(), 0, nb * 8, ctx.stream));
   size_t smem = nb * (8 + 4);
+  ProfScope ps(ctx, KC_SHELL_STATS, 0.);
   hipLaunchKernelGGL(k_shell_stats, dim3(grid), dim3(RED_THREADS), smem,
                      ctx.stream, ctx.g, edges.get(), nb, cnt.get(), part.get());
   TRVG_LAUNCH_CHECK();
@@ -211,33 +358,65 @@ template <typename T>
 static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
                                      const c2<T>* FB, int nbf, const c2<T>* G,
                                      const std::vector<int2>& pairs) {
+  const long long nm = ctx.g.nmesh;
   const int P = int(pairs.size());
-  std::vector<cplx> out(P);
-  const int group = RED_THREADS * PR_MAXP;
-  size_t smem = size_t(PR_CELLS) * (na + nbf) * sizeof(double2);
-  if (smem > 65536)
-    throw Error(TRVG_ERR_PARAM, "Too many distinct shells for one pair reduction.");
-  long long nchunks = (ctx.g.nmesh + PR_CELLS - 1) / PR_CELLS;
-  unsigned grid = unsigned(std::min<long long>(nchunks, (long long)ctx.sm_count * 4));
-  for (int p0 = 0; p0 < P; p0 += group) {
-    int np = std::min(group, P - p0);
-    DevArray<int2> dp(np);
-    TRVG_HIP(hipMemcpyAsync(dp.get(), pairs.data() + p0, np * sizeof(int2),
-                            hipMemcpyHostToDevice, ctx.stream));
-    DevArray<double2> part((size_t)grid * np), sum(np);
-    hipLaunchKernelGGL((k_pair_reduce<T>), dim3(grid), dim3(RED_THREADS), smem,
-                       ctx.stream, FA, na, FB, nbf, G, ctx.g.nmesh, dp.get(),
-                       np, part.get());
-    TRVG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sum_partials2, dim3((np + 255) / 256), dim3(256), 0,
-                       ctx.stream, part.get(), int(grid), np, sum.get());
-    TRVG_LAUNCH_CHECK();
-    std::vector<double2> h(np);
-    TRVG_HIP(hipMemcpyAsync(h.data(), sum.get(), np * sizeof(double2),
-                            hipMemcpyDeviceToHost, ctx.stream));
-    TRVG_HIP(hipStreamSynchronize(ctx.stream));
-    for (int i = 0; i < np; i++) out[p0 + i] = cplx(h[i].x, h[i].y);
+  const int rbn = (na + 3) / 4, cbn = (nbf + 3) / 4;
+  // Active 4x4 tiles of the (ia, ib) grid.
+  std::vector<char> active((size_t)rbn * cbn, 0);
+  for (const int2& pr : pairs) active[(size_t)(pr.x / 4) * cbn + pr.y / 4] = 1;
+  // Sums for every pair of every active tile, indexed [ia * nbf + ib].
+  std::vector<cplx> full((size_t)na * nbf, cplx(0., 0.));
+  const int CB = std::min(cbn, 8);                         // column blocks per launch
+  const int RB = std::max(1, std::min((64 - 4 * CB) / 4, PT_MAXT / CB));
+  const long long nchunks = (nm + PT_CELLS - 1) / PT_CELLS;
+  const unsigned grid = unsigned(std::min<long long>(nchunks, (long long)ctx.sm_count * 4));
+  for (int c0b = 0; c0b < cbn; c0b += CB) {
+    const int c1b = std::min(cbn, c0b + CB);
+    for (int r0b = 0; r0b < rbn; r0b += RB) {
+      const int r1b = std::min(rbn, r0b + RB);
+      std::vector<int2> tl;
+      for (int rb = r0b; rb < r1b; rb++)
+        for (int cb = c0b; cb < c1b; cb++)
+          if (active[(size_t)rb * cbn + cb]) tl.push_back(make_int2(4 * (rb - r0b), 4 * (cb - c0b)));
+      if (tl.empty()) continue;
+      const int ra0 = 4 * r0b, ca0 = 4 * c0b;
+      const int nas = std::min(na, 4 * r1b) - ra0;
+      const int nbs = std::min(nbf, 4 * c1b) - ca0;
+      const int nt = int(tl.size());
+      DevArray<int2> dt(nt);
+      TRVG_HIP(hipMemcpyAsync(dt.get(), tl.data(), nt * sizeof(int2),
+                              hipMemcpyHostToDevice, ctx.stream));
+      DevArray<double2> part((size_t)grid * nt * 16), sum((size_t)nt * 16);
+      size_t smem = std::max<size_t>((size_t)(nas + nbs) * PT_CELLS * sizeof(c2<T>),
+                                     256 * sizeof(double2));
+      {
+        ProfScope ps(ctx, KC_PAIR_REDUCE,
+                     double(nas + nbs + 1) * nm * sizeof(c2<T>));
+        hipLaunchKernelGGL((k_pair_tiles2<T>), dim3(grid), dim3(256), smem,
+                           ctx.stream, FA + (long long)ra0 * nm, nas,
+                           FB + (long long)ca0 * nm, nbs, G, nm, dt.get(), nt,
+                           part.get());
+        TRVG_LAUNCH_CHECK();
+      }
+      hipLaunchKernelGGL(k_sum_partials2, dim3((nt * 16 + 255) / 256), dim3(256),
+                         0, ctx.stream, part.get(), int(grid), nt * 16, sum.get());
+      TRVG_LAUNCH_CHECK();
+      std::vector<double2> h((size_t)nt * 16);
+      TRVG_HIP(hipMemcpyAsync(h.data(), sum.get(), h.size() * sizeof(double2),
+                              hipMemcpyDeviceToHost, ctx.stream));
+      TRVG_HIP(hipStreamSynchronize(ctx.stream));
+      for (int t = 0; t < nt; t++)
+        for (int i = 0; i < 4; i++)
+          for (int j = 0; j < 4; j++) {
+            int a = ra0 + tl[t].x + i, b = ca0 + tl[t].y + j;
+            if (a < na && b < nbf)
+              full[(size_t)a * nbf + b] = cplx(h[t * 16 + i * 4 + j].x,
+                                               h[t * 16 + i * 4 + j].y);
+          }
+    }
   }
+  std::vector<cplx> out(P);
+  for (int d = 0; d < P; d++) out[d] = full[(size_t)pairs[d].x * nbf + pairs[d].y];
   return out;
 }
 
@@ -259,6 +438,7 @@ static TwoPt twopt_fourier(Context& ctx, const c2<T>* fa, const c2<T>* fb,
   DevArray<double> part((size_t)grid * nb * 4), sum(nb * 4);
   TRVG_HIP(hipMemsetAsync(cnt.get(), 0, nb * 8, ctx.stream));
   size_t smem = nb * (4 * 8 + 4);
+  ProfScope ps(ctx, KC_TWOPT, 2. * ctx.g.nmesh * sizeof(c2<T>));
   hipLaunchKernelGGL((k_twopt_fourier<T>), dim3(grid), dim3(RED_THREADS), smem,
                      ctx.stream, fa, fb, ctx.g, ctx.tab,
                      dmk(sbar.real(), sbar.imag()), ell, m, edges.get(), nb,
@@ -310,6 +490,7 @@ static ShotBins3pcf shot_bins_3pcf(Context& ctx, const c2<T>* t, int la, int ma,
   DevArray<double> part((size_t)grid * nb * 3), sum(nb * 3);
   TRVG_HIP(hipMemsetAsync(cnt.get(), 0, nb * 8, ctx.stream));
   size_t smem = nb * (3 * 8 + 4);
+  ProfScope ps(ctx, KC_TWOPT, double(ctx.g.nmesh) * sizeof(c2<T>));
   hipLaunchKernelGGL((k_3pcf_shot_bins<T>), dim3(grid), dim3(RED_THREADS), smem,
                      ctx.stream, t, ctx.g, la, ma, lb, mb, edges.get(), nb,
                      cnt.get(), part.get());
@@ -365,11 +546,15 @@ static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
     hipLaunchKernelGGL(k_zero_d, dim3(grid_for(ctx, 2 * nq)), dim3(RED_THREADS), 0,
                        ctx.stream, Q.get(), 2 * nq);
     TRVG_LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_radial_hist<T>), dim3(grid_for(ctx, g.nmesh, 8)),
+    ProfScope ps1(ctx, KC_RADIAL, double(g.nmesh) * sizeof(c2<T>));
+    const long long nrep = (long long)(g.n[0] / 2 + 1) * (g.n[1] / 2 + 1) *
+                           (g.n[2] / 2 + 1);
+    hipLaunchKernelGGL((k_radial_hist_oct<T>), dim3(grid_for(ctx, nrep, 8)),
                        dim3(RED_THREADS), 0, ctx.stream, t, g, la, ma, lb, mb,
                        Q.get());
     TRVG_LAUNCH_CHECK();
     DevArray<double2> res(P);
+    ProfScope ps2(ctx, KC_SHOT_PAIRS, double(P) * nq * 16.);
     hipLaunchKernelGGL(k_shot_pairs_hist, dim3(P), dim3(RED_THREADS), 0,
                        ctx.stream, Q.get(), nq, g.dr[0], sa.dev(), sb.dev(),
                        dka.get(), dkb.get(), g.vol_cell, res.get());
@@ -386,6 +571,7 @@ static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
   for (int p0 = 0; p0 < P; p0 += GP) {
     int np = std::min(GP, P - p0);
     DevArray<double2> part((size_t)grid * np), sum(np);
+    ProfScope ps(ctx, KC_SHOT_PAIRS, double(g.nmesh) * sizeof(c2<T>));
     hipLaunchKernelGGL((k_shot_pairs_direct<T>), dim3(grid), dim3(RED_THREADS), 0,
                        ctx.stream, t, g, la, ma, lb, mb, sa.dev(), sb.dev(),
                        dka.get() + p0, dkb.get() + p0, np, part.get());
@@ -473,6 +659,62 @@ static std::vector<int> distinct(const std::vector<int>& v) {
 }
 
 // ---------------------------------------------------------------------------
+// Shell-field slabs, reused across terms: a slab holds the inverse-
+// transformed shell fields of one (l, m) for a set of bins.  B_202 needs the
+// (0, 0) side in every term, B_000 / zeta_000 share one slab for both sides.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct ShellSlab {
+  int ell = -1, m = 0;
+  std::vector<int> index;   // bin -> field index in `data`, -1 if absent
+  int nfields = 0;
+  DevArray<c2<T>> data;
+  bool covers(int l, int mm, const std::vector<int>& bins) const {
+    if (ell != l || m != mm) return false;
+    for (int b : bins)
+      if (b >= int(index.size()) || index[b] < 0) return false;
+    return true;
+  }
+};
+
+template <typename T>
+class SlabCache {
+ public:
+  // Returns a slab of (l, m) covering `bins`, building it with
+  // make(field_ptr, bin) if needed.  `keep` is a slab that must survive.
+  template <class Make>
+  ShellSlab<T>* get(Context& ctx, int l, int m, const std::vector<int>& bins,
+                    int nb, const ShellSlab<T>* keep, Make&& make) {
+    for (auto& e : entries_)
+      if (e->covers(l, m, bins)) return e.get();
+    if (entries_.size() >= 2) {
+      for (size_t i = 0; i < entries_.size(); i++)
+        if (entries_[i].get() != keep) {
+          entries_.erase(entries_.begin() + i);
+          break;
+        }
+    }
+    auto e = std::make_unique<ShellSlab<T>>();
+    e->ell = l;
+    e->m = m;
+    e->index.assign(nb, -1);
+    e->nfields = int(bins.size());
+    const long long nm = ctx.g.nmesh;
+    e->data.reset((size_t)nm * bins.size());
+    for (size_t q = 0; q < bins.size(); q++) {
+      make(e->data.get() + nm * q, bins[q]);
+      e->index[bins[q]] = int(q);
+    }
+    entries_.push_back(std::move(e));
+    return entries_.back().get();
+  }
+
+ private:
+  std::vector<std::unique_ptr<ShellSlab<T>>> entries_;
+};
+
+// ---------------------------------------------------------------------------
 // Bispectrum
 // ---------------------------------------------------------------------------
 
@@ -497,9 +739,11 @@ Result run_bispec(Context& ctx, const DeviceCatalogue& d,
   out.raw.assign(dim, 0.); out.shot.assign(dim, 0.);
 
   // Common fields.
-  DevArray<c2<T>> dn00(nm), N00(nm);
-  build_field_k<T>(ctx, dn00.get(), d, r, survey, alpha, 0, 0, false, true);
-  build_field_k<T>(ctx, N00.get(), d, r, survey, alpha, 0, 0, true, false);
+  // Common fields in FP64 whatever the shell precision T: the shot-noise
+  // path (two-point statistics, S_ij_k) subtracts nearly equal quantities.
+  DevArray<double2> dn00(nm), N00(nm);
+  build_field_k<double>(ctx, dn00.get(), d, r, survey, alpha, 0, 0, false, true);
+  build_field_k<double>(ctx, N00.get(), d, r, survey, alpha, 0, 0, true, false);
 
   ShellStats ss = shell_stats(ctx, kb);
   for (int dv = 0; dv < dim; dv++) {
@@ -516,58 +760,44 @@ Result run_bispec(Context& ctx, const DeviceCatalogue& d,
   build_spline(p.ell2, sjb, ctx.stream);
 
   std::vector<int> rows = distinct(ra), cols = distinct(cb);
-  DevArray<c2<T>> G(nm), tfield(nm), dnLM, NLM;
+  DevArray<c2<T>> G(nm);
+  DevArray<double2> tfield(nm), dnLM, NLM;
+  SlabCache<T> slabs;
 
   for (const Term& tm : enumerate_terms(p, survey)) {
     const bool lm_zero = !survey || (p.ELL == 0 && tm.M == 0);
-    const c2<T>* dnLMp = dn00.get();
-    const c2<T>* NLMp = N00.get();
+    const double2* dnLMp = dn00.get();
+    const double2* NLMp = N00.get();
     if (!lm_zero) {
       if (!dnLM.get()) { dnLM.reset(nm); NLM.reset(nm); }
-      build_field_k<T>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
-      build_field_k<T>(ctx, NLM.get(), d, r, true, alpha, p.ELL, tm.M, true, false);
+      build_field_k<double>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
+      build_field_k<double>(ctx, NLM.get(), d, r, true, alpha, p.ELL, tm.M, true, false);
       dnLMp = dnLM.get();
       NLMp = NLM.get();
     }
 
     // G_LM(x): compensated, inverse-transformed (threept.cpp:452-459).
-    LoadDeconv<T> ldg{dnLMp, ctx.tab, 1. / g.vol};
-    fft3d<T>(ctx, G.get(), ldg, +1);
+    LoadDeconv<T, double> ldg{dnLMp, ctx.tab, 1. / g.vol};
+    fft3d<T>(ctx, G.get(), ldg, +1, double(nm) * sizeof(double2));
 
     // Shell fields F_a (l1, m1) for the row bins, F_b (l2, m2) for columns.
-    const bool same_side = (p.ell1 == p.ell2 && tm.m1 == tm.m2);
-    DevArray<c2<T>> FA(nm * rows.size());
-    std::vector<int> ia(nb, -1), ib(nb, -1);
-    for (size_t q = 0; q < rows.size(); q++) {
-      int b = rows[q];
-      LoadShell<T> ls{dn00.get(), ctx.tab, g, kb.bin_edges[b], kb.bin_edges[b + 1],
-                      p.ell1, tm.m1};
-      fft3d<T>(ctx, FA.get() + nm * q, ls, +1);
-      ia[b] = int(q);
-    }
-    DevArray<c2<T>> FBown;
-    const c2<T>* FBp = FA.get();
-    int nfb = int(rows.size());
-    bool cols_in_rows = std::all_of(cols.begin(), cols.end(),
-                                    [&](int b) { return ia[b] >= 0; });
-    if (same_side && cols_in_rows) {
-      for (int b : cols) ib[b] = ia[b];
-    } else {
-      FBown.reset(nm * cols.size());
-      for (size_t q = 0; q < cols.size(); q++) {
-        int b = cols[q];
-        LoadShell<T> ls{dn00.get(), ctx.tab, g, kb.bin_edges[b],
-                        kb.bin_edges[b + 1], p.ell2, tm.m2};
-        fft3d<T>(ctx, FBown.get() + nm * q, ls, +1);
-        ib[b] = int(q);
-      }
-      FBp = FBown.get();
-      nfb = int(cols.size());
-    }
+    auto make_shell = [&](int l, int mm) {
+      return [&, l, mm](c2<T>* out_f, int b) {
+        LoadShell<T, double> ls{dn00.get(), ctx.tab, g, kb.bin_edges[b],
+                                kb.bin_edges[b + 1], l, mm};
+        Prune pr = make_prune(g, kb.bin_edges[b + 1]);
+        fft3d<T>(ctx, out_f, ls, +1, double(ss.nmodes[b]) * sizeof(double2), &pr);
+      };
+    };
+    ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr,
+                                make_shell(p.ell1, tm.m1));
+    ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A,
+                                make_shell(p.ell2, tm.m2));
     std::vector<int2> pairs(dim);
-    for (int dv = 0; dv < dim; dv++) pairs[dv] = make_int2(ia[ra[dv]], ib[cb[dv]]);
-    std::vector<cplx> S = pair_reduce<T>(ctx, FA.get(), int(rows.size()), FBp, nfb,
-                                         G.get(), pairs);
+    for (int dv = 0; dv < dim; dv++)
+      pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
+    std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields,
+                                         B->data.get(), B->nfields, G.get(), pairs);
     for (int dv = 0; dv < dim; dv++) {
       cplx s = S[dv] / (double(ss.nmodes[ra[dv]]) * double(ss.nmodes[cb[dv]]));
       out.raw[dv] += tm.coupling * g.vol_cell * (s + tm.f_mirror * std::conj(s));
@@ -586,7 +816,7 @@ Result run_bispec(Context& ctx, const DeviceCatalogue& d,
       for (int dv = 0; dv < dim; dv++) out.shot[dv] += sijk;
     }
     if (p.ell2 == 0) {
-      TwoPt st = twopt_fourier<T>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb);
+      TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb);
       for (int dv = 0; dv < dim; dv++) {
         int b = ra[dv];
         cplx s = tm.coupling * (st.pk[b] - st.sn[b]);
@@ -594,17 +824,17 @@ Result run_bispec(Context& ctx, const DeviceCatalogue& d,
       }
     }
     if (p.ell1 == 0) {
-      TwoPt st = twopt_fourier<T>(ctx, dn00.get(), NLMp, Sbar, p.ell2, tm.m2, kb);
+      TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell2, tm.m2, kb);
       for (int dv = 0; dv < dim; dv++) {
         int b = cb[dv];
         cplx s = tm.coupling * (st.pk[b] - st.sn[b]);
         out.shot[dv] += s + tm.f_mirror * std::conj(s);
       }
     }
-    LoadTwopt<T> lt{dnLMp, N00.get(), ctx.tab, dmk(Sbar.real(), Sbar.imag()),
-                    1. / g.vol};
-    fft3d<T>(ctx, tfield.get(), lt, +1);
-    std::vector<cplx> sij = shot_pairs_bispec<T>(ctx, tfield.get(), p.ell1, tm.m1,
+    LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab, dmk(Sbar.real(), Sbar.imag()),
+                         1. / g.vol};
+    fft3d<double>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
+    std::vector<cplx> sij = shot_pairs_bispec<double>(ctx, tfield.get(), p.ell1, tm.m1,
                                                  p.ell2, tm.m2, sja, sjb,
                                                  out.e1, out.e2);
     for (int dv = 0; dv < dim; dv++) {
@@ -644,25 +874,27 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
   out.n1.resize(dim); out.n2.resize(dim);
   out.raw.assign(dim, 0.); out.shot.assign(dim, 0.);
 
-  DevArray<c2<T>> dn00(nm), N00(nm);
-  build_field_k<T>(ctx, dn00.get(), d, r, survey, alpha, 0, 0, false, true);
-  build_field_k<T>(ctx, N00.get(), d, r, survey, alpha, 0, 0, true, false);
+  DevArray<double2> dn00(nm), N00(nm);
+  build_field_k<double>(ctx, dn00.get(), d, r, survey, alpha, 0, 0, false, true);
+  build_field_k<double>(ctx, N00.get(), d, r, survey, alpha, 0, 0, true, false);
 
   Spline sja, sjb;
   build_spline(p.ell1, sja, ctx.stream);
   build_spline(p.ell2, sjb, ctx.stream);
 
   std::vector<int> rows = distinct(ra), cols = distinct(cb);
-  DevArray<c2<T>> G(nm), tfield(nm), dnLM;
+  DevArray<c2<T>> G(nm);
+  DevArray<double2> tfield(nm), dnLM;
+  SlabCache<T> slabs;
   std::vector<double> reff(nb, 0.);
   bool first = true;
 
   for (const Term& tm : enumerate_terms(p, survey)) {
     const bool lm_zero = !survey || (p.ELL == 0 && tm.M == 0);
-    const c2<T>* dnLMp = dn00.get();
+    const double2* dnLMp = dn00.get();
     if (!lm_zero) {
       if (!dnLM.get()) dnLM.reset(nm);
-      build_field_k<T>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
+      build_field_k<double>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
       dnLMp = dnLM.get();
     }
     cplx Sbar;
@@ -674,10 +906,10 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
     }
 
     // Shot noise first: defines r_eff and n_pairs (threept.cpp:1223-1367).
-    LoadTwopt<T> lt{dnLMp, N00.get(), ctx.tab, dmk(Sbar.real(), Sbar.imag()),
-                    1. / g.vol};
-    fft3d<T>(ctx, tfield.get(), lt, +1);
-    ShotBins3pcf sb = shot_bins_3pcf<T>(ctx, tfield.get(), p.ell1, tm.m1, p.ell2,
+    LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab, dmk(Sbar.real(), Sbar.imag()),
+                         1. / g.vol};
+    fft3d<double>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
+    ShotBins3pcf sb = shot_bins_3pcf<double>(ctx, tfield.get(), p.ell1, tm.m1, p.ell2,
                                         tm.m2, rb);
     for (int dv = 0; dv < dim; dv++) {
       if (ra[dv] != cb[dv]) continue;
@@ -699,40 +931,24 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
       first = false;
     }
 
-    LoadDeconv<T> ldg{dnLMp, ctx.tab, 1. / g.vol};
-    fft3d<T>(ctx, G.get(), ldg, +1);
+    LoadDeconv<T, double> ldg{dnLMp, ctx.tab, 1. / g.vol};
+    fft3d<T>(ctx, G.get(), ldg, +1, double(nm) * sizeof(double2));
 
-    const bool same_side = (p.ell1 == p.ell2 && tm.m1 == tm.m2);
-    DevArray<c2<T>> FA(nm * rows.size());
-    std::vector<int> ia(nb, -1), ib(nb, -1);
-    for (size_t q = 0; q < rows.size(); q++) {
-      int b = rows[q];
-      LoadSjl<T> ls{dn00.get(), ctx.tab, g, sja.dev(), reff[b], p.ell1, tm.m1};
-      fft3d<T>(ctx, FA.get() + nm * q, ls, +1);
-      ia[b] = int(q);
-    }
-    DevArray<c2<T>> FBown;
-    const c2<T>* FBp = FA.get();
-    int nfb = int(rows.size());
-    bool cols_in_rows = std::all_of(cols.begin(), cols.end(),
-                                    [&](int b) { return ia[b] >= 0; });
-    if (same_side && cols_in_rows) {
-      for (int b : cols) ib[b] = ia[b];
-    } else {
-      FBown.reset(nm * cols.size());
-      for (size_t q = 0; q < cols.size(); q++) {
-        int b = cols[q];
-        LoadSjl<T> ls{dn00.get(), ctx.tab, g, sjb.dev(), reff[b], p.ell2, tm.m2};
-        fft3d<T>(ctx, FBown.get() + nm * q, ls, +1);
-        ib[b] = int(q);
-      }
-      FBp = FBown.get();
-      nfb = int(cols.size());
-    }
+    auto make_sjl = [&](int l, int mm, const Spline& sp) {
+      return [&, l, mm](c2<T>* out_f, int b) {
+        LoadSjl<T, double> ls{dn00.get(), ctx.tab, g, sp.dev(), reff[b], l, mm};
+        fft3d<T>(ctx, out_f, ls, +1, double(nm) * sizeof(double2));
+      };
+    };
+    ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr,
+                                make_sjl(p.ell1, tm.m1, sja));
+    ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A,
+                                make_sjl(p.ell2, tm.m2, sjb));
     std::vector<int2> pairs(dim);
-    for (int dv = 0; dv < dim; dv++) pairs[dv] = make_int2(ia[ra[dv]], ib[cb[dv]]);
-    std::vector<cplx> S = pair_reduce<T>(ctx, FA.get(), int(rows.size()), FBp, nfb,
-                                         G.get(), pairs);
+    for (int dv = 0; dv < dim; dv++)
+      pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
+    std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields,
+                                         B->data.get(), B->nfields, G.get(), pairs);
     for (int dv = 0; dv < dim; dv++) {
       cplx s = S[dv];
       out.raw[dv] += tm.coupling * g.vol_cell * factor_phase
@@ -750,12 +966,13 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
 // field.cpp:2017-2065) before the 1/alpha^3 factor.
 template <typename T>
 double run_mesh_norm(Context& ctx, const DeviceCatalogue& c) {
-  DevArray<c2<T>> f(ctx.g.nmesh);
-  zero_field<T>(ctx, f.get());
-  assign<T>(ctx, f.get(), c, WM_W, 0, 0, 1.);
+  // A single assignment and reduction: always FP64.
+  DevArray<double2> f(ctx.g.nmesh);
+  zero_field<double>(ctx, f.get());
+  assign<double>(ctx, f.get(), c, WM_W, 0, 0, 1.);
   unsigned grid = grid_for(ctx, ctx.g.nmesh, 4);
   DevArray<double> part(grid), sum(1);
-  hipLaunchKernelGGL((k_cube_sum<T>), dim3(grid), dim3(RED_THREADS), 0,
+  hipLaunchKernelGGL((k_cube_sum<double>), dim3(grid), dim3(RED_THREADS), 0,
                      ctx.stream, f.get(), ctx.g.nmesh, part.get());
   TRVG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, ctx.stream,
@@ -782,8 +999,10 @@ template Result run_3pcf<float>(Context&, const DeviceCatalogue&,
 template double run_mesh_norm<double>(Context&, const DeviceCatalogue&);
 template double run_mesh_norm<float>(Context&, const DeviceCatalogue&);
 template void fft3d<double, LoadSame<double>>(Context&, double2*,
-                                              const LoadSame<double>&, int);
+                                              const LoadSame<double>&, int,
+                                              double, const Prune*);
 template void fft3d<float, LoadSame<float>>(Context&, float2*,
-                                            const LoadSame<float>&, int);
+                                            const LoadSame<float>&, int, double,
+                                            const Prune*);
 
 }  // namespace trvg

@@ -21,6 +21,30 @@ namespace trvg {
 
 constexpr int FFT_THREADS = 256;
 
+// Support pruning for band-limited inverse transforms: input cells exist only
+// where |k| < k_hi, so (with kcut2 = k_hi^2 (1 + 1e-9))
+//   line (i, j) of the z pass is needed only if kx^2 + ky^2 < kcut2 (P1);
+//   slab i of the y pass only if kx^2 < kcut2 (P2).
+// Passes skip what is not needed, and later passes read only rows that
+// earlier passes wrote, treating the rest as zero.
+struct Prune {
+  int on;
+  double kcut2;
+  double dk0, dk1;
+  int n0, n1;
+  // Explicitly rounded operations: every pass must evaluate the predicates
+  // identically (no contraction differences between kernels).
+  __device__ __forceinline__ bool p2(int i) const {
+    double kx = __dmul_rn(double(shifted(i, n0)), dk0);
+    return __dmul_rn(kx, kx) < kcut2;
+  }
+  __device__ __forceinline__ bool p1(int i, int j) const {
+    double kx = __dmul_rn(double(shifted(i, n0)), dk0);
+    double ky = __dmul_rn(double(shifted(j, n1)), dk1);
+    return __dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky)) < kcut2;
+  }
+};
+
 // In-LDS Stockham transform of L lines of length N (N = 2^LOGN) stored as
 // buf[line * (N + 1) + n] (one pad element per line against bank conflicts).  W[m] = e^{sign 2 pi i m / N} (m < N) in LDS.
 template <typename T, int LOGN, int EPT>
@@ -123,7 +147,7 @@ __device__ __forceinline__ void lds_fft_lines(c2<T>* buf, const c2<T>* W,
 template <typename T, int LOGN, int EPT, class Loader>
 __global__ void __launch_bounds__(FFT_THREADS)
 fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
-           int n0, int n1, int sign) {
+           int n0, int n1, int sign, Prune pr) {
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   constexpr int NLINES = E / N;
@@ -132,9 +156,18 @@ fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* W = buf + NLINES * LS;
   const int t = threadIdx.x;
-  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
   const long long line0 = (long long)blockIdx.x * NLINES;
   const long long base = line0 * N;
+  if (pr.on) {
+    // Skip the workgroup if none of its lines is needed (uniform exit).
+    bool any = false;
+    for (int l = 0; l < NLINES && !any; l++) {
+      long long line = line0 + l;
+      any = pr.p1(int(line / n1), int(line % n1));
+    }
+    if (!any) return;
+  }
+  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
     int idx = t + e * FFT_THREADS;
@@ -160,11 +193,14 @@ fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
 //   y pass: S = n2, inner = n2, outer_stride = n1*n2;
 //   x pass: S = n1*n2, inner = n1*n2, outer_stride = 0.
 // Each workgroup takes L = E/N consecutive lines.
+//
+// prune_mode (with pr.on): 1 = y pass (slab i = id / inner must satisfy P2,
+// row j read only where P1(i, j)); 2 = x pass (row i read only where P2(i)).
 template <typename T, int LOGN, int EPT>
 __global__ void __launch_bounds__(FFT_THREADS)
 fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
                  long long S, long long outer_stride, long long inner,
-                 int sign) {
+                 int sign, Prune pr, int prune_mode) {
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   constexpr int NLINES = E / N;
@@ -173,8 +209,13 @@ fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* W = buf + NLINES * LS;
   const int t = threadIdx.x;
-  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
   const long long id0 = (long long)blockIdx.x * NLINES;
+  if (pr.on && prune_mode == 1) {
+    bool any = false;
+    for (int l = 0; l < NLINES && !any; l++) any = pr.p2(int((id0 + l) / inner));
+    if (!any) return;
+  }
+  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
   // Global element (n, l): line id0 + l, position n; LDS line-major [l][n].
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
@@ -182,7 +223,16 @@ fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
     int n = idx / NLINES, l = idx % NLINES;
     long long id = id0 + l;
     long long base = (id / inner) * outer_stride + id % inner;
-    buf[l * LS + n] = data[base + (long long)n * S];
+    bool need = true;
+    if (pr.on) {
+      if (prune_mode == 1) {
+        int i = int(id / inner);
+        need = pr.p2(i) && pr.p1(i, n);
+      } else if (prune_mode == 2) {
+        need = pr.p2(n);
+      }
+    }
+    buf[l * LS + n] = need ? data[base + (long long)n * S] : mk<T>(T(0), T(0));
   }
   __syncthreads();
   lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
@@ -192,6 +242,7 @@ fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
     int n = idx / NLINES, l = idx % NLINES;
     long long id = id0 + l;
     long long base = (id / inner) * outer_stride + id % inner;
+    if (pr.on && prune_mode == 1 && !pr.p2(int(id / inner))) continue;
     data[base + (long long)n * S] = buf[l * LS + n];
   }
 }

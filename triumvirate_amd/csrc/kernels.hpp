@@ -154,13 +154,13 @@ struct LoadScaleShift {
 
 // src / W(k) * scale (apply_assignment_compensation then the 1/V of
 // inv_fourier_transform, field.cpp:1657-1720, 1764-1785).
-template <typename T>
+template <typename T, typename S = T>
 struct LoadDeconv {
-  const c2<T>* src;
+  const c2<S>* src;
   AxisTables tab;
   double scale;
   __device__ c2<T> operator()(long long g, int i, int j, int k) const {
-    c2<T> v = src[g];
+    c2<S> v = src[g];
     double f = scale / window_at(tab, i, j, k);
     return mk<T>(T(double(v.x) * f), T(double(v.y) * f));
   }
@@ -169,9 +169,9 @@ struct LoadDeconv {
 // Shell filter: y_lm(k^) src(k) / W(k) for k_lo <= |k| < k_hi, else 0
 // (inv_fourier_transform_ylm_wgtd_field_band_limited, field.cpp:1792-1906;
 // the 1/n_modes factor is applied to the pair sums instead).
-template <typename T>
+template <typename T, typename S = T>
 struct LoadShell {
-  const c2<T>* src;
+  const c2<S>* src;
   AxisTables tab;
   Geom geo;
   double k_lo, k_hi;
@@ -182,7 +182,7 @@ struct LoadShell {
     double kz = grid_coord(k, geo.n[2], geo.dk[2]);
     double km = mag3_exact(kx, ky, kz);
     if (!(k_lo <= km && km < k_hi)) return mk<T>(T(0), T(0));
-    c2<T> v = src[g];
+    c2<S> v = src[g];
     double iw = 1. / window_at(tab, i, j, k);
     double2 fk = dmk(double(v.x) * iw, double(v.y) * iw);
     double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
@@ -193,9 +193,9 @@ struct LoadShell {
 
 // j_l(|k| r) y_lm(k^) src(k) / W(k) / V
 // (inv_fourier_transform_sjl_ylm_wgtd_field, field.cpp:1908-2010).
-template <typename T>
+template <typename T, typename S = T>
 struct LoadSjl {
-  const c2<T>* src;
+  const c2<S>* src;
   AxisTables tab;
   Geom geo;
   SplineDev sj;
@@ -206,7 +206,7 @@ struct LoadSjl {
     double ky = grid_coord(j, geo.n[1], geo.dk[1]);
     double kz = grid_coord(k, geo.n[2], geo.dk[2]);
     double km = mag3_exact(kx, ky, kz);
-    c2<T> v = src[g];
+    c2<S> v = src[g];
     double iw = 1. / window_at(tab, i, j, k);
     double2 fk = dmk(double(v.x) * iw, double(v.y) * iw);
     double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
@@ -677,6 +677,461 @@ __global__ void k_copy(c2<T>* __restrict__ dst, const c2<T>* __restrict__ src,
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
        c += (long long)gridDim.x * blockDim.x)
     dst[c] = src[c];
+}
+
+}  // namespace trvg
+
+namespace trvg {
+
+// ---------------------------------------------------------------------------
+// Tile-binned assignment.
+//
+// Particles are counting-sorted by the 8^3-cell tile that holds their base
+// cell index (the int(loc) of every scheme).  A workgroup accumulates one
+// chunk of a tile's particles into an 11^3 LDS block (the tile plus the
+// [-1, +2] window halo) in FP64 and flushes it to the mesh with one global
+// atomic per touched cell.  For base indices inside the mesh every scheme's
+// wrap rule (field.cpp:655-1033) is cell = (idx + offset) mod n, which is
+// what the flush applies.  Particles whose base index falls outside the mesh
+// (positions outside [0, L)) are "stragglers" and keep the reference's
+// flattened-index rule through k_assign.
+// ---------------------------------------------------------------------------
+
+constexpr int TILE = 8;
+constexpr int TILE_H = TILE + 3;                    // halo -1 .. +2
+constexpr int TILE_CELLS = TILE_H * TILE_H * TILE_H;  // 1331
+
+struct TileGeom {
+  int nt[3];          // tiles per axis
+  long long ntiles;
+};
+
+// Base cell index per axis of a particle (the int(loc) of field.cpp).
+__device__ __forceinline__ void base_cell(const Geom& g, double x, double y,
+                                          double z, int idx[3], double s[3]) {
+  const double pos[3] = {x, y, z};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    double loc = __ddiv_rn(__dmul_rn(double(g.n[a]), pos[a]), g.L[a]);
+    idx[a] = int(loc);
+    s[a] = __dadd_rn(loc, -double(idx[a]));
+  }
+}
+
+// Tile id per particle (ntiles for stragglers) and per-tile counts.
+static __global__ void __launch_bounds__(256)
+k_tile_count(DevParticles p, Geom g, TileGeom tg, int* __restrict__ tile_of,
+             unsigned int* __restrict__ counts) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < p.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    int idx[3];
+    double s[3];
+    base_cell(g, p.x[i], p.y[i], p.z[i], idx, s);
+    long long t = tg.ntiles;
+    if (idx[0] >= 0 && idx[0] < g.n[0] && idx[1] >= 0 && idx[1] < g.n[1] &&
+        idx[2] >= 0 && idx[2] < g.n[2]) {
+      t = ((long long)(idx[0] / TILE) * tg.nt[1] + idx[1] / TILE) * tg.nt[2] +
+          idx[2] / TILE;
+    }
+    tile_of[i] = int(t);
+    atomicAdd(&counts[t], 1u);
+  }
+}
+
+// Exclusive scan of n counts by one workgroup (n up to a few 1e6).
+static __global__ void __launch_bounds__(1024)
+k_scan_counts(const unsigned int* __restrict__ counts, long long n,
+              unsigned int* __restrict__ offsets) {
+  __shared__ unsigned int sh[1024];
+  __shared__ unsigned int carry;
+  if (threadIdx.x == 0) carry = 0u;
+  __syncthreads();
+  for (long long base = 0; base < n; base += 1024) {
+    long long i = base + threadIdx.x;
+    unsigned int v = i < n ? counts[i] : 0u;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      unsigned int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n) offsets[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += sh[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) offsets[n] = carry;
+}
+
+// Scatter particle columns into tile order.
+static __global__ void __launch_bounds__(256)
+k_tile_scatter(DevParticles p, const int* __restrict__ tile_of,
+               unsigned int* __restrict__ cursor, double* __restrict__ sx,
+               double* __restrict__ sy, double* __restrict__ sz,
+               double* __restrict__ sw, double* __restrict__ slos) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < p.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    unsigned int d = atomicAdd(&cursor[tile_of[i]], 1u);
+    sx[d] = p.x[i];
+    sy[d] = p.y[i];
+    sz[d] = p.z[i];
+    sw[d] = p.w[i];
+    if (slos) {
+      slos[3 * (long long)d] = p.los[3 * i];
+      slos[3 * (long long)d + 1] = p.los[3 * i + 1];
+      slos[3 * (long long)d + 2] = p.los[3 * i + 2];
+    }
+  }
+}
+
+// One work item: a chunk [start, start + count) of one tile's particles.
+struct TileWork {
+  int tile;
+  int count;
+  long long start;
+};
+
+template <typename T, int ORDER>
+__global__ void __launch_bounds__(256)
+k_assign_tiled(c2<T>* __restrict__ mesh, DevParticles p, Geom g, TileGeom tg,
+               const TileWork* __restrict__ work, int mode, int ell, int m,
+               double scale, int cplx_w) {
+  __shared__ double acc_re[TILE_CELLS];
+  __shared__ double acc_im[TILE_CELLS];
+  const TileWork wk = work[blockIdx.x];
+  for (int c = threadIdx.x; c < TILE_CELLS; c += blockDim.x) {
+    acc_re[c] = 0.;
+    acc_im[c] = 0.;
+  }
+  const int t = wk.tile;
+  const int tz = t % tg.nt[2];
+  const int ty = (t / tg.nt[2]) % tg.nt[1];
+  const int tx = t / (tg.nt[2] * tg.nt[1]);
+  const int o0[3] = {tx * TILE - 1, ty * TILE - 1, tz * TILE - 1};
+  const double inv_vol_cell = 1 / g.vol_cell;
+  __syncthreads();
+  for (int q = threadIdx.x; q < wk.count; q += blockDim.x) {
+    const long long pid = wk.start + q;
+    int idx[3];
+    double s[3];
+    base_cell(g, p.x[pid], p.y[pid], p.z[pid], idx, s);
+    int lo[3];            // local LDS index of the first covered cell
+    double win[ORDER][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const double sa = s[a];
+      int first = idx[a];
+      if (ORDER == 1) {
+        if (sa >= 0.5) first = idx[a] + 1;
+        win[0][a] = 1.;
+      } else if (ORDER == 2) {
+        win[0][a] = 1. - sa;
+        win[1][a] = sa;
+      } else if (ORDER == 3) {
+        if (sa < 0.5) {
+          first = idx[a] - 1;
+          win[0][a] = 0.5 * (0.5 - sa) * (0.5 - sa);
+          win[1][a] = 0.75 - sa * sa;
+          win[2][a] = 0.5 * (0.5 + sa) * (0.5 + sa);
+        } else {
+          double sp = 1. - sa;
+          win[0][a] = 0.5 * (0.5 + sp) * (0.5 + sp);
+          win[1][a] = 0.75 - sp * sp;
+          win[2][a] = 0.5 * (0.5 - sp) * (0.5 - sp);
+        }
+      } else {
+        first = idx[a] - 1;
+        double sm = 1. - sa;
+        win[0][a] = 1. / 6 * sm * sm * sm;
+        win[1][a] = 1. / 6 * (4. - 6. * sa * sa + 3. * sa * sa * sa);
+        win[2][a] = 1. / 6 * (4. - 6. * sm * sm + 3. * sm * sm * sm);
+        win[3][a] = 1. / 6 * sa * sa * sa;
+      }
+      lo[a] = first - o0[a];
+    }
+    double2 wt = particle_weight(p, pid, mode, ell, m);
+    const double wre = inv_vol_cell * wt.x * scale;
+    const double wim = inv_vol_cell * wt.y * scale;
+#pragma unroll
+    for (int il = 0; il < ORDER; il++) {
+#pragma unroll
+      for (int jl = 0; jl < ORDER; jl++) {
+        const int row = ((lo[0] + il) * TILE_H + (lo[1] + jl)) * TILE_H + lo[2];
+        const double wij = win[il][0] * win[jl][1];
+#pragma unroll
+        for (int kl = 0; kl < ORDER; kl++) {
+          const double wv = wij * win[kl][2];
+          atomicAdd(&acc_re[row + kl], wre * wv);
+          if (cplx_w) atomicAdd(&acc_im[row + kl], wim * wv);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < TILE_CELLS; c += blockDim.x) {
+    const double vr = acc_re[c], vi = acc_im[c];
+    if (vr == 0. && vi == 0.) continue;
+    const int lz = c % TILE_H, ly = (c / TILE_H) % TILE_H, lx = c / (TILE_H * TILE_H);
+    int gx = o0[0] + lx, gy = o0[1] + ly, gz = o0[2] + lz;
+    gx = (gx % g.n[0] + g.n[0]) % g.n[0];
+    gy = (gy % g.n[1] + g.n[1]) % g.n[1];
+    gz = (gz % g.n[2] + g.n[2]) % g.n[2];
+    T* cell = reinterpret_cast<T*>(mesh + ((long long)gx * g.n[1] + gy) * g.n[2] + gz);
+    unsafeAtomicAdd(cell, T(vr));
+    if (cplx_w) unsafeAtomicAdd(cell + 1, T(vi));
+  }
+}
+
+}  // namespace trvg
+
+namespace trvg {
+
+// ---------------------------------------------------------------------------
+// Register-tiled pair product-reduction.
+//   S[ia][ib] = sum_x FA[ia](x) FB[ib](x) G(x)
+// A launch covers a list of 4x4 tiles of the (ia, ib) grid (up to PT_MAXT
+// tiles).  Each block stages PT_CELLS cells of every field in LDS (one HBM
+// read per field and cell), forms H = FB * G once, and thread (tile, group)
+// accumulates its 16 pair sums in FP64 registers over its cells of every
+// chunk the block visits.  Output: per-block partials [gridDim.x][ntiles*16].
+// ---------------------------------------------------------------------------
+
+constexpr int PT_CELLS = 64;
+constexpr int PT_MAXT = 32;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pair_tiles(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
+             int nbf, const c2<T>* __restrict__ G, long long nmesh,
+             const int2* __restrict__ tiles, int ntiles,
+             double2* __restrict__ part) {
+  extern __shared__ double2 shp[];
+  double2* sa = shp;                          // [na][PT_CELLS]
+  double2* sh = shp + (size_t)na * PT_CELLS;  // [nbf][PT_CELLS]
+  const int ngroups = 256 / ntiles;
+  const int tile = threadIdx.x % ntiles;
+  const int group = threadIdx.x / ntiles;
+  const bool active = group < ngroups;
+  const int2 t0 = tiles[tile];
+  double ar[16], ai[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) ar[q] = ai[q] = 0.;
+  int ia[4], ib[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    ia[q] = min(t0.x + q, na - 1);
+    ib[q] = min(t0.y + q, nbf - 1);
+  }
+  const long long nchunks = (nmesh + PT_CELLS - 1) / PT_CELLS;
+  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const long long c0 = ch * PT_CELLS;
+    __syncthreads();
+    for (int e = threadIdx.x; e < na * PT_CELLS; e += 256) {
+      int f = e / PT_CELLS, c = e % PT_CELLS;
+      long long gi = c0 + c;
+      double2 v = dmk(0., 0.);
+      if (gi < nmesh) {
+        c2<T> a = FA[(long long)f * nmesh + gi];
+        v = dmk(a.x, a.y);
+      }
+      sa[e] = v;
+    }
+    for (int e = threadIdx.x; e < nbf * PT_CELLS; e += 256) {
+      int f = e / PT_CELLS, c = e % PT_CELLS;
+      long long gi = c0 + c;
+      double2 h = dmk(0., 0.);
+      if (gi < nmesh) {
+        c2<T> b = FB[(long long)f * nmesh + gi];
+        c2<T> gg = G[gi];
+        h = dmul(dmk(b.x, b.y), dmk(gg.x, gg.y));
+      }
+      sh[e] = h;
+    }
+    __syncthreads();
+    if (active) {
+      for (int c = group; c < PT_CELLS; c += ngroups) {
+        double2 x[4], y[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          x[q] = sa[ia[q] * PT_CELLS + c];
+          y[q] = sh[ib[q] * PT_CELLS + c];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            ar[i * 4 + j] += x[i].x * y[j].x - x[i].y * y[j].y;
+            ai[i * 4 + j] += x[i].x * y[j].y + x[i].y * y[j].x;
+          }
+        }
+      }
+    }
+  }
+  // Reduce over groups through LDS (reusing the staging buffer).
+  __syncthreads();
+  double2* red = shp;  // [256][16] double2 = 64 KiB at most; staging >= that
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < 16; q++) red[threadIdx.x * 16 + q] = dmk(ar[q], ai[q]);
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < ntiles * 16; o += 256) {
+    int tl = o / 16, q = o % 16;
+    double sx = 0., sy = 0.;
+    for (int gr = 0; gr < ngroups; gr++) {
+      double2 v = red[(gr * ntiles + tl) * 16 + q];
+      sx += v.x;
+      sy += v.y;
+    }
+    part[(long long)blockIdx.x * ntiles * 16 + o] = dmk(sx, sy);
+  }
+}
+
+}  // namespace trvg
+
+namespace trvg {
+
+// Radial histogram over octant representatives: the cells (+-a, +-b, +-c)
+// share n = a^2 + b^2 + c^2, so each thread sums its (up to 8) mirror cells
+// and issues one atomic pair per representative.
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS)
+k_radial_hist_oct(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
+                  int mb, double* __restrict__ Q) {
+  const int h0 = g.n[0] / 2 + 1, h1 = g.n[1] / 2 + 1, h2 = g.n[2] / 2 + 1;
+  const long long nrep = (long long)h0 * h1 * h2;
+  const bool trivial = (la == 0 && ma == 0 && lb == 0 && mb == 0);
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nrep;
+       r += (long long)gridDim.x * blockDim.x) {
+    const int c = int(r % h2);
+    const int b = int((r / h2) % h1);
+    const int a = int(r / ((long long)h2 * h1));
+    int is[2], js[2], ks[2];
+    const int ni = (a == 0 || 2 * a == g.n[0]) ? 1 : 2;
+    const int nj = (b == 0 || 2 * b == g.n[1]) ? 1 : 2;
+    const int nk = (c == 0 || 2 * c == g.n[2]) ? 1 : 2;
+    is[0] = a; is[1] = g.n[0] - a;
+    js[0] = b; js[1] = g.n[1] - b;
+    ks[0] = c; ks[1] = g.n[2] - c;
+    double sx = 0., sy = 0.;
+    for (int u = 0; u < ni; u++)
+      for (int v = 0; v < nj; v++)
+        for (int w = 0; w < nk; w++) {
+          const int i = is[u], j = js[v], k = ks[w];
+          const long long cell = ((long long)i * g.n[1] + j) * g.n[2] + k;
+          c2<T> tv = t[cell];
+          double2 val = dmk(tv.x, tv.y);
+          if (!trivial) {
+            double rx = shifted(i, g.n[0]) * g.dr[0];
+            double ry = shifted(j, g.n[1]) * g.dr[1];
+            double rz = shifted(k, g.n[2]) * g.dr[2];
+            double2 ya = ylm_reduced_dev(la, ma, rx, ry, rz);
+            double2 yb = ylm_reduced_dev(lb, mb, rx, ry, rz);
+            val = dmul(val, dmul(ya, yb));
+          }
+          sx += val.x;
+          sy += val.y;
+        }
+    const long long nsq = (long long)a * a + (long long)b * b + (long long)c * c;
+    unsafeAtomicAdd(&Q[2 * nsq], sx);
+    unsafeAtomicAdd(&Q[2 * nsq + 1], sy);
+  }
+}
+
+}  // namespace trvg
+
+namespace trvg {
+
+// Pair reduction, v2 (see k_pair_tiles for the data flow).  Staging is in the
+// field precision T; threads are mapped group-major (the threads of one tile
+// read consecutive cells); each thread forms its chunk partial in T and
+// promotes it to FP64 accumulators once per chunk; the cross-group
+// reduction goes through a 4 KiB buffer, 16 rounds.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pair_tiles2(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
+              int nbf, const c2<T>* __restrict__ G, long long nmesh,
+              const int2* __restrict__ tiles, int ntiles,
+              double2* __restrict__ part) {
+  extern __shared__ char sm2_raw[];
+  c2<T>* sa = reinterpret_cast<c2<T>*>(sm2_raw);           // [na][PT_CELLS]
+  c2<T>* sh = sa + (size_t)na * PT_CELLS;                   // [nbf][PT_CELLS]
+  const int ngroups = 256 / ntiles;
+  const int group = threadIdx.x % ngroups;
+  const int tile = threadIdx.x / ngroups;
+  const bool active = tile < ntiles;
+  const int2 t0 = tiles[active ? tile : 0];
+  double ar[16], ai[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) ar[q] = ai[q] = 0.;
+  int ia[4], ib[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    ia[q] = min(t0.x + q, na - 1) * PT_CELLS;
+    ib[q] = min(t0.y + q, nbf - 1) * PT_CELLS;
+  }
+  const long long nchunks = (nmesh + PT_CELLS - 1) / PT_CELLS;
+  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const long long c0 = ch * PT_CELLS;
+    __syncthreads();
+    for (int e = threadIdx.x; e < na * PT_CELLS; e += 256) {
+      int f = e / PT_CELLS, c = e % PT_CELLS;
+      long long gi = c0 + c;
+      sa[e] = gi < nmesh ? FA[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
+    }
+    for (int e = threadIdx.x; e < nbf * PT_CELLS; e += 256) {
+      int f = e / PT_CELLS, c = e % PT_CELLS;
+      long long gi = c0 + c;
+      c2<T> h = mk<T>(T(0), T(0));
+      if (gi < nmesh) h = cmul<T>(FB[(long long)f * nmesh + gi], G[gi]);
+      sh[e] = h;
+    }
+    __syncthreads();
+    if (active) {
+      T pr[16], pi[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) pr[q] = pi[q] = T(0);
+      for (int c = group; c < PT_CELLS; c += ngroups) {
+        c2<T> x[4], y[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          x[q] = sa[ia[q] + c];
+          y[q] = sh[ib[q] + c];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            pr[i * 4 + j] += x[i].x * y[j].x - x[i].y * y[j].y;
+            pi[i * 4 + j] += x[i].x * y[j].y + x[i].y * y[j].x;
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        ar[q] += double(pr[q]);
+        ai[q] += double(pi[q]);
+      }
+    }
+  }
+  __syncthreads();
+  double2* red = reinterpret_cast<double2*>(sm2_raw);   // [256]
+  for (int q = 0; q < 16; q++) {
+    red[threadIdx.x] = active ? dmk(ar[q], ai[q]) : dmk(0., 0.);
+    __syncthreads();
+    if (threadIdx.x < ntiles) {
+      double sx = 0., sy = 0.;
+      for (int gr = 0; gr < ngroups; gr++) {
+        double2 v = red[threadIdx.x * ngroups + gr];
+        sx += v.x;
+        sy += v.y;
+      }
+      part[(long long)blockIdx.x * ntiles * 16 + threadIdx.x * 16 + q] = dmk(sx, sy);
+    }
+    __syncthreads();
+  }
 }
 
 }  // namespace trvg
