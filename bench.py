@@ -123,15 +123,21 @@ def main():
     multipoles = [((0, 0, 0), 'triu'), ((2, 0, 2), 'full')]
 
     def step(prec):
+        # One engine call computes both multipoles on shared fields.
+        plist = [params_struct(args.ngrid, deg, shape, prec)
+                 for deg, shape in multipoles]
+        res = [_abi.ResultBuffers(_abi.result_dim(p.shape, 20, 0), 'bispec')
+               for p in plist]
+        arr = (_abi.Params * len(plist))(*plist)
+        outs = (_abi.BispecResult * len(plist))(*[r.struct for r in res])
+        norms = np.full(len(plist), norm.value)
+        check(lib.trvg_compute_bispec_multi_dev(
+            hd, hr, arr, len(plist), C.byref(kbv.struct), _abi.dptr(norms),
+            outs))
         out = {}
-        for deg, shape in multipoles:
-            p = params_struct(args.ngrid, deg, shape, prec)
-            dim = _abi.result_dim(p.shape, 20, 0)
-            res = _abi.ResultBuffers(dim, 'bispec')
-            check(lib.trvg_compute_bispec_dev(hd, hr, C.byref(p),
-                                              C.byref(kbv.struct), norm.value,
-                                              C.byref(res.struct)))
-            out[deg] = res.as_dict()
+        for (deg, _), r, o in zip(multipoles, res, outs):
+            r.struct.dim = o.dim
+            out[deg] = r.as_dict()
         return out
 
     pairs_per_step = 210 + 400

@@ -1,0 +1,401 @@
+// triumvirate_gpu.hpp -- C++ drop-in for Triumvirate's three-point API,
+// implemented over the C ABI of include/trv_gpu.h (header only).
+//
+// The names, member layouts and function signatures follow the reference:
+//   trv::ParticleData / trv::ParticleCatalogue  (include/particles.hpp:63-78)
+//   trv::LineOfSight                             (include/dataobjs.hpp:186)
+//   trv::Binning                                 (include/dataobjs.hpp:58)
+//   trv::BispecMeasurements / ThreePCFMeasurements (include/dataobjs.hpp:249-283)
+//   trv::ParameterSet (numerical subset + validate semantics,
+//                      include/parameters.hpp:58-240)
+//   trv::compute_bispec, compute_3pcf, compute_bispec_in_gpp_box,
+//   compute_3pcf_in_gpp_box, calc_bispec_normalisation_from_particles,
+//   calc_bispec_normalisation_from_mesh      (include/threept.hpp:138-294)
+// Errors are thrown as trv::sys::InvalidParameterError /
+// trv::sys::InvalidDataError (include/monitor.hpp:731-779), or
+// std::runtime_error for device failures.
+#ifndef TRIUMVIRATE_GPU_HPP_INCLUDED
+#define TRIUMVIRATE_GPU_HPP_INCLUDED
+
+#include <cmath>
+#include <complex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "trv_gpu.h"
+
+namespace trv {
+
+namespace sys {
+class InvalidParameterError : public std::invalid_argument {
+ public:
+  explicit InvalidParameterError(const std::string& m) : std::invalid_argument(m) {}
+};
+class InvalidDataError : public std::runtime_error {
+ public:
+  explicit InvalidDataError(const std::string& m) : std::runtime_error(m) {}
+};
+
+inline void check_status(int st) {
+  if (st == TRVG_OK) return;
+  std::string msg = trvg_last_error();
+  if (st == TRVG_ERR_PARAM) throw InvalidParameterError(msg);
+  if (st == TRVG_ERR_DATA) throw InvalidDataError(msg);
+  throw std::runtime_error("[trvg status " + std::to_string(st) + "] " + msg);
+}
+}  // namespace sys
+
+struct ParticleData {
+  double pos[3];
+  double nz;
+  double ws;
+  double wc;
+  double w;
+};
+
+struct LineOfSight {
+  double pos[3];
+};
+
+class ParticleCatalogue {
+ public:
+  std::string source;
+  ParticleData* pdata = nullptr;
+  int ntotal = 0;
+  double wtotal = 0.;
+  double wstotal = 0.;
+  double pos_min[3] = {0., 0., 0.};
+  double pos_max[3] = {0., 0., 0.};
+  double pos_span[3] = {0., 0., 0.};
+
+  ParticleCatalogue() = default;
+  ParticleCatalogue(const ParticleCatalogue&) = delete;
+  ParticleCatalogue& operator=(const ParticleCatalogue&) = delete;
+  ~ParticleCatalogue() { delete[] pdata; }
+
+  ParticleData& operator[](int pid) { return pdata[pid]; }
+
+  // ParticleCatalogue::load_particle_data (src/particles.cpp:512-565).
+  int load_particle_data(std::vector<double> x, std::vector<double> y,
+                         std::vector<double> z, std::vector<double> nz,
+                         std::vector<double> ws, std::vector<double> wc) {
+    source = "extdata";
+    std::size_t n = x.size();
+    if (!(n == y.size() && n == z.size() && n == nz.size() && n == ws.size() &&
+          n == wc.size()))
+      throw sys::InvalidDataError(
+          "Inconsistent particle data dimensions (source=extdata).");
+    delete[] pdata;
+    ntotal = static_cast<int>(n);
+    pdata = new ParticleData[n];
+    for (std::size_t i = 0; i < n; i++) {
+      pdata[i].pos[0] = x[i];
+      pdata[i].pos[1] = y[i];
+      pdata[i].pos[2] = z[i];
+      pdata[i].nz = nz[i];
+      pdata[i].ws = ws[i];
+      pdata[i].wc = wc[i];
+      pdata[i].w = ws[i] * wc[i];
+    }
+    wtotal = wstotal = 0.;
+    for (int i = 0; i < ntotal; i++) {
+      wtotal += pdata[i].w;
+      wstotal += pdata[i].ws;
+    }
+    for (int a = 0; a < 3; a++) {
+      pos_min[a] = pos_max[a] = ntotal ? pdata[0].pos[a] : 0.;
+      for (int i = 0; i < ntotal; i++) {
+        pos_min[a] = std::fmin(pos_min[a], pdata[i].pos[a]);
+        pos_max[a] = std::fmax(pos_max[a], pdata[i].pos[a]);
+      }
+      pos_span[a] = pos_max[a] - pos_min[a];
+    }
+    return 0;
+  }
+
+  // C-ABI view of the 56-byte records in place (stride 7 doubles).
+  trvg_catalogue abi_view() const {
+    if (pdata == nullptr)
+      throw sys::InvalidDataError("Particle data are uninitialised.");
+    trvg_catalogue v;
+    v.ntotal = ntotal;
+    v.stride = sizeof(ParticleData) / sizeof(double);
+    v.x = &pdata[0].pos[0];
+    v.y = &pdata[0].pos[1];
+    v.z = &pdata[0].pos[2];
+    v.nz = &pdata[0].nz;
+    v.ws = &pdata[0].ws;
+    v.wc = &pdata[0].wc;
+    return v;
+  }
+};
+
+// Numerical subset of trv::ParameterSet with the validation rules the
+// three-point path depends on (src/parameters.cpp:466-1270).
+class ParameterSet {
+ public:
+  double boxsize[3] = {0., 0., 0.};
+  int ngrid[3] = {0, 0, 0};
+  double volume = 0.;
+  long long nmesh = 0;
+  std::string assignment = "tsc";
+  int assignment_order = 3;
+  std::string interlace = "false";
+  std::string catalogue_type;
+  std::string statistic_type;
+  std::string npoint;
+  std::string space;
+  int ell1 = 0, ell2 = 0, ELL = 0;
+  std::string form = "diag";
+  std::string shape = "diag";
+  int idx_bin = 0;
+  int num_bins = 0;
+  double bin_min = 0., bin_max = 0.;
+  std::string binning = "lin";
+  std::string norm_convention = "particle";
+  int verbose = 20;
+  int precision = TRVG_PREC_FP64;   // engine extension: 64 or 32
+
+  int validate() {
+    if (assignment == "ngp") assignment_order = 1;
+    else if (assignment == "cic") assignment_order = 2;
+    else if (assignment == "tsc") assignment_order = 3;
+    else if (assignment == "pcs") assignment_order = 4;
+    else
+      throw sys::InvalidParameterError(
+          "Mesh assignment scheme must be 'ngp', 'cic', 'tsc' or 'pcs': "
+          "`assignment` = '" + assignment + "'.");
+    if (interlace == "true" || interlace == "on") interlace = "true";
+    else if (interlace == "false" || interlace == "off") interlace = "false";
+    else
+      throw sys::InvalidParameterError(
+          "Interlacing must be 'true'/'on' or 'false'/'off'.");
+    if (statistic_type == "bispec") { npoint = "3pt"; space = "fourier"; }
+    else if (statistic_type == "3pcf") { npoint = "3pt"; space = "config"; }
+    if (!(form == "full" || form == "diag" || form == "off-diag" || form == "row"))
+      throw sys::InvalidParameterError(
+          "Three-point statistic form is not recognised: `form` = '" + form + "'.");
+    shape = (form == "full" && ell1 == ell2) ? "triu" : form;
+    volume = boxsize[0] * boxsize[1] * boxsize[2];
+    nmesh = static_cast<long long>(ngrid[0]) * ngrid[1] * ngrid[2];
+    if (npoint == "3pt" && interlace == "true") interlace = "false";
+    return 0;
+  }
+
+  trvg_params abi_view() const {
+    trvg_params p{};
+    for (int a = 0; a < 3; a++) {
+      p.boxsize[a] = boxsize[a];
+      p.ngrid[a] = ngrid[a];
+    }
+    p.assignment_order = assignment_order;
+    p.interlace = (interlace == "true") ? 1 : 0;
+    p.ell1 = ell1;
+    p.ell2 = ell2;
+    p.ELL = ELL;
+    if (shape == "diag") p.shape = TRVG_SHAPE_DIAG;
+    else if (shape == "off-diag") p.shape = TRVG_SHAPE_OFFDIAG;
+    else if (shape == "row") p.shape = TRVG_SHAPE_ROW;
+    else if (shape == "full") p.shape = TRVG_SHAPE_FULL;
+    else if (shape == "triu") p.shape = TRVG_SHAPE_TRIU;
+    else throw sys::InvalidParameterError("Unknown shape '" + shape + "'.");
+    p.idx_bin = idx_bin;
+    p.precision = precision;
+    p.verbose = verbose;
+    p.device = -1;
+    return p;
+  }
+};
+
+// trv::Binning, linear and logarithmic schemes
+// (src/dataobjs.cpp:134-160 arithmetic).
+class Binning {
+ public:
+  std::string space, scheme;
+  double bin_min = 0., bin_max = 0.;
+  int num_bins = 0;
+  std::vector<double> bin_edges, bin_centres, bin_widths;
+
+  Binning(std::string space_, std::string scheme_)
+      : space(std::move(space_)), scheme(std::move(scheme_)) {}
+  explicit Binning(const ParameterSet& p) : Binning(p.space, p.binning) {
+    set_bins(p.bin_min, p.bin_max, p.num_bins);
+  }
+
+  void set_bins(double coord_min, double coord_max, int nbin) {
+    if (coord_min < 0.)
+      throw sys::InvalidParameterError("Bin range must be non-negative.");
+    if (nbin <= 0)
+      throw sys::InvalidParameterError("Number of bins must be positive.");
+    bin_min = coord_min;
+    bin_max = coord_max;
+    num_bins = nbin;
+    bin_edges.clear();
+    bin_centres.clear();
+    bin_widths.clear();
+    if (scheme == "lin") {
+      double dbin = (bin_max - bin_min) / double(num_bins);
+      for (int i = 0; i < num_bins; i++) {
+        double left = bin_min + dbin * i;
+        bin_edges.push_back(left);
+        bin_centres.push_back(left + dbin / 2.);
+        bin_widths.push_back(dbin);
+      }
+    } else if (scheme == "log") {
+      if (bin_min == 0.)
+        throw sys::InvalidParameterError(
+            "Cannot use logarithmic binning when the lowest edge is zero.");
+      double dln = (std::log(bin_max) - std::log(bin_min)) / double(num_bins);
+      for (int i = 0; i < num_bins; i++) {
+        double left = bin_min * std::exp(dln * i);
+        double right = bin_min * std::exp(dln * (i + 1));
+        bin_edges.push_back(left);
+        bin_centres.push_back((left + right) / 2.);
+        bin_widths.push_back(right - left);
+      }
+    } else {
+      throw sys::InvalidParameterError(
+          "Unrecognised/unsupported binning `scheme`: " + scheme + ".");
+    }
+    bin_edges.push_back(bin_max);
+  }
+
+  trvg_binning abi_view() const {
+    trvg_binning b;
+    b.num_bins = num_bins;
+    b.bin_edges = bin_edges.data();
+    b.bin_centres = bin_centres.data();
+    return b;
+  }
+};
+
+struct BispecMeasurements {
+  int dim = 0;
+  std::vector<double> k1_bin, k2_bin, k1_eff, k2_eff;
+  std::vector<int> nmodes_1, nmodes_2;
+  std::vector<std::complex<double>> bk_raw, bk_shot;
+};
+
+struct ThreePCFMeasurements {
+  int dim = 0;
+  std::vector<double> r1_bin, r2_bin, r1_eff, r2_eff;
+  std::vector<int> npairs_1, npairs_2;
+  std::vector<std::complex<double>> zeta_raw, zeta_shot;
+};
+
+namespace detail {
+
+inline int dim_for(const trvg_params& p, const Binning& b) {
+  int dim = 0;
+  sys::check_status(trvg_result_dim(p.shape, b.num_bins, p.idx_bin, &dim));
+  return dim;
+}
+
+template <class Call>
+BispecMeasurements run_bispec(const trvg_params& p, const Binning& b,
+                              Call&& call) {
+  BispecMeasurements m;
+  int dim = dim_for(p, b);
+  m.k1_bin.resize(dim); m.k2_bin.resize(dim);
+  m.k1_eff.resize(dim); m.k2_eff.resize(dim);
+  m.nmodes_1.resize(dim); m.nmodes_2.resize(dim);
+  m.bk_raw.resize(dim); m.bk_shot.resize(dim);
+  trvg_bispec_result r{dim, m.k1_bin.data(), m.k2_bin.data(), m.k1_eff.data(),
+                       m.k2_eff.data(), m.nmodes_1.data(), m.nmodes_2.data(),
+                       reinterpret_cast<double*>(m.bk_raw.data()),
+                       reinterpret_cast<double*>(m.bk_shot.data())};
+  sys::check_status(call(&r));
+  m.dim = r.dim;
+  return m;
+}
+
+template <class Call>
+ThreePCFMeasurements run_3pcf(const trvg_params& p, const Binning& b,
+                              Call&& call) {
+  ThreePCFMeasurements m;
+  int dim = dim_for(p, b);
+  m.r1_bin.resize(dim); m.r2_bin.resize(dim);
+  m.r1_eff.resize(dim); m.r2_eff.resize(dim);
+  m.npairs_1.resize(dim); m.npairs_2.resize(dim);
+  m.zeta_raw.resize(dim); m.zeta_shot.resize(dim);
+  trvg_3pcf_result r{dim, m.r1_bin.data(), m.r2_bin.data(), m.r1_eff.data(),
+                     m.r2_eff.data(), m.npairs_1.data(), m.npairs_2.data(),
+                     reinterpret_cast<double*>(m.zeta_raw.data()),
+                     reinterpret_cast<double*>(m.zeta_shot.data())};
+  sys::check_status(call(&r));
+  m.dim = r.dim;
+  return m;
+}
+
+}  // namespace detail
+
+inline double calc_bispec_normalisation_from_particles(
+    ParticleCatalogue& particles, double alpha = 1.) {
+  trvg_catalogue c = particles.abi_view();
+  double nf = 0.;
+  sys::check_status(trvg_calc_bispec_normalisation_from_particles(&c, alpha, &nf));
+  return nf;
+}
+
+inline double calc_bispec_normalisation_from_mesh(ParticleCatalogue& particles,
+                                                  ParameterSet& params,
+                                                  double alpha = 1.) {
+  trvg_catalogue c = particles.abi_view();
+  trvg_params p = params.abi_view();
+  double nf = 0.;
+  sys::check_status(trvg_calc_bispec_normalisation_from_mesh(&c, &p, alpha, &nf));
+  return nf;
+}
+
+inline BispecMeasurements compute_bispec(
+    ParticleCatalogue& catalogue_data, ParticleCatalogue& catalogue_rand,
+    LineOfSight* los_data, LineOfSight* los_rand, ParameterSet& params,
+    Binning& kbinning, double norm_factor) {
+  trvg_catalogue d = catalogue_data.abi_view(), r = catalogue_rand.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = kbinning.abi_view();
+  return detail::run_bispec(p, kbinning, [&](trvg_bispec_result* out) {
+    return trvg_compute_bispec(&d, &r, &los_data[0].pos[0], &los_rand[0].pos[0],
+                               &p, &b, norm_factor, out);
+  });
+}
+
+inline ThreePCFMeasurements compute_3pcf(
+    ParticleCatalogue& catalogue_data, ParticleCatalogue& catalogue_rand,
+    LineOfSight* los_data, LineOfSight* los_rand, ParameterSet& params,
+    Binning& rbinning, double norm_factor) {
+  trvg_catalogue d = catalogue_data.abi_view(), r = catalogue_rand.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = rbinning.abi_view();
+  return detail::run_3pcf(p, rbinning, [&](trvg_3pcf_result* out) {
+    return trvg_compute_3pcf(&d, &r, &los_data[0].pos[0], &los_rand[0].pos[0],
+                             &p, &b, norm_factor, out);
+  });
+}
+
+inline BispecMeasurements compute_bispec_in_gpp_box(
+    ParticleCatalogue& catalogue_data, ParameterSet& params, Binning kbinning,
+    double norm_factor) {
+  trvg_catalogue d = catalogue_data.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = kbinning.abi_view();
+  return detail::run_bispec(p, kbinning, [&](trvg_bispec_result* out) {
+    return trvg_compute_bispec_in_gpp_box(&d, &p, &b, norm_factor, out);
+  });
+}
+
+inline ThreePCFMeasurements compute_3pcf_in_gpp_box(
+    ParticleCatalogue& catalogue_data, ParameterSet& params, Binning& rbinning,
+    double norm_factor) {
+  trvg_catalogue d = catalogue_data.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = rbinning.abi_view();
+  return detail::run_3pcf(p, rbinning, [&](trvg_3pcf_result* out) {
+    return trvg_compute_3pcf_in_gpp_box(&d, &p, &b, norm_factor, out);
+  });
+}
+
+}  // namespace trv
+
+#endif  // TRIUMVIRATE_GPU_HPP_INCLUDED

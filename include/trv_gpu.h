@@ -231,6 +231,29 @@ int trvg_compute_3pcf_dev(const trvg_device_catalogue* data,
                           const trvg_binning* rbinning, double norm_factor,
                           trvg_3pcf_result* out);
 
+/* Several bispectrum multipoles in one call on the same catalogues:
+ * params[i] give (ell1, ell2, ELL, shape, idx_bin) per multipole and must
+ * agree on box, mesh, assignment and precision.  The multipoles share the
+ * common fields, shell statistics and shell fields (e.g. the (0, 0) shells
+ * of B_000 are the l2 = 0 side of B_202).  rand == NULL selects the
+ * periodic-box estimator.  outs[i] receives multipole i (norm_factors[i]). */
+int trvg_compute_bispec_multi_dev(const trvg_device_catalogue* data,
+                                  const trvg_device_catalogue* rand,
+                                  const trvg_params* params, int nmultipoles,
+                                  const trvg_binning* kbinning,
+                                  const double* norm_factors,
+                                  trvg_bispec_result* outs);
+
+/* Host-array form of trvg_compute_bispec_multi_dev (rand/los NULL for the
+ * periodic box). */
+int trvg_compute_bispec_multi(const trvg_catalogue* data,
+                              const trvg_catalogue* rand,
+                              const double* los_data, const double* los_rand,
+                              const trvg_params* params, int nmultipoles,
+                              const trvg_binning* kbinning,
+                              const double* norm_factors,
+                              trvg_bispec_result* outs);
+
 /* Frees the cached device work buffers (they are otherwise kept for reuse
  * by later calls). */
 int trvg_release_workspace(void);

@@ -265,3 +265,37 @@ def test_errors_map_to_reference_exceptions(gpu_backend):
     view = _abi.CatalogueView(x, y, z, np.zeros_like(nz), ws, wc)
     with pytest.raises(T.InvalidDataError):
         gpu_backend.norm_particles(view, 1.)
+
+
+@pytest.mark.parametrize('geo', ['lpp', 'gpp'])
+def test_multipole_call_equals_single_calls(gpu_backend, oracle_backend, geo):
+    """compute_bispec_multipoles (shared fields and shells) gives the same
+    numbers as separate single-multipole calls and as the oracle."""
+    from triumvirate_amd import threept as T
+    from triumvirate_amd.dataobjs import Binning
+    print(f"multi {geo}", flush=True)
+    ps = cf.make_test_paramset(assignment='pcs',
+                               ngrid={'x': 32, 'y': 32, 'z': 32},
+                               boxsize={'x': 800., 'y': 800., 'z': 800.})
+    b = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.11, num_bins=5)
+    mps = [((0, 0, 0), 'full'), ((2, 0, 2), 'full'), ((0, 0, 0), 'row', 2)]
+
+    def run(be, deg=None, form=None, idx=None, multipoles=None):
+        if geo == 'gpp':
+            d = P.uniform_box_catalogue(3000, 800., seed=5)
+            return T._compute_3pt_stats_sim_like(
+                be, 'bispec', d, paramset=ps, degrees=deg or (0, 0, 0),
+                binning=b, form=form or 'full', idx_bin=idx,
+                multipoles=multipoles)
+        d, r = P.survey_pair(2000, 8000, 700., seed=9, weights=True)
+        return T._compute_3pt_stats_survey_like(
+            be, 'bispec', d, r, paramset=ps, degrees=deg or (0, 0, 0),
+            binning=b, form=form or 'full', idx_bin=idx, multipoles=multipoles)
+
+    multi = run(gpu_backend, multipoles=mps)
+    for mp, got in zip(mps, multi):
+        single = run(gpu_backend, mp[0], mp[1], mp[2] if len(mp) > 2 else None)
+        single.pop('_norm_factors', None)
+        P.compare_results(got, single, 'bispec', rtol=1e-12)
+        ref = run(oracle_backend, mp[0], mp[1], mp[2] if len(mp) > 2 else None)
+        P.compare_results(got, ref, 'bispec', rtol=1e-10)

@@ -16,4 +16,5 @@ from .threept import (  # noqa: F401
     compute_3pcf_in_gpp_box,
     compute_bispec,
     compute_bispec_in_gpp_box,
+    compute_bispec_multipoles,
 )

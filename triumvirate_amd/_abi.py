@@ -209,6 +209,13 @@ def declare_extensions(lib):
     lib.trvg_mesh_assign.restype = C.c_int
     lib.trvg_fft3d.argtypes = [P(Params), _dp, C.c_int]
     lib.trvg_fft3d.restype = C.c_int
+    lib.trvg_compute_bispec_multi_dev.argtypes = [
+        vp, vp, P(Params), C.c_int, P(Binning), _dp, P(BispecResult)]
+    lib.trvg_compute_bispec_multi_dev.restype = C.c_int
+    lib.trvg_compute_bispec_multi.argtypes = [
+        P(Catalogue), P(Catalogue), _dp, _dp, P(Params), C.c_int, P(Binning),
+        _dp, P(BispecResult)]
+    lib.trvg_compute_bispec_multi.restype = C.c_int
     lib.trvg_profile_enable.argtypes = [C.c_int]
     lib.trvg_profile_enable.restype = C.c_int
     lib.trvg_profile_read.argtypes = [C.c_int, _dp, P(C.c_longlong), _dp]

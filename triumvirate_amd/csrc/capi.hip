@@ -342,6 +342,93 @@ const char* trvg_profile_class_name(int kernel_class) {
 
 int trvg_profile_class_count(void) { return KC_COUNT; }
 
+}  // extern "C"
+
+namespace {
+
+std::vector<Multipole> check_multipoles(const trvg_params* params, int n,
+                                        const trvg_binning* kb,
+                                        const double* norms, trvg_params* base) {
+  if (params == nullptr || n <= 0 || norms == nullptr)
+    throw Error(TRVG_ERR_PARAM, "At least one multipole is required.");
+  std::vector<Multipole> mps;
+  for (int i = 0; i < n; i++) {
+    trvg_params q = three_point_params(&params[i], kb->num_bins);
+    if (i == 0) *base = q;
+    for (int a = 0; a < 3; a++)
+      if (q.boxsize[a] != base->boxsize[a] || q.ngrid[a] != base->ngrid[a])
+        throw Error(TRVG_ERR_PARAM, "Multipoles must share box and mesh.");
+    if (q.assignment_order != base->assignment_order ||
+        q.precision != base->precision)
+      throw Error(TRVG_ERR_PARAM,
+                  "Multipoles must share assignment scheme and precision.");
+    mps.push_back(Multipole{q.ell1, q.ell2, q.ELL, q.shape, q.idx_bin, norms[i]});
+  }
+  return mps;
+}
+
+void run_multi(Context& ctx, const DeviceCatalogue& d, const DeviceCatalogue* r,
+               const trvg_binning& kb, const std::vector<Multipole>& mps,
+               trvg_bispec_result* outs) {
+  std::vector<Result> res =
+      (ctx.p.precision == TRVG_PREC_FP32)
+          ? run_bispec_multi<float>(ctx, d, r, r != nullptr, kb, mps)
+          : run_bispec_multi<double>(ctx, d, r, r != nullptr, kb, mps);
+  for (size_t i = 0; i < res.size(); i++) write_bispec(res[i], &outs[i]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int trvg_compute_bispec_multi_dev(const trvg_device_catalogue* data,
+                                  const trvg_device_catalogue* rand,
+                                  const trvg_params* params, int n,
+                                  const trvg_binning* kb,
+                                  const double* norms,
+                                  trvg_bispec_result* outs) {
+  return guarded([&] {
+    require_device();
+    check_binning(kb);
+    if (data == nullptr) throw Error(TRVG_ERR_DATA, "Null data catalogue.");
+    if (outs == nullptr) throw Error(TRVG_ERR_PARAM, "Null results.");
+    trvg_params base;
+    std::vector<Multipole> mps = check_multipoles(params, n, kb, norms, &base);
+    if (base.device < 0) base.device = data->device;
+    if (rand && (data->cat.los.get() == nullptr || rand->cat.los.get() == nullptr))
+      throw Error(TRVG_ERR_DATA, "Survey catalogues need lines of sight.");
+    Context ctx(base);
+    run_multi(ctx, data->cat, rand ? &rand->cat : nullptr, *kb, mps, outs);
+  });
+}
+
+int trvg_compute_bispec_multi(const trvg_catalogue* data,
+                              const trvg_catalogue* rand,
+                              const double* los_data, const double* los_rand,
+                              const trvg_params* params, int n,
+                              const trvg_binning* kb, const double* norms,
+                              trvg_bispec_result* outs) {
+  return guarded([&] {
+    require_device();
+    check_binning(kb);
+    if (outs == nullptr) throw Error(TRVG_ERR_PARAM, "Null results.");
+    trvg_params base;
+    std::vector<Multipole> mps = check_multipoles(params, n, kb, norms, &base);
+    Context ctx(base);
+    DeviceCatalogue d, r;
+    if (rand) {
+      require_los(los_data, "data");
+      require_los(los_rand, "random");
+      upload_catalogue(data, los_data, d, ctx.stream);
+      upload_catalogue(rand, los_rand, r, ctx.stream);
+      run_multi(ctx, d, &r, *kb, mps, outs);
+    } else {
+      upload_catalogue(data, nullptr, d, ctx.stream);
+      run_multi(ctx, d, nullptr, *kb, mps, outs);
+    }
+  });
+}
+
 int trvg_release_workspace(void) {
   return guarded([&] { pool().release_all(); });
 }

@@ -239,6 +239,20 @@ struct Result {
   std::vector<cplx> raw, shot;
 };
 
+// One multipole of a multi-multipole call.
+struct Multipole {
+  int ell1, ell2, ELL, shape, idx_bin;
+  double norm;
+};
+
+// Several bispectrum multipoles on one catalogue pair, sharing the common
+// fields, shell statistics and shell slabs.
+template <typename T>
+std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
+                                     const DeviceCatalogue* r, bool survey,
+                                     const trvg_binning& kb,
+                                     const std::vector<Multipole>& mps);
+
 template <typename T>
 Result run_bispec(Context& ctx, const DeviceCatalogue& d,
                   const DeviceCatalogue* r, bool survey,

@@ -366,8 +366,10 @@ static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
   for (const int2& pr : pairs) active[(size_t)(pr.x / 4) * cbn + pr.y / 4] = 1;
   // Sums for every pair of every active tile, indexed [ia * nbf + ib].
   std::vector<cplx> full((size_t)na * nbf, cplx(0., 0.));
-  const int CB = std::min(cbn, 8);                         // column blocks per launch
-  const int RB = std::max(1, std::min((64 - 4 * CB) / 4, PT_MAXT / CB));
+  // Column blocks per launch (at most 32 fields a side: the v3 kernel
+  // stages PT3_SLOTS * 4 fields per side) and row blocks to match.
+  const int CB = std::min(cbn, 8);
+  const int RB = std::max(1, std::min(std::min((64 - 4 * CB) / 4, PT_MAXT / CB), 8));
   const long long nchunks = (nm + PT_CELLS - 1) / PT_CELLS;
   const unsigned grid = unsigned(std::min<long long>(nchunks, (long long)ctx.sm_count * 4));
   for (int c0b = 0; c0b < cbn; c0b += CB) {
@@ -392,7 +394,7 @@ static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
       {
         ProfScope ps(ctx, KC_PAIR_REDUCE,
                      double(nas + nbs + 1) * nm * sizeof(c2<T>));
-        hipLaunchKernelGGL((k_pair_tiles2<T>), dim3(grid), dim3(256), smem,
+        hipLaunchKernelGGL((k_pair_tiles3<T>), dim3(grid), dim3(256), smem,
                            ctx.stream, FA + (long long)ra0 * nm, nas,
                            FB + (long long)ca0 * nm, nbs, G, nm, dt.get(), nt,
                            part.get());
@@ -681,14 +683,22 @@ struct ShellSlab {
 template <typename T>
 class SlabCache {
  public:
+  explicit SlabCache(size_t capacity = 2) : cap_(capacity < 2 ? 2 : capacity) {}
+
   // Returns a slab of (l, m) covering `bins`, building it with
-  // make(field_ptr, bin) if needed.  `keep` is a slab that must survive.
+  // make(field_ptr, bin) if needed; evicts the least recently used slab
+  // other than `keep` when full.
   template <class Make>
   ShellSlab<T>* get(Context& ctx, int l, int m, const std::vector<int>& bins,
                     int nb, const ShellSlab<T>* keep, Make&& make) {
-    for (auto& e : entries_)
-      if (e->covers(l, m, bins)) return e.get();
-    if (entries_.size() >= 2) {
+    for (size_t i = 0; i < entries_.size(); i++)
+      if (entries_[i]->covers(l, m, bins)) {
+        auto e = std::move(entries_[i]);      // move to the back (MRU)
+        entries_.erase(entries_.begin() + i);
+        entries_.push_back(std::move(e));
+        return entries_.back().get();
+      }
+    if (entries_.size() >= cap_) {
       for (size_t i = 0; i < entries_.size(); i++)
         if (entries_[i].get() != keep) {
           entries_.erase(entries_.begin() + i);
@@ -711,6 +721,7 @@ class SlabCache {
   }
 
  private:
+  size_t cap_;
   std::vector<std::unique_ptr<ShellSlab<T>>> entries_;
 };
 
@@ -719,134 +730,155 @@ class SlabCache {
 // ---------------------------------------------------------------------------
 
 template <typename T>
-Result run_bispec(Context& ctx, const DeviceCatalogue& d,
-                  const DeviceCatalogue* r, bool survey,
-                  const trvg_binning& kb, double norm_factor) {
-  const trvg_params& p = ctx.p;
+std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
+                                     const DeviceCatalogue* r, bool survey,
+                                     const trvg_binning& kb,
+                                     const std::vector<Multipole>& mps) {
   const Geom& g = ctx.g;
   const long long nm = g.nmesh;
   const double alpha = survey ? d.wstotal / r->wstotal : 1.;
-  const cplx factor_phase = ipow(p.ell1 + p.ell2);
   const int nb = kb.num_bins;
-  std::vector<int> ra, cb;
-  pair_bins(p.shape, nb, p.idx_bin, ra, cb);
-  const int dim = int(ra.size());
 
-  Result out;
-  out.dim = dim;
-  out.b1.resize(dim); out.b2.resize(dim); out.e1.resize(dim); out.e2.resize(dim);
-  out.n1.resize(dim); out.n2.resize(dim);
-  out.raw.assign(dim, 0.); out.shot.assign(dim, 0.);
-
-  // Common fields.
   // Common fields in FP64 whatever the shell precision T: the shot-noise
   // path (two-point statistics, S_ij_k) subtracts nearly equal quantities.
   DevArray<double2> dn00(nm), N00(nm);
   build_field_k<double>(ctx, dn00.get(), d, r, survey, alpha, 0, 0, false, true);
   build_field_k<double>(ctx, N00.get(), d, r, survey, alpha, 0, 0, true, false);
-
   ShellStats ss = shell_stats(ctx, kb);
-  for (int dv = 0; dv < dim; dv++) {
-    out.b1[dv] = kb.bin_centres[ra[dv]];
-    out.b2[dv] = kb.bin_centres[cb[dv]];
-    out.e1[dv] = ss.keff[ra[dv]];
-    out.e2[dv] = ss.keff[cb[dv]];
-    out.n1[dv] = ss.nmodes[ra[dv]];
-    out.n2[dv] = ss.nmodes[cb[dv]];
-  }
 
-  Spline sja, sjb;
-  build_spline(p.ell1, sja, ctx.stream);
-  build_spline(p.ell2, sjb, ctx.stream);
+  std::map<int, std::unique_ptr<Spline>> splines;
+  auto spline = [&](int ell) -> const Spline& {
+    auto it = splines.find(ell);
+    if (it != splines.end()) return *it->second;
+    auto sp = std::make_unique<Spline>();
+    build_spline(ell, *sp, ctx.stream);
+    return *(splines[ell] = std::move(sp));
+  };
 
-  std::vector<int> rows = distinct(ra), cols = distinct(cb);
   DevArray<c2<T>> G(nm);
   DevArray<double2> tfield(nm), dnLM, NLM;
-  SlabCache<T> slabs;
-
-  for (const Term& tm : enumerate_terms(p, survey)) {
-    const bool lm_zero = !survey || (p.ELL == 0 && tm.M == 0);
-    const double2* dnLMp = dn00.get();
-    const double2* NLMp = N00.get();
-    if (!lm_zero) {
-      if (!dnLM.get()) { dnLM.reset(nm); NLM.reset(nm); }
-      build_field_k<double>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
-      build_field_k<double>(ctx, NLM.get(), d, r, true, alpha, p.ELL, tm.M, true, false);
-      dnLMp = dnLM.get();
-      NLMp = NLM.get();
-    }
-
-    // G_LM(x): compensated, inverse-transformed (threept.cpp:452-459).
-    LoadDeconv<T, double> ldg{dnLMp, ctx.tab, 1. / g.vol};
-    fft3d<T>(ctx, G.get(), ldg, +1, double(nm) * sizeof(double2));
-
-    // Shell fields F_a (l1, m1) for the row bins, F_b (l2, m2) for columns.
-    auto make_shell = [&](int l, int mm) {
-      return [&, l, mm](c2<T>* out_f, int b) {
-        LoadShell<T, double> ls{dn00.get(), ctx.tab, g, kb.bin_edges[b],
-                                kb.bin_edges[b + 1], l, mm};
-        Prune pr = make_prune(g, kb.bin_edges[b + 1]);
-        fft3d<T>(ctx, out_f, ls, +1, double(ss.nmodes[b]) * sizeof(double2), &pr);
-      };
+  SlabCache<T> slabs(mps.size() > 1 ? 3 : 2);
+  auto make_shell = [&](int l, int mm) {
+    return [&, l, mm](c2<T>* out_f, int b) {
+      LoadShell<T, double> ls{dn00.get(), ctx.tab, g, kb.bin_edges[b],
+                              kb.bin_edges[b + 1], l, mm};
+      Prune pr = make_prune(g, kb.bin_edges[b + 1]);
+      fft3d<T>(ctx, out_f, ls, +1, double(ss.nmodes[b]) * sizeof(double2), &pr);
     };
-    ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr,
-                                make_shell(p.ell1, tm.m1));
-    ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A,
-                                make_shell(p.ell2, tm.m2));
-    std::vector<int2> pairs(dim);
-    for (int dv = 0; dv < dim; dv++)
-      pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
-    std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields,
-                                         B->data.get(), B->nfields, G.get(), pairs);
-    for (int dv = 0; dv < dim; dv++) {
-      cplx s = S[dv] / (double(ss.nmodes[ra[dv]]) * double(ss.nmodes[cb[dv]]));
-      out.raw[dv] += tm.coupling * g.vol_cell * (s + tm.f_mirror * std::conj(s));
-    }
+  };
 
-    // Shot noise (threept.cpp:771-966).
-    cplx Sbar;
-    if (survey) {
-      Sbar = sbar_catalogue(ctx, d, p.ELL, tm.M)
-             + std::pow(alpha, 3) * sbar_catalogue(ctx, *r, p.ELL, tm.M);
-    } else {
-      Sbar = double(d.n);
-    }
-    if (p.ell1 == 0 && p.ell2 == 0) {
-      cplx sijk = tm.coupling * Sbar;
-      for (int dv = 0; dv < dim; dv++) out.shot[dv] += sijk;
-    }
-    if (p.ell2 == 0) {
-      TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb);
-      for (int dv = 0; dv < dim; dv++) {
-        int b = ra[dv];
-        cplx s = tm.coupling * (st.pk[b] - st.sn[b]);
-        out.shot[dv] += s + tm.f_mirror * std::conj(s);
-      }
-    }
-    if (p.ell1 == 0) {
-      TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell2, tm.m2, kb);
-      for (int dv = 0; dv < dim; dv++) {
-        int b = cb[dv];
-        cplx s = tm.coupling * (st.pk[b] - st.sn[b]);
-        out.shot[dv] += s + tm.f_mirror * std::conj(s);
-      }
-    }
-    LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab, dmk(Sbar.real(), Sbar.imag()),
-                         1. / g.vol};
-    fft3d<double>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
-    std::vector<cplx> sij = shot_pairs_bispec<double>(ctx, tfield.get(), p.ell1, tm.m1,
-                                                 p.ell2, tm.m2, sja, sjb,
-                                                 out.e1, out.e2);
+  std::vector<Result> results;
+  for (const Multipole& mp : mps) {
+    trvg_params p = ctx.p;
+    p.ell1 = mp.ell1; p.ell2 = mp.ell2; p.ELL = mp.ELL;
+    p.shape = mp.shape; p.idx_bin = mp.idx_bin;
+    const cplx factor_phase = ipow(p.ell1 + p.ell2);
+    std::vector<int> ra, cb;
+    pair_bins(p.shape, nb, p.idx_bin, ra, cb);
+    const int dim = int(ra.size());
+    std::vector<int> rows = distinct(ra), cols = distinct(cb);
+
+    Result out;
+    out.dim = dim;
+    out.b1.resize(dim); out.b2.resize(dim); out.e1.resize(dim); out.e2.resize(dim);
+    out.n1.resize(dim); out.n2.resize(dim);
+    out.raw.assign(dim, 0.); out.shot.assign(dim, 0.);
     for (int dv = 0; dv < dim; dv++) {
-      cplx s = tm.coupling * sij[dv];
-      out.shot[dv] += factor_phase * (s + tm.f_mirror_w3j * std::conj(s));
+      out.b1[dv] = kb.bin_centres[ra[dv]];
+      out.b2[dv] = kb.bin_centres[cb[dv]];
+      out.e1[dv] = ss.keff[ra[dv]];
+      out.e2[dv] = ss.keff[cb[dv]];
+      out.n1[dv] = ss.nmodes[ra[dv]];
+      out.n2[dv] = ss.nmodes[cb[dv]];
     }
+    const Spline& sja = spline(p.ell1);
+    const Spline& sjb = spline(p.ell2);
+
+    for (const Term& tm : enumerate_terms(p, survey)) {
+      const bool lm_zero = !survey || (p.ELL == 0 && tm.M == 0);
+      const double2* dnLMp = dn00.get();
+      const double2* NLMp = N00.get();
+      if (!lm_zero) {
+        if (!dnLM.get()) { dnLM.reset(nm); NLM.reset(nm); }
+        build_field_k<double>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
+        build_field_k<double>(ctx, NLM.get(), d, r, true, alpha, p.ELL, tm.M, true, false);
+        dnLMp = dnLM.get();
+        NLMp = NLM.get();
+      }
+
+      // G_LM(x): compensated, inverse-transformed (threept.cpp:452-459).
+      LoadDeconv<T, double> ldg{dnLMp, ctx.tab, 1. / g.vol};
+      fft3d<T>(ctx, G.get(), ldg, +1, double(nm) * sizeof(double2));
+
+      // Shell fields F_a (l1, m1) for the row bins, F_b (l2, m2) for columns.
+      ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr,
+                                  make_shell(p.ell1, tm.m1));
+      ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A,
+                                  make_shell(p.ell2, tm.m2));
+      std::vector<int2> pairs(dim);
+      for (int dv = 0; dv < dim; dv++)
+        pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
+      std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields,
+                                           B->data.get(), B->nfields, G.get(), pairs);
+      for (int dv = 0; dv < dim; dv++) {
+        cplx sv = S[dv] / (double(ss.nmodes[ra[dv]]) * double(ss.nmodes[cb[dv]]));
+        out.raw[dv] += tm.coupling * g.vol_cell * (sv + tm.f_mirror * std::conj(sv));
+      }
+
+      // Shot noise (threept.cpp:771-966).
+      cplx Sbar;
+      if (survey) {
+        Sbar = sbar_catalogue(ctx, d, p.ELL, tm.M)
+               + std::pow(alpha, 3) * sbar_catalogue(ctx, *r, p.ELL, tm.M);
+      } else {
+        Sbar = double(d.n);
+      }
+      if (p.ell1 == 0 && p.ell2 == 0) {
+        cplx sijk = tm.coupling * Sbar;
+        for (int dv = 0; dv < dim; dv++) out.shot[dv] += sijk;
+      }
+      if (p.ell2 == 0) {
+        TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb);
+        for (int dv = 0; dv < dim; dv++) {
+          int bb = ra[dv];
+          cplx sv = tm.coupling * (st.pk[bb] - st.sn[bb]);
+          out.shot[dv] += sv + tm.f_mirror * std::conj(sv);
+        }
+      }
+      if (p.ell1 == 0) {
+        TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell2, tm.m2, kb);
+        for (int dv = 0; dv < dim; dv++) {
+          int bb = cb[dv];
+          cplx sv = tm.coupling * (st.pk[bb] - st.sn[bb]);
+          out.shot[dv] += sv + tm.f_mirror * std::conj(sv);
+        }
+      }
+      LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab,
+                           dmk(Sbar.real(), Sbar.imag()), 1. / g.vol};
+      fft3d<double>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
+      std::vector<cplx> sij = shot_pairs_bispec<double>(
+          ctx, tfield.get(), p.ell1, tm.m1, p.ell2, tm.m2, sja, sjb, out.e1, out.e2);
+      for (int dv = 0; dv < dim; dv++) {
+        cplx sv = tm.coupling * sij[dv];
+        out.shot[dv] += factor_phase * (sv + tm.f_mirror_w3j * std::conj(sv));
+      }
+    }
+    for (int dv = 0; dv < dim; dv++) {
+      out.raw[dv] *= mp.norm;
+      out.shot[dv] *= mp.norm;
+    }
+    results.push_back(std::move(out));
   }
-  for (int dv = 0; dv < dim; dv++) {
-    out.raw[dv] *= norm_factor;
-    out.shot[dv] *= norm_factor;
-  }
-  return out;
+  return results;
+}
+
+template <typename T>
+Result run_bispec(Context& ctx, const DeviceCatalogue& d,
+                  const DeviceCatalogue* r, bool survey,
+                  const trvg_binning& kb, double norm_factor) {
+  Multipole mp{ctx.p.ell1, ctx.p.ell2, ctx.p.ELL, ctx.p.shape, ctx.p.idx_bin,
+               norm_factor};
+  return std::move(run_bispec_multi<T>(ctx, d, r, survey, kb, {mp})[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -984,6 +1016,12 @@ double run_mesh_norm(Context& ctx, const DeviceCatalogue& c) {
   return 1. / (h * ctx.g.vol_cell);
 }
 
+template std::vector<Result> run_bispec_multi<double>(
+    Context&, const DeviceCatalogue&, const DeviceCatalogue*, bool,
+    const trvg_binning&, const std::vector<Multipole>&);
+template std::vector<Result> run_bispec_multi<float>(
+    Context&, const DeviceCatalogue&, const DeviceCatalogue*, bool,
+    const trvg_binning&, const std::vector<Multipole>&);
 template Result run_bispec<double>(Context&, const DeviceCatalogue&,
                                    const DeviceCatalogue*, bool,
                                    const trvg_binning&, double);

@@ -1135,3 +1135,109 @@ k_pair_tiles2(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB
 }
 
 }  // namespace trvg
+
+namespace trvg {
+
+// Pair reduction, v3: as v2, with the next chunk's global loads issued into
+// registers before the current chunk is reduced (software pipelining).
+// Staging map: thread t owns cell c = t % PT_CELLS of the chunk and fields
+// f = t / PT_CELLS + k * (256 / PT_CELLS), so each thread loads G[c] once.
+constexpr int PT3_SLOTS = 8;   // per-thread staged values per side (<= 32 fields)
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pair_tiles3(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
+              int nbf, const c2<T>* __restrict__ G, long long nmesh,
+              const int2* __restrict__ tiles, int ntiles,
+              double2* __restrict__ part) {
+  extern __shared__ char sm3_raw[];
+  c2<T>* sa = reinterpret_cast<c2<T>*>(sm3_raw);           // [na][PT_CELLS]
+  c2<T>* sh = sa + (size_t)na * PT_CELLS;                   // [nbf][PT_CELLS]
+  constexpr int FSTEP = 256 / PT_CELLS;                     // 4
+  const int ngroups = 256 / ntiles;
+  const int group = threadIdx.x % ngroups;
+  const int tile = threadIdx.x / ngroups;
+  const bool active = tile < ntiles;
+  const int2 t0 = tiles[active ? tile : 0];
+  double ar[16], ai[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) ar[q] = ai[q] = 0.;
+  int ia[4], ib[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    ia[q] = min(t0.x + q, na - 1) * PT_CELLS;
+    ib[q] = min(t0.y + q, nbf - 1) * PT_CELLS;
+  }
+  const int sc = threadIdx.x % PT_CELLS;   // staged cell
+  const int sf = threadIdx.x / PT_CELLS;   // first staged field
+  c2<T> ra[PT3_SLOTS], rb[PT3_SLOTS], rg;
+  const long long nchunks = (nmesh + PT_CELLS - 1) / PT_CELLS;
+  auto fetch = [&](long long ch) {
+    const long long gi = ch * PT_CELLS + sc;
+    const bool ok = (ch < nchunks) && (gi < nmesh);
+#pragma unroll
+    for (int k = 0; k < PT3_SLOTS; k++) {
+      const int f = sf + k * FSTEP;
+      ra[k] = (ok && f < na) ? FA[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
+      rb[k] = (ok && f < nbf) ? FB[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
+    }
+    rg = ok ? G[gi] : mk<T>(T(0), T(0));
+  };
+  long long ch = blockIdx.x;
+  fetch(ch);
+  for (; ch < nchunks; ch += gridDim.x) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PT3_SLOTS; k++) {
+      const int f = sf + k * FSTEP;
+      if (f < na) sa[f * PT_CELLS + sc] = ra[k];
+      if (f < nbf) sh[f * PT_CELLS + sc] = cmul<T>(rb[k], rg);
+    }
+    __syncthreads();
+    fetch(ch + gridDim.x);
+    if (active) {
+      T pr[16], pi[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) pr[q] = pi[q] = T(0);
+      for (int c = group; c < PT_CELLS; c += ngroups) {
+        c2<T> x[4], y[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          x[q] = sa[ia[q] + c];
+          y[q] = sh[ib[q] + c];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            pr[i * 4 + j] += x[i].x * y[j].x - x[i].y * y[j].y;
+            pi[i * 4 + j] += x[i].x * y[j].y + x[i].y * y[j].x;
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        ar[q] += double(pr[q]);
+        ai[q] += double(pi[q]);
+      }
+    }
+  }
+  __syncthreads();
+  double2* red = reinterpret_cast<double2*>(sm3_raw);   // [256]
+  for (int q = 0; q < 16; q++) {
+    red[threadIdx.x] = active ? dmk(ar[q], ai[q]) : dmk(0., 0.);
+    __syncthreads();
+    if (threadIdx.x < ntiles) {
+      double sx = 0., sy = 0.;
+      for (int gr = 0; gr < ngroups; gr++) {
+        double2 v = red[threadIdx.x * ngroups + gr];
+        sx += v.x;
+        sy += v.y;
+      }
+      part[(long long)blockIdx.x * ntiles * 16 + threadIdx.x * 16 + q] = dmk(sx, sy);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace trvg

@@ -90,6 +90,22 @@ class CLibBackend:
                        C.byref(res.struct)))
         return res.as_dict()
 
+    def multi(self, data, rand, los_d, los_r, params_list, binning, norms):
+        """Several bispectrum multipoles in one engine call (engine
+        extension trvg_compute_bispec_multi)."""
+        n = len(params_list)
+        arr = (_abi.Params * n)(*params_list)
+        res = [self._result(p, binning, 'bispec') for p in params_list]
+        outs = (_abi.BispecResult * n)(*[r.struct for r in res])
+        nrm = np.ascontiguousarray(norms, dtype=np.float64)
+        self._check(self._fn('compute_bispec_multi')(
+            C.byref(data.struct), C.byref(rand.struct) if rand else None,
+            _abi.dptr(los_d), _abi.dptr(los_r), arr, n,
+            C.byref(binning.struct), _abi.dptr(nrm), outs))
+        for r, o in zip(res, outs):
+            r.struct.dim = o.dim
+        return [r.as_dict() for r in res]
+
     def sim(self, kind, data, params, binning, norm):
         res = self._result(params, binning, kind)
         fn = self._fn('compute_bispec_in_gpp_box' if kind == 'bispec'
@@ -231,12 +247,29 @@ def _select_norm(paramset, norm_part, norm_mesh):
     return 0.   # 'mesh-mixed' is n/a for three-point statistics
 
 
+def _multipole_structs(paramset, multipoles):
+    """ABI parameter structs for a list of (degrees, form[, idx_bin])."""
+    out = []
+    for mp in multipoles:
+        deg, form = mp[0], mp[1]
+        idx = mp[2] if len(mp) > 2 else None
+        ell1, ell2, ELL = _parse_degrees(deg)
+        pdict = {k: (dict(v) if isinstance(v, dict) else v)
+                 for k, v in paramset.items()}
+        pdict['degrees'] = {'ell1': ell1, 'ell2': ell2, 'ELL': ELL}
+        pdict['form'] = form
+        pdict['idx_bin'] = idx
+        out.append(_params_struct(ParameterSet(param_dict=pdict)))
+    return out
+
+
 def _compute_3pt_stats_survey_like(backend, kind, catalogue_data,
                                    catalogue_rand, los_data=None,
                                    los_rand=None, paramset=None,
                                    params_sampling=None, degrees=None,
                                    binning=None, form=None, idx_bin=None,
-                                   precision=None, logger=None):
+                                   precision=None, logger=None,
+                                   multipoles=None):
     stype = 'bispec' if kind == 'bispec' else '3pcf'
     paramset = _amalgamate_parameters(
         paramset=paramset, params_sampling=params_sampling, degrees=degrees,
@@ -279,6 +312,11 @@ def _compute_3pt_stats_survey_like(backend, kind, catalogue_data,
     norm_mesh = backend.norm_mesh(rand_v, params, alpha)
     norm_factor = _select_norm(paramset, norm_part, norm_mesh)
 
+    if multipoles is not None:
+        plist = _multipole_structs(paramset, multipoles)
+        return backend.multi(data_v, rand_v, los_data, los_rand, plist,
+                             _binning_view(binning),
+                             [norm_factor] * len(plist))
     results = backend.survey(kind, data_v, rand_v, los_data, los_rand, params,
                              _binning_view(binning), norm_factor)
     results['_norm_factors'] = (norm_part, norm_mesh)
@@ -288,7 +326,8 @@ def _compute_3pt_stats_survey_like(backend, kind, catalogue_data,
 def _compute_3pt_stats_sim_like(backend, kind, catalogue_data, paramset=None,
                                 params_sampling=None, degrees=None,
                                 binning=None, form=None, idx_bin=None,
-                                precision=None, logger=None):
+                                precision=None, logger=None,
+                                multipoles=None):
     stype = 'bispec' if kind == 'bispec' else '3pcf'
     paramset = _amalgamate_parameters(
         paramset=paramset, params_sampling=params_sampling, degrees=degrees,
@@ -314,6 +353,11 @@ def _compute_3pt_stats_sim_like(backend, kind, catalogue_data, paramset=None,
     norm_mesh = backend.norm_mesh(data_v, params, 1.)
     norm_factor = _select_norm(paramset, norm_part, norm_mesh)
 
+    if multipoles is not None:
+        plist = _multipole_structs(paramset, multipoles)
+        return backend.multi(data_v, None, None, None, plist,
+                             _binning_view(binning),
+                             [norm_factor] * len(plist))
     results = backend.sim(kind, data_v, params, _binning_view(binning),
                           norm_factor)
     results['_norm_factors'] = (norm_part, norm_mesh)
@@ -389,3 +433,30 @@ def compute_3pcf_in_gpp_box(catalogue_data, degrees=None, binning=None,
         params_sampling=sampling_params, degrees=degrees, binning=binning,
         form=form, idx_bin=idx_bin, precision=precision, logger=logger)
     return _finish(res, save)
+
+
+def compute_bispec_multipoles(catalogue_data, catalogue_rand=None,
+                              multipoles=(((0, 0, 0), 'full'),
+                                          ((2, 0, 2), 'full')),
+                              binning=None, los_data=None, los_rand=None,
+                              sampling_params=None, paramset=None,
+                              boxsize=None, precision=None, logger=None):
+    """Several bispectrum multipoles in one engine call (no reference
+    counterpart; each entry equals the corresponding compute_bispec /
+    compute_bispec_in_gpp_box result).
+
+    `multipoles` lists (degrees, form) or (degrees, form, idx_bin).  With
+    `catalogue_rand` the survey-like (local plane-parallel) estimator is used,
+    otherwise the periodic-box one.  Returns a list of result dicts.
+    """
+    first = multipoles[0]
+    kw = dict(paramset=paramset, params_sampling=sampling_params,
+              degrees=first[0], binning=binning, form=first[1],
+              idx_bin=first[2] if len(first) > 2 else None,
+              precision=precision, logger=logger, multipoles=multipoles)
+    if catalogue_rand is not None:
+        return _compute_3pt_stats_survey_like(
+            _engine_backend(), 'bispec', catalogue_data, catalogue_rand,
+            los_data=los_data, los_rand=los_rand, **kw)
+    return _compute_3pt_stats_sim_like(_engine_backend(), 'bispec',
+                                       catalogue_data, **kw)
