@@ -49,6 +49,21 @@ def make_inputs(nd, nr, ngen):
     return data, rand, los_d, los_r
 
 
+def traffic_for(kernel_class):
+    """HBM bytes per launch of a kernel class from the committed counter
+    summary (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
+    kernel), or None when no summary covers it."""
+    path = os.path.join(ROOT, 'profiles', 'hbm_traffic_bispec_b000_b202_256.md')
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        parts = ln.split()
+        if len(parts) == 3 and parts[0] == 'traffic_bytes_per_launch' \
+                and parts[1] == kernel_class:
+            return float(parts[2])
+    return None
+
+
 def params_struct(ngrid, degrees, shape, prec):
     from triumvirate_amd import _abi
     p = _abi.Params()
@@ -74,6 +89,8 @@ def main():
     ap.add_argument('--ngrid', type=int, default=256)
     ap.add_argument('--ndata', type=int, default=1_000_000)
     ap.add_argument('--nrand', type=int, default=5_000_000)
+    ap.add_argument('--skip-fp64', action='store_true',
+                    help='skip the FP64 steps (profiler runs)')
     ap.add_argument('--cpu-pairs', type=int, default=6,
                     help='bin pairs in the CPU-baseline sample (0: skip)')
     args = ap.parse_args()
@@ -142,48 +159,72 @@ def main():
 
     pairs_per_step = 210 + 400
 
-    # FP64 reference run (drop-in precision) and its timing.
-    log("FP64 step ...")
-    t = time.perf_counter()
-    r64 = step(64)
-    t64 = time.perf_counter() - t
-    log(f"FP64 step {t64 * 1e3:.1f} ms")
+    # FP64 (the drop-in precision): one cold call, then one timed warm call.
+    r64, t64 = None, None
+    if not args.skip_fp64:
+        log("FP64 steps ...")
+        step(64)
+        t = time.perf_counter()
+        r64 = step(64)
+        t64 = time.perf_counter() - t
+        log(f"FP64 step {t64 * 1e3:.1f} ms")
 
     for _ in range(args.warmup):
         step(32)
-    log(f"timing {args.steps} FP32 steps ...")
+
+    # Breakdown step (outside the timed region): every kernel class timed.
     ncls = lib.trvg_profile_class_count()
+    names = [lib.trvg_profile_class_name(kc).decode() for kc in range(ncls)]
+
+    def read_profile(nsteps):
+        out = {}
+        for kc in range(ncls):
+            ms, n, by = C.c_double(), C.c_longlong(), C.c_double()
+            check(lib.trvg_profile_read(kc, C.byref(ms), C.byref(n), C.byref(by)))
+            if n.value:
+                out[names[kc]] = {
+                    'class': kc,
+                    'ms_per_step': ms.value / nsteps,
+                    'launches_per_step': n.value / nsteps,
+                    'avg_us': 1e3 * ms.value / n.value,
+                    'bytes_per_launch': by.value / n.value,
+                    'achieved_GBs': (by.value / n.value) /
+                    (ms.value / n.value * 1e-3) / 1e9 if by.value > 0 else None,
+                }
+        return out
+
+    check(lib.trvg_profile_only(-1))
+    check(lib.trvg_profile_enable(1))
+    t = time.perf_counter()
+    step(32)
+    ms_breakdown = (time.perf_counter() - t) * 1e3
+    kernels = read_profile(1)
+    check(lib.trvg_profile_enable(0))
+    # The roofline kernel: the HBM-bound class with the most time per step.
+    dom = max((k for k in kernels if kernels[k]['achieved_GBs']),
+              key=lambda k: kernels[k]['ms_per_step'])
+
+    # Timed region: K steps, HIP events only around the roofline kernel.
+    log(f"timing {args.steps} FP32 steps (events on {dom}) ...")
+    check(lib.trvg_profile_only(kernels[dom]['class']))
     check(lib.trvg_profile_enable(1))
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r32 = step(32)
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    kernels = {}
-    for kc in range(ncls):
-        ms, n, by = C.c_double(), C.c_longlong(), C.c_double()
-        check(lib.trvg_profile_read(kc, C.byref(ms), C.byref(n), C.byref(by)))
-        if n.value:
-            name = lib.trvg_profile_class_name(kc).decode()
-            kernels[name] = {
-                'ms_per_step': ms.value / args.steps,
-                'launches_per_step': n.value / args.steps,
-                'avg_us': 1e3 * ms.value / n.value,
-                'bytes_per_launch': by.value / n.value,
-                'achieved_GBs': (by.value / n.value) / (ms.value / n.value * 1e-3) / 1e9
-                if by.value > 0 else None,
-            }
+    kd = read_profile(args.steps)[dom]
     check(lib.trvg_profile_enable(0))
-    log(f"FP32 step {ms_per_step:.1f} ms (profiled)")
+    check(lib.trvg_profile_only(-1))
+    log(f"FP32 step {ms_per_step:.2f} ms")
 
-    # Untimed-by-profiler check of the profiling overhead.
-    t = time.perf_counter()
-    step(32)
-    ms_unprof = (time.perf_counter() - t) * 1e3
+    # Plain copy kernel of 1 GiB: the scale for hardware byte counters.
+    cal_ms = C.c_double()
+    check(lib.trvg_copy_calibration(1 << 30, C.byref(cal_ms)))
 
     # FP32 error against FP64, per multipole.
     errs = {}
-    for deg, _ in multipoles:
+    for deg, _ in (multipoles if r64 else []):
         tag = 'B' + ''.join(map(str, deg))
         e = {}
         for key in ('bk_raw', 'bk_shot'):
@@ -196,17 +237,14 @@ def main():
                                                 r64[deg]['nmodes_1']))
         errs[tag] = e
 
-    # Roofline: the dominant HBM-bound kernel class by time.
-    dom = max((k for k in kernels if kernels[k]['achieved_GBs']),
-              key=lambda k: kernels[k]['ms_per_step'])
-    kd = kernels[dom]
     roofline = {
         'bound': 'hbm', 'kernel': dom,
         'achieved': round(kd['achieved_GBs'], 1), 'peak': HBM_PEAK_GBS,
         'unit': 'GB/s', 'frac': round(kd['achieved_GBs'] / HBM_PEAK_GBS, 4),
-        'traffic': None,
+        'traffic': traffic_for(dom),
         'bytes_per_launch': kd['bytes_per_launch'],
         'avg_launch_us': round(kd['avg_us'], 2),
+        'launches_per_step': kd['launches_per_step'],
     }
 
     # CPU baseline: oracle reference-schedule per-pair loop on host cores.
@@ -260,9 +298,11 @@ def main():
         'roofline': roofline,
         'cpu_baseline': cpu,
         'fp64': {'value': round(pairs_per_step / t64, 2),
-                 'ms_per_step': round(t64 * 1e3, 3)},
+                 'ms_per_step': round(t64 * 1e3, 3)} if t64 else None,
         'fp32_rel_err': errs,
-        'ms_per_step_unprofiled': round(ms_unprof, 3),
+        'ms_breakdown_step': round(ms_breakdown, 3),
+        'copy_calibration': {'bytes': 1 << 30, 'ms': round(cal_ms.value, 4),
+                             'GBs': round(2 * (1 << 30) / cal_ms.value / 1e6, 1)},
         'upload_ms': round(t_up * 1e3, 1),
         'pcie_inclusive_value': round(
             pairs_per_step / (ms_per_step / 1e3 + t_up), 2),

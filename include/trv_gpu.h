@@ -264,6 +264,9 @@ int trvg_release_workspace(void);
  * traffic of the launch) are accumulated per kernel class.  Classes are
  * numbered 0 .. trvg_profile_class_count() - 1. */
 int trvg_profile_enable(int on);
+/* Restricts the profiler to one kernel class (-1: all classes), so that a
+ * timed run carries events only around the launches being measured. */
+int trvg_profile_only(int kernel_class);
 int trvg_profile_read(int kernel_class, double* total_ms, long long* launches,
                       double* algorithmic_bytes);
 const char* trvg_profile_class_name(int kernel_class);
@@ -283,6 +286,51 @@ int trvg_mesh_assign(const trvg_catalogue* cat, const double* los,
  * (sign -1 forward, +1 backward), computed on the device in
  * params->precision. */
 int trvg_fft3d(const trvg_params* params, double* mesh, int sign);
+
+/* ------------------------------------------------------------------------
+ * Differentiable periodic-box bispectrum (no reference counterpart: the
+ * reference computes forward values only).  The forward chain is that of
+ * compute_bispec_in_gpp_box (threept.cpp:1473-2188) for B_000 in the diag
+ * form, from a real mesh delta(x) instead of a catalogue; the adjoints are
+ * hand-written.  All pointers except bk / nmodes / grad_bk are DEVICE
+ * pointers; meshes are real, n0 x n1 x n2 row-major, in params->precision
+ * (float for TRVG_PREC_FP32, double otherwise); positions are n x 3 FP64.
+ * Calls are synchronous on the device.
+ * ------------------------------------------------------------------------ */
+
+/* mesh (+)= sum_p w_p W(x - x_p) / V_cell with the assignment scheme of
+ * params (field.cpp:569-1112 index and window rules); weights may be NULL
+ * (unit weights); accumulate = 0 zeroes the mesh first. */
+int trvg_diff_assign(const trvg_params* params, const double* pos,
+                     const double* weights, long long n, void* mesh,
+                     int accumulate);
+
+/* Adjoint of trvg_diff_assign: grad_pos (n x 3) and grad_weights (n), each
+ * optional, from the mesh cotangent grad_mesh. */
+int trvg_diff_assign_backward(const trvg_params* params, const double* pos,
+                              const double* weights, long long n,
+                              const void* grad_mesh, double* grad_pos,
+                              double* grad_weights);
+
+/* B_b = norm V_cell / n_b^2 sum_x F_b(x)^2 G(x) for every k-bin b (host
+ * outputs bk[num_bins], optional nmodes[num_bins]); F_b and G are the
+ * real parts of the shell field and compensated field of delta. */
+int trvg_diff_bispec_diag(const trvg_params* params,
+                          const trvg_binning* kbinning, double norm_factor,
+                          const void* mesh, double* bk, int* nmodes);
+
+/* grad_mesh = sum_b grad_bk[b] dB_b/ddelta (shells recomputed, not
+ * stored). */
+int trvg_diff_bispec_diag_backward(const trvg_params* params,
+                                   const trvg_binning* kbinning,
+                                   double norm_factor, const void* mesh,
+                                   const double* grad_bk, void* grad_mesh);
+
+/* Streams `bytes` (rounded down to 16-byte units) from one device buffer
+ * to another with a plain copy kernel and returns its duration in
+ * milliseconds: the reference point for scaling hardware byte counters
+ * (a copy moves exactly `bytes` in and `bytes` out). */
+int trvg_copy_calibration(long long bytes, double* ms);
 
 #ifdef __cplusplus
 }  /* extern "C" */

@@ -17,6 +17,9 @@ TRVG_ERR_INTERNAL = 5
 
 SHAPE_CODES = {'diag': 0, 'off-diag': 1, 'row': 2, 'full': 3, 'triu': 4}
 
+TRVG_PREC_FP64 = 64
+TRVG_PREC_FP32 = 32
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
@@ -218,6 +221,22 @@ def declare_extensions(lib):
     lib.trvg_compute_bispec_multi.restype = C.c_int
     lib.trvg_profile_enable.argtypes = [C.c_int]
     lib.trvg_profile_enable.restype = C.c_int
+    _vp = C.c_void_p
+    lib.trvg_diff_assign.argtypes = [P(Params), _vp, _vp, C.c_longlong, _vp, C.c_int]
+    lib.trvg_diff_assign.restype = C.c_int
+    lib.trvg_diff_assign_backward.argtypes = [P(Params), _vp, _vp, C.c_longlong,
+                                              _vp, _vp, _vp]
+    lib.trvg_diff_assign_backward.restype = C.c_int
+    lib.trvg_diff_bispec_diag.argtypes = [P(Params), P(Binning), C.c_double,
+                                          _vp, _dp, P(C.c_int)]
+    lib.trvg_diff_bispec_diag.restype = C.c_int
+    lib.trvg_diff_bispec_diag_backward.argtypes = [P(Params), P(Binning),
+                                                   C.c_double, _vp, _dp, _vp]
+    lib.trvg_diff_bispec_diag_backward.restype = C.c_int
+    lib.trvg_copy_calibration.argtypes = [C.c_longlong, _dp]
+    lib.trvg_copy_calibration.restype = C.c_int
+    lib.trvg_profile_only.argtypes = [C.c_int]
+    lib.trvg_profile_only.restype = C.c_int
     lib.trvg_profile_read.argtypes = [C.c_int, _dp, P(C.c_longlong), _dp]
     lib.trvg_profile_read.restype = C.c_int
     lib.trvg_profile_class_name.argtypes = [C.c_int]

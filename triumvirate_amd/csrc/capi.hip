@@ -327,6 +327,14 @@ int trvg_profile_enable(int on) {
   return guarded([&] { Profiler::get().enable(on != 0); });
 }
 
+int trvg_profile_only(int kernel_class) {
+  return guarded([&] {
+    if (kernel_class < -1 || kernel_class >= KC_COUNT)
+      throw Error(TRVG_ERR_PARAM, "Unknown kernel class.");
+    Profiler::get().only(kernel_class);
+  });
+}
+
 int trvg_profile_read(int kernel_class, double* total_ms, long long* launches,
                       double* algorithmic_bytes) {
   return guarded([&] {
@@ -471,6 +479,121 @@ int trvg_mesh_assign(const trvg_catalogue* c, const double* los,
 
 // In-place 3-D FFT of an interleaved complex host mesh (sign -1 forward,
 // +1 backward, unnormalised), run on the device in params->precision.
+int trvg_diff_assign(const trvg_params* params, const double* pos,
+                     const double* weights, long long n, void* mesh,
+                     int accumulate) {
+  return guarded([&] {
+    require_device();
+    trvg_params q = mesh_params(params);
+    if (n < 0 || (n > 0 && pos == nullptr) || mesh == nullptr)
+      throw Error(TRVG_ERR_PARAM, "Invalid particle or mesh pointers.");
+    Context ctx(q);
+    if (q.precision == TRVG_PREC_FP32)
+      diff_assign<float>(ctx, static_cast<float*>(mesh), pos, weights, n, accumulate != 0);
+    else
+      diff_assign<double>(ctx, static_cast<double*>(mesh), pos, weights, n, accumulate != 0);
+  });
+}
+
+int trvg_diff_assign_backward(const trvg_params* params, const double* pos,
+                              const double* weights, long long n,
+                              const void* grad_mesh, double* grad_pos,
+                              double* grad_weights) {
+  return guarded([&] {
+    require_device();
+    trvg_params q = mesh_params(params);
+    if (n < 0 || (n > 0 && pos == nullptr) || grad_mesh == nullptr)
+      throw Error(TRVG_ERR_PARAM, "Invalid particle or mesh pointers.");
+    Context ctx(q);
+    if (q.precision == TRVG_PREC_FP32)
+      diff_assign_adjoint<float>(ctx, static_cast<const float*>(grad_mesh), pos,
+                                 weights, n, grad_pos, grad_weights);
+    else
+      diff_assign_adjoint<double>(ctx, static_cast<const double*>(grad_mesh), pos,
+                                  weights, n, grad_pos, grad_weights);
+  });
+}
+
+int trvg_diff_bispec_diag(const trvg_params* params,
+                          const trvg_binning* kbinning, double norm_factor,
+                          const void* mesh, double* bk, int* nmodes) {
+  return guarded([&] {
+    require_device();
+    check_binning(kbinning);
+    trvg_params q = mesh_params(params);
+    if (mesh == nullptr || bk == nullptr)
+      throw Error(TRVG_ERR_PARAM, "Null mesh or output.");
+    Context ctx(q);
+    std::vector<int> nm;
+    std::vector<double> r =
+        (q.precision == TRVG_PREC_FP32)
+            ? diff_bispec_forward<float>(ctx, *kbinning, norm_factor,
+                                         static_cast<const float*>(mesh), nm)
+            : diff_bispec_forward<double>(ctx, *kbinning, norm_factor,
+                                          static_cast<const double*>(mesh), nm);
+    for (int b = 0; b < kbinning->num_bins; b++) {
+      bk[b] = r[b];
+      if (nmodes) nmodes[b] = nm[b];
+    }
+  });
+}
+
+int trvg_diff_bispec_diag_backward(const trvg_params* params,
+                                   const trvg_binning* kbinning,
+                                   double norm_factor, const void* mesh,
+                                   const double* grad_bk, void* grad_mesh) {
+  return guarded([&] {
+    require_device();
+    check_binning(kbinning);
+    trvg_params q = mesh_params(params);
+    if (mesh == nullptr || grad_bk == nullptr || grad_mesh == nullptr)
+      throw Error(TRVG_ERR_PARAM, "Null mesh, cotangent or gradient.");
+    Context ctx(q);
+    if (q.precision == TRVG_PREC_FP32)
+      diff_bispec_backward<float>(ctx, *kbinning, norm_factor,
+                                  static_cast<const float*>(mesh), grad_bk,
+                                  static_cast<float*>(grad_mesh));
+    else
+      diff_bispec_backward<double>(ctx, *kbinning, norm_factor,
+                                   static_cast<const double*>(mesh), grad_bk,
+                                   static_cast<double*>(grad_mesh));
+  });
+}
+
+int trvg_copy_calibration(long long bytes, double* ms) {
+  return guarded([&] {
+    require_device();
+    if (bytes < 16 || ms == nullptr)
+      throw Error(TRVG_ERR_PARAM, "copy calibration needs bytes >= 16 and an output.");
+    const long long n = bytes / 16;
+    DevArray<double2> a(n), b(n);
+    hipStream_t s = nullptr;
+    TRVG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    TRVG_HIP(hipEventCreate(&e0));
+    TRVG_HIP(hipEventCreate(&e1));
+    int dev = 0, ncu = 256;
+    TRVG_HIP(hipGetDevice(&dev));
+    TRVG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // Best of three (the first launch also loads the code object).
+    float best = 0.f;
+    for (int rep = 0; rep < 3; rep++) {
+      TRVG_HIP(hipEventRecord(e0, s));
+      hipLaunchKernelGGL((k_copy<double>), dim3(unsigned(ncu) * 16), dim3(256), 0, s,
+                         b.get(), a.get(), n);
+      TRVG_HIP(hipEventRecord(e1, s));
+      TRVG_HIP(hipEventSynchronize(e1));
+      float t = 0.f;
+      TRVG_HIP(hipEventElapsedTime(&t, e0, e1));
+      if (rep == 0 || t < best) best = t;
+    }
+    *ms = best;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    TRVG_HIP(hipStreamDestroy(s));
+  });
+}
+
 int trvg_fft3d(const trvg_params* params, double* mesh, int sign) {
   return guarded([&] {
     require_device();

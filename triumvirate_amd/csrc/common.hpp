@@ -108,6 +108,21 @@ __host__ __device__ __forceinline__ int shifted(int i, int n) {
 // |v| = sqrt(x*x + y*y + z*z) with every operation IEEE round-to-nearest
 // and no contraction, so that bin membership tests agree bit-for-bit with
 // the reference's plain FP64 arithmetic (maths.cpp:57-59).
+// k_lo <= |k| < k_hi with |k| = sqrt_rn(|k|^2), evaluated on |k|^2:
+// k2_lo (k2_hi) is the least double s with sqrt_rn(s) >= k_lo (k_hi), so
+// the comparison of |k|^2 selects exactly the same modes (shell_band()).
+struct ShellBand {
+  double k2_lo, k2_hi;
+  __device__ __forceinline__ bool contains(double k2) const {
+    return k2 >= k2_lo && k2 < k2_hi;
+  }
+};
+
+// |v|^2 with the rounding sequence of mag3_exact.
+__device__ __forceinline__ double mag2_exact(double x, double y, double z) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+}
+
 __device__ __forceinline__ double mag3_exact(double x, double y, double z) {
   double s = __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)),
                        __dmul_rn(z, z));

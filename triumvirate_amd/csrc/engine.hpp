@@ -127,6 +127,9 @@ struct Spline {
 
 void build_spline(int ell, Spline& sp, hipStream_t s);
 
+// Process-wide spline for degree ell on device dev (built on first use).
+const Spline& cached_spline(int dev, int ell, hipStream_t s);
+
 // Host natural-spline node values and coefficients (for tests).
 void spline_nodes(int ell, std::vector<double>& y, std::vector<double>& c,
                   double& split);
@@ -138,8 +141,8 @@ void spline_nodes(int ell, std::vector<double>& y, std::vector<double>& c,
 // ---------------------------------------------------------------------------
 
 enum KernelClass {
-  KC_FFT_Z = 0,        // first FFT pass (with fused loader)
-  KC_FFT_STRIDED,      // y / x FFT passes
+  KC_FFT_Z = 0,        // first FFT pass (with fused loader), FP32 fields
+  KC_FFT_STRIDED,      // y / x FFT passes, FP32 fields
   KC_ASSIGN,           // particle -> mesh assignment
   KC_PAIR_REDUCE,      // sum_x F_a F_b G over bin pairs
   KC_TWOPT,            // two-point binned statistics
@@ -147,6 +150,8 @@ enum KernelClass {
   KC_RADIAL,           // shot-noise radial histogram
   KC_SHOT_PAIRS,       // shot-noise per-pair sums
   KC_MISC,             // zero fills, partial sums, particle sums
+  KC_FFT_Z64,          // first FFT pass, FP64 fields
+  KC_FFT_STRIDED64,    // y / x FFT passes, FP64 fields
   KC_COUNT
 };
 
@@ -162,13 +167,16 @@ class Profiler {
  public:
   static Profiler& get();
   bool enabled() const { return on_; }
+  bool enabled_for(int kc) const { return on_ && (only_ < 0 || only_ == kc); }
   void enable(bool on);
+  void only(int kc) { only_ = kc; }
   void add(int kc, double ms, double bytes);
   void read(int kc, double* ms, long long* n, double* bytes);
 
  private:
   std::mutex mu_;
   bool on_ = false;
+  int only_ = -1;
   double ms_[KC_COUNT] = {};
   long long n_[KC_COUNT] = {};
   double bytes_[KC_COUNT] = {};
@@ -182,8 +190,8 @@ struct Context {
   trvg_params p;
   Geom g;
   hipStream_t stream = nullptr;
-  DevArray<double> tab_data;   // per-axis sinc, sin^2, u
-  AxisTables tab;
+  int dev = 0;
+  AxisTables tab;               // per-axis sinc, sin^2, u (cached tables)
   int sm_count = 256;
   std::vector<ProfRecord> prof;
 
@@ -205,6 +213,39 @@ class ProfScope {
   double bytes_;
   hipEvent_t a_ = nullptr;
 };
+
+// Least double s >= 0 with sqrt_rn(s) >= t (-1 when every s passes), and
+// the band of a bin [k_lo, k_hi) in |k|^2.
+double sq_threshold(double t);
+inline ShellBand shell_band(double k_lo, double k_hi) {
+  return ShellBand{sq_threshold(k_lo), sq_threshold(k_hi)};
+}
+
+// Launch helpers shared by the drivers.
+inline unsigned grid_for(const Context& ctx, long long work, int per_cu = 8) {
+  long long b = (work + RED_THREADS - 1) / RED_THREADS;
+  long long cap = (long long)ctx.sm_count * per_cu;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return unsigned(b);
+}
+
+// Column sums of a [nrows][width] partials array (fixed order, so results
+// do not depend on scheduling).
+template <typename V>
+inline void reduce_cols(Context& ctx, const V* part, int nrows, int width, V* out) {
+  hipLaunchKernelGGL((k_reduce_cols<V>), dim3((width + RC_COLS - 1) / RC_COLS),
+                     dim3(RC_COLS * RC_ROWS), 0, ctx.stream, part, nrows, width, out);
+  TRVG_LAUNCH_CHECK();
+}
+
+// Mode counts and mean |k| of each k-bin (exactly rounded |k|; k_eff is
+// NaN for an empty bin, as the reference divides by zero).
+struct ShellStats {
+  std::vector<int> nmodes;
+  std::vector<double> keff;
+};
+ShellStats shell_stats(Context& ctx, const trvg_binning& kb);
 
 // Maths shared with the drivers.
 double wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3);
@@ -252,6 +293,21 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
                                      const DeviceCatalogue* r, bool survey,
                                      const trvg_binning& kb,
                                      const std::vector<Multipole>& mps);
+
+// Differentiable periodic-box B_000 diag form (engine_diff.hip).
+template <typename T>
+std::vector<double> diff_bispec_forward(Context& ctx, const trvg_binning& kb,
+                                        double norm, const T* mesh,
+                                        std::vector<int>& nmodes);
+template <typename T>
+void diff_bispec_backward(Context& ctx, const trvg_binning& kb, double norm,
+                          const T* mesh, const double* grad_bk, T* grad_mesh);
+template <typename T>
+void diff_assign(Context& ctx, T* mesh, const double* pos, const double* w,
+                 long long n, bool accumulate);
+template <typename T>
+void diff_assign_adjoint(Context& ctx, const T* gmesh, const double* pos,
+                         const double* w, long long n, double* gpos, double* gw);
 
 template <typename T>
 Result run_bispec(Context& ctx, const DeviceCatalogue& d,

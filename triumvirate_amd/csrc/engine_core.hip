@@ -2,7 +2,11 @@
 // catalogue upload, spline construction, window tables, Wigner 3-j and the
 // binning-shape index maps.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+
+#include <map>
+#include <tuple>
 
 #include "engine.hpp"
 
@@ -132,6 +136,28 @@ void spline_nodes(int ell, std::vector<double>& y, std::vector<double>& c,
   for (int i = 0; i < m; i++) c[i + 1] = sol[i];
 }
 
+double sq_threshold(double t) {
+  if (!(t > 0.)) return -1.;
+  if (std::isinf(t)) return t;
+  double s = t * t;
+  while (s > 0. && std::sqrt(s) >= t) s = std::nextafter(s, 0.);
+  while (std::sqrt(s) < t) s = std::nextafter(s, HUGE_VAL);
+  return s;
+}
+
+static std::mutex g_spline_mu;
+static std::map<std::pair<int, int>, std::unique_ptr<Spline>> g_splines;
+
+const Spline& cached_spline(int dev, int ell, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_spline_mu);
+  auto key = std::make_pair(dev, ell);
+  auto it = g_splines.find(key);
+  if (it != g_splines.end()) return *it->second;
+  auto sp = std::make_unique<Spline>();
+  build_spline(ell, *sp, s);
+  return *(g_splines[key] = std::move(sp));
+}
+
 void build_spline(int ell, Spline& sp, hipStream_t s) {
   std::vector<double> y, c;
   double split;
@@ -150,22 +176,53 @@ void build_spline(int ell, Spline& sp, hipStream_t s) {
 // Context
 // ---------------------------------------------------------------------------
 
-Context::Context(const trvg_params& params) : p(params) {
-  if (p.device >= 0) TRVG_HIP(hipSetDevice(p.device));
-  g = make_geom(p);
-  TRVG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  int dev = 0;
-  TRVG_HIP(hipGetDevice(&dev));
-  TRVG_HIP(hipDeviceGetAttribute(&sm_count, hipDeviceAttributeMultiprocessorCount, dev));
-  // Per-axis tables: sinc, sin^2, u for each axis (field.cpp:1138-1149,
-  // 3441-3465).
+// Streams are reused across calls: a free list per device (all at the
+// default priority).
+static std::mutex g_stream_mu;
+static std::map<int, std::vector<hipStream_t>> g_free_streams;
+
+static hipStream_t acquire_stream(int dev) {
+  {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    auto& v = g_free_streams[dev];
+    if (!v.empty()) {
+      hipStream_t s = v.back();
+      v.pop_back();
+      return s;
+    }
+  }
+  hipStream_t s = nullptr;
+  TRVG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+static void release_stream(int dev, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_stream_mu);
+  g_free_streams[dev].push_back(s);
+}
+
+// Per-axis tables depend only on the mesh sizes: built once per (device,
+// sizes) and kept for the process lifetime.
+struct AxisTableData {
+  DevArray<double> data;
+  long long offs[3][3];
+};
+static std::mutex g_tab_mu;
+static std::map<std::tuple<int, int, int, int>, std::unique_ptr<AxisTableData>> g_tabs;
+
+static const AxisTableData& axis_tables(int dev, const Geom& g, hipStream_t stream) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  auto key = std::make_tuple(dev, g.n[0], g.n[1], g.n[2]);
+  auto it = g_tabs.find(key);
+  if (it != g_tabs.end()) return *it->second;
+  // sinc, sin^2, u for each axis (field.cpp:1138-1149, 3441-3465).
+  auto t = std::make_unique<AxisTableData>();
   long long tot = 3LL * (g.n[0] + g.n[1] + g.n[2]);
   std::vector<double> h(tot);
   long long off = 0;
-  long long offs[3][3];
   for (int kind = 0; kind < 3; kind++) {
     for (int a = 0; a < 3; a++) {
-      offs[kind][a] = off;
+      t->offs[kind][a] = off;
       for (int i = 0; i < g.n[a]; i++) {
         int ii = shifted(i, g.n[a]);
         double u = M_PI * ii / double(g.n[a]);
@@ -178,13 +235,23 @@ Context::Context(const trvg_params& params) : p(params) {
       off += g.n[a];
     }
   }
-  tab_data.reset(tot);
-  TRVG_HIP(hipMemcpyAsync(tab_data.get(), h.data(), tot * 8, hipMemcpyHostToDevice, stream));
+  t->data.reset(tot);
+  TRVG_HIP(hipMemcpyAsync(t->data.get(), h.data(), tot * 8, hipMemcpyHostToDevice, stream));
   TRVG_HIP(hipStreamSynchronize(stream));
+  return *(g_tabs[key] = std::move(t));
+}
+
+Context::Context(const trvg_params& params) : p(params) {
+  if (p.device >= 0) TRVG_HIP(hipSetDevice(p.device));
+  g = make_geom(p);
+  TRVG_HIP(hipGetDevice(&dev));
+  stream = acquire_stream(dev);
+  TRVG_HIP(hipDeviceGetAttribute(&sm_count, hipDeviceAttributeMultiprocessorCount, dev));
+  const AxisTableData& t = axis_tables(dev, g, stream);
   for (int a = 0; a < 3; a++) {
-    tab.sinc[a] = tab_data.get() + offs[0][a];
-    tab.sin2[a] = tab_data.get() + offs[1][a];
-    tab.u[a] = tab_data.get() + offs[2][a];
+    tab.sinc[a] = t.data.get() + t.offs[0][a];
+    tab.sin2[a] = t.data.get() + t.offs[1][a];
+    tab.u[a] = t.data.get() + t.offs[2][a];
   }
   tab.order = p.assignment_order;
   tab.interlace = p.interlace;
@@ -197,7 +264,7 @@ Context::~Context() {
       flush_profile();
     } catch (...) {
     }
-    (void)hipStreamDestroy(stream);
+    release_stream(dev, stream);
   }
 }
 
@@ -220,8 +287,9 @@ void Context::flush_profile() {
 
 const char* kernel_class_name(int kc) {
   static const char* names[KC_COUNT] = {
-      "fft_pass_z", "fft_pass_strided", "assign", "pair_reduce", "twopt_bins",
-      "shell_stats", "shot_radial_hist", "shot_pairs", "misc"};
+      "fft_pass_z_f32", "fft_pass_strided_f32", "assign", "pair_reduce",
+      "twopt_bins", "shell_stats", "shot_radial_hist", "shot_pairs", "misc",
+      "fft_pass_z_f64", "fft_pass_strided_f64"};
   return (kc >= 0 && kc < KC_COUNT) ? names[kc] : "";
 }
 
@@ -256,7 +324,7 @@ void Profiler::read(int kc, double* ms, long long* n, double* bytes) {
 
 ProfScope::ProfScope(Context& ctx, int kc, double bytes)
     : ctx_(ctx), kc_(kc), bytes_(bytes) {
-  if (!Profiler::get().enabled()) return;
+  if (!Profiler::get().enabled_for(kc)) return;
   TRVG_HIP(hipEventCreate(&a_));
   TRVG_HIP(hipEventRecord(a_, ctx_.stream));
 }

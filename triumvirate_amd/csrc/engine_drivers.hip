@@ -16,6 +16,7 @@
 #include <map>
 #include <tuple>
 
+#include "diff_kernels.hpp"
 #include "engine.hpp"
 
 namespace trvg {
@@ -58,7 +59,7 @@ static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign,
   constexpr int NL = E / N;
   size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>);
   long long nblocks = ctx.g.nmesh / E;
-  ProfScope ps(ctx, KC_FFT_Z, read_bytes + write_bytes);
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, read_bytes + write_bytes);
   hipLaunchKernelGGL((fft_pass_z<T, LOGN, EPT, Loader>), dim3(unsigned(nblocks)),
                      dim3(FFT_THREADS), smem, ctx.stream, data, load,
                      twiddles<T>(N, sign), ctx.g.n[0], ctx.g.n[1], sign, pr);
@@ -75,7 +76,7 @@ static void launch_strided(Context& ctx, c2<T>* data, long long S,
   constexpr int NL = E / N;
   size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>);
   long long nblocks = ctx.g.nmesh / E;
-  ProfScope ps(ctx, KC_FFT_STRIDED, bytes);
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
   hipLaunchKernelGGL((fft_pass_strided<T, LOGN, EPT>), dim3(unsigned(nblocks)),
                      dim3(FFT_THREADS), smem, ctx.stream, data,
                      twiddles<T>(N, sign), S, outer_stride, inner, sign, pr,
@@ -157,14 +158,6 @@ void fft3d(Context& ctx, c2<T>* data, const Loader& load, int sign,
 // ---------------------------------------------------------------------------
 // Small launch helpers
 // ---------------------------------------------------------------------------
-
-static unsigned grid_for(const Context& ctx, long long work, int per_cu = 8) {
-  long long b = (work + RED_THREADS - 1) / RED_THREADS;
-  long long cap = (long long)ctx.sm_count * per_cu;
-  if (b > cap) b = cap;
-  if (b < 1) b = 1;
-  return unsigned(b);
-}
 
 template <typename T>
 static void zero_field(Context& ctx, c2<T>* f) {
@@ -318,12 +311,7 @@ static void build_field_k(Context& ctx, c2<T>* f, const DeviceCatalogue& d,
 // Reductions
 // ---------------------------------------------------------------------------
 
-struct ShellStats {
-  std::vector<int> nmodes;
-  std::vector<double> keff;
-};
-
-static ShellStats shell_stats(Context& ctx, const trvg_binning& kb) {
+ShellStats shell_stats(Context& ctx, const trvg_binning& kb) {
   const int nb = kb.num_bins;
   DevArray<double> edges(nb + 1);
   TRVG_HIP(hipMemcpyAsync(edges.get(), kb.bin_edges, (nb + 1) * 8,
@@ -337,9 +325,7 @@ static ShellStats shell_stats(Context& ctx, const trvg_binning& kb) {
   hipLaunchKernelGGL(k_shell_stats, dim3(grid), dim3(RED_THREADS), smem,
                      ctx.stream, ctx.g, edges.get(), nb, cnt.get(), part.get());
   TRVG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, dim3((nb + 255) / 256), dim3(256), 0,
-                     ctx.stream, part.get(), int(grid), nb, ks.get());
-  TRVG_LAUNCH_CHECK();
+  reduce_cols(ctx, part.get(), int(grid), nb, ks.get());
   std::vector<unsigned long long> hc(nb);
   ShellStats s;
   s.keff.resize(nb);
@@ -400,9 +386,7 @@ static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
                            part.get());
         TRVG_LAUNCH_CHECK();
       }
-      hipLaunchKernelGGL(k_sum_partials2, dim3((nt * 16 + 255) / 256), dim3(256),
-                         0, ctx.stream, part.get(), int(grid), nt * 16, sum.get());
-      TRVG_LAUNCH_CHECK();
+      reduce_cols(ctx, part.get(), int(grid), nt * 16, sum.get());
       std::vector<double2> h((size_t)nt * 16);
       TRVG_HIP(hipMemcpyAsync(h.data(), sum.get(), h.size() * sizeof(double2),
                               hipMemcpyDeviceToHost, ctx.stream));
@@ -430,7 +414,8 @@ struct TwoPt {
 // FieldStats::compute_ylm_wgtd_2pt_stats_in_fourier (field.cpp:2511-2703).
 template <typename T>
 static TwoPt twopt_fourier(Context& ctx, const c2<T>* fa, const c2<T>* fb,
-                           cplx sbar, int ell, int m, const trvg_binning& kb) {
+                           cplx sbar, int ell, int m, const trvg_binning& kb,
+                           double modes_in_range) {
   const int nb = kb.num_bins;
   DevArray<double> edges(nb + 1);
   TRVG_HIP(hipMemcpyAsync(edges.get(), kb.bin_edges, (nb + 1) * 8,
@@ -440,15 +425,14 @@ static TwoPt twopt_fourier(Context& ctx, const c2<T>* fa, const c2<T>* fb,
   DevArray<double> part((size_t)grid * nb * 4), sum(nb * 4);
   TRVG_HIP(hipMemsetAsync(cnt.get(), 0, nb * 8, ctx.stream));
   size_t smem = nb * (4 * 8 + 4);
-  ProfScope ps(ctx, KC_TWOPT, 2. * ctx.g.nmesh * sizeof(c2<T>));
+  // Compulsory reads: both fields at the modes inside the binning range.
+  ProfScope ps(ctx, KC_TWOPT, 2. * modes_in_range * sizeof(c2<T>));
   hipLaunchKernelGGL((k_twopt_fourier<T>), dim3(grid), dim3(RED_THREADS), smem,
                      ctx.stream, fa, fb, ctx.g, ctx.tab,
                      dmk(sbar.real(), sbar.imag()), ell, m, edges.get(), nb,
                      cnt.get(), part.get());
   TRVG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, dim3((nb * 4 + 255) / 256), dim3(256), 0,
-                     ctx.stream, part.get(), int(grid), nb * 4, sum.get());
-  TRVG_LAUNCH_CHECK();
+  reduce_cols(ctx, part.get(), int(grid), nb * 4, sum.get());
   std::vector<unsigned long long> hc(nb);
   std::vector<double> hs(nb * 4);
   TRVG_HIP(hipMemcpyAsync(hc.data(), cnt.get(), nb * 8, hipMemcpyDeviceToHost, ctx.stream));
@@ -497,9 +481,7 @@ static ShotBins3pcf shot_bins_3pcf(Context& ctx, const c2<T>* t, int la, int ma,
                      ctx.stream, t, ctx.g, la, ma, lb, mb, edges.get(), nb,
                      cnt.get(), part.get());
   TRVG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, dim3((nb * 3 + 255) / 256), dim3(256), 0,
-                     ctx.stream, part.get(), int(grid), nb * 3, sum.get());
-  TRVG_LAUNCH_CHECK();
+  reduce_cols(ctx, part.get(), int(grid), nb * 3, sum.get());
   std::vector<unsigned long long> hc(nb);
   std::vector<double> hs(nb * 3);
   TRVG_HIP(hipMemcpyAsync(hc.data(), cnt.get(), nb * 8, hipMemcpyDeviceToHost, ctx.stream));
@@ -556,7 +538,8 @@ static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
                        Q.get());
     TRVG_LAUNCH_CHECK();
     DevArray<double2> res(P);
-    ProfScope ps2(ctx, KC_SHOT_PAIRS, double(P) * nq * 16.);
+    // The histogram is read once from HBM (then from L2 by every pair).
+    ProfScope ps2(ctx, KC_SHOT_PAIRS, double(nq) * 16. + double(P) * 16.);
     hipLaunchKernelGGL(k_shot_pairs_hist, dim3(P), dim3(RED_THREADS), 0,
                        ctx.stream, Q.get(), nq, g.dr[0], sa.dev(), sb.dev(),
                        dka.get(), dkb.get(), g.vol_cell, res.get());
@@ -578,9 +561,7 @@ static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
                        ctx.stream, t, g, la, ma, lb, mb, sa.dev(), sb.dev(),
                        dka.get() + p0, dkb.get() + p0, np, part.get());
     TRVG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sum_partials2, dim3(1), dim3(256), 0, ctx.stream,
-                       part.get(), int(grid), np, sum.get());
-    TRVG_LAUNCH_CHECK();
+    reduce_cols(ctx, part.get(), int(grid), np, sum.get());
     std::vector<double2> h(np);
     TRVG_HIP(hipMemcpyAsync(h.data(), sum.get(), np * sizeof(double2),
                             hipMemcpyDeviceToHost, ctx.stream));
@@ -745,14 +726,11 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
   build_field_k<double>(ctx, dn00.get(), d, r, survey, alpha, 0, 0, false, true);
   build_field_k<double>(ctx, N00.get(), d, r, survey, alpha, 0, 0, true, false);
   ShellStats ss = shell_stats(ctx, kb);
+  double modes_in_range = 0.;
+  for (int n : ss.nmodes) modes_in_range += n;
 
-  std::map<int, std::unique_ptr<Spline>> splines;
   auto spline = [&](int ell) -> const Spline& {
-    auto it = splines.find(ell);
-    if (it != splines.end()) return *it->second;
-    auto sp = std::make_unique<Spline>();
-    build_spline(ell, *sp, ctx.stream);
-    return *(splines[ell] = std::move(sp));
+    return cached_spline(ctx.dev, ell, ctx.stream);
   };
 
   DevArray<c2<T>> G(nm);
@@ -760,8 +738,8 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
   SlabCache<T> slabs(mps.size() > 1 ? 3 : 2);
   auto make_shell = [&](int l, int mm) {
     return [&, l, mm](c2<T>* out_f, int b) {
-      LoadShell<T, double> ls{dn00.get(), ctx.tab, g, kb.bin_edges[b],
-                              kb.bin_edges[b + 1], l, mm};
+      LoadShell<T, double> ls{dn00.get(), ctx.tab, g,
+                              shell_band(kb.bin_edges[b], kb.bin_edges[b + 1]), l, mm};
       Prune pr = make_prune(g, kb.bin_edges[b + 1]);
       fft3d<T>(ctx, out_f, ls, +1, double(ss.nmodes[b]) * sizeof(double2), &pr);
     };
@@ -838,7 +816,8 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
         for (int dv = 0; dv < dim; dv++) out.shot[dv] += sijk;
       }
       if (p.ell2 == 0) {
-        TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb);
+        TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb,
+                                         modes_in_range);
         for (int dv = 0; dv < dim; dv++) {
           int bb = ra[dv];
           cplx sv = tm.coupling * (st.pk[bb] - st.sn[bb]);
@@ -846,7 +825,8 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
         }
       }
       if (p.ell1 == 0) {
-        TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell2, tm.m2, kb);
+        TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell2, tm.m2, kb,
+                                         modes_in_range);
         for (int dv = 0; dv < dim; dv++) {
           int bb = cb[dv];
           cplx sv = tm.coupling * (st.pk[bb] - st.sn[bb]);
@@ -910,9 +890,8 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
   build_field_k<double>(ctx, dn00.get(), d, r, survey, alpha, 0, 0, false, true);
   build_field_k<double>(ctx, N00.get(), d, r, survey, alpha, 0, 0, true, false);
 
-  Spline sja, sjb;
-  build_spline(p.ell1, sja, ctx.stream);
-  build_spline(p.ell2, sjb, ctx.stream);
+  const Spline& sja = cached_spline(ctx.dev, p.ell1, ctx.stream);
+  const Spline& sjb = cached_spline(ctx.dev, p.ell2, ctx.stream);
 
   std::vector<int> rows = distinct(ra), cols = distinct(cb);
   DevArray<c2<T>> G(nm);
@@ -1007,9 +986,7 @@ double run_mesh_norm(Context& ctx, const DeviceCatalogue& c) {
   hipLaunchKernelGGL((k_cube_sum<double>), dim3(grid), dim3(RED_THREADS), 0,
                      ctx.stream, f.get(), ctx.g.nmesh, part.get());
   TRVG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, ctx.stream,
-                     part.get(), int(grid), 1, sum.get());
-  TRVG_LAUNCH_CHECK();
+  reduce_cols(ctx, part.get(), int(grid), 1, sum.get());
   double h = 0.;
   TRVG_HIP(hipMemcpyAsync(&h, sum.get(), 8, hipMemcpyDeviceToHost, ctx.stream));
   TRVG_HIP(hipStreamSynchronize(ctx.stream));
@@ -1042,5 +1019,20 @@ template void fft3d<double, LoadSame<double>>(Context&, double2*,
 template void fft3d<float, LoadSame<float>>(Context&, float2*,
                                             const LoadSame<float>&, int, double,
                                             const Prune*);
+
+// Loaders of the differentiable path (engine_diff.hip).
+#define TRVG_FFT3D_INST(T, L)                                                   \
+  template void fft3d<T, L>(Context&, c2<T>*, const L&, int, double, const Prune*);
+TRVG_FFT3D_INST(double, LoadReal<double>)
+TRVG_FFT3D_INST(float, LoadReal<float>)
+TRVG_FFT3D_INST(double, LoadShellCotangent<double>)
+TRVG_FFT3D_INST(float, LoadShellCotangent<float>)
+#define TRVG_COMMA ,
+TRVG_FFT3D_INST(double, LoadDeconv<double TRVG_COMMA double>)
+TRVG_FFT3D_INST(float, LoadDeconv<float TRVG_COMMA float>)
+TRVG_FFT3D_INST(double, LoadShell<double TRVG_COMMA double>)
+TRVG_FFT3D_INST(float, LoadShell<float TRVG_COMMA float>)
+#undef TRVG_COMMA
+#undef TRVG_FFT3D_INST
 
 }  // namespace trvg

@@ -168,20 +168,21 @@ struct LoadDeconv {
 
 // Shell filter: y_lm(k^) src(k) / W(k) for k_lo <= |k| < k_hi, else 0
 // (inv_fourier_transform_ylm_wgtd_field_band_limited, field.cpp:1792-1906;
-// the 1/n_modes factor is applied to the pair sums instead).
+// the 1/n_modes factor is applied to the pair sums instead).  The shell
+// test runs on |k|^2 against ShellBand thresholds, which selects exactly
+// the modes whose correctly rounded |k| passes the reference's test.
 template <typename T, typename S = T>
 struct LoadShell {
   const c2<S>* src;
   AxisTables tab;
   Geom geo;
-  double k_lo, k_hi;
+  ShellBand band;
   int ell, m;
   __device__ c2<T> operator()(long long g, int i, int j, int k) const {
     double kx = grid_coord(i, geo.n[0], geo.dk[0]);
     double ky = grid_coord(j, geo.n[1], geo.dk[1]);
     double kz = grid_coord(k, geo.n[2], geo.dk[2]);
-    double km = mag3_exact(kx, ky, kz);
-    if (!(k_lo <= km && km < k_hi)) return mk<T>(T(0), T(0));
+    if (!band.contains(mag2_exact(kx, ky, kz))) return mk<T>(T(0), T(0));
     c2<S> v = src[g];
     double iw = 1. / window_at(tab, i, j, k);
     double2 fk = dmk(double(v.x) * iw, double(v.y) * iw);
@@ -289,13 +290,35 @@ k_shell_stats(Geom g, const double* __restrict__ edges, int nb,
 }
 
 // Sum per-block partials in a fixed order (deterministic).
-static __global__ void k_sum_partials(const double* __restrict__ part, int nblocks,
-                               int width, double* __restrict__ out) {
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= width) return;
-  double s = 0.;
-  for (int b = 0; b < nblocks; b++) s += part[(long long)b * width + col];
-  out[col] = s;
+// Column sums of partials [nrows][width]: a block takes RC_COLS adjacent
+// columns (contiguous loads) with RC_ROWS row lanes, then a fixed-shape LDS
+// tree; the summation order depends only on (nrows, width).
+constexpr int RC_COLS = 4;
+constexpr int RC_ROWS = 64;
+
+__device__ __forceinline__ double vzero(double) { return 0.; }
+__device__ __forceinline__ double2 vzero(double2) { return make_double2(0., 0.); }
+__device__ __forceinline__ double vadd(double a, double b) { return a + b; }
+__device__ __forceinline__ double2 vadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+
+template <typename V>
+static __global__ void __launch_bounds__(RC_COLS * RC_ROWS)
+k_reduce_cols(const V* __restrict__ part, int nrows, int width, V* __restrict__ out) {
+  __shared__ V red[RC_ROWS][RC_COLS];
+  const int c = threadIdx.x % RC_COLS, r = threadIdx.x / RC_COLS;
+  const int col = blockIdx.x * RC_COLS + c;
+  V s = vzero(V());
+  if (col < width)
+    for (int b = r; b < nrows; b += RC_ROWS) s = vadd(s, part[(long long)b * width + col]);
+  red[r][c] = s;
+  __syncthreads();
+  for (int h = RC_ROWS / 2; h > 0; h >>= 1) {
+    if (r < h) red[r][c] = vadd(red[r][c], red[r + h][c]);
+    __syncthreads();
+  }
+  if (r == 0 && col < width) out[col] = red[0][c];
 }
 
 // ---------------------------------------------------------------------------
@@ -370,18 +393,6 @@ k_pair_reduce(const c2<T>* __restrict__ FA, int na,
   }
 }
 
-static __global__ void k_sum_partials2(const double2* __restrict__ part, int nblocks,
-                                int width, double2* __restrict__ out) {
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= width) return;
-  double sx = 0., sy = 0.;
-  for (int b = 0; b < nblocks; b++) {
-    double2 v = part[(long long)b * width + col];
-    sx += v.x;
-    sy += v.y;
-  }
-  out[col] = dmk(sx, sy);
-}
 
 // ---------------------------------------------------------------------------
 // Two-point statistics in Fourier space with the reference's two-level
