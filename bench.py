@@ -89,6 +89,11 @@ def main():
     ap.add_argument('--ngrid', type=int, default=256)
     ap.add_argument('--ndata', type=int, default=1_000_000)
     ap.add_argument('--nrand', type=int, default=5_000_000)
+    ap.add_argument('--workload', choices=('bispec', 'grad', '3pcf'),
+                    default='bispec',
+                    help='bispec: B_000 + B_202 survey at 256^3 (default); '
+                         'grad: differentiable B_000 fwd+bwd, 1e6 particles, '
+                         '256^3 CIC; 3pcf: zeta_000 + zeta_110 survey at 512^3')
     ap.add_argument('--skip-fp64', action='store_true',
                     help='skip the FP64 steps (profiler runs)')
     ap.add_argument('--cpu-pairs', type=int, default=6,
@@ -108,6 +113,18 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.workload == 'grad':
+        print(json.dumps(bench_grad(args, lib)))
+        return
+    if args.workload == '3pcf':
+        print(json.dumps(bench_3pcf(args, lib)))
+        return
+    print(json.dumps(bench_bispec(args, lib)))
+
+
+def bench_bispec(args, lib):
+    from triumvirate_amd import _abi
+    from triumvirate_amd.dataobjs import Binning
     data, rand, los_d, los_r = make_inputs(args.ndata, args.nrand, args.ngrid)
     kb = Binning('fourier', 'lin', bin_min=0.005, bin_max=0.205, num_bins=20)
     kbv = _abi.BinningView(kb.bin_edges, kb.bin_centres)
@@ -310,7 +327,213 @@ def main():
     }
     for h in handles:
         lib.trvg_catalogue_free(h)
-    print(json.dumps(line))
+    return line
+
+
+def _check(lib, st):
+    if st != 0:
+        raise RuntimeError(lib.trvg_last_error().decode())
+
+
+def _upload(lib, cat, los):
+    from triumvirate_amd import _abi
+    x, y, z, nz, ws, wc = cat.columns()
+    view = _abi.CatalogueView(x, y, z, nz, ws, wc)
+    h = C.c_void_p()
+    _check(lib, lib.trvg_catalogue_upload(C.byref(view.struct),
+                                          _abi.dptr(los), 0, C.byref(h)))
+    return h
+
+
+def _sync_time(fn, reps):
+    """Mean wall time of fn() over reps calls (each call is synchronous)."""
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    return (time.perf_counter() - t) / reps, out
+
+
+def bench_grad(args, lib):
+    """Differentiable B_000 diag: 1e6 lognormal particles in a periodic
+    L = 1000 Mpc/h box, CIC on a 256^3 FP32 mesh, 15 bins [0.01, 0.16];
+    loss = sum_b w_b Re B_000(k_b, k_b), w_b ~ N(0, 1) (seed 5).  Times the
+    forward and forward+backward against the plain GPU FP64 forward of the
+    same statistic, and checks position gradients against central finite
+    differences (eps = 1e-3 Mpc/h, 64 particles, FP64 mesh)."""
+    import torch
+    from triumvirate_amd import _abi, mocks
+    from triumvirate_amd import threept as T
+    from triumvirate_amd.autodiff import DifferentiableBispectrum
+    from triumvirate_amd.catalogue import ParticleCatalogue
+    from triumvirate_amd.dataobjs import Binning
+    from triumvirate_amd.parameters import ParameterSet
+    L, ng = 1000., 256
+    x, y, z = mocks.lognormal_box(1_000_000, L, 256, seed=4)
+    cat = ParticleCatalogue(x, y, z, nz=len(x) / L ** 3)
+    kb = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.16, num_bins=15)
+    ps = ParameterSet(param_dict={
+        'boxsize': {'x': L, 'y': L, 'z': L},
+        'ngrid': {'x': ng, 'y': ng, 'z': ng}, 'assignment': 'cic',
+        'norm_convention': 'particle',
+        'degrees': {'ell1': None, 'ell2': None, 'ELL': 0}})
+    backend = T.CLibBackend(lib)
+    ref = T._compute_3pt_stats_sim_like(backend, 'bispec', cat, paramset=ps,
+                                        degrees=(0, 0, 0), binning=kb,
+                                        form='diag')
+    norm = ref['_norm_factors'][0]
+    px, py, pz, _, _, _ = cat.columns()        # re-centred by the driver
+    pos = torch.tensor(np.stack([px, py, pz], axis=1), device='cuda')
+    log(f"grad inputs: {pos.shape[0]} particles")
+
+    # Plain GPU FP64 forward of the same statistic (device-resident
+    # catalogue, as in the timed forward below).
+    h = _upload(lib, cat, None)
+    p = T._params_struct(T._amalgamate_parameters(
+        paramset=ps, degrees=(0, 0, 0), binning=kb, form='diag',
+        catalogue_type='sim', statistic_type='bispec'))
+    res = _abi.ResultBuffers(_abi.result_dim(p.shape, 15, 0), 'bispec')
+    kbv = _abi.BinningView(kb.bin_edges, kb.bin_centres)
+
+    def plain64():
+        _check(lib, lib.trvg_compute_bispec_dev(h, None, C.byref(p),
+                                                C.byref(kbv.struct), norm,
+                                                C.byref(res.struct)))
+    plain64()
+    t_plain, _ = _sync_time(plain64, 3)
+    lib.trvg_catalogue_free(h)
+
+    wb = torch.tensor(np.random.default_rng(5).standard_normal(15), device='cuda')
+    B32 = DifferentiableBispectrum(L, ng, kb, 'cic', 'fp32', norm)
+
+    def fwd():
+        with torch.no_grad():
+            return B32(pos)
+
+    def fwd_bwd():
+        pp = pos.clone().requires_grad_(True)
+        (B32(pp) * wb).sum().backward()
+        torch.cuda.synchronize()
+        return pp.grad
+
+    fwd()
+    fwd_bwd()
+    t_fwd, bk32 = _sync_time(fwd, args.steps)
+    t_fb, g32 = _sync_time(fwd_bwd, args.steps)
+    bk32 = bk32.cpu().numpy()
+    ref_re = ref['bk_raw'].real
+    err_fwd = float(np.max(np.abs(bk32 - ref_re)) / np.max(np.abs(ref_re)))
+
+    # FP64 differentiable path: gradient reference and finite differences.
+    B64 = DifferentiableBispectrum(L, ng, kb, 'cic', 'fp64', norm)
+    pp = pos.clone().requires_grad_(True)
+    (B64(pp) * wb).sum().backward()
+    g64 = pp.grad.cpu().numpy()
+    err_g32 = float(np.max(np.abs(g32.cpu().numpy() - g64)) / np.max(np.abs(g64)))
+    rng = np.random.default_rng(6)
+    idx = rng.choice(pos.shape[0], 64, replace=False)
+    eps = 1e-3
+    fd_err = 0.
+    gscale = np.max(np.abs(g64[idx]))
+    t = time.perf_counter()
+    with torch.no_grad():
+        for i in idx:
+            for a in range(3):
+                q = pos.clone()
+                q[i, a] += eps
+                lp = float((B64(q) * wb).sum())
+                q[i, a] -= 2 * eps
+                lm = float((B64(q) * wb).sum())
+                fd = (lp - lm) / (2 * eps)
+                fd_err = max(fd_err, abs(fd - g64[i, a]) / gscale)
+    log(f"FD check {time.perf_counter() - t:.1f} s, max err {fd_err:.2e}")
+    return {
+        'metric': 'differentiable B_000 diag fwd+bwd time (1e6 particles, '
+                  '256^3 CIC, FP32 mesh)',
+        'value': round(t_fb * 1e3, 3), 'unit': 'ms', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': 1, 'ms_per_step': round(t_fb * 1e3, 3),
+        'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic (seeded lognormal box)',
+        'config': {'workload': 'periodic box, 1e6 lognormal particles (seed 4), '
+                               'L=1000, 256^3 CIC, 15 diag k-bins [0.01,0.16], '
+                               'loss = sum_b w_b Re B_000(k_b,k_b)',
+                   'n_particles': int(pos.shape[0])},
+        'forward_ms': round(t_fwd * 1e3, 3),
+        'plain_fp64_forward_ms': round(t_plain * 1e3, 3),
+        'fwd_bwd_over_plain_fp64': round(t_fb / t_plain, 3),
+        'fp32_forward_rel_err_vs_fp64_plain': err_fwd,
+        'fp32_grad_rel_err_vs_fp64': err_g32,
+        'fd_check': {'particles': 64, 'eps': eps, 'precision': 'fp64',
+                     'max_err_rel_to_max_grad': fd_err},
+    }
+
+
+def bench_3pcf(args, lib):
+    """Survey-like zeta_000 and zeta_110 (full form -> upper triangle, 465
+    pairs each), 30 linear r-bins [10, 200] Mpc/h, 1e7 lognormal data + 3e7
+    randoms in the quarter-sky shell, L = 3200 Mpc/h, 512^3 TSC, FP32
+    fields with FP64 sums (one FP64 run for the error)."""
+    from triumvirate_amd import _abi
+    from triumvirate_amd.dataobjs import Binning
+    data, rand, los_d, los_r = make_inputs(10_000_000, 30_000_000, 256)
+    rb = Binning('config', 'lin', bin_min=10., bin_max=200., num_bins=30)
+    rbv = _abi.BinningView(rb.bin_edges, rb.bin_centres)
+    t_up = time.perf_counter()
+    hd, hr = _upload(lib, data, los_d), _upload(lib, rand, los_r)
+    t_up = time.perf_counter() - t_up
+    x, y, z, nz, ws, wc = rand.columns()
+    rv = _abi.CatalogueView(x, y, z, nz, ws, wc)
+    norm = C.c_double()
+    _check(lib, lib.trvg_calc_bispec_normalisation_from_particles(
+        C.byref(rv.struct), data.wstotal / rand.wstotal, C.byref(norm)))
+    ng = 512
+
+    def run(deg, prec):
+        p = params_struct(ng, deg, 'full', prec)
+        for i in range(3):
+            p.boxsize[i] = 3200.
+        p.assignment_order = 3
+        p.verbose = 20
+        res = _abi.ResultBuffers(_abi.result_dim(p.shape, 30, 0), '3pcf')
+        _check(lib, lib.trvg_compute_3pcf_dev(hd, hr, C.byref(p),
+                                              C.byref(rbv.struct), norm.value,
+                                              C.byref(res.struct)))
+        return res.as_dict()
+
+    degs = [(0, 0, 0), (1, 1, 0)]
+    r64 = {d: run(d, 64) for d in degs}
+    log("FP64 done")
+    for d in degs:
+        run(d, 32)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        r32 = {d: run(d, 32) for d in degs}
+    ms = (time.perf_counter() - t) * 1e3 / args.steps
+    pairs = sum(r32[d]['zeta_raw'].shape[0] for d in degs)
+    errs = {}
+    for d in degs:
+        tag = 'zeta' + ''.join(map(str, d))
+        errs[tag] = {k: float(np.max(np.abs(r32[d][k] - r64[d][k])) /
+                               np.max(np.abs(r64[d][k])))
+                     for k in ('zeta_raw', 'zeta_shot')}
+        errs[tag]['npairs_equal'] = bool(np.array_equal(r32[d]['npairs_1'],
+                                                        r64[d]['npairs_1']))
+    lib.trvg_catalogue_free(hd)
+    lib.trvg_catalogue_free(hr)
+    return {
+        'metric': 'zeta_l bin-pair evals/s at 512^3 mesh (FP32 fields)',
+        'value': round(pairs / (ms / 1e3), 2), 'unit': 'bin-pair evals/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': 1,
+        'ms_per_step': round(ms, 3), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic (seeded lognormal mock + uniform randoms)',
+        'config': {'workload': 'survey-like lpp zeta_000 + zeta_110 (full -> '
+                               'triu, 465 pairs each), 30 lin r-bins [10,200], '
+                               '1e7 lognormal data + 3e7 randoms, L=3200, '
+                               '512^3 TSC, interlace off',
+                   'pairs_per_step': pairs, 'n_data': int(data.ntotal),
+                   'n_rand': int(rand.ntotal)},
+        'fp32_rel_err': errs, 'upload_ms': round(t_up * 1e3, 1),
+    }
 
 
 if __name__ == '__main__':

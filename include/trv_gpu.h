@@ -288,6 +288,38 @@ int trvg_mesh_assign(const trvg_catalogue* cat, const double* los,
 int trvg_fft3d(const trvg_params* params, double* mesh, int sign);
 
 /* ------------------------------------------------------------------------
+ * Batched periodic-box bispectrum (no reference counterpart: the reference
+ * computes one catalogue per call).  A plan fixes the mesh, binning and a
+ * list of multipoles (`multipoles[q]`: geometry, assignment, precision and
+ * degrees / shape / idx_bin of multipole q; all share the mesh) for batches
+ * of up to `batch` catalogues holding at most `capacity` particles in
+ * total.  Each result equals compute_bispec_in_gpp_box
+ * (threept.cpp:1473-2188) of that catalogue with the particle
+ * normalisation (nz = N / V, threept.cpp:96-136).  With autotune != 0 the
+ * first run times candidate launch shapes and keeps the fastest; with
+ * use_graph != 0 the whole batch is captured into a HIP graph on the first
+ * run and replayed afterwards.
+ * ------------------------------------------------------------------------ */
+typedef struct trvg_batch_plan trvg_batch_plan;
+
+int trvg_batch_plan_create(const trvg_params* multipoles, int nmultipoles,
+                           const trvg_binning* kbinning, int batch,
+                           long long capacity, int autotune, int use_graph,
+                           trvg_batch_plan** plan);
+
+/* Positions of catalogue b are x/y/z[o_b .. o_b + counts[b]) with
+ * o_b = counts[0] + ... + counts[b-1] (host arrays, already in [0, L));
+ * outs[b * nmultipoles + q] receives multipole q of catalogue b. */
+int trvg_batch_run(trvg_batch_plan* plan, int ncatalogues,
+                   const long long* counts, const double* x, const double* y,
+                   const double* z, trvg_bispec_result* outs);
+
+/* tuned[5]: assignment, pair-reduction and binned-statistics blocks per
+ * catalogue, pair row-block cap, and whether the graph is captured. */
+int trvg_batch_plan_info(const trvg_batch_plan* plan, int* tuned);
+int trvg_batch_plan_free(trvg_batch_plan* plan);
+
+/* ------------------------------------------------------------------------
  * Differentiable periodic-box bispectrum (no reference counterpart: the
  * reference computes forward values only).  The forward chain is that of
  * compute_bispec_in_gpp_box (threept.cpp:1473-2188) for B_000 in the diag

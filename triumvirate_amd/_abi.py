@@ -222,6 +222,17 @@ def declare_extensions(lib):
     lib.trvg_profile_enable.argtypes = [C.c_int]
     lib.trvg_profile_enable.restype = C.c_int
     _vp = C.c_void_p
+    lib.trvg_batch_plan_create.argtypes = [P(Params), C.c_int, P(Binning), C.c_int,
+                                           C.c_longlong, C.c_int, C.c_int,
+                                           P(C.c_void_p)]
+    lib.trvg_batch_plan_create.restype = C.c_int
+    lib.trvg_batch_run.argtypes = [_vp, C.c_int, P(C.c_longlong), _dp, _dp, _dp,
+                                   P(BispecResult)]
+    lib.trvg_batch_run.restype = C.c_int
+    lib.trvg_batch_plan_info.argtypes = [_vp, P(C.c_int)]
+    lib.trvg_batch_plan_info.restype = C.c_int
+    lib.trvg_batch_plan_free.argtypes = [_vp]
+    lib.trvg_batch_plan_free.restype = C.c_int
     lib.trvg_diff_assign.argtypes = [P(Params), _vp, _vp, C.c_longlong, _vp, C.c_int]
     lib.trvg_diff_assign.restype = C.c_int
     lib.trvg_diff_assign_backward.argtypes = [P(Params), _vp, _vp, C.c_longlong,

@@ -1,0 +1,276 @@
+// batch_kernels.hpp -- kernels of the batched periodic-box bispectrum: many
+// catalogues ("mocks") of one geometry per launch.  Mock b of a batch is
+// blockIdx.y (or, in the FFT work lists, part of the field index), and every
+// per-mock quantity that varies between batches (particle offsets, counts,
+// normalisations) is read from device memory, so the whole batch is one
+// fixed launch sequence that a HIP graph can capture and replay.
+#pragma once
+
+#include "diff_kernels.hpp"
+#include "kernels.hpp"
+
+namespace trvg {
+
+// Unit-weight assignment of each mock's particles [offs[b], offs[b+1]) to
+// its mesh (real part), cell += W / V_cell (field.cpp:569-1112 rules, as
+// k_assign with WM_UNIT).
+template <int ORDER>
+static __global__ void __launch_bounds__(256)
+k_assign_batch(double2* __restrict__ mesh, long long nm,
+               const double* __restrict__ x, const double* __restrict__ y,
+               const double* __restrict__ z, const long long* __restrict__ offs,
+               Geom g) {
+  const int b = blockIdx.y;
+  double2* out = mesh + (long long)b * nm;
+  const double inv_vol_cell = 1 / g.vol_cell;
+  const long long p0 = offs[b], p1 = offs[b + 1];
+  for (long long p = p0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; p < p1;
+       p += (long long)gridDim.x * blockDim.x) {
+    const double pos[3] = {x[p], y[p], z[p]};
+    int ix[3][ORDER];
+    double wn[3][ORDER], dw[3][ORDER];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+      stencil_1d<ORDER>(pos[a], g.n[a], g.L[a], ix[a], wn[a], dw[a]);
+#pragma unroll
+    for (int il = 0; il < ORDER; il++)
+#pragma unroll
+      for (int jl = 0; jl < ORDER; jl++) {
+        const long long row = ((long long)ix[0][il] * g.n[1] + ix[1][jl]) * g.n[2];
+        const double wij = inv_vol_cell * wn[0][il] * wn[1][jl];
+#pragma unroll
+        for (int kl = 0; kl < ORDER; kl++) {
+          const long long gid = row + ix[2][kl];
+          if (gid < 0 || gid >= g.nmesh) continue;
+          unsafeAtomicAdd(reinterpret_cast<double*>(out + gid), wij * wn[2][kl]);
+        }
+      }
+  }
+}
+
+// Zero a FP64 array (graph-safe replacement of a memset).
+static __global__ void __launch_bounds__(256)
+k_fill_zero_d(double* __restrict__ a, long long n) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x)
+    a[c] = 0.;
+}
+
+static __global__ void __launch_bounds__(256)
+k_fill_zero_u64(unsigned long long* __restrict__ a, long long n) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x)
+    a[c] = 0ull;
+}
+
+// N00(k) of every mock from its dn00(k): equal except at k = 0, where the
+// unsubtracted field carries the particle count (V_cell FFT[n] = V_cell
+// FFT[n - N/V] + N delta_k0).
+static __global__ void __launch_bounds__(256)
+k_n00_from_dn(double2* __restrict__ n00, const double2* __restrict__ dn,
+              long long total, long long nm, const double* __restrict__ nmock) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < total;
+       c += (long long)gridDim.x * blockDim.x) {
+    double2 v = dn[c];
+    if (c % nm == 0) v.x += nmock[c / nm];
+    n00[c] = v;
+  }
+}
+
+// ---- batched loaders (field(q) gives the per-field loader) ---------------
+
+// (v - N_b / V) V_cell for mock b.
+struct LoadScaleShiftMocks {
+  const double2* src;
+  long long stride;
+  const double* nmock;
+  double vol, vol_cell;
+  __device__ LoadScaleShift<double> field(int b) const {
+    return LoadScaleShift<double>{src + (long long)b * stride,
+                                  -(nmock[b] / vol), vol_cell};
+  }
+};
+
+// Shell field (mock f / nbins, bin f % nbins).
+template <typename T>
+struct LoadShellMocks {
+  const double2* src;
+  long long stride;
+  AxisTables tab;
+  Geom geo;
+  const ShellBand* bands;
+  int nbins, ell, m;
+  __device__ LoadShell<T, double> field(int f) const {
+    return LoadShell<T, double>{src + (long long)(f / nbins) * stride, tab, geo,
+                                bands[f % nbins], ell, m};
+  }
+};
+
+// Compensated, 1/V-scaled field of mock b.
+template <typename T>
+struct LoadDeconvMocks {
+  const double2* src;
+  long long stride;
+  AxisTables tab;
+  double scale;
+  __device__ LoadDeconv<T, double> field(int b) const {
+    return LoadDeconv<T, double>{src + (long long)b * stride, tab, scale};
+  }
+};
+
+// Shot-noise field of mock b with Sbar = N_b (periodic box).
+struct LoadTwoptMocks {
+  const double2* fa;
+  const double2* fb;
+  long long stride;
+  AxisTables tab;
+  const double* nmock;
+  double inv_vol;
+  __device__ LoadTwopt<double> field(int b) const {
+    const long long o = (long long)b * stride;
+    return LoadTwopt<double>{fa + o, fb + o, tab, dmk(nmock[b], 0.), inv_vol};
+  }
+};
+
+// ---- batched reductions ----------------------------------------------------
+
+template <typename T>
+static __global__ void __launch_bounds__(256)
+k_pair_tiles3_batch(const c2<T>* __restrict__ FA, int na, long long sa,
+                    const c2<T>* __restrict__ FB, int nbf, long long sb,
+                    const c2<T>* __restrict__ G, long long sg, long long nmesh,
+                    const int2* __restrict__ tiles, int ntiles,
+                    double2* __restrict__ part, long long sp) {
+  const long long b = blockIdx.y;
+  k_pair_tiles3_body<T>(FA + b * sa, na, FB + b * sb, nbf, G + b * sg, nmesh,
+                        tiles, ntiles, part + b * sp);
+}
+
+template <typename V>
+static __global__ void __launch_bounds__(RC_COLS * RC_ROWS)
+k_reduce_cols_batch(const V* __restrict__ part, int nrows, int width,
+                    V* __restrict__ out) {
+  const long long b = blockIdx.y;
+  k_reduce_cols_body<V>(part + b * nrows * width, nrows, width, out + b * width);
+}
+
+// Two-point statistics of mock b (fa = dn00, fb = N00, Sbar = N_b).
+static __global__ void __launch_bounds__(RED_THREADS)
+k_twopt_fourier_batch(const double2* __restrict__ fa,
+                      const double2* __restrict__ fb, long long stride, Geom g,
+                      AxisTables tab, const double* __restrict__ nmock, int ell,
+                      int m, const double* __restrict__ edges, int nb,
+                      unsigned long long* __restrict__ counts,
+                      double* __restrict__ part) {
+  const long long b = blockIdx.y;
+  k_twopt_fourier_body<double>(fa + b * stride, fb + b * stride, g, tab,
+                               dmk(nmock[b], 0.), ell, m, edges, nb,
+                               counts + b * nb, part + b * gridDim.x * nb * 4);
+}
+
+static __global__ void __launch_bounds__(RED_THREADS)
+k_radial_hist_oct_batch(const double2* __restrict__ t, long long stride, Geom g,
+                        int la, int ma, int lb, int mb, double* __restrict__ Q,
+                        long long nq) {
+  const long long b = blockIdx.y;
+  k_radial_hist_oct_body<double>(t + b * stride, g, la, ma, lb, mb, Q + b * 2 * nq);
+}
+
+static __global__ void __launch_bounds__(RED_THREADS)
+k_shot_pairs_hist_batch(const double* __restrict__ Q, long long nq, double dr,
+                        SplineDev sa, SplineDev sb, const double* __restrict__ ka,
+                        const double* __restrict__ kb, double vol_cell,
+                        double2* __restrict__ out, int P) {
+  const long long b = blockIdx.y;
+  k_shot_pairs_hist_body(Q + b * 2 * nq, nq, dr, sa, sb, ka, kb, vol_cell,
+                         out + b * P);
+}
+
+// ---- epilogues: per-mock accumulation of the estimator terms ------------
+
+// raw[b][dv] += c V_cell (s + f conj s), s = S[b][idx[dv]] / (n1 n2), for
+// the entries dv of one tile group (ent[e] = (dv, idx)).
+static __global__ void __launch_bounds__(256)
+k_epi_pairs(double2* __restrict__ raw, int dim, const double2* __restrict__ S,
+            int width, const int2* __restrict__ ent, int nent,
+            const double* __restrict__ n1, const double* __restrict__ n2,
+            double coupling, double f_mirror, double vol_cell) {
+  const int b = blockIdx.y;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nent;
+       e += gridDim.x * blockDim.x) {
+    const int dv = ent[e].x;
+    const double2 v = S[(long long)b * width + ent[e].y];
+    const double d = n1[dv] * n2[dv];
+    const double sx = v.x / d, sy = v.y / d;
+    double2& r = raw[(long long)b * dim + dv];
+    r.x += coupling * vol_cell * (sx + f_mirror * sx);
+    r.y += coupling * vol_cell * (sy - f_mirror * sy);
+  }
+}
+
+// shot[b][dv] += c N_b (the S_ijk term of l1 = l2 = 0, Sbar = N).
+static __global__ void __launch_bounds__(256)
+k_epi_const(double2* __restrict__ shot, int dim, const double* __restrict__ nmock,
+            double coupling) {
+  const int b = blockIdx.y;
+  for (int dv = blockIdx.x * blockDim.x + threadIdx.x; dv < dim;
+       dv += gridDim.x * blockDim.x)
+    shot[(long long)b * dim + dv].x += coupling * nmock[b];
+}
+
+// Two-point term: shot[b][dv] += s + f conj s, s = c (pk[bb] - sn[bb]) with
+// per-bin means of the summed [b][nb][4] statistics (0 for an empty bin).
+static __global__ void __launch_bounds__(256)
+k_epi_twopt(double2* __restrict__ shot, int dim, const double* __restrict__ sums,
+            const unsigned long long* __restrict__ counts, int nb,
+            const int* __restrict__ bin_of, double coupling, double f_mirror) {
+  const int b = blockIdx.y;
+  for (int dv = blockIdx.x * blockDim.x + threadIdx.x; dv < dim;
+       dv += gridDim.x * blockDim.x) {
+    const int bb = bin_of[dv];
+    const unsigned long long c = counts[(long long)b * nb + bb];
+    double px = 0., py = 0., sx = 0., sy = 0.;
+    if (c != 0ull) {
+      const double n = double(c);
+      const double* s = sums + ((long long)b * nb + bb) * 4;
+      px = s[0] / n; py = s[1] / n; sx = s[2] / n; sy = s[3] / n;
+    }
+    const double vx = coupling * (px - sx), vy = coupling * (py - sy);
+    double2& r = shot[(long long)b * dim + dv];
+    r.x += vx + f_mirror * vx;
+    r.y += vy - f_mirror * vy;
+  }
+}
+
+// S_ij_k term: shot[b][dv] += phase (s + f conj s), s = c sij[b][dv].
+static __global__ void __launch_bounds__(256)
+k_epi_sij(double2* __restrict__ shot, int dim, const double2* __restrict__ sij,
+          double coupling, double f_w3j, double2 phase) {
+  const int b = blockIdx.y;
+  for (int dv = blockIdx.x * blockDim.x + threadIdx.x; dv < dim;
+       dv += gridDim.x * blockDim.x) {
+    const double2 v = sij[(long long)b * dim + dv];
+    const double sx = coupling * v.x, sy = coupling * v.y;
+    const double2 w = dmk(sx + f_w3j * sx, sy - f_w3j * sy);
+    double2& r = shot[(long long)b * dim + dv];
+    r.x += phase.x * w.x - phase.y * w.y;
+    r.y += phase.x * w.y + phase.y * w.x;
+  }
+}
+
+// raw, shot *= norm[b]; entries zeroed first by the caller's fill.
+static __global__ void __launch_bounds__(256)
+k_epi_norm(double2* __restrict__ raw, double2* __restrict__ shot, int dim,
+           const double* __restrict__ norm) {
+  const int b = blockIdx.y;
+  for (int dv = blockIdx.x * blockDim.x + threadIdx.x; dv < dim;
+       dv += gridDim.x * blockDim.x) {
+    const double f = norm[b];
+    double2& r = raw[(long long)b * dim + dv];
+    double2& s = shot[(long long)b * dim + dv];
+    r.x *= f; r.y *= f;
+    s.x *= f; s.y *= f;
+  }
+}
+
+}  // namespace trvg

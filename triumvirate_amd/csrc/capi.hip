@@ -129,6 +129,12 @@ void require_los(const double* los, const char* which) {
 
 }  // namespace
 
+// Batched periodic-box plan.
+struct trvg_batch_plan {
+  std::unique_ptr<BatchPlanBase> impl;
+  int nmultipoles;
+};
+
 // Opaque device-resident catalogue.
 struct trvg_device_catalogue {
   int device = 0;
@@ -479,6 +485,56 @@ int trvg_mesh_assign(const trvg_catalogue* c, const double* los,
 
 // In-place 3-D FFT of an interleaved complex host mesh (sign -1 forward,
 // +1 backward, unnormalised), run on the device in params->precision.
+int trvg_batch_plan_create(const trvg_params* multipoles, int nmultipoles,
+                           const trvg_binning* kbinning, int batch,
+                           long long capacity, int autotune, int use_graph,
+                           trvg_batch_plan** plan) {
+  return guarded([&] {
+    require_device();
+    check_binning(kbinning);
+    if (plan == nullptr || multipoles == nullptr || nmultipoles < 1)
+      throw Error(TRVG_ERR_PARAM, "Invalid plan arguments.");
+    if (batch < 1 || capacity < 1)
+      throw Error(TRVG_ERR_PARAM, "Batch size and capacity must be positive.");
+    std::vector<Multipole> mps;
+    trvg_params base = three_point_params(&multipoles[0], kbinning->num_bins);
+    for (int q = 0; q < nmultipoles; q++) {
+      trvg_params m = three_point_params(&multipoles[q], kbinning->num_bins);
+      for (int a = 0; a < 3; a++)
+        if (m.ngrid[a] != base.ngrid[a] || m.boxsize[a] != base.boxsize[a])
+          throw Error(TRVG_ERR_PARAM, "Multipoles of a plan must share the mesh.");
+      if (m.assignment_order != base.assignment_order || m.precision != base.precision)
+        throw Error(TRVG_ERR_PARAM, "Multipoles of a plan must share assignment and precision.");
+      mps.push_back(Multipole{m.ell1, m.ell2, m.ELL, m.shape, m.idx_bin, 1.});
+    }
+    *plan = new trvg_batch_plan{std::unique_ptr<BatchPlanBase>(
+        make_batch_plan(base, mps, *kbinning, batch, capacity, autotune, use_graph)),
+        nmultipoles};
+  });
+}
+
+int trvg_batch_run(trvg_batch_plan* plan, int ncatalogues, const long long* counts,
+                   const double* x, const double* y, const double* z,
+                   trvg_bispec_result* outs) {
+  return guarded([&] {
+    if (plan == nullptr || counts == nullptr || x == nullptr || y == nullptr ||
+        z == nullptr || outs == nullptr)
+      throw Error(TRVG_ERR_PARAM, "Null argument.");
+    plan->impl->run(ncatalogues, counts, x, y, z, outs);
+  });
+}
+
+int trvg_batch_plan_info(const trvg_batch_plan* plan, int* tuned) {
+  return guarded([&] {
+    if (plan == nullptr || tuned == nullptr) throw Error(TRVG_ERR_PARAM, "Null argument.");
+    plan->impl->info(tuned);
+  });
+}
+
+int trvg_batch_plan_free(trvg_batch_plan* plan) {
+  return guarded([&] { delete plan; });
+}
+
 int trvg_diff_assign(const trvg_params* params, const double* pos,
                      const double* weights, long long n, void* mesh,
                      int accumulate) {

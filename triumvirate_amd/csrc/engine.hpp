@@ -294,6 +294,21 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
                                      const trvg_binning& kb,
                                      const std::vector<Multipole>& mps);
 
+// Batched periodic-box bispectrum plans (engine_batch.hip).
+struct BatchPlanBase {
+  virtual ~BatchPlanBase() = default;
+  // Catalogues b < nmocks with counts[b] particles, positions concatenated;
+  // outs[b * nmultipoles + q] receives multipole q of catalogue b.
+  virtual void run(int nmocks, const long long* counts, const double* x,
+                   const double* y, const double* z, trvg_bispec_result* outs) = 0;
+  // tuned[0..4]: assignment / pair / statistics blocks per catalogue, pair
+  // row-block cap, graph captured (0/1).
+  virtual void info(int* tuned) const = 0;
+};
+BatchPlanBase* make_batch_plan(const trvg_params& p, const std::vector<Multipole>& mps,
+                               const trvg_binning& kb, int batch, long long capacity,
+                               int autotune, int use_graph);
+
 // Differentiable periodic-box B_000 diag form (engine_diff.hip).
 template <typename T>
 std::vector<double> diff_bispec_forward(Context& ctx, const trvg_binning& kb,

@@ -325,6 +325,10 @@ void Profiler::read(int kc, double* ms, long long* n, double* bytes) {
 ProfScope::ProfScope(Context& ctx, int kc, double bytes)
     : ctx_(ctx), kc_(kc), bytes_(bytes) {
   if (!Profiler::get().enabled_for(kc)) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(ctx_.stream, &cs) != hipSuccess ||
+      cs != hipStreamCaptureStatusNone)
+    return;   // no events inside a graph being captured
   TRVG_HIP(hipEventCreate(&a_));
   TRVG_HIP(hipEventRecord(a_, ctx_.stream));
 }

@@ -16,6 +16,7 @@
 #include <map>
 #include <tuple>
 
+#include "batch_kernels.hpp"
 #include "diff_kernels.hpp"
 #include "engine.hpp"
 
@@ -74,7 +75,8 @@ static void launch_strided(Context& ctx, c2<T>* data, long long S,
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   constexpr int NL = E / N;
-  size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>);
+  // Lines, twiddles, then one byte per row for the pruning flags.
+  size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>) + N;
   long long nblocks = ctx.g.nmesh / E;
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
   hipLaunchKernelGGL((fft_pass_strided<T, LOGN, EPT>), dim3(unsigned(nblocks)),
@@ -118,6 +120,243 @@ Prune make_prune(const Geom& g, double k_hi) {
   pr.n0 = g.n[0];
   pr.n1 = g.n[1];
   return pr;
+}
+
+// ---------------------------------------------------------------------------
+// Batched shell transforms: one launch per pass for a whole slab of bins,
+// over only the blocks whose lines lie in each bin's band.
+// ---------------------------------------------------------------------------
+
+// Blocks of the z and y passes holding a line inside a pruning radius, for
+// one geometry (computed once per radius).  The host predicate carries a
+// relative margin over the kernels' exactly rounded one, so every block the
+// kernels would use is listed (a listed block with no needed line only
+// writes zeros that no later pass reads).
+struct PruneLists {
+  std::vector<int> zblocks, yblocks;
+  double d_lines = 0., slabs = 0.;   // needed z lines and y slabs
+};
+
+static const PruneLists& prune_lists(const Geom& g, double kcut2, int nl_z,
+                                     int nl_y) {
+  static std::mutex mu;
+  static std::map<std::vector<double>, std::unique_ptr<PruneLists>> cache;
+  std::vector<double> key = {double(g.n[0]), double(g.n[1]), double(g.n[2]),
+                             g.dk[0], g.dk[1], kcut2, double(nl_z), double(nl_y)};
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return *it->second;
+  auto pl = std::make_unique<PruneLists>();
+  const double kc = kcut2 * (1. + 1.e-12);
+  std::vector<char> slab_in(g.n[0]), line_in((size_t)g.n[0] * g.n[1]);
+  for (int i = 0; i < g.n[0]; i++) {
+    const double kx = shifted(i, g.n[0]) * g.dk[0];
+    slab_in[i] = kx * kx < kc;
+    pl->slabs += (kx * kx < kcut2);
+    for (int j = 0; j < g.n[1]; j++) {
+      const double ky = shifted(j, g.n[1]) * g.dk[1];
+      line_in[(size_t)i * g.n[1] + j] = kx * kx + ky * ky < kc;
+      pl->d_lines += (kx * kx + ky * ky < kcut2);
+    }
+  }
+  const long long nlines = (long long)g.n[0] * g.n[1];
+  for (long long b = 0; b * nl_z < nlines; b++) {
+    bool any = false;
+    for (int l = 0; l < nl_z && !any; l++) any = line_in[b * nl_z + l];
+    if (any) pl->zblocks.push_back(int(b));
+  }
+  const long long ylines = (long long)g.n[0] * g.n[2];   // (i, k) lines
+  for (long long b = 0; b * nl_y < ylines; b++) {
+    bool any = false;
+    for (int l = 0; l < nl_y && !any; l++) any = slab_in[(b * nl_y + l) / g.n[2]];
+    if (any) pl->yblocks.push_back(int(b));
+  }
+  return *(cache[key] = std::move(pl));
+}
+
+template <typename T, int LOGN, class BatchLoader>
+static void launch_z_batch(Context& ctx, c2<T>* base, const BatchLoader& load,
+                           int sign, const Prune& pr, const double* kcut2,
+                           const int2* work, int nwork, double bytes) {
+  constexpr int EPT = FftEpt<T>::v;
+  constexpr int N = 1 << LOGN;
+  constexpr int NL = FFT_THREADS * EPT / N;
+  size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>);
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, bytes);
+  hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, BatchLoader>),
+                     dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream,
+                     base, ctx.g.nmesh, load, twiddles<T>(N, sign), ctx.g.n[1],
+                     sign, pr, kcut2, work);
+  TRVG_LAUNCH_CHECK();
+}
+
+template <typename T, int LOGN>
+static void launch_strided_batch(Context& ctx, c2<T>* base, long long S,
+                                 long long outer_stride, long long inner,
+                                 int sign, const Prune& pr, const double* kcut2,
+                                 const int2* work, int nwork, int prune_mode,
+                                 double bytes) {
+  constexpr int EPT = FftEpt<T>::v;
+  constexpr int N = 1 << LOGN;
+  constexpr int NL = FFT_THREADS * EPT / N;
+  // Lines, twiddles, then one byte per row for the pruning flags.
+  size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>) + N;
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
+  hipLaunchKernelGGL((fft_pass_strided_batch<T, LOGN, EPT>), dim3(unsigned(nwork)),
+                     dim3(FFT_THREADS), smem, ctx.stream, base, ctx.g.nmesh,
+                     twiddles<T>(N, sign), S, outer_stride, inner, sign, pr,
+                     kcut2, work, prune_mode);
+  TRVG_LAUNCH_CHECK();
+}
+
+// Transform of several fields in one launch per pass: field f of the work
+// lists lives at base + f * nmesh and loads through load.field(f); with
+// `prune`, kcut2[f] is field f's pruning radius and the lists hold only the
+// blocks that radius needs (z and y passes).
+template <typename T, class BatchLoader>
+void fft3d_batch(Context& ctx, c2<T>* base, const BatchLoader& load, int sign,
+                 bool prune, const double* kcut2, const int2* wz, int nz,
+                 const int2* wy, int ny, const int2* wx, int nx) {
+  const Geom& g = ctx.g;
+  Prune pr{};
+  pr.on = prune ? 1 : 0;
+  pr.dk0 = g.dk[0];
+  pr.dk1 = g.dk[1];
+  pr.n0 = g.n[0];
+  pr.n1 = g.n[1];
+  const double blk = double(FFT_THREADS * FftEpt<T>::v) * sizeof(c2<T>);
+  if (nz > 0)
+    TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
+        ctx, base, load, sign, pr, kcut2, wz, nz, 2. * blk * nz)));
+  if (ny > 0)
+    TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
+        ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], sign, pr, kcut2, wy,
+        ny, 1, 2. * blk * ny)));
+  if (nx > 0)
+    TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_strided_batch<T, LG>(
+        ctx, base, (long long)g.n[1] * g.n[2], 0, (long long)g.n[1] * g.n[2], sign,
+        pr, kcut2, wx, nx, 2, 2. * blk * nx)));
+}
+
+// Work lists of a shell slab transform over `ngroups` groups of `nbins`
+// fields (field f = group * nbins + bin): the bands of the bins, each
+// field's pruning radius (+inf when the band covers the whole plane), and
+// the z / y / x blocks.
+template <typename T>
+void shell_slab_work(const Geom& g, const trvg_binning& kb, int nbins,
+                     int ngroups, std::vector<ShellBand>& bands,
+                     std::vector<double>& kcut2, std::vector<int2>& wz,
+                     std::vector<int2>& wy, std::vector<int2>& wx) {
+  const int E = FFT_THREADS * FftEpt<T>::v;
+  const int nl_z = E / g.n[2], nl_y = E / g.n[1];
+  const long long nblk = g.nmesh / E;
+  bands.resize(nbins);
+  std::vector<double> kc(nbins);
+  std::vector<const PruneLists*> pls(nbins);
+  for (int q = 0; q < nbins; q++) {
+    bands[q] = shell_band(kb.bin_edges[q], kb.bin_edges[q + 1]);
+    Prune pq = make_prune(g, kb.bin_edges[q + 1]);
+    kc[q] = pq.on ? pq.kcut2 : HUGE_VAL;
+    pls[q] = &prune_lists(g, kc[q], nl_z, nl_y);
+  }
+  kcut2.clear(); wz.clear(); wy.clear(); wx.clear();
+  for (int gi = 0; gi < ngroups; gi++)
+    for (int q = 0; q < nbins; q++) {
+      const int f = gi * nbins + q;
+      kcut2.push_back(kc[q]);
+      for (int w : pls[q]->zblocks) wz.push_back(make_int2(f, w));
+      for (int w : pls[q]->yblocks) wy.push_back(make_int2(f, w));
+      for (long long w = 0; w < nblk; w++) wx.push_back(make_int2(f, int(w)));
+    }
+}
+
+template void shell_slab_work<double>(const Geom&, const trvg_binning&, int, int,
+                                      std::vector<ShellBand>&, std::vector<double>&,
+                                      std::vector<int2>&, std::vector<int2>&,
+                                      std::vector<int2>&);
+template void shell_slab_work<float>(const Geom&, const trvg_binning&, int, int,
+                                     std::vector<ShellBand>&, std::vector<double>&,
+                                     std::vector<int2>&, std::vector<int2>&,
+                                     std::vector<int2>&);
+#define TRVG_FFT3D_BATCH_INST(T, L)                                             \
+  template void fft3d_batch<T, L>(Context&, c2<T>*, const L&, int, bool,        \
+                                  const double*, const int2*, int, const int2*, \
+                                  int, const int2*, int);
+TRVG_FFT3D_BATCH_INST(double, LoadScaleShiftMocks)
+TRVG_FFT3D_BATCH_INST(double, LoadTwoptMocks)
+TRVG_FFT3D_BATCH_INST(double, LoadShellMocks<double>)
+TRVG_FFT3D_BATCH_INST(float, LoadShellMocks<float>)
+TRVG_FFT3D_BATCH_INST(double, LoadDeconvMocks<double>)
+TRVG_FFT3D_BATCH_INST(float, LoadDeconvMocks<float>)
+#undef TRVG_FFT3D_BATCH_INST
+
+// Shell fields of (ell, m) for `bins`, field q at base + q * nmesh: the
+// backward transform of y_lm(k^) src(k) / W(k) on each bin's shell, all
+// bins per pass in one launch.
+template <typename T>
+void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
+                     const trvg_binning& kb, const std::vector<int>& bins,
+                     const std::vector<int>& nmodes, int ell, int m) {
+  const Geom& g = ctx.g;
+  const int nf = int(bins.size());
+  if (nf == 0) return;
+  constexpr int EPT = FftEpt<T>::v;
+  const int E = FFT_THREADS * EPT;
+  const int nl_z = E / g.n[2], nl_y = E / g.n[1];
+  const long long nblk = g.nmesh / E;
+  std::vector<ShellBand> bands(nf);
+  std::vector<double> kc(nf);
+  std::vector<int2> wz, wy, wx;
+  const double cb = sizeof(c2<T>);
+  double bz = 0., by = 0., bx = 0.;
+  for (int q = 0; q < nf; q++) {
+    const int b = bins[q];
+    bands[q] = shell_band(kb.bin_edges[b], kb.bin_edges[b + 1]);
+    Prune pq = make_prune(g, kb.bin_edges[b + 1]);
+    kc[q] = pq.on ? pq.kcut2 : HUGE_VAL;
+    const PruneLists& pl = prune_lists(g, kc[q], nl_z, nl_y);
+    for (int w : pl.zblocks) wz.push_back(make_int2(q, w));
+    for (int w : pl.yblocks) wy.push_back(make_int2(q, w));
+    for (long long w = 0; w < nblk; w++) wx.push_back(make_int2(q, int(w)));
+    const double zw = pl.d_lines * g.n[2] * cb;
+    const double slab_vals = pl.slabs * g.n[1] * g.n[2];
+    bz += double(nmodes[b]) * sizeof(double2) + zw;
+    by += zw + slab_vals * cb;
+    bx += slab_vals * cb + double(g.nmesh) * cb;
+  }
+  DevArray<ShellBand> dbands(nf);
+  DevArray<double> dkc(nf);
+  DevArray<int2> dw(wz.size() + wy.size() + wx.size());
+  TRVG_HIP(hipMemcpyAsync(dbands.get(), bands.data(), nf * sizeof(ShellBand),
+                          hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(dkc.get(), kc.data(), nf * sizeof(double),
+                          hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(dw.get(), wz.data(), wz.size() * sizeof(int2),
+                          hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(dw.get() + wz.size(), wy.data(), wy.size() * sizeof(int2),
+                          hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(dw.get() + wz.size() + wy.size(), wx.data(),
+                          wx.size() * sizeof(int2), hipMemcpyHostToDevice, ctx.stream));
+  Prune pr{};
+  pr.on = 1;
+  pr.dk0 = g.dk[0];
+  pr.dk1 = g.dk[1];
+  pr.n0 = g.n[0];
+  pr.n1 = g.n[1];
+  LoadShellBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
+  const int2* pz = dw.get();
+  const int2* py = pz + wz.size();
+  const int2* px = py + wy.size();
+  if (!wz.empty())
+    TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
+        ctx, base, ld, +1, pr, dkc.get(), pz, int(wz.size()), bz)));
+  if (!wy.empty())
+    TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
+        ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], +1, pr, dkc.get(),
+        py, int(wy.size()), 1, by)));
+  TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_strided_batch<T, LG>(
+      ctx, base, (long long)g.n[1] * g.n[2], 0, (long long)g.n[1] * g.n[2], +1,
+      pr, dkc.get(), px, int(wx.size()), 2, bx)));
 }
 
 template <typename T, class Loader>
@@ -667,8 +906,9 @@ class SlabCache {
   explicit SlabCache(size_t capacity = 2) : cap_(capacity < 2 ? 2 : capacity) {}
 
   // Returns a slab of (l, m) covering `bins`, building it with
-  // make(field_ptr, bin) if needed; evicts the least recently used slab
-  // other than `keep` when full.
+  // make(slab_base, bins) if needed (field q of the slab at
+  // slab_base + q * nmesh); evicts the least recently used slab other than
+  // `keep` when full.
   template <class Make>
   ShellSlab<T>* get(Context& ctx, int l, int m, const std::vector<int>& bins,
                     int nb, const ShellSlab<T>* keep, Make&& make) {
@@ -693,10 +933,8 @@ class SlabCache {
     e->nfields = int(bins.size());
     const long long nm = ctx.g.nmesh;
     e->data.reset((size_t)nm * bins.size());
-    for (size_t q = 0; q < bins.size(); q++) {
-      make(e->data.get() + nm * q, bins[q]);
-      e->index[bins[q]] = int(q);
-    }
+    make(e->data.get(), bins);
+    for (size_t q = 0; q < bins.size(); q++) e->index[bins[q]] = int(q);
     entries_.push_back(std::move(e));
     return entries_.back().get();
   }
@@ -737,11 +975,8 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
   DevArray<double2> tfield(nm), dnLM, NLM;
   SlabCache<T> slabs(mps.size() > 1 ? 3 : 2);
   auto make_shell = [&](int l, int mm) {
-    return [&, l, mm](c2<T>* out_f, int b) {
-      LoadShell<T, double> ls{dn00.get(), ctx.tab, g,
-                              shell_band(kb.bin_edges[b], kb.bin_edges[b + 1]), l, mm};
-      Prune pr = make_prune(g, kb.bin_edges[b + 1]);
-      fft3d<T>(ctx, out_f, ls, +1, double(ss.nmodes[b]) * sizeof(double2), &pr);
+    return [&, l, mm](c2<T>* base, const std::vector<int>& bins) {
+      shell_slab_ifft<T>(ctx, base, dn00.get(), kb, bins, ss.nmodes, l, mm);
     };
   };
 
@@ -946,9 +1181,11 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
     fft3d<T>(ctx, G.get(), ldg, +1, double(nm) * sizeof(double2));
 
     auto make_sjl = [&](int l, int mm, const Spline& sp) {
-      return [&, l, mm](c2<T>* out_f, int b) {
-        LoadSjl<T, double> ls{dn00.get(), ctx.tab, g, sp.dev(), reff[b], l, mm};
-        fft3d<T>(ctx, out_f, ls, +1, double(nm) * sizeof(double2));
+      return [&, l, mm](c2<T>* base, const std::vector<int>& bins) {
+        for (size_t q = 0; q < bins.size(); q++) {
+          LoadSjl<T, double> ls{dn00.get(), ctx.tab, g, sp.dev(), reff[bins[q]], l, mm};
+          fft3d<T>(ctx, base + nm * q, ls, +1, double(nm) * sizeof(double2));
+        }
       };
     };
     ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr,

@@ -142,12 +142,15 @@ __device__ __forceinline__ void lds_fft_lines(c2<T>* buf, const c2<T>* W,
   (void)NLINES;
 }
 
-// First pass along z (contiguous lines), with a fused loader.
+// First pass along z (contiguous lines), with a fused loader, for block
+// `blk` of the mesh at `data`.
 // Loader: c2<T> operator()(long long g, int i, int j, int k) const.
 template <typename T, int LOGN, int EPT, class Loader>
-__global__ void __launch_bounds__(FFT_THREADS)
-fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
-           int n0, int n1, int sign, Prune pr) {
+__device__ __forceinline__ void fft_z_block(c2<T>* __restrict__ data,
+                                            const Loader& load,
+                                            const c2<T>* __restrict__ tw,
+                                            int n1, int sign, const Prune& pr,
+                                            long long blk) {
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   constexpr int NLINES = E / N;
@@ -156,7 +159,7 @@ fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* W = buf + NLINES * LS;
   const int t = threadIdx.x;
-  const long long line0 = (long long)blockIdx.x * NLINES;
+  const long long line0 = blk * NLINES;
   const long long base = line0 * N;
   if (pr.on) {
     // Skip the workgroup if none of its lines is needed (uniform exit).
@@ -168,13 +171,23 @@ fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
     if (!any) return;
   }
   for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
+  // Lines of one block share i when n1 is a multiple of NLINES (line0 is).
+  const bool one_row = (n1 % NLINES) == 0;
+  const int i0 = int(line0 / n1), j0 = int(line0 % n1);
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
     int idx = t + e * FFT_THREADS;
     long long g = base + idx;
-    long long line = line0 + idx / N;
     int k = idx % N;
-    int i = int(line / n1), j = int(line % n1);
+    int i, j;
+    if (one_row) {
+      i = i0;
+      j = j0 + idx / N;
+    } else {
+      long long line = line0 + idx / N;
+      i = int(line / n1);
+      j = int(line % n1);
+    }
     buf[(idx / N) * LS + k] = load(g, i, j, k);
   }
   __syncthreads();
@@ -184,7 +197,30 @@ fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
     int idx = t + e * FFT_THREADS;
     data[base + idx] = buf[(idx / N) * LS + idx % N];
   }
+}
+
+template <typename T, int LOGN, int EPT, class Loader>
+__global__ void __launch_bounds__(FFT_THREADS)
+fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
+           int n0, int n1, int sign, Prune pr) {
+  fft_z_block<T, LOGN, EPT>(data, load, tw, n1, sign, pr, blockIdx.x);
   (void)n0;
+}
+
+// Batched first pass over several fields (a slab of meshes `fstride` apart):
+// work[b] = (field, block) lists only the blocks with needed lines, so
+// blocks outside every band cost nothing.  BatchLoader::field(q) returns
+// the loader of field q; kcut2[q] is that field's pruning radius.
+template <typename T, int LOGN, int EPT, class BatchLoader>
+__global__ void __launch_bounds__(FFT_THREADS)
+fft_pass_z_batch(c2<T>* __restrict__ base, long long fstride, BatchLoader load,
+                 const c2<T>* __restrict__ tw, int n1, int sign, Prune pr,
+                 const double* __restrict__ kcut2, const int2* __restrict__ work) {
+  const int2 wk = work[blockIdx.x];
+  Prune p = pr;
+  p.kcut2 = kcut2[wk.x];
+  fft_z_block<T, LOGN, EPT>(base + wk.x * fstride, load.field(wk.x), tw, n1,
+                            sign, p, wk.y);
 }
 
 // Strided pass: lines of length N with element stride S.  Line `id` starts
@@ -197,10 +233,10 @@ fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
 // prune_mode (with pr.on): 1 = y pass (slab i = id / inner must satisfy P2,
 // row j read only where P1(i, j)); 2 = x pass (row i read only where P2(i)).
 template <typename T, int LOGN, int EPT>
-__global__ void __launch_bounds__(FFT_THREADS)
-fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
-                 long long S, long long outer_stride, long long inner,
-                 int sign, Prune pr, int prune_mode) {
+__device__ __forceinline__ void fft_strided_block(
+    c2<T>* __restrict__ data, const c2<T>* __restrict__ tw, long long S,
+    long long outer_stride, long long inner, int sign, const Prune& pr,
+    int prune_mode, long long blk) {
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   constexpr int NLINES = E / N;
@@ -208,28 +244,53 @@ fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
   constexpr int LS = N + 1;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* W = buf + NLINES * LS;
+  unsigned char* row_in = reinterpret_cast<unsigned char*>(W + N);   // [N]
   const int t = threadIdx.x;
-  const long long id0 = (long long)blockIdx.x * NLINES;
+  const long long id0 = blk * NLINES;
+  // The block's lines lie in one slice when inner is a multiple of NLINES
+  // (id0 is): then addresses and the slice's pruning predicates are
+  // computed once per block instead of per element.
+  const bool one_slice = (inner % NLINES) == 0;
+  const long long slice0 = id0 / inner;
+  const long long blk_base = slice0 * outer_stride + id0 % inner;
   if (pr.on && prune_mode == 1) {
     bool any = false;
-    for (int l = 0; l < NLINES && !any; l++) any = pr.p2(int((id0 + l) / inner));
+    if (one_slice) any = pr.p2(int(slice0));
+    for (int l = 0; l < NLINES && !any && !one_slice; l++)
+      any = pr.p2(int((id0 + l) / inner));
     if (!any) return;
   }
-  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
+  for (int m = t; m < N; m += FFT_THREADS) {
+    W[m] = tw[m];
+    bool need = true;
+    if (pr.on && one_slice) {
+      if (prune_mode == 1) need = pr.p1(int(slice0), m);
+      else if (prune_mode == 2) need = pr.p2(m);
+    }
+    row_in[m] = need;
+  }
+  __syncthreads();
   // Global element (n, l): line id0 + l, position n; LDS line-major [l][n].
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
     int idx = t + e * FFT_THREADS;
     int n = idx / NLINES, l = idx % NLINES;
-    long long id = id0 + l;
-    long long base = (id / inner) * outer_stride + id % inner;
-    bool need = true;
-    if (pr.on) {
-      if (prune_mode == 1) {
-        int i = int(id / inner);
-        need = pr.p2(i) && pr.p1(i, n);
-      } else if (prune_mode == 2) {
-        need = pr.p2(n);
+    long long base;
+    bool need;
+    if (one_slice) {
+      base = blk_base + l;
+      need = row_in[n];
+    } else {
+      long long id = id0 + l;
+      base = (id / inner) * outer_stride + id % inner;
+      need = true;
+      if (pr.on) {
+        if (prune_mode == 1) {
+          int i = int(id / inner);
+          need = pr.p2(i) && pr.p1(i, n);
+        } else if (prune_mode == 2) {
+          need = pr.p2(n);
+        }
       }
     }
     buf[l * LS + n] = need ? data[base + (long long)n * S] : mk<T>(T(0), T(0));
@@ -240,11 +301,40 @@ fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
   for (int e = 0; e < EPT; e++) {
     int idx = t + e * FFT_THREADS;
     int n = idx / NLINES, l = idx % NLINES;
-    long long id = id0 + l;
-    long long base = (id / inner) * outer_stride + id % inner;
-    if (pr.on && prune_mode == 1 && !pr.p2(int(id / inner))) continue;
+    long long base;
+    if (one_slice) {
+      base = blk_base + l;
+    } else {
+      long long id = id0 + l;
+      base = (id / inner) * outer_stride + id % inner;
+      if (pr.on && prune_mode == 1 && !pr.p2(int(id / inner))) continue;
+    }
     data[base + (long long)n * S] = buf[l * LS + n];
   }
+}
+
+template <typename T, int LOGN, int EPT>
+__global__ void __launch_bounds__(FFT_THREADS)
+fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
+                 long long S, long long outer_stride, long long inner,
+                 int sign, Prune pr, int prune_mode) {
+  fft_strided_block<T, LOGN, EPT>(data, tw, S, outer_stride, inner, sign, pr,
+                                  prune_mode, blockIdx.x);
+}
+
+// Batched strided pass over a slab of fields, work[b] = (field, block).
+template <typename T, int LOGN, int EPT>
+__global__ void __launch_bounds__(FFT_THREADS)
+fft_pass_strided_batch(c2<T>* __restrict__ base, long long fstride,
+                       const c2<T>* __restrict__ tw, long long S,
+                       long long outer_stride, long long inner, int sign,
+                       Prune pr, const double* __restrict__ kcut2,
+                       const int2* __restrict__ work, int prune_mode) {
+  const int2 wk = work[blockIdx.x];
+  Prune p = pr;
+  p.kcut2 = kcut2[wk.x];
+  fft_strided_block<T, LOGN, EPT>(base + wk.x * fstride, tw, S, outer_stride,
+                                  inner, sign, p, prune_mode, wk.y);
 }
 
 // Identity loader (plain in-place transform).

@@ -192,6 +192,19 @@ struct LoadShell {
   }
 };
 
+// LoadShell for field q of a batched slab transform (band of bin q).
+template <typename T, typename S = T>
+struct LoadShellBatch {
+  const c2<S>* src;
+  AxisTables tab;
+  Geom geo;
+  const ShellBand* bands;
+  int ell, m;
+  __device__ LoadShell<T, S> field(int q) const {
+    return LoadShell<T, S>{src, tab, geo, bands[q], ell, m};
+  }
+};
+
 // j_l(|k| r) y_lm(k^) src(k) / W(k) / V
 // (inv_fourier_transform_sjl_ylm_wgtd_field, field.cpp:1908-2010).
 template <typename T, typename S = T>
@@ -304,8 +317,7 @@ __device__ __forceinline__ double2 vadd(double2 a, double2 b) {
 }
 
 template <typename V>
-static __global__ void __launch_bounds__(RC_COLS * RC_ROWS)
-k_reduce_cols(const V* __restrict__ part, int nrows, int width, V* __restrict__ out) {
+__device__ __forceinline__ void k_reduce_cols_body(const V* __restrict__ part, int nrows, int width, V* __restrict__ out) {
   __shared__ V red[RC_ROWS][RC_COLS];
   const int c = threadIdx.x % RC_COLS, r = threadIdx.x / RC_COLS;
   const int col = blockIdx.x * RC_COLS + c;
@@ -319,6 +331,12 @@ k_reduce_cols(const V* __restrict__ part, int nrows, int width, V* __restrict__ 
     __syncthreads();
   }
   if (r == 0 && col < width) out[col] = red[0][c];
+}
+
+template <typename V>
+static __global__ void __launch_bounds__(RC_COLS * RC_ROWS)
+k_reduce_cols(const V* __restrict__ part, int nrows, int width, V* __restrict__ out) {
+  k_reduce_cols_body<V>(part, nrows, width, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -403,8 +421,7 @@ k_pair_reduce(const c2<T>* __restrict__ FA, int na,
 // ---------------------------------------------------------------------------
 
 template <typename T>
-static __global__ void __launch_bounds__(RED_THREADS)
-k_twopt_fourier(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
+__device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
                 Geom g, AxisTables tab, double2 sbar, int ell, int m,
                 const double* __restrict__ edges, int nb,
                 unsigned long long* __restrict__ counts,
@@ -453,6 +470,16 @@ k_twopt_fourier(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
     for (int q = 0; q < 4; q++)
       part[((long long)blockIdx.x * nb + b) * 4 + q] = sh2[4 * b + q];
   }
+}
+
+template <typename T>
+static __global__ void __launch_bounds__(RED_THREADS)
+k_twopt_fourier(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
+                Geom g, AxisTables tab, double2 sbar, int ell, int m,
+                const double* __restrict__ edges, int nb,
+                unsigned long long* __restrict__ counts,
+                double* __restrict__ part) {
+  k_twopt_fourier_body<T>(fa, fb, g, tab, sbar, ell, m, edges, nb, counts, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -537,8 +564,7 @@ k_radial_hist(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
 }
 
 // One block per pair: S_p = V_cell sum_n j_a(ka_p r_n) j_b(kb_p r_n) Q[n].
-static __global__ void __launch_bounds__(RED_THREADS)
-k_shot_pairs_hist(const double* __restrict__ Q, long long nq, double dr,
+__device__ __forceinline__ void k_shot_pairs_hist_body(const double* __restrict__ Q, long long nq, double dr,
                   SplineDev sa, SplineDev sb, const double* __restrict__ ka,
                   const double* __restrict__ kb, double vol_cell,
                   double2* __restrict__ out) {
@@ -565,6 +591,14 @@ k_shot_pairs_hist(const double* __restrict__ Q, long long nq, double dr,
     __syncthreads();
   }
   if (threadIdx.x == 0) out[p] = dmk(rs[0] * vol_cell, is[0] * vol_cell);
+}
+
+static __global__ void __launch_bounds__(RED_THREADS)
+k_shot_pairs_hist(const double* __restrict__ Q, long long nq, double dr,
+                  SplineDev sa, SplineDev sb, const double* __restrict__ ka,
+                  const double* __restrict__ kb, double vol_cell,
+                  double2* __restrict__ out) {
+  k_shot_pairs_hist_body(Q, nq, dr, sa, sb, ka, kb, vol_cell, out);
 }
 
 // General cells: direct sum over the mesh for a group of pairs.
@@ -1008,8 +1042,7 @@ namespace trvg {
 // share n = a^2 + b^2 + c^2, so each thread sums its (up to 8) mirror cells
 // and issues one atomic pair per representative.
 template <typename T>
-__global__ void __launch_bounds__(RED_THREADS)
-k_radial_hist_oct(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
+__device__ __forceinline__ void k_radial_hist_oct_body(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
                   int mb, double* __restrict__ Q) {
   const int h0 = g.n[0] / 2 + 1, h1 = g.n[1] / 2 + 1, h2 = g.n[2] / 2 + 1;
   const long long nrep = (long long)h0 * h1 * h2;
@@ -1049,6 +1082,13 @@ k_radial_hist_oct(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
     unsafeAtomicAdd(&Q[2 * nsq], sx);
     unsafeAtomicAdd(&Q[2 * nsq + 1], sy);
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS)
+k_radial_hist_oct(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
+                  int mb, double* __restrict__ Q) {
+  k_radial_hist_oct_body<T>(t, g, la, ma, lb, mb, Q);
 }
 
 }  // namespace trvg
@@ -1156,8 +1196,7 @@ namespace trvg {
 constexpr int PT3_SLOTS = 8;   // per-thread staged values per side (<= 32 fields)
 
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_pair_tiles3(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
+__device__ __forceinline__ void k_pair_tiles3_body(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
               int nbf, const c2<T>* __restrict__ G, long long nmesh,
               const int2* __restrict__ tiles, int ntiles,
               double2* __restrict__ part) {
@@ -1249,6 +1288,15 @@ k_pair_tiles3(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB
     }
     __syncthreads();
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pair_tiles3(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
+              int nbf, const c2<T>* __restrict__ G, long long nmesh,
+              const int2* __restrict__ tiles, int ntiles,
+              double2* __restrict__ part) {
+  k_pair_tiles3_body<T>(FA, na, FB, nbf, G, nmesh, tiles, ntiles, part);
 }
 
 }  // namespace trvg
