@@ -89,11 +89,14 @@ def main():
     ap.add_argument('--ngrid', type=int, default=256)
     ap.add_argument('--ndata', type=int, default=1_000_000)
     ap.add_argument('--nrand', type=int, default=5_000_000)
-    ap.add_argument('--workload', choices=('bispec', 'grad', '3pcf'),
+    ap.add_argument('--workload', choices=('bispec', 'grad', '3pcf', 'mocks'),
                     default='bispec',
                     help='bispec: B_000 + B_202 survey at 256^3 (default); '
                          'grad: differentiable B_000 fwd+bwd, 1e6 particles, '
-                         '256^3 CIC; 3pcf: zeta_000 + zeta_110 survey at 512^3')
+                         '256^3 CIC; 3pcf: zeta_000 + zeta_110 survey at 512^3; '
+                         'mocks: 256 periodic lognormal mocks, batched, 128^3')
+    ap.add_argument('--nmocks', type=int, default=256)
+    ap.add_argument('--batch', type=int, default=64)
     ap.add_argument('--skip-fp64', action='store_true',
                     help='skip the FP64 steps (profiler runs)')
     ap.add_argument('--cpu-pairs', type=int, default=6,
@@ -118,6 +121,9 @@ def main():
         return
     if args.workload == '3pcf':
         print(json.dumps(bench_3pcf(args, lib)))
+        return
+    if args.workload == 'mocks':
+        print(json.dumps(bench_mocks(args, lib)))
         return
     print(json.dumps(bench_bispec(args, lib)))
 
@@ -488,7 +494,8 @@ def bench_3pcf(args, lib):
     ng = 512
 
     def run(deg, prec):
-        p = params_struct(ng, deg, 'full', prec)
+        # full form with l1 == l2 -> upper triangle (parameters.cpp:830-850)
+        p = params_struct(ng, deg, 'triu' if deg[0] == deg[1] else 'full', prec)
         for i in range(3):
             p.boxsize[i] = 3200.
         p.assignment_order = 3
@@ -533,6 +540,107 @@ def bench_3pcf(args, lib):
                    'pairs_per_step': pairs, 'n_data': int(data.ntotal),
                    'n_rand': int(rand.ntotal)},
         'fp32_rel_err': errs, 'upload_ms': round(t_up * 1e3, 1),
+    }
+
+
+def bench_mocks(args, lib):
+    """256 periodic-box lognormal mocks (seeds 0..255, Poisson mean 3e5,
+    uneven sizes), L = 1000 Mpc/h, 128^3 TSC, B_000 (full -> upper triangle,
+    120 pairs) + B_202 (full, 225 pairs), 15 linear k-bins [0.01, 0.16]:
+    batched plan (FP32 fields, FP64 sums, autotuned, graph-replayed) against
+    the plain GPU FP64 single-catalogue path on a sample of mocks."""
+    from triumvirate_amd import _abi, mocks
+    from triumvirate_amd.batch import BatchedBispectrum
+    from triumvirate_amd.catalogue import ParticleCatalogue
+    from triumvirate_amd.dataobjs import Binning
+    from triumvirate_amd.threept import _multipole_structs
+    from triumvirate_amd.parameters import ParameterSet
+    L, ng = 1000., 128
+    t = time.perf_counter()
+    cats = [mocks.lognormal_box(300_000, L, 128, seed=s) for s in range(args.nmocks)]
+    sizes = [len(c[0]) for c in cats]
+    log(f"mocks: {len(cats)} catalogues, {min(sizes)}..{max(sizes)} particles "
+        f"({time.perf_counter() - t:.1f} s)")
+    kb = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.16, num_bins=15)
+    mps = [((0, 0, 0), 'full'), ((2, 0, 2), 'full')]
+    bb = BatchedBispectrum(L, ng, kb, mps, 'tsc', 'fp32', batch=args.batch,
+                           capacity=args.batch * max(sizes))
+    t = time.perf_counter()
+    prepared = bb.prepare(cats)                       # periodise + pack (host)
+    t_prep = time.perf_counter() - t
+    resident = bb.upload(prepared)                    # catalogues in HBM
+    bb.run(resident[:1])                              # tune + capture
+    t = time.perf_counter()
+    res = bb.run(resident)
+    t_batch = time.perf_counter() - t
+    t = time.perf_counter()
+    bb.run(prepared)                                  # from host arrays
+    t_host_inputs = time.perf_counter() - t
+    pairs = sum(r['bk_raw'].shape[0] for r in res[0]) * len(cats)
+
+    # Plain GPU FP64 path: one catalogue per call (device-resident upload
+    # excluded), both multipoles per call, on a sample of mocks.
+    ps = ParameterSet(param_dict={
+        'boxsize': {'x': L, 'y': L, 'z': L},
+        'ngrid': {'x': ng, 'y': ng, 'z': ng}, 'assignment': 'tsc',
+        'norm_convention': 'particle', 'precision': 'fp64',
+        'degrees': {'ell1': 0, 'ell2': 0, 'ELL': 0}, 'form': 'diag',
+        'range': [0.01, 0.16], 'num_bins': 15})
+    plist = _multipole_structs(ps, mps)
+    kbv = _abi.BinningView(kb.bin_edges, kb.bin_centres)
+    nsample = min(8, len(cats))
+    t_plain, err = 0., {'B000': 0., 'B202': 0.}
+    for i in range(nsample):
+        pc = ParticleCatalogue(*[a.copy() for a in cats[i]])
+        pc.periodise([L, L, L])
+        pc.compute_mean_density(boxsize=[L, L, L])
+        h = _upload(lib, pc, None)
+        x, y, z, nz, ws, wc = pc.columns()
+        view = _abi.CatalogueView(x, y, z, nz, ws, wc)
+        norm = C.c_double()
+        _check(lib, lib.trvg_calc_bispec_normalisation_from_particles(
+            C.byref(view.struct), 1., C.byref(norm)))
+        rb = [_abi.ResultBuffers(_abi.result_dim(p.shape, 15, 0), 'bispec')
+              for p in plist]
+        arr = (_abi.Params * 2)(*plist)
+        outs = (_abi.BispecResult * 2)(*[r.struct for r in rb])
+        norms = np.full(2, norm.value)
+        t = time.perf_counter()
+        _check(lib, lib.trvg_compute_bispec_multi_dev(
+            h, None, arr, 2, C.byref(kbv.struct), _abi.dptr(norms), outs))
+        if i > 0:                                      # first call warms up
+            t_plain += time.perf_counter() - t
+        lib.trvg_catalogue_free(h)
+        for q, tag in enumerate(err):
+            rb[q].struct.dim = outs[q].dim
+            ref = rb[q].as_dict()
+            for key in ('bk_raw', 'bk_shot'):
+                e = float(np.max(np.abs(res[i][q][key] - ref[key])) /
+                          np.max(np.abs(ref[key])))
+                err[tag] = max(err[tag], e)
+    per_mock_plain = t_plain / max(1, nsample - 1)
+    value = pairs / t_batch
+    return {
+        'metric': 'batched-mock B_l bin-pair evals/s (256 periodic lognormal '
+                  'mocks, 128^3, FP32 fields)',
+        'value': round(value, 1), 'unit': 'bin-pair evals/s', 'n_gpus': 1,
+        'steps': 1, 'warmup': 1, 'ms_per_step': round(t_batch * 1e3, 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic (seeded lognormal boxes)',
+        'config': {'workload': f'{len(cats)} periodic lognormal mocks (seeds '
+                               '0..N-1, Poisson mean 3e5), L=1000, 128^3 TSC, '
+                               'B_000 (triu 120) + B_202 (full 225), 15 lin '
+                               'k-bins [0.01,0.16]',
+                   'n_mocks': len(cats), 'batch': args.batch,
+                   'particles_min': min(sizes), 'particles_max': max(sizes),
+                   'pairs_per_step': pairs},
+        'mocks_per_s': round(len(cats) / t_batch, 2),
+        'host_prepare_ms': round(t_prep * 1e3, 1),
+        'from_host_arrays_ms': round(t_host_inputs * 1e3, 1),
+        'plain_fp64_ms_per_mock': round(per_mock_plain * 1e3, 3),
+        'speedup_vs_plain_fp64': round(per_mock_plain * len(cats) / t_batch, 2),
+        'fp32_rel_err_vs_plain_fp64': err,
+        'tuning': bb.tuning,
     }
 
 

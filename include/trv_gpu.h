@@ -314,6 +314,13 @@ int trvg_batch_run(trvg_batch_plan* plan, int ncatalogues,
                    const long long* counts, const double* x, const double* y,
                    const double* z, trvg_bispec_result* outs);
 
+/* As trvg_batch_run with x / y / z DEVICE pointers (catalogues already in
+ * device memory, e.g. resident for many batches). */
+int trvg_batch_run_device(trvg_batch_plan* plan, int ncatalogues,
+                          const long long* counts, const double* x,
+                          const double* y, const double* z,
+                          trvg_bispec_result* outs);
+
 /* tuned[5]: assignment, pair-reduction and binned-statistics blocks per
  * catalogue, pair row-block cap, and whether the graph is captured. */
 int trvg_batch_plan_info(const trvg_batch_plan* plan, int* tuned);

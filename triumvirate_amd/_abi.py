@@ -229,6 +229,9 @@ def declare_extensions(lib):
     lib.trvg_batch_run.argtypes = [_vp, C.c_int, P(C.c_longlong), _dp, _dp, _dp,
                                    P(BispecResult)]
     lib.trvg_batch_run.restype = C.c_int
+    lib.trvg_batch_run_device.argtypes = [_vp, C.c_int, P(C.c_longlong), _vp, _vp,
+                                          _vp, P(BispecResult)]
+    lib.trvg_batch_run_device.restype = C.c_int
     lib.trvg_batch_plan_info.argtypes = [_vp, P(C.c_int)]
     lib.trvg_batch_plan_info.restype = C.c_int
     lib.trvg_batch_plan_free.argtypes = [_vp]

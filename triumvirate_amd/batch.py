@@ -44,7 +44,7 @@ class BatchedBispectrum:
         Catalogues per batch (device memory grows linearly with it).
     capacity : int, optional
         Particles per batch; by default the largest batch of the first
-        :meth:`compute` call.
+        :meth:`compute` call (a later, larger batch rebuilds the plan).
     autotune, graph : bool
         Time candidate launch shapes on the first batch; capture the batch
         into a HIP graph and replay it.
@@ -122,34 +122,54 @@ class BatchedBispectrum:
         x, y, z, _, _, _ = pc.columns()
         return x, y, z
 
-    def compute(self, catalogues):
-        """Bispectra of every catalogue: list (per catalogue) of lists (per
-        multipole) of result dicts, as ``compute_bispec_in_gpp_box``."""
+    def prepare(self, catalogues):
+        """Periodise the catalogues (as the reference's simulation-box driver)
+        and pack them into batches: list of (counts, x, y, z)."""
         pos = [self._positions(c) for c in catalogues]
-        groups = [pos[i:i + self.batch] for i in range(0, len(pos), self.batch)]
-        need = max(sum(len(p[0]) for p in g) for g in groups)
+        out = []
+        for i in range(0, len(pos), self.batch):
+            g = pos[i:i + self.batch]
+            out.append((np.array([len(p[0]) for p in g], dtype=np.int64),
+                        np.ascontiguousarray(np.concatenate([p[0] for p in g])),
+                        np.ascontiguousarray(np.concatenate([p[1] for p in g])),
+                        np.ascontiguousarray(np.concatenate([p[2] for p in g]))))
+        return out
+
+    def upload(self, batches):
+        """Device copies (torch tensors on cuda:0) of prepared batches, for
+        :meth:`run` with catalogues resident in device memory."""
+        import torch
+        return [(c, torch.from_numpy(x).to('cuda'), torch.from_numpy(y).to('cuda'),
+                 torch.from_numpy(z).to('cuda')) for c, x, y, z in batches]
+
+    def run(self, batches):
+        """Results of prepared batches (host arrays or uploaded tensors)."""
+        need = max(int(b[0].sum()) for b in batches)
+        if self._plan is not None and need > self.capacity:
+            # A larger batch than the plan holds: rebuild it (its tuning
+            # and captured graph are redone on the next batch).
+            self._lib.trvg_batch_plan_free(self._plan)
+            self._plan = None
+            self.capacity = need
         if self._plan is None:
-            self._create(self.capacity or need)
-        elif need > self.capacity:
-            raise ValueError(f"A batch of {need} particles exceeds the plan "
-                             f"capacity {self.capacity}.")
+            self._create(max(self.capacity or 0, need))
         nmp = len(self._params)
         out = []
-        for g in groups:
-            counts = np.array([len(p[0]) for p in g], dtype=np.int64)
-            x = np.ascontiguousarray(np.concatenate([p[0] for p in g]))
-            y = np.ascontiguousarray(np.concatenate([p[1] for p in g]))
-            z = np.ascontiguousarray(np.concatenate([p[2] for p in g]))
+        for counts, x, y, z in batches:
+            nc = len(counts)
             bufs = [[_abi.ResultBuffers(_abi.result_dim(p.shape,
                                                         self.binning.num_bins,
                                                         p.idx_bin), 'bispec')
-                     for p in self._params] for _ in g]
-            arr = (_abi.BispecResult * (len(g) * nmp))(
+                     for p in self._params] for _ in range(nc)]
+            arr = (_abi.BispecResult * (nc * nmp))(
                 *[b.struct for row in bufs for b in row])
-            self._check(self._lib.trvg_batch_run(
-                self._plan, len(g),
-                counts.ctypes.data_as(C.POINTER(C.c_longlong)),
-                _abi.dptr(x), _abi.dptr(y), _abi.dptr(z), arr))
+            cp = counts.ctypes.data_as(C.POINTER(C.c_longlong))
+            if isinstance(x, np.ndarray):
+                self._check(self._lib.trvg_batch_run(
+                    self._plan, nc, cp, _abi.dptr(x), _abi.dptr(y), _abi.dptr(z), arr))
+            else:
+                self._check(self._lib.trvg_batch_run_device(
+                    self._plan, nc, cp, x.data_ptr(), y.data_ptr(), z.data_ptr(), arr))
             for i, row in enumerate(bufs):
                 res = []
                 for q, b in enumerate(row):
@@ -157,3 +177,8 @@ class BatchedBispectrum:
                     res.append(b.as_dict())
                 out.append(res)
         return out
+
+    def compute(self, catalogues):
+        """Bispectra of every catalogue: list (per catalogue) of lists (per
+        multipole) of result dicts, as ``compute_bispec_in_gpp_box``."""
+        return self.run(self.prepare(catalogues))

@@ -524,6 +524,18 @@ int trvg_batch_run(trvg_batch_plan* plan, int ncatalogues, const long long* coun
   });
 }
 
+int trvg_batch_run_device(trvg_batch_plan* plan, int ncatalogues,
+                          const long long* counts, const double* x,
+                          const double* y, const double* z,
+                          trvg_bispec_result* outs) {
+  return guarded([&] {
+    if (plan == nullptr || counts == nullptr || x == nullptr || y == nullptr ||
+        z == nullptr || outs == nullptr)
+      throw Error(TRVG_ERR_PARAM, "Null argument.");
+    plan->impl->run_device(ncatalogues, counts, x, y, z, outs);
+  });
+}
+
 int trvg_batch_plan_info(const trvg_batch_plan* plan, int* tuned) {
   return guarded([&] {
     if (plan == nullptr || tuned == nullptr) throw Error(TRVG_ERR_PARAM, "Null argument.");

@@ -301,6 +301,10 @@ struct BatchPlanBase {
   // outs[b * nmultipoles + q] receives multipole q of catalogue b.
   virtual void run(int nmocks, const long long* counts, const double* x,
                    const double* y, const double* z, trvg_bispec_result* outs) = 0;
+  // As run(), with x / y / z device pointers.
+  virtual void run_device(int nmocks, const long long* counts, const double* x,
+                          const double* y, const double* z,
+                          trvg_bispec_result* outs) = 0;
   // tuned[0..4]: assignment / pair / statistics blocks per catalogue, pair
   // row-block cap, graph captured (0/1).
   virtual void info(int* tuned) const = 0;

@@ -206,6 +206,18 @@ class BatchPlan : public BatchPlanBase {
 
   void run(int nmocks, const long long* counts, const double* x, const double* y,
            const double* z, trvg_bispec_result* outs) override {
+    run_impl(nmocks, counts, x, y, z, false, outs);
+  }
+
+  void run_device(int nmocks, const long long* counts, const double* x,
+                  const double* y, const double* z, trvg_bispec_result* outs) override {
+    run_impl(nmocks, counts, x, y, z, true, outs);
+  }
+
+ private:
+  void run_impl(int nmocks, const long long* counts, const double* x,
+                const double* y, const double* z, bool on_device,
+                trvg_bispec_result* outs) {
     if (nmocks < 1 || nmocks > B_)
       throw Error(TRVG_ERR_PARAM, "Batch holds 1 to " + std::to_string(B_) + " catalogues.");
     long long total = 0;
@@ -220,9 +232,11 @@ class BatchPlan : public BatchPlanBase {
     for (int b = 0; b < nmocks; b++)
       if (counts[b] == 0) throw Error(TRVG_ERR_DATA, "Empty catalogue in batch.");
     // Stage positions and per-mock metadata (unused slots: no particles).
-    std::memcpy(h_pos_, x, total * 8);
-    std::memcpy(h_pos_ + cap_, y, total * 8);
-    std::memcpy(h_pos_ + 2 * cap_, z, total * 8);
+    if (!on_device) {
+      std::memcpy(h_pos_, x, total * 8);
+      std::memcpy(h_pos_ + cap_, y, total * 8);
+      std::memcpy(h_pos_ + 2 * cap_, z, total * 8);
+    }
     h_offs_[0] = 0;
     const double V = ctx_.g.vol;
     for (int b = 0; b < B_; b++) {
@@ -234,9 +248,18 @@ class BatchPlan : public BatchPlanBase {
       h_meta_[B_ + b] = n ? 1. / (double(n) * nz * nz) : 0.;
     }
     hipStream_t s = ctx_.stream;
-    TRVG_HIP(hipMemcpyAsync(d_x_.get(), h_pos_, total * 8, hipMemcpyHostToDevice, s));
-    TRVG_HIP(hipMemcpyAsync(d_y_.get(), h_pos_ + cap_, total * 8, hipMemcpyHostToDevice, s));
-    TRVG_HIP(hipMemcpyAsync(d_z_.get(), h_pos_ + 2 * cap_, total * 8, hipMemcpyHostToDevice, s));
+    if (on_device) {
+      // Inputs already in device memory (written by other streams before
+      // this call returns to them: synchronise the device first).
+      TRVG_HIP(hipDeviceSynchronize());
+      TRVG_HIP(hipMemcpyAsync(d_x_.get(), x, total * 8, hipMemcpyDeviceToDevice, s));
+      TRVG_HIP(hipMemcpyAsync(d_y_.get(), y, total * 8, hipMemcpyDeviceToDevice, s));
+      TRVG_HIP(hipMemcpyAsync(d_z_.get(), z, total * 8, hipMemcpyDeviceToDevice, s));
+    } else {
+      TRVG_HIP(hipMemcpyAsync(d_x_.get(), h_pos_, total * 8, hipMemcpyHostToDevice, s));
+      TRVG_HIP(hipMemcpyAsync(d_y_.get(), h_pos_ + cap_, total * 8, hipMemcpyHostToDevice, s));
+      TRVG_HIP(hipMemcpyAsync(d_z_.get(), h_pos_ + 2 * cap_, total * 8, hipMemcpyHostToDevice, s));
+    }
     TRVG_HIP(hipMemcpyAsync(d_offs_.get(), h_offs_, (B_ + 1) * 8, hipMemcpyHostToDevice, s));
     TRVG_HIP(hipMemcpyAsync(d_nmock_.get(), h_meta_, B_ * 8, hipMemcpyHostToDevice, s));
     TRVG_HIP(hipMemcpyAsync(d_norm_.get(), h_meta_ + B_, B_ * 8, hipMemcpyHostToDevice, s));
@@ -275,7 +298,6 @@ class BatchPlan : public BatchPlanBase {
     }
   }
 
- private:
   template <typename E>
   void upload(DevArray<E>& d, const std::vector<E>& h) {
     d.reset(h.size());
@@ -286,7 +308,7 @@ class BatchPlan : public BatchPlanBase {
 
   template <typename U>
   void full_lists(int nfields, std::vector<int2>& w) {
-    constexpr int E = FFT_THREADS * (sizeof(U) == 8 ? 8 : 16);
+    constexpr int E = FFT_THREADS * FftEpt<U>::v;
     const long long nblk = nm_ / E;
     w.clear();
     for (int f = 0; f < nfields; f++)

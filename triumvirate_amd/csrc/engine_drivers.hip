@@ -48,8 +48,14 @@ static const c2<T>* twiddles(int n, int sign) {
   return d;
 }
 
-template <typename T> struct FftEpt { static constexpr int v = 8; };
-template <> struct FftEpt<float> { static constexpr int v = 16; };
+
+template <typename T, int LOGN>
+static size_t fft_smem(bool row_flags) {
+  constexpr int N = 1 << LOGN;
+  constexpr int NL = FFT_THREADS * FftEpt<T>::v / N;
+  // Lines, twiddles, then (strided passes) one byte per row of pruning flags.
+  return size_t(NL * (N + 1) + N) * sizeof(c2<T>) + (row_flags ? N : 0);
+}
 
 template <typename T, int LOGN, class Loader>
 static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign,
@@ -57,13 +63,18 @@ static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign,
   constexpr int EPT = FftEpt<T>::v;
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
-  constexpr int NL = E / N;
-  size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>);
-  long long nblocks = ctx.g.nmesh / E;
+  const long long nblocks = ctx.g.nmesh / E;
+  const size_t smem = fft_smem<T, LOGN>(false);
+  const c2<T>* tw = twiddles<T>(N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, read_bytes + write_bytes);
-  hipLaunchKernelGGL((fft_pass_z<T, LOGN, EPT, Loader>), dim3(unsigned(nblocks)),
-                     dim3(FFT_THREADS), smem, ctx.stream, data, load,
-                     twiddles<T>(N, sign), ctx.g.n[0], ctx.g.n[1], sign, pr);
+  if (ctx.g.n[1] % (E / N) == 0)
+    hipLaunchKernelGGL((fft_pass_z<T, LOGN, EPT, true, Loader>), dim3(unsigned(nblocks)),
+                       dim3(FFT_THREADS), smem, ctx.stream, data, load, tw,
+                       ctx.g.n[1], sign, pr);
+  else
+    hipLaunchKernelGGL((fft_pass_z<T, LOGN, EPT, false, Loader>), dim3(unsigned(nblocks)),
+                       dim3(FFT_THREADS), smem, ctx.stream, data, load, tw,
+                       ctx.g.n[1], sign, pr);
   TRVG_LAUNCH_CHECK();
 }
 
@@ -74,15 +85,18 @@ static void launch_strided(Context& ctx, c2<T>* data, long long S,
   constexpr int EPT = FftEpt<T>::v;
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
-  constexpr int NL = E / N;
-  // Lines, twiddles, then one byte per row for the pruning flags.
-  size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>) + N;
-  long long nblocks = ctx.g.nmesh / E;
+  const long long nblocks = ctx.g.nmesh / E;
+  const size_t smem = fft_smem<T, LOGN>(true);
+  const c2<T>* tw = twiddles<T>(N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
-  hipLaunchKernelGGL((fft_pass_strided<T, LOGN, EPT>), dim3(unsigned(nblocks)),
-                     dim3(FFT_THREADS), smem, ctx.stream, data,
-                     twiddles<T>(N, sign), S, outer_stride, inner, sign, pr,
-                     prune_mode);
+  if (inner % (E / N) == 0)
+    hipLaunchKernelGGL((fft_pass_strided<T, LOGN, EPT, true>), dim3(unsigned(nblocks)),
+                       dim3(FFT_THREADS), smem, ctx.stream, data, tw, S, outer_stride,
+                       inner, sign, pr, prune_mode);
+  else
+    hipLaunchKernelGGL((fft_pass_strided<T, LOGN, EPT, false>), dim3(unsigned(nblocks)),
+                       dim3(FFT_THREADS), smem, ctx.stream, data, tw, S, outer_stride,
+                       inner, sign, pr, prune_mode);
   TRVG_LAUNCH_CHECK();
 }
 
@@ -180,13 +194,18 @@ static void launch_z_batch(Context& ctx, c2<T>* base, const BatchLoader& load,
                            const int2* work, int nwork, double bytes) {
   constexpr int EPT = FftEpt<T>::v;
   constexpr int N = 1 << LOGN;
-  constexpr int NL = FFT_THREADS * EPT / N;
-  size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>);
+  constexpr int E = FFT_THREADS * EPT;
+  const size_t smem = fft_smem<T, LOGN>(false);
+  const c2<T>* tw = twiddles<T>(N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, bytes);
-  hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, BatchLoader>),
-                     dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream,
-                     base, ctx.g.nmesh, load, twiddles<T>(N, sign), ctx.g.n[1],
-                     sign, pr, kcut2, work);
+  if (ctx.g.n[1] % (E / N) == 0)
+    hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, true, BatchLoader>),
+                       dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream,
+                       base, ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);
+  else
+    hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, false, BatchLoader>),
+                       dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream,
+                       base, ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);
   TRVG_LAUNCH_CHECK();
 }
 
@@ -198,14 +217,18 @@ static void launch_strided_batch(Context& ctx, c2<T>* base, long long S,
                                  double bytes) {
   constexpr int EPT = FftEpt<T>::v;
   constexpr int N = 1 << LOGN;
-  constexpr int NL = FFT_THREADS * EPT / N;
-  // Lines, twiddles, then one byte per row for the pruning flags.
-  size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>) + N;
+  constexpr int E = FFT_THREADS * EPT;
+  const size_t smem = fft_smem<T, LOGN>(true);
+  const c2<T>* tw = twiddles<T>(N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
-  hipLaunchKernelGGL((fft_pass_strided_batch<T, LOGN, EPT>), dim3(unsigned(nwork)),
-                     dim3(FFT_THREADS), smem, ctx.stream, base, ctx.g.nmesh,
-                     twiddles<T>(N, sign), S, outer_stride, inner, sign, pr,
-                     kcut2, work, prune_mode);
+  if (inner % (E / N) == 0)
+    hipLaunchKernelGGL((fft_pass_strided_batch<T, LOGN, EPT, true>), dim3(unsigned(nwork)),
+                       dim3(FFT_THREADS), smem, ctx.stream, base, ctx.g.nmesh, tw, S,
+                       outer_stride, inner, sign, pr, kcut2, work, prune_mode);
+  else
+    hipLaunchKernelGGL((fft_pass_strided_batch<T, LOGN, EPT, false>), dim3(unsigned(nwork)),
+                       dim3(FFT_THREADS), smem, ctx.stream, base, ctx.g.nmesh, tw, S,
+                       outer_stride, inner, sign, pr, kcut2, work, prune_mode);
   TRVG_LAUNCH_CHECK();
 }
 

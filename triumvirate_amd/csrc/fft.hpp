@@ -21,6 +21,20 @@ namespace trvg {
 
 constexpr int FFT_THREADS = 256;
 
+// Elements per thread of a pass (a workgroup holds 256 * EPT elements).
+// Eight for both precisions: the FP32 passes are latency-bound, and the
+// smaller LDS footprint admits twice the workgroups per CU.
+template <typename T> struct FftEpt { static constexpr int v = 8; };
+
+// Workgroups are dispatched to the 8 XCDs round-robin by index; mapping
+// index b to logical block (b % 8) * (n / 8) + b / 8 gives each XCD (and
+// its L2) a contiguous range of blocks, so neighbouring blocks that share
+// cache lines run under one L2.  Identity when n is not a multiple of 8.
+__device__ __forceinline__ long long xcd_block(long long b, long long n) {
+  if (n % 8 != 0) return b;
+  return (b % 8) * (n / 8) + b / 8;
+}
+
 // Support pruning for band-limited inverse transforms: input cells exist only
 // where |k| < k_hi, so (with kcut2 = k_hi^2 (1 + 1e-9))
 //   line (i, j) of the z pass is needed only if kx^2 + ky^2 < kcut2 (P1);
@@ -145,7 +159,10 @@ __device__ __forceinline__ void lds_fft_lines(c2<T>* buf, const c2<T>* W,
 // First pass along z (contiguous lines), with a fused loader, for block
 // `blk` of the mesh at `data`.
 // Loader: c2<T> operator()(long long g, int i, int j, int k) const.
-template <typename T, int LOGN, int EPT, class Loader>
+// FAST (n1 a multiple of the lines per block, as for every mesh whose z
+// size fits NLINES lines per row): the block's lines share i, so indices
+// need no division.
+template <typename T, int LOGN, int EPT, bool FAST, class Loader>
 __device__ __forceinline__ void fft_z_block(c2<T>* __restrict__ data,
                                             const Loader& load,
                                             const c2<T>* __restrict__ tw,
@@ -161,66 +178,68 @@ __device__ __forceinline__ void fft_z_block(c2<T>* __restrict__ data,
   const int t = threadIdx.x;
   const long long line0 = blk * NLINES;
   const long long base = line0 * N;
+  const int i0 = int(line0 / n1), j0 = int(line0 % n1);
   if (pr.on) {
     // Skip the workgroup if none of its lines is needed (uniform exit).
     bool any = false;
     for (int l = 0; l < NLINES && !any; l++) {
-      long long line = line0 + l;
-      any = pr.p1(int(line / n1), int(line % n1));
+      if (FAST) {
+        any = pr.p1(i0, j0 + l);
+      } else {
+        const long long line = line0 + l;
+        any = pr.p1(int(line / n1), int(line % n1));
+      }
     }
     if (!any) return;
   }
   for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
-  // Lines of one block share i when n1 is a multiple of NLINES (line0 is).
-  const bool one_row = (n1 % NLINES) == 0;
-  const int i0 = int(line0 / n1), j0 = int(line0 % n1);
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
-    int idx = t + e * FFT_THREADS;
-    long long g = base + idx;
-    int k = idx % N;
+    const int idx = t + e * FFT_THREADS;
+    const int k = idx % N;
     int i, j;
-    if (one_row) {
+    if (FAST) {
       i = i0;
       j = j0 + idx / N;
     } else {
-      long long line = line0 + idx / N;
+      const long long line = line0 + idx / N;
       i = int(line / n1);
       j = int(line % n1);
     }
-    buf[(idx / N) * LS + k] = load(g, i, j, k);
+    buf[(idx / N) * LS + k] = load(base + idx, i, j, k);
   }
   __syncthreads();
   lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
+  c2<T>* out = data + base;
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
-    int idx = t + e * FFT_THREADS;
-    data[base + idx] = buf[(idx / N) * LS + idx % N];
+    const int idx = t + e * FFT_THREADS;
+    out[idx] = buf[(idx / N) * LS + idx % N];
   }
 }
 
-template <typename T, int LOGN, int EPT, class Loader>
+template <typename T, int LOGN, int EPT, bool FAST, class Loader>
 __global__ void __launch_bounds__(FFT_THREADS)
 fft_pass_z(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
-           int n0, int n1, int sign, Prune pr) {
-  fft_z_block<T, LOGN, EPT>(data, load, tw, n1, sign, pr, blockIdx.x);
-  (void)n0;
+           int n1, int sign, Prune pr) {
+  fft_z_block<T, LOGN, EPT, FAST>(data, load, tw, n1, sign, pr,
+                                  xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Batched first pass over several fields (a slab of meshes `fstride` apart):
 // work[b] = (field, block) lists only the blocks with needed lines, so
 // blocks outside every band cost nothing.  BatchLoader::field(q) returns
 // the loader of field q; kcut2[q] is that field's pruning radius.
-template <typename T, int LOGN, int EPT, class BatchLoader>
+template <typename T, int LOGN, int EPT, bool FAST, class BatchLoader>
 __global__ void __launch_bounds__(FFT_THREADS)
 fft_pass_z_batch(c2<T>* __restrict__ base, long long fstride, BatchLoader load,
                  const c2<T>* __restrict__ tw, int n1, int sign, Prune pr,
                  const double* __restrict__ kcut2, const int2* __restrict__ work) {
-  const int2 wk = work[blockIdx.x];
+  const int2 wk = work[xcd_block(blockIdx.x, gridDim.x)];
   Prune p = pr;
   p.kcut2 = kcut2[wk.x];
-  fft_z_block<T, LOGN, EPT>(base + wk.x * fstride, load.field(wk.x), tw, n1,
-                            sign, p, wk.y);
+  fft_z_block<T, LOGN, EPT, FAST>(base + wk.x * fstride, load.field(wk.x), tw,
+                                  n1, sign, p, wk.y);
 }
 
 // Strided pass: lines of length N with element stride S.  Line `id` starts
@@ -232,7 +251,11 @@ fft_pass_z_batch(c2<T>* __restrict__ base, long long fstride, BatchLoader load,
 //
 // prune_mode (with pr.on): 1 = y pass (slab i = id / inner must satisfy P2,
 // row j read only where P1(i, j)); 2 = x pass (row i read only where P2(i)).
-template <typename T, int LOGN, int EPT>
+//
+// FAST (inner a multiple of L): the block's lines lie in one slice, so the
+// slice's pruning flags go to LDS once per block and each thread steps a
+// pointer by a constant per element (thread t always owns line t % L).
+template <typename T, int LOGN, int EPT, bool FAST>
 __device__ __forceinline__ void fft_strided_block(
     c2<T>* __restrict__ data, const c2<T>* __restrict__ tw, long long S,
     long long outer_stride, long long inner, int sign, const Prune& pr,
@@ -240,6 +263,7 @@ __device__ __forceinline__ void fft_strided_block(
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   constexpr int NLINES = E / N;
+  constexpr int NSTEP = FFT_THREADS / NLINES;   // rows advanced per element
   extern __shared__ char smem_raw[];
   constexpr int LS = N + 1;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
@@ -247,50 +271,59 @@ __device__ __forceinline__ void fft_strided_block(
   unsigned char* row_in = reinterpret_cast<unsigned char*>(W + N);   // [N]
   const int t = threadIdx.x;
   const long long id0 = blk * NLINES;
-  // The block's lines lie in one slice when inner is a multiple of NLINES
-  // (id0 is): then addresses and the slice's pruning predicates are
-  // computed once per block instead of per element.
-  const bool one_slice = (inner % NLINES) == 0;
-  const long long slice0 = id0 / inner;
-  const long long blk_base = slice0 * outer_stride + id0 % inner;
+  if (FAST) {
+    const long long slice0 = id0 / inner;
+    if (pr.on && prune_mode == 1 && !pr.p2(int(slice0))) return;
+    for (int m = t; m < N; m += FFT_THREADS) {
+      W[m] = tw[m];
+      bool need = true;
+      if (pr.on) {
+        if (prune_mode == 1) need = pr.p1(int(slice0), m);
+        else if (prune_mode == 2) need = pr.p2(m);
+      }
+      row_in[m] = need;
+    }
+    __syncthreads();
+    const int l = t % NLINES, n0 = t / NLINES;
+    c2<T>* line = data + (slice0 * outer_stride + id0 % inner + l);
+    const long long step = (long long)NSTEP * S;
+    c2<T>* p = line + (long long)n0 * S;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const int n = n0 + e * NSTEP;
+      buf[l * LS + n] = row_in[n] ? *p : mk<T>(T(0), T(0));
+      p += step;
+    }
+    __syncthreads();
+    lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
+    p = line + (long long)n0 * S;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      *p = buf[l * LS + n0 + e * NSTEP];
+      p += step;
+    }
+    return;
+  }
   if (pr.on && prune_mode == 1) {
     bool any = false;
-    if (one_slice) any = pr.p2(int(slice0));
-    for (int l = 0; l < NLINES && !any && !one_slice; l++)
-      any = pr.p2(int((id0 + l) / inner));
+    for (int l = 0; l < NLINES && !any; l++) any = pr.p2(int((id0 + l) / inner));
     if (!any) return;
   }
-  for (int m = t; m < N; m += FFT_THREADS) {
-    W[m] = tw[m];
-    bool need = true;
-    if (pr.on && one_slice) {
-      if (prune_mode == 1) need = pr.p1(int(slice0), m);
-      else if (prune_mode == 2) need = pr.p2(m);
-    }
-    row_in[m] = need;
-  }
-  __syncthreads();
+  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
   // Global element (n, l): line id0 + l, position n; LDS line-major [l][n].
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
-    int idx = t + e * FFT_THREADS;
-    int n = idx / NLINES, l = idx % NLINES;
-    long long base;
-    bool need;
-    if (one_slice) {
-      base = blk_base + l;
-      need = row_in[n];
-    } else {
-      long long id = id0 + l;
-      base = (id / inner) * outer_stride + id % inner;
-      need = true;
-      if (pr.on) {
-        if (prune_mode == 1) {
-          int i = int(id / inner);
-          need = pr.p2(i) && pr.p1(i, n);
-        } else if (prune_mode == 2) {
-          need = pr.p2(n);
-        }
+    const int idx = t + e * FFT_THREADS;
+    const int n = idx / NLINES, l = idx % NLINES;
+    const long long id = id0 + l;
+    const long long base = (id / inner) * outer_stride + id % inner;
+    bool need = true;
+    if (pr.on) {
+      if (prune_mode == 1) {
+        const int i = int(id / inner);
+        need = pr.p2(i) && pr.p1(i, n);
+      } else if (prune_mode == 2) {
+        need = pr.p2(n);
       }
     }
     buf[l * LS + n] = need ? data[base + (long long)n * S] : mk<T>(T(0), T(0));
@@ -299,42 +332,37 @@ __device__ __forceinline__ void fft_strided_block(
   lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
-    int idx = t + e * FFT_THREADS;
-    int n = idx / NLINES, l = idx % NLINES;
-    long long base;
-    if (one_slice) {
-      base = blk_base + l;
-    } else {
-      long long id = id0 + l;
-      base = (id / inner) * outer_stride + id % inner;
-      if (pr.on && prune_mode == 1 && !pr.p2(int(id / inner))) continue;
-    }
+    const int idx = t + e * FFT_THREADS;
+    const int n = idx / NLINES, l = idx % NLINES;
+    const long long id = id0 + l;
+    const long long base = (id / inner) * outer_stride + id % inner;
+    if (pr.on && prune_mode == 1 && !pr.p2(int(id / inner))) continue;
     data[base + (long long)n * S] = buf[l * LS + n];
   }
 }
 
-template <typename T, int LOGN, int EPT>
+template <typename T, int LOGN, int EPT, bool FAST>
 __global__ void __launch_bounds__(FFT_THREADS)
 fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
                  long long S, long long outer_stride, long long inner,
                  int sign, Prune pr, int prune_mode) {
-  fft_strided_block<T, LOGN, EPT>(data, tw, S, outer_stride, inner, sign, pr,
-                                  prune_mode, blockIdx.x);
+  fft_strided_block<T, LOGN, EPT, FAST>(data, tw, S, outer_stride, inner, sign, pr,
+                                        prune_mode, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Batched strided pass over a slab of fields, work[b] = (field, block).
-template <typename T, int LOGN, int EPT>
+template <typename T, int LOGN, int EPT, bool FAST>
 __global__ void __launch_bounds__(FFT_THREADS)
 fft_pass_strided_batch(c2<T>* __restrict__ base, long long fstride,
                        const c2<T>* __restrict__ tw, long long S,
                        long long outer_stride, long long inner, int sign,
                        Prune pr, const double* __restrict__ kcut2,
                        const int2* __restrict__ work, int prune_mode) {
-  const int2 wk = work[blockIdx.x];
+  const int2 wk = work[xcd_block(blockIdx.x, gridDim.x)];
   Prune p = pr;
   p.kcut2 = kcut2[wk.x];
-  fft_strided_block<T, LOGN, EPT>(base + wk.x * fstride, tw, S, outer_stride,
-                                  inner, sign, p, prune_mode, wk.y);
+  fft_strided_block<T, LOGN, EPT, FAST>(base + wk.x * fstride, tw, S, outer_stride,
+                                        inner, sign, p, prune_mode, wk.y);
 }
 
 // Identity loader (plain in-place transform).
