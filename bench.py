@@ -99,7 +99,7 @@ def main():
     ap.add_argument('--batch', type=int, default=64)
     ap.add_argument('--skip-fp64', action='store_true',
                     help='skip the FP64 steps (profiler runs)')
-    ap.add_argument('--cpu-pairs', type=int, default=6,
+    ap.add_argument('--cpu-pairs', type=int, default=40,
                     help='bin pairs in the CPU-baseline sample (0: skip)')
     args = ap.parse_args()
     if args.gpus != 1:
