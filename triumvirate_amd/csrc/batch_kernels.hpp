@@ -134,7 +134,7 @@ struct LoadTwoptMocks {
 
 // ---- batched reductions ----------------------------------------------------
 
-template <typename T>
+template <typename T, int MODE>
 static __global__ void __launch_bounds__(256)
 k_pair_tiles3_batch(const c2<T>* __restrict__ FA, int na, long long sa,
                     const c2<T>* __restrict__ FB, int nbf, long long sb,
@@ -142,8 +142,12 @@ k_pair_tiles3_batch(const c2<T>* __restrict__ FA, int na, long long sa,
                     const int2* __restrict__ tiles, int ntiles,
                     double2* __restrict__ part, long long sp) {
   const long long b = blockIdx.y;
-  k_pair_tiles3_body<T>(FA + b * sa, na, FB + b * sb, nbf, G + b * sg, nmesh,
-                        tiles, ntiles, part + b * sp);
+  if constexpr (MODE == 2)
+    k_pair_real_body<T>(FA + b * sa, na, FB + b * sb, nbf, G + b * sg, nmesh, tiles,
+                        ntiles, part + b * sp);
+  else
+    k_pair_tiles3_body<T, MODE>(FA + b * sa, na, FB + b * sb, nbf, G + b * sg, nmesh,
+                                tiles, ntiles, part + b * sp);
 }
 
 template <typename V>

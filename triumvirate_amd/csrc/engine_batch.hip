@@ -446,10 +446,9 @@ class BatchPlan : public BatchPlanBase {
   void launch_pairs(TermPlan& tp, PairGroup& gr) {
     const c2<T>* A = slab_data_[tp.slab_a].get();
     const c2<T>* Bs = slab_data_[tp.slab_b].get();
-    const size_t smem = std::max<size_t>((size_t)(gr.nas + gr.nbs) * PT_CELLS * sizeof(c2<T>),
-                                         256 * sizeof(double2));
+    const size_t smem = pair_smem<T>(0, gr.nas, gr.nbs);
     const long long ss = (long long)nb_ * nm_;
-    hipLaunchKernelGGL((k_pair_tiles3_batch<T>), dim3(tune_.pair_blocks, B_), dim3(256),
+    hipLaunchKernelGGL((k_pair_tiles3_batch<T, 0>), dim3(tune_.pair_blocks, B_), dim3(256),
                        smem, ctx_.stream, A + (long long)gr.ra0 * nm_, gr.nas, ss,
                        Bs + (long long)gr.ca0 * nm_, gr.nbs, ss, G_.get(), nm_, nm_,
                        gr.tiles.get(), gr.nt, pair_part_.get(),

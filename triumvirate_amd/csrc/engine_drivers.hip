@@ -313,22 +313,32 @@ TRVG_FFT3D_BATCH_INST(double, LoadDeconvMocks<double>)
 TRVG_FFT3D_BATCH_INST(float, LoadDeconvMocks<float>)
 #undef TRVG_FFT3D_BATCH_INST
 
-// Shell fields of (ell, m) for `bins`, field q at base + q * nmesh: the
-// backward transform of y_lm(k^) src(k) / W(k) on each bin's shell, all
-// bins per pass in one launch.
+// Shell fields of a real source field are real for m = 0 and even l (the
+// spectrum y_l0(k^) f(k) / W(k) is Hermitian).
+static bool real_shells(int ell, int m) { return m == 0 && ell % 2 == 0; }
+
+// Shell fields of (ell, m) for `bins`: the backward transform of
+// y_lm(k^) src(k) / W(k) on each bin's shell, all bins per pass in one
+// launch.  Unpacked: field q at base + q * nmesh.  Packed (real shells:
+// real source, even l, m = 0): bins q = 2p, 2p + 1 are the real and
+// imaginary parts of base + p * nmesh.
 template <typename T>
 void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
                      const trvg_binning& kb, const std::vector<int>& bins,
-                     const std::vector<int>& nmodes, int ell, int m) {
+                     const std::vector<int>& nmodes, int ell, int m,
+                     bool packed) {
   const Geom& g = ctx.g;
   const int nf = int(bins.size());
   if (nf == 0) return;
+  const int nt = packed ? (nf + 1) / 2 : nf;    // transforms
+  const int per = packed ? 2 : 1;               // bins per transform
   constexpr int EPT = FftEpt<T>::v;
   const int E = FFT_THREADS * EPT;
   const int nl_z = E / g.n[2], nl_y = E / g.n[1];
   const long long nblk = g.nmesh / E;
-  std::vector<ShellBand> bands(nf);
-  std::vector<double> kc(nf);
+  const ShellBand none{HUGE_VAL, HUGE_VAL};
+  std::vector<ShellBand> bands((size_t)nt * per, none);
+  std::vector<double> kc(nt, 0.);
   std::vector<int2> wz, wy, wx;
   const double cb = sizeof(c2<T>);
   double bz = 0., by = 0., bx = 0.;
@@ -336,23 +346,26 @@ void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
     const int b = bins[q];
     bands[q] = shell_band(kb.bin_edges[b], kb.bin_edges[b + 1]);
     Prune pq = make_prune(g, kb.bin_edges[b + 1]);
-    kc[q] = pq.on ? pq.kcut2 : HUGE_VAL;
-    const PruneLists& pl = prune_lists(g, kc[q], nl_z, nl_y);
-    for (int w : pl.zblocks) wz.push_back(make_int2(q, w));
-    for (int w : pl.yblocks) wy.push_back(make_int2(q, w));
-    for (long long w = 0; w < nblk; w++) wx.push_back(make_int2(q, int(w)));
+    kc[q / per] = std::max(kc[q / per], pq.on ? pq.kcut2 : HUGE_VAL);
+    bz += double(nmodes[b]) * sizeof(double2);
+  }
+  for (int f = 0; f < nt; f++) {
+    const PruneLists& pl = prune_lists(g, kc[f], nl_z, nl_y);
+    for (int w : pl.zblocks) wz.push_back(make_int2(f, w));
+    for (int w : pl.yblocks) wy.push_back(make_int2(f, w));
+    for (long long w = 0; w < nblk; w++) wx.push_back(make_int2(f, int(w)));
     const double zw = pl.d_lines * g.n[2] * cb;
     const double slab_vals = pl.slabs * g.n[1] * g.n[2];
-    bz += double(nmodes[b]) * sizeof(double2) + zw;
+    bz += zw;
     by += zw + slab_vals * cb;
     bx += slab_vals * cb + double(g.nmesh) * cb;
   }
-  DevArray<ShellBand> dbands(nf);
-  DevArray<double> dkc(nf);
+  DevArray<ShellBand> dbands(bands.size());
+  DevArray<double> dkc(nt);
   DevArray<int2> dw(wz.size() + wy.size() + wx.size());
-  TRVG_HIP(hipMemcpyAsync(dbands.get(), bands.data(), nf * sizeof(ShellBand),
+  TRVG_HIP(hipMemcpyAsync(dbands.get(), bands.data(), bands.size() * sizeof(ShellBand),
                           hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dkc.get(), kc.data(), nf * sizeof(double),
+  TRVG_HIP(hipMemcpyAsync(dkc.get(), kc.data(), nt * sizeof(double),
                           hipMemcpyHostToDevice, ctx.stream));
   TRVG_HIP(hipMemcpyAsync(dw.get(), wz.data(), wz.size() * sizeof(int2),
                           hipMemcpyHostToDevice, ctx.stream));
@@ -366,13 +379,20 @@ void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
   pr.dk1 = g.dk[1];
   pr.n0 = g.n[0];
   pr.n1 = g.n[1];
-  LoadShellBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
   const int2* pz = dw.get();
   const int2* py = pz + wz.size();
   const int2* px = py + wy.size();
-  if (!wz.empty())
-    TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
-        ctx, base, ld, +1, pr, dkc.get(), pz, int(wz.size()), bz)));
+  if (!wz.empty()) {
+    if (packed) {
+      LoadShellPairBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
+      TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
+          ctx, base, ld, +1, pr, dkc.get(), pz, int(wz.size()), bz)));
+    } else {
+      LoadShellBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
+      TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
+          ctx, base, ld, +1, pr, dkc.get(), pz, int(wz.size()), bz)));
+    }
+  }
   if (!wy.empty())
     TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
         ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], +1, pr, dkc.get(),
@@ -602,10 +622,23 @@ ShellStats shell_stats(Context& ctx, const trvg_binning& kb) {
   return s;
 }
 
+// S_p = sum_x F_a(x) F_b(x) G(x) for pairs p = (a, b) of logical fields.
+// A side / B side packed: real fields stored two per complex mesh (logical
+// field 2q + c is component c of mesh q); g_real: G is real.
 template <typename T>
 static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
-                                     const c2<T>* FB, int nbf, const c2<T>* G,
+                                     bool a_packed, const c2<T>* FB, int nbf,
+                                     bool b_packed, const c2<T>* G, bool g_real,
                                      const std::vector<int2>& pairs) {
+  if (a_packed && !b_packed) {
+    // The product is symmetric in (a, b): put the packed side second.
+    std::vector<int2> sw(pairs.size());
+    for (size_t d = 0; d < pairs.size(); d++) sw[d] = make_int2(pairs[d].y, pairs[d].x);
+    return pair_reduce<T>(ctx, FB, nbf, false, FA, na, true, G, g_real, sw);
+  }
+  if (a_packed && !g_real)
+    throw Error(TRVG_ERR_INTERNAL, "Real shell pairs need a real G field.");
+  const int mode = (a_packed && b_packed) ? 2 : (b_packed ? 1 : 0);
   const long long nm = ctx.g.nmesh;
   const int P = int(pairs.size());
   const int rbn = (na + 3) / 4, cbn = (nbf + 3) / 4;
@@ -637,15 +670,23 @@ static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
       TRVG_HIP(hipMemcpyAsync(dt.get(), tl.data(), nt * sizeof(int2),
                               hipMemcpyHostToDevice, ctx.stream));
       DevArray<double2> part((size_t)grid * nt * 16), sum((size_t)nt * 16);
-      size_t smem = std::max<size_t>((size_t)(nas + nbs) * PT_CELLS * sizeof(c2<T>),
-                                     256 * sizeof(double2));
+      const size_t smem = pair_smem<T>(mode, nas, nbs);
       {
-        ProfScope ps(ctx, KC_PAIR_REDUCE,
-                     double(nas + nbs + 1) * nm * sizeof(c2<T>));
-        hipLaunchKernelGGL((k_pair_tiles3<T>), dim3(grid), dim3(256), smem,
-                           ctx.stream, FA + (long long)ra0 * nm, nas,
-                           FB + (long long)ca0 * nm, nbs, G, nm, dt.get(), nt,
-                           part.get());
+        const double ma = mode == 2 ? (nas + 1) / 2 : nas;
+        const double mb = mode >= 1 ? (nbs + 1) / 2 : nbs;
+        ProfScope ps(ctx, KC_PAIR_REDUCE, (ma + mb + 1.) * nm * sizeof(c2<T>));
+        // Packed sides start at mesh ra0 / 2 (group starts are multiples of 4).
+        const c2<T>* fa = FA + (long long)(mode == 2 ? ra0 / 2 : ra0) * nm;
+        const c2<T>* fb = FB + (long long)(mode >= 1 ? ca0 / 2 : ca0) * nm;
+        if (mode == 2)
+          hipLaunchKernelGGL((k_pair_tiles3<T, 2>), dim3(grid), dim3(256), smem,
+                             ctx.stream, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get());
+        else if (mode == 1)
+          hipLaunchKernelGGL((k_pair_tiles3<T, 1>), dim3(grid), dim3(256), smem,
+                             ctx.stream, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get());
+        else
+          hipLaunchKernelGGL((k_pair_tiles3<T, 0>), dim3(grid), dim3(256), smem,
+                             ctx.stream, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get());
         TRVG_LAUNCH_CHECK();
       }
       reduce_cols(ctx, part.get(), int(grid), nt * 16, sum.get());
@@ -914,6 +955,7 @@ struct ShellSlab {
   int ell = -1, m = 0;
   std::vector<int> index;   // bin -> field index in `data`, -1 if absent
   int nfields = 0;
+  bool packed = false;      // real fields, two per complex mesh
   DevArray<c2<T>> data;
   bool covers(int l, int mm, const std::vector<int>& bins) const {
     if (ell != l || m != mm) return false;
@@ -930,11 +972,11 @@ class SlabCache {
 
   // Returns a slab of (l, m) covering `bins`, building it with
   // make(slab_base, bins) if needed (field q of the slab at
-  // slab_base + q * nmesh); evicts the least recently used slab other than
-  // `keep` when full.
+  // slab_base + q * nmesh, or component q % 2 of mesh q / 2 when packed);
+  // evicts the least recently used slab other than `keep` when full.
   template <class Make>
   ShellSlab<T>* get(Context& ctx, int l, int m, const std::vector<int>& bins,
-                    int nb, const ShellSlab<T>* keep, Make&& make) {
+                    int nb, const ShellSlab<T>* keep, bool packed, Make&& make) {
     for (size_t i = 0; i < entries_.size(); i++)
       if (entries_[i]->covers(l, m, bins)) {
         auto e = std::move(entries_[i]);      // move to the back (MRU)
@@ -954,8 +996,9 @@ class SlabCache {
     e->m = m;
     e->index.assign(nb, -1);
     e->nfields = int(bins.size());
+    e->packed = packed;
     const long long nm = ctx.g.nmesh;
-    e->data.reset((size_t)nm * bins.size());
+    e->data.reset((size_t)nm * (packed ? (bins.size() + 1) / 2 : bins.size()));
     make(e->data.get(), bins);
     for (size_t q = 0; q < bins.size(); q++) e->index[bins[q]] = int(q);
     entries_.push_back(std::move(e));
@@ -999,7 +1042,8 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
   SlabCache<T> slabs(mps.size() > 1 ? 3 : 2);
   auto make_shell = [&](int l, int mm) {
     return [&, l, mm](c2<T>* base, const std::vector<int>& bins) {
-      shell_slab_ifft<T>(ctx, base, dn00.get(), kb, bins, ss.nmodes, l, mm);
+      shell_slab_ifft<T>(ctx, base, dn00.get(), kb, bins, ss.nmodes, l, mm,
+                         real_shells(l, mm));
     };
   };
 
@@ -1048,14 +1092,19 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
 
       // Shell fields F_a (l1, m1) for the row bins, F_b (l2, m2) for columns.
       ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr,
+                                  real_shells(p.ell1, tm.m1),
                                   make_shell(p.ell1, tm.m1));
       ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A,
+                                  real_shells(p.ell2, tm.m2),
                                   make_shell(p.ell2, tm.m2));
       std::vector<int2> pairs(dim);
       for (int dv = 0; dv < dim; dv++)
         pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
-      std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields,
-                                           B->data.get(), B->nfields, G.get(), pairs);
+      // G_LM is real for M = 0 (real weights Y_L0) and in a periodic box.
+      const bool g_real = !survey || tm.M == 0;
+      std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields, A->packed,
+                                           B->data.get(), B->nfields, B->packed,
+                                           G.get(), g_real, pairs);
       for (int dv = 0; dv < dim; dv++) {
         cplx sv = S[dv] / (double(ss.nmodes[ra[dv]]) * double(ss.nmodes[cb[dv]]));
         out.raw[dv] += tm.coupling * g.vol_cell * (sv + tm.f_mirror * std::conj(sv));
@@ -1211,15 +1260,16 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
         }
       };
     };
-    ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr,
+    ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr, false,
                                 make_sjl(p.ell1, tm.m1, sja));
-    ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A,
+    ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A, false,
                                 make_sjl(p.ell2, tm.m2, sjb));
     std::vector<int2> pairs(dim);
     for (int dv = 0; dv < dim; dv++)
       pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
-    std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields,
-                                         B->data.get(), B->nfields, G.get(), pairs);
+    std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields, false,
+                                         B->data.get(), B->nfields, false, G.get(),
+                                         false, pairs);
     for (int dv = 0; dv < dim; dv++) {
       cplx s = S[dv];
       out.raw[dv] += tm.coupling * g.vol_cell * factor_phase

@@ -192,6 +192,44 @@ struct LoadShell {
   }
 };
 
+// Two real shells in one transform: for a real source field, even l and
+// m = 0 the shell field of each bin is real, so the transform of
+// S_0(k) + i S_1(k) returns F_0(x) + i F_1(x).  The bins are disjoint, so
+// each mode belongs to at most one of the two bands.
+template <typename T, typename S = T>
+struct LoadShellPair {
+  const c2<S>* src;
+  AxisTables tab;
+  Geom geo;
+  ShellBand band0, band1;
+  int ell, m;
+  __device__ c2<T> operator()(long long g, int i, int j, int k) const {
+    double kx = grid_coord(i, geo.n[0], geo.dk[0]);
+    double ky = grid_coord(j, geo.n[1], geo.dk[1]);
+    double kz = grid_coord(k, geo.n[2], geo.dk[2]);
+    const double k2 = mag2_exact(kx, ky, kz);
+    const bool in0 = band0.contains(k2);
+    if (!in0 && !band1.contains(k2)) return mk<T>(T(0), T(0));
+    c2<S> v = src[g];
+    double iw = 1. / window_at(tab, i, j, k);
+    double2 fk = dmk(double(v.x) * iw, double(v.y) * iw);
+    double2 r = dmul(ylm_reduced_dev(ell, m, kx, ky, kz), fk);
+    return in0 ? mk<T>(T(r.x), T(r.y)) : mk<T>(T(-r.y), T(r.x));
+  }
+};
+
+template <typename T, typename S = T>
+struct LoadShellPairBatch {
+  const c2<S>* src;
+  AxisTables tab;
+  Geom geo;
+  const ShellBand* bands;   // two per packed field
+  int ell, m;
+  __device__ LoadShellPair<T, S> field(int p) const {
+    return LoadShellPair<T, S>{src, tab, geo, bands[2 * p], bands[2 * p + 1], ell, m};
+  }
+};
+
 // LoadShell for field q of a batched slab transform (band of bin q).
 template <typename T, typename S = T>
 struct LoadShellBatch {
@@ -340,79 +378,6 @@ k_reduce_cols(const V* __restrict__ part, int nrows, int width, V* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Pair product-reduction  S_p = sum_x FA[ia_p](x) FB[ib_p](x) G(x)
-// (threept.cpp:493-517).  A chunk of PR_CELLS cells of every needed field is
-// staged in LDS (cell-major), H = FB * G is formed once per chunk, and each
-// thread accumulates up to PR_MAXP pairs in FP64 registers across all chunks
-// the block visits.  Output: per-block partials [gridDim.x][P] (double2).
-// ---------------------------------------------------------------------------
-
-constexpr int PR_CELLS = 32;
-constexpr int PR_MAXP = 8;
-
-template <typename T>
-static __global__ void __launch_bounds__(RED_THREADS)
-k_pair_reduce(const c2<T>* __restrict__ FA, int na,
-              const c2<T>* __restrict__ FB, int nbf,
-              const c2<T>* __restrict__ G, long long nmesh,
-              const int2* __restrict__ pairs, int P,
-              double2* __restrict__ part) {
-  extern __shared__ double2 sh[];
-  double2* sa = sh;                          // [PR_CELLS][na]
-  double2* sb = sh + PR_CELLS * na;          // [PR_CELLS][nbf]
-  double2 acc[PR_MAXP];
-  int ia[PR_MAXP], ib[PR_MAXP];
-#pragma unroll
-  for (int s = 0; s < PR_MAXP; s++) {
-    acc[s] = dmk(0., 0.);
-    int p = threadIdx.x + s * RED_THREADS;
-    ia[s] = p < P ? pairs[p].x : 0;
-    ib[s] = p < P ? pairs[p].y : 0;
-  }
-  const long long nchunks = (nmesh + PR_CELLS - 1) / PR_CELLS;
-  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    const long long c0 = ch * PR_CELLS;
-    __syncthreads();
-    for (int e = threadIdx.x; e < PR_CELLS * na; e += RED_THREADS) {
-      int f = e / PR_CELLS, c = e % PR_CELLS;
-      long long gidx = c0 + c;
-      c2<T> v = gidx < nmesh ? FA[(long long)f * nmesh + gidx] : mk<T>(T(0), T(0));
-      sa[c * na + f] = dmk(v.x, v.y);
-    }
-    for (int e = threadIdx.x; e < PR_CELLS * nbf; e += RED_THREADS) {
-      int f = e / PR_CELLS, c = e % PR_CELLS;
-      long long gidx = c0 + c;
-      double2 h = dmk(0., 0.);
-      if (gidx < nmesh) {
-        c2<T> v = FB[(long long)f * nmesh + gidx];
-        c2<T> gg = G[gidx];
-        h = dmul(dmk(v.x, v.y), dmk(gg.x, gg.y));
-      }
-      sb[c * nbf + f] = h;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < PR_MAXP; s++) {
-      if (threadIdx.x + s * RED_THREADS >= P) break;
-      double2 a = acc[s];
-      for (int c = 0; c < PR_CELLS; c++) {
-        double2 x = sa[c * na + ia[s]];
-        double2 y = sb[c * nbf + ib[s]];
-        a.x += x.x * y.x - x.y * y.y;
-        a.y += x.x * y.y + x.y * y.x;
-      }
-      acc[s] = a;
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < PR_MAXP; s++) {
-    int p = threadIdx.x + s * RED_THREADS;
-    if (p < P) part[(long long)blockIdx.x * P + p] = acc[s];
-  }
-}
-
-
-// ---------------------------------------------------------------------------
 // Two-point statistics in Fourier space with the reference's two-level
 // binning (field.cpp:2511-2703): fine bin ik = int(|k| / 1e-5) (< 1e6);
 // the fine bin enters coarse bin b when edges[b] <= ik * 1e-5 < edges[b+1].
@@ -538,30 +503,6 @@ k_3pcf_shot_bins(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
 // through n = i^2 + j^2 + k^2, so the angular-weighted field is first
 // histogrammed into Q[n] and each pair then costs one pass over n.
 // ---------------------------------------------------------------------------
-
-template <typename T>
-static __global__ void __launch_bounds__(RED_THREADS)
-k_radial_hist(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
-              int mb, double* __restrict__ Q) {
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-       c < g.nmesh; c += (long long)gridDim.x * blockDim.x) {
-    int k = int(c % g.n[2]);
-    long long ij = c / g.n[2];
-    int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
-    int si = shifted(i, g.n[0]), sj = shifted(j, g.n[1]), sk = shifted(k, g.n[2]);
-    long long nsq = (long long)si * si + (long long)sj * sj + (long long)sk * sk;
-    double rx = si * g.dr[0], ry = sj * g.dr[1], rz = sk * g.dr[2];
-    c2<T> tv = t[c];
-    double2 v = dmk(tv.x, tv.y);
-    if (!(la == 0 && ma == 0 && lb == 0 && mb == 0)) {
-      double2 ya = ylm_reduced_dev(la, ma, rx, ry, rz);
-      double2 yb = ylm_reduced_dev(lb, mb, rx, ry, rz);
-      v = dmul(v, dmul(ya, yb));
-    }
-    unsafeAtomicAdd(&Q[2 * nsq], v.x);
-    unsafeAtomicAdd(&Q[2 * nsq + 1], v.y);
-  }
-}
 
 // One block per pair: S_p = V_cell sum_n j_a(ka_p r_n) j_b(kb_p r_n) Q[n].
 __device__ __forceinline__ void k_shot_pairs_hist_body(const double* __restrict__ Q, long long nq, double dr,
@@ -946,98 +887,6 @@ namespace trvg {
 constexpr int PT_CELLS = 64;
 constexpr int PT_MAXT = 32;
 
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_pair_tiles(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
-             int nbf, const c2<T>* __restrict__ G, long long nmesh,
-             const int2* __restrict__ tiles, int ntiles,
-             double2* __restrict__ part) {
-  extern __shared__ double2 shp[];
-  double2* sa = shp;                          // [na][PT_CELLS]
-  double2* sh = shp + (size_t)na * PT_CELLS;  // [nbf][PT_CELLS]
-  const int ngroups = 256 / ntiles;
-  const int tile = threadIdx.x % ntiles;
-  const int group = threadIdx.x / ntiles;
-  const bool active = group < ngroups;
-  const int2 t0 = tiles[tile];
-  double ar[16], ai[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) ar[q] = ai[q] = 0.;
-  int ia[4], ib[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    ia[q] = min(t0.x + q, na - 1);
-    ib[q] = min(t0.y + q, nbf - 1);
-  }
-  const long long nchunks = (nmesh + PT_CELLS - 1) / PT_CELLS;
-  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    const long long c0 = ch * PT_CELLS;
-    __syncthreads();
-    for (int e = threadIdx.x; e < na * PT_CELLS; e += 256) {
-      int f = e / PT_CELLS, c = e % PT_CELLS;
-      long long gi = c0 + c;
-      double2 v = dmk(0., 0.);
-      if (gi < nmesh) {
-        c2<T> a = FA[(long long)f * nmesh + gi];
-        v = dmk(a.x, a.y);
-      }
-      sa[e] = v;
-    }
-    for (int e = threadIdx.x; e < nbf * PT_CELLS; e += 256) {
-      int f = e / PT_CELLS, c = e % PT_CELLS;
-      long long gi = c0 + c;
-      double2 h = dmk(0., 0.);
-      if (gi < nmesh) {
-        c2<T> b = FB[(long long)f * nmesh + gi];
-        c2<T> gg = G[gi];
-        h = dmul(dmk(b.x, b.y), dmk(gg.x, gg.y));
-      }
-      sh[e] = h;
-    }
-    __syncthreads();
-    if (active) {
-      for (int c = group; c < PT_CELLS; c += ngroups) {
-        double2 x[4], y[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          x[q] = sa[ia[q] * PT_CELLS + c];
-          y[q] = sh[ib[q] * PT_CELLS + c];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-#pragma unroll
-          for (int j = 0; j < 4; j++) {
-            ar[i * 4 + j] += x[i].x * y[j].x - x[i].y * y[j].y;
-            ai[i * 4 + j] += x[i].x * y[j].y + x[i].y * y[j].x;
-          }
-        }
-      }
-    }
-  }
-  // Reduce over groups through LDS (reusing the staging buffer).
-  __syncthreads();
-  double2* red = shp;  // [256][16] double2 = 64 KiB at most; staging >= that
-  if (active) {
-#pragma unroll
-    for (int q = 0; q < 16; q++) red[threadIdx.x * 16 + q] = dmk(ar[q], ai[q]);
-  }
-  __syncthreads();
-  for (int o = threadIdx.x; o < ntiles * 16; o += 256) {
-    int tl = o / 16, q = o % 16;
-    double sx = 0., sy = 0.;
-    for (int gr = 0; gr < ngroups; gr++) {
-      double2 v = red[(gr * ntiles + tl) * 16 + q];
-      sx += v.x;
-      sy += v.y;
-    }
-    part[(long long)blockIdx.x * ntiles * 16 + o] = dmk(sx, sy);
-  }
-}
-
-}  // namespace trvg
-
-namespace trvg {
-
 // Radial histogram over octant representatives: the cells (+-a, +-b, +-c)
 // share n = a^2 + b^2 + c^2, so each thread sums its (up to 8) mirror cells
 // and issues one atomic pair per representative.
@@ -1095,107 +944,27 @@ k_radial_hist_oct(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
 
 namespace trvg {
 
-// Pair reduction, v2 (see k_pair_tiles for the data flow).  Staging is in the
-// field precision T; threads are mapped group-major (the threads of one tile
-// read consecutive cells); each thread forms its chunk partial in T and
-// promotes it to FP64 accumulators once per chunk; the cross-group
-// reduction goes through a 4 KiB buffer, 16 rounds.
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_pair_tiles2(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
-              int nbf, const c2<T>* __restrict__ G, long long nmesh,
-              const int2* __restrict__ tiles, int ntiles,
-              double2* __restrict__ part) {
-  extern __shared__ char sm2_raw[];
-  c2<T>* sa = reinterpret_cast<c2<T>*>(sm2_raw);           // [na][PT_CELLS]
-  c2<T>* sh = sa + (size_t)na * PT_CELLS;                   // [nbf][PT_CELLS]
-  const int ngroups = 256 / ntiles;
-  const int group = threadIdx.x % ngroups;
-  const int tile = threadIdx.x / ngroups;
-  const bool active = tile < ntiles;
-  const int2 t0 = tiles[active ? tile : 0];
-  double ar[16], ai[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) ar[q] = ai[q] = 0.;
-  int ia[4], ib[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    ia[q] = min(t0.x + q, na - 1) * PT_CELLS;
-    ib[q] = min(t0.y + q, nbf - 1) * PT_CELLS;
-  }
-  const long long nchunks = (nmesh + PT_CELLS - 1) / PT_CELLS;
-  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    const long long c0 = ch * PT_CELLS;
-    __syncthreads();
-    for (int e = threadIdx.x; e < na * PT_CELLS; e += 256) {
-      int f = e / PT_CELLS, c = e % PT_CELLS;
-      long long gi = c0 + c;
-      sa[e] = gi < nmesh ? FA[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
-    }
-    for (int e = threadIdx.x; e < nbf * PT_CELLS; e += 256) {
-      int f = e / PT_CELLS, c = e % PT_CELLS;
-      long long gi = c0 + c;
-      c2<T> h = mk<T>(T(0), T(0));
-      if (gi < nmesh) h = cmul<T>(FB[(long long)f * nmesh + gi], G[gi]);
-      sh[e] = h;
-    }
-    __syncthreads();
-    if (active) {
-      T pr[16], pi[16];
-#pragma unroll
-      for (int q = 0; q < 16; q++) pr[q] = pi[q] = T(0);
-      for (int c = group; c < PT_CELLS; c += ngroups) {
-        c2<T> x[4], y[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          x[q] = sa[ia[q] + c];
-          y[q] = sh[ib[q] + c];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-#pragma unroll
-          for (int j = 0; j < 4; j++) {
-            pr[i * 4 + j] += x[i].x * y[j].x - x[i].y * y[j].y;
-            pi[i * 4 + j] += x[i].x * y[j].y + x[i].y * y[j].x;
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        ar[q] += double(pr[q]);
-        ai[q] += double(pi[q]);
-      }
-    }
-  }
-  __syncthreads();
-  double2* red = reinterpret_cast<double2*>(sm2_raw);   // [256]
-  for (int q = 0; q < 16; q++) {
-    red[threadIdx.x] = active ? dmk(ar[q], ai[q]) : dmk(0., 0.);
-    __syncthreads();
-    if (threadIdx.x < ntiles) {
-      double sx = 0., sy = 0.;
-      for (int gr = 0; gr < ngroups; gr++) {
-        double2 v = red[threadIdx.x * ngroups + gr];
-        sx += v.x;
-        sy += v.y;
-      }
-      part[(long long)blockIdx.x * ntiles * 16 + threadIdx.x * 16 + q] = dmk(sx, sy);
-    }
-    __syncthreads();
-  }
-}
-
 }  // namespace trvg
 
 namespace trvg {
 
-// Pair reduction, v3: as v2, with the next chunk's global loads issued into
-// registers before the current chunk is reduced (software pipelining).
-// Staging map: thread t owns cell c = t % PT_CELLS of the chunk and fields
-// f = t / PT_CELLS + k * (256 / PT_CELLS), so each thread loads G[c] once.
+// Register-tiled pair product-reduction
+//   S[ia][ib] = sum_x FA[ia](x) FB[ib](x) G(x).
+// A launch covers a list of 4x4 tiles of the (ia, ib) grid (up to PT_MAXT
+// tiles).  Each block stages PT_CELLS cells of every field in LDS (one HBM
+// read per field and cell), forms H = FB * G once, and thread (tile, group)
+// accumulates its 16 pair sums over its cells of every chunk the block
+// visits: in T within a chunk, promoted to FP64 accumulators once per
+// chunk.  The next chunk's global loads are issued into registers before
+// the current chunk is reduced.  Staging map: thread t owns cell
+// c = t % PT_CELLS of the chunk and fields f = t / PT_CELLS + k * (256 /
+// PT_CELLS), so each thread loads G[c] once.  Output: per-block partials
+// [gridDim.x][ntiles * 16].
 constexpr int PT3_SLOTS = 8;   // per-thread staged values per side (<= 32 fields)
 
-template <typename T>
+// MODE 0: complex fields.  MODE 1: the B fields are real and packed two per
+// complex mesh (logical field 2p + c is component c of mesh p).
+template <typename T, int MODE>
 __device__ __forceinline__ void k_pair_tiles3_body(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
               int nbf, const c2<T>* __restrict__ G, long long nmesh,
               const int2* __restrict__ tiles, int ntiles,
@@ -1222,14 +991,18 @@ __device__ __forceinline__ void k_pair_tiles3_body(const c2<T>* __restrict__ FA,
   const int sf = threadIdx.x / PT_CELLS;   // first staged field
   c2<T> ra[PT3_SLOTS], rb[PT3_SLOTS], rg;
   const long long nchunks = (nmesh + PT_CELLS - 1) / PT_CELLS;
+  constexpr bool A_PACKED = false, B_PACKED = (MODE == 1);
+  // Staged meshes per side: packed sides hold two logical fields per mesh.
+  const int ma = A_PACKED ? (na + 1) / 2 : na;
+  const int mb = B_PACKED ? (nbf + 1) / 2 : nbf;
   auto fetch = [&](long long ch) {
     const long long gi = ch * PT_CELLS + sc;
     const bool ok = (ch < nchunks) && (gi < nmesh);
 #pragma unroll
     for (int k = 0; k < PT3_SLOTS; k++) {
       const int f = sf + k * FSTEP;
-      ra[k] = (ok && f < na) ? FA[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
-      rb[k] = (ok && f < nbf) ? FB[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
+      ra[k] = (ok && f < ma) ? FA[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
+      rb[k] = (ok && f < mb) ? FB[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
     }
     rg = ok ? G[gi] : mk<T>(T(0), T(0));
   };
@@ -1240,8 +1013,19 @@ __device__ __forceinline__ void k_pair_tiles3_body(const c2<T>* __restrict__ FA,
 #pragma unroll
     for (int k = 0; k < PT3_SLOTS; k++) {
       const int f = sf + k * FSTEP;
-      if (f < na) sa[f * PT_CELLS + sc] = ra[k];
-      if (f < nbf) sh[f * PT_CELLS + sc] = cmul<T>(rb[k], rg);
+      if (A_PACKED) {
+        if (2 * f < na) sa[2 * f * PT_CELLS + sc] = mk<T>(ra[k].x, T(0));
+        if (2 * f + 1 < na) sa[(2 * f + 1) * PT_CELLS + sc] = mk<T>(ra[k].y, T(0));
+      } else if (f < na) {
+        sa[f * PT_CELLS + sc] = ra[k];
+      }
+      if (MODE == 1) {
+        if (2 * f < nbf) sh[2 * f * PT_CELLS + sc] = mk<T>(rb[k].x * rg.x, rb[k].x * rg.y);
+        if (2 * f + 1 < nbf)
+          sh[(2 * f + 1) * PT_CELLS + sc] = mk<T>(rb[k].y * rg.x, rb[k].y * rg.y);
+      } else if (f < nbf) {
+        sh[f * PT_CELLS + sc] = cmul<T>(rb[k], rg);
+      }
     }
     __syncthreads();
     fetch(ch + gridDim.x);
@@ -1290,13 +1074,120 @@ __device__ __forceinline__ void k_pair_tiles3_body(const c2<T>* __restrict__ FA,
   }
 }
 
+// All-real pair reduction: A and B real and packed two per complex mesh, G
+// real (shells of even l, m = 0 with an M = 0 or periodic-box G).  Same
+// tiles and output as the complex body with imaginary parts zero; values
+// are staged as scalars and chunks are PR2_CELLS cells, since each pair
+// costs one FMA per cell here.
+constexpr int PR2_CELLS = 128;
+
 template <typename T>
+__device__ __forceinline__ void k_pair_real_body(
+    const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB, int nbf,
+    const c2<T>* __restrict__ G, long long nmesh, const int2* __restrict__ tiles,
+    int ntiles, double2* __restrict__ part) {
+  extern __shared__ char smr_raw[];
+  T* sa = reinterpret_cast<T*>(smr_raw);                 // [na][PR2_CELLS]
+  T* sh = sa + (size_t)na * PR2_CELLS;                    // [nbf][PR2_CELLS]
+  constexpr int FSTEP = 256 / PR2_CELLS;                  // packed meshes per pass
+  constexpr int SLOTS = 8;                                // 16 meshes = 32 fields
+  const int ngroups = 256 / ntiles;
+  const int group = threadIdx.x % ngroups;
+  const int tile = threadIdx.x / ngroups;
+  const bool active = tile < ntiles;
+  const int2 t0 = tiles[active ? tile : 0];
+  double ar[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) ar[q] = 0.;
+  int ia[4], ib[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    ia[q] = min(t0.x + q, na - 1) * PR2_CELLS;
+    ib[q] = min(t0.y + q, nbf - 1) * PR2_CELLS;
+  }
+  const int sc = threadIdx.x % PR2_CELLS;
+  const int sf = threadIdx.x / PR2_CELLS;
+  const int ma = (na + 1) / 2, mb = (nbf + 1) / 2;
+  c2<T> ra[SLOTS], rb[SLOTS];
+  T rg;
+  const long long nchunks = (nmesh + PR2_CELLS - 1) / PR2_CELLS;
+  auto fetch = [&](long long ch) {
+    const long long gi = ch * PR2_CELLS + sc;
+    const bool ok = (ch < nchunks) && (gi < nmesh);
+#pragma unroll
+    for (int k = 0; k < SLOTS; k++) {
+      const int f = sf + k * FSTEP;
+      ra[k] = (ok && f < ma) ? FA[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
+      rb[k] = (ok && f < mb) ? FB[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
+    }
+    rg = ok ? G[gi].x : T(0);
+  };
+  long long ch = blockIdx.x;
+  fetch(ch);
+  for (; ch < nchunks; ch += gridDim.x) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SLOTS; k++) {
+      const int f = sf + k * FSTEP;
+      if (2 * f < na) sa[2 * f * PR2_CELLS + sc] = ra[k].x;
+      if (2 * f + 1 < na) sa[(2 * f + 1) * PR2_CELLS + sc] = ra[k].y;
+      if (2 * f < nbf) sh[2 * f * PR2_CELLS + sc] = rb[k].x * rg;
+      if (2 * f + 1 < nbf) sh[(2 * f + 1) * PR2_CELLS + sc] = rb[k].y * rg;
+    }
+    __syncthreads();
+    fetch(ch + gridDim.x);
+    if (active) {
+      T pr[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) pr[q] = T(0);
+      for (int c = group; c < PR2_CELLS; c += ngroups) {
+        T x[4], y[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          x[q] = sa[ia[q] + c];
+          y[q] = sh[ib[q] + c];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) pr[i * 4 + j] += x[i] * y[j];
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++) ar[q] += double(pr[q]);
+    }
+  }
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(smr_raw);   // [256]
+  for (int q = 0; q < 16; q++) {
+    red[threadIdx.x] = active ? ar[q] : 0.;
+    __syncthreads();
+    if (threadIdx.x < ntiles) {
+      double sx = 0.;
+      for (int gr = 0; gr < ngroups; gr++) sx += red[threadIdx.x * ngroups + gr];
+      part[(long long)blockIdx.x * ntiles * 16 + threadIdx.x * 16 + q] = dmk(sx, 0.);
+    }
+    __syncthreads();
+  }
+}
+
+// Bytes of dynamic LDS of a pair-reduction launch of a given MODE.
+template <typename T>
+inline size_t pair_smem(int mode, int nas, int nbs) {
+  const size_t staged = mode == 2 ? (size_t)(nas + nbs) * PR2_CELLS * sizeof(T)
+                                  : (size_t)(nas + nbs) * PT_CELLS * sizeof(c2<T>);
+  return staged > 256 * sizeof(double2) ? staged : 256 * sizeof(double2);
+}
+
+template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
 k_pair_tiles3(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
               int nbf, const c2<T>* __restrict__ G, long long nmesh,
               const int2* __restrict__ tiles, int ntiles,
               double2* __restrict__ part) {
-  k_pair_tiles3_body<T>(FA, na, FB, nbf, G, nmesh, tiles, ntiles, part);
+  if constexpr (MODE == 2)
+    k_pair_real_body<T>(FA, na, FB, nbf, G, nmesh, tiles, ntiles, part);
+  else
+    k_pair_tiles3_body<T, MODE>(FA, na, FB, nbf, G, nmesh, tiles, ntiles, part);
 }
 
 }  // namespace trvg
