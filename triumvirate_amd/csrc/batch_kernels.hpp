@@ -106,6 +106,22 @@ struct LoadShellMocks {
   }
 };
 
+// Packed real shell pair (mock f / nt, bins 2 (f % nt) and 2 (f % nt) + 1).
+template <typename T>
+struct LoadShellPairMocks {
+  const double2* src;
+  long long stride;
+  AxisTables tab;
+  Geom geo;
+  const ShellBand* bands;
+  int nt, ell, m;
+  __device__ LoadShellPair<T, double> field(int f) const {
+    const int t = f % nt;
+    return LoadShellPair<T, double>{src + (long long)(f / nt) * stride, tab, geo,
+                                    bands[2 * t], bands[2 * t + 1], ell, m};
+  }
+};
+
 // Compensated, 1/V-scaled field of mock b.
 template <typename T>
 struct LoadDeconvMocks {

@@ -25,7 +25,7 @@ namespace trvg {
 // Declared in engine_drivers.hip.
 template <typename T>
 void shell_slab_work(const Geom& g, const trvg_binning& kb, int nbins,
-                     int nfield_groups, std::vector<ShellBand>& bands,
+                     int nfield_groups, bool packed, std::vector<ShellBand>& bands,
                      std::vector<double>& kcut2, std::vector<int2>& wz,
                      std::vector<int2>& wy, std::vector<int2>& wx);
 template <typename T, class BatchLoader>
@@ -69,8 +69,17 @@ struct PairGroup {
 struct TermPlan {
   Term t;
   int slab_a, slab_b;
+  // Pair kernel: 0 both sides complex, 1 second side packed real, 2 both
+  // sides packed real; swap: the first side is slab_b (the packed side of a
+  // mixed pair goes second).
+  int mode = 0;
+  bool swap = false;
   std::vector<std::unique_ptr<PairGroup>> groups;
 };
+
+// Shells of (l, m) are real for even l and m = 0 (real Y_l0, Hermitian
+// dn00): such slabs hold two bins per complex mesh.
+bool real_slab(int l, int m) { return m == 0 && l % 2 == 0; }
 
 struct MultipolePlan {
   int l1, l2, L, shape, idx_bin, dim;
@@ -133,19 +142,23 @@ class BatchPlan : public BatchPlanBase {
     TRVG_HIP(hipMemcpyAsync(d_edges_.get(), edges_.data(), (nb_ + 1) * 8,
                             hipMemcpyHostToDevice, ctx_.stream));
 
-    // Work lists: unpruned passes over B fields, and the shell slabs.
+    // Work lists: unpruned passes over B fields, and the shell slabs
+    // (complex and packed real).
     full_lists(B_, wfull_);
-    std::vector<ShellBand> bands;
-    std::vector<double> kc;
-    std::vector<int2> wz, wy, wx;
-    shell_slab_work<T>(g, kb_, nb_, B_, bands, kc, wz, wy, wx);
-    nz_ = int(wz.size()); ny_ = int(wy.size()); nx_ = int(wx.size());
-    upload(d_bands_, bands);
-    upload(d_kcut2_, kc);
-    std::vector<int2> all(wz);
-    all.insert(all.end(), wy.begin(), wy.end());
-    all.insert(all.end(), wx.begin(), wx.end());
-    upload(d_wshell_, all);
+    for (int pk = 0; pk < 2; pk++) {
+      SlabWork& sw = swork_[pk];
+      std::vector<ShellBand> bands;
+      std::vector<double> kc;
+      std::vector<int2> wz, wy, wx;
+      shell_slab_work<T>(g, kb_, nb_, B_, pk == 1, bands, kc, wz, wy, wx);
+      sw.nz = int(wz.size()); sw.ny = int(wy.size()); sw.nx = int(wx.size());
+      upload(sw.bands, bands);
+      upload(sw.kcut2, kc);
+      std::vector<int2> all(wz);
+      all.insert(all.end(), wy.begin(), wy.end());
+      all.insert(all.end(), wx.begin(), wx.end());
+      upload(sw.work, all);
+    }
 
     // Multipoles, terms, slabs, pair groups.
     for (const Multipole& mp : mps) {
@@ -173,6 +186,10 @@ class BatchPlan : public BatchPlanBase {
         tp.t = t;
         tp.slab_a = slab_index(mp.ell1, t.m1);
         tp.slab_b = slab_index(mp.ell2, t.m2);
+        // G is real for every periodic-box term (M = 0).
+        const bool pa = slab_packed_[tp.slab_a], pb = slab_packed_[tp.slab_b];
+        tp.mode = (pa && pb) ? 2 : (pa || pb) ? 1 : 0;
+        tp.swap = pa && !pb;
         P->terms.push_back(std::move(tp));
       }
       (void)cached_spline(ctx_.dev, mp.ell1, ctx_.stream);
@@ -329,9 +346,14 @@ class BatchPlan : public BatchPlanBase {
     for (size_t i = 0; i < slabs_.size(); i++)
       if (slabs_[i].first == l && slabs_[i].second == m) return int(i);
     slabs_.push_back({l, m});
+    slab_packed_.push_back(real_slab(l, m));
     slab_data_.emplace_back();
-    slab_data_.back().reset((size_t)B_ * nb_ * nm_);
+    slab_data_.back().reset((size_t)B_ * slab_meshes(int(slabs_.size() - 1)) * nm_);
     return int(slabs_.size() - 1);
+  }
+
+  // Meshes per mock of slab i.
+  int slab_meshes(int i) const { return slab_packed_[i] ? (nb_ + 1) / 2 : nb_;
   }
 
   void build_pair_groups() {
@@ -340,9 +362,12 @@ class BatchPlan : public BatchPlanBase {
         tp.groups.clear();
         const int na = nb_, nbf = nb_;
         const int rbn = (na + 3) / 4, cbn = (nbf + 3) / 4;
+        // Tile rows index the first side's bins (slab_b's when swapped).
+        const std::vector<int>& rows = tp.swap ? P->cb : P->ra;
+        const std::vector<int>& cols = tp.swap ? P->ra : P->cb;
         std::vector<char> active((size_t)rbn * cbn, 0);
         for (int dv = 0; dv < P->dim; dv++)
-          active[(size_t)(P->ra[dv] / 4) * cbn + P->cb[dv] / 4] = 1;
+          active[(size_t)(rows[dv] / 4) * cbn + cols[dv] / 4] = 1;
         const int CB = std::min(cbn, 8);
         const int RB = std::max(1, std::min(std::min((64 - 4 * CB) / 4, PT_MAXT / CB),
                                             tune_.rb_cap));
@@ -364,7 +389,7 @@ class BatchPlan : public BatchPlanBase {
             gr->nt = int(tl.size());
             std::vector<int2> ent;
             for (int dv = 0; dv < P->dim; dv++) {
-              const int a = P->ra[dv] - gr->ra0, c = P->cb[dv] - gr->ca0;
+              const int a = rows[dv] - gr->ra0, c = cols[dv] - gr->ca0;
               if (a < 0 || c < 0 || a >= 4 * (r1b - r0b) || c >= 4 * (c1b - c0b)) continue;
               for (int t = 0; t < gr->nt; t++)
                 if (tl[t].x == 4 * (a / 4) && tl[t].y == 4 * (c / 4))
@@ -402,7 +427,7 @@ class BatchPlan : public BatchPlanBase {
     TRVG_LAUNCH_CHECK();
     launch_assign();
     LoadScaleShiftMocks ls{D_.get(), nm_, d_nmock_.get(), g.vol, g.vol_cell};
-    fft3d_batch<double>(ctx_, D_.get(), ls, -1, false, d_kcut2_.get(),
+    fft3d_batch<double>(ctx_, D_.get(), ls, -1, false, swork_[0].kcut2.get(),
                         wfull_.get(), nfull64_, wfull_.get(), nfull64_,
                         wfull_.get(), nfull64_);
     hipLaunchKernelGGL(k_n00_from_dn, dim3(grid_for(ctx_, total)), dim3(256), 0, s,
@@ -410,15 +435,23 @@ class BatchPlan : public BatchPlanBase {
     TRVG_LAUNCH_CHECK();
     // Shell slabs (all bins of every mock per pass).
     for (size_t i = 0; i < slabs_.size(); i++) {
-      LoadShellMocks<T> lsh{D_.get(), nm_, ctx_.tab, g, d_bands_.get(), nb_,
-                            slabs_[i].first, slabs_[i].second};
-      const int2* wz = d_wshell_.get();
-      fft3d_batch<T>(ctx_, slab_data_[i].get(), lsh, +1, true, d_kcut2_.get(),
-                     wz, nz_, wz + nz_, ny_, wz + nz_ + ny_, nx_);
+      const SlabWork& sw = swork_[slab_packed_[i] ? 1 : 0];
+      const int2* wz = sw.work.get();
+      if (slab_packed_[i]) {
+        LoadShellPairMocks<T> lsh{D_.get(), nm_, ctx_.tab, g, sw.bands.get(),
+                                  (nb_ + 1) / 2, slabs_[i].first, slabs_[i].second};
+        fft3d_batch<T>(ctx_, slab_data_[i].get(), lsh, +1, true, sw.kcut2.get(),
+                       wz, sw.nz, wz + sw.nz, sw.ny, wz + sw.nz + sw.ny, sw.nx);
+      } else {
+        LoadShellMocks<T> lsh{D_.get(), nm_, ctx_.tab, g, sw.bands.get(), nb_,
+                              slabs_[i].first, slabs_[i].second};
+        fft3d_batch<T>(ctx_, slab_data_[i].get(), lsh, +1, true, sw.kcut2.get(),
+                       wz, sw.nz, wz + sw.nz, sw.ny, wz + sw.nz + sw.ny, sw.nx);
+      }
     }
     // G(x) = IFFT[dn00 / (W V)] (every periodic-box term uses dn00).
     LoadDeconvMocks<T> ld{D_.get(), nm_, ctx_.tab, 1. / g.vol};
-    fft3d_batch<T>(ctx_, G_.get(), ld, +1, false, d_kcut2_.get(), wfullT_.get(),
+    fft3d_batch<T>(ctx_, G_.get(), ld, +1, false, swork_[0].kcut2.get(), wfullT_.get(),
                    nfullT_, wfullT_.get(), nfullT_, wfullT_.get(), nfullT_);
     for (auto& P : mps_) run_multipole(*P);
   }
@@ -444,15 +477,30 @@ class BatchPlan : public BatchPlanBase {
   }
 
   void launch_pairs(TermPlan& tp, PairGroup& gr) {
-    const c2<T>* A = slab_data_[tp.slab_a].get();
-    const c2<T>* Bs = slab_data_[tp.slab_b].get();
-    const size_t smem = pair_smem<T>(0, gr.nas, gr.nbs);
-    const long long ss = (long long)nb_ * nm_;
-    hipLaunchKernelGGL((k_pair_tiles3_batch<T, 0>), dim3(tune_.pair_blocks, B_), dim3(256),
-                       smem, ctx_.stream, A + (long long)gr.ra0 * nm_, gr.nas, ss,
-                       Bs + (long long)gr.ca0 * nm_, gr.nbs, ss, G_.get(), nm_, nm_,
-                       gr.tiles.get(), gr.nt, pair_part_.get(),
-                       (long long)tune_.pair_blocks * gr.nt * 16);
+    const int ia = tp.swap ? tp.slab_b : tp.slab_a;
+    const int ib = tp.swap ? tp.slab_a : tp.slab_b;
+    // Packed sides start at mesh ra0 / 2 (group starts are multiples of 4).
+    const int a0 = slab_packed_[ia] ? gr.ra0 / 2 : gr.ra0;
+    const int b0 = slab_packed_[ib] ? gr.ca0 / 2 : gr.ca0;
+    const c2<T>* A = slab_data_[ia].get() + (long long)a0 * nm_;
+    const c2<T>* Bs = slab_data_[ib].get() + (long long)b0 * nm_;
+    const long long sa = (long long)slab_meshes(ia) * nm_;
+    const long long sb = (long long)slab_meshes(ib) * nm_;
+    const size_t smem = pair_smem<T>(tp.mode, gr.nas, gr.nbs);
+    const dim3 grid(tune_.pair_blocks, B_);
+    const long long sp = (long long)tune_.pair_blocks * gr.nt * 16;
+    if (tp.mode == 2)
+      hipLaunchKernelGGL((k_pair_tiles3_batch<T, 2>), grid, dim3(256), smem, ctx_.stream,
+                         A, gr.nas, sa, Bs, gr.nbs, sb, G_.get(), nm_, nm_, gr.tiles.get(),
+                         gr.nt, pair_part_.get(), sp);
+    else if (tp.mode == 1)
+      hipLaunchKernelGGL((k_pair_tiles3_batch<T, 1>), grid, dim3(256), smem, ctx_.stream,
+                         A, gr.nas, sa, Bs, gr.nbs, sb, G_.get(), nm_, nm_, gr.tiles.get(),
+                         gr.nt, pair_part_.get(), sp);
+    else
+      hipLaunchKernelGGL((k_pair_tiles3_batch<T, 0>), grid, dim3(256), smem, ctx_.stream,
+                         A, gr.nas, sa, Bs, gr.nbs, sb, G_.get(), nm_, nm_, gr.tiles.get(),
+                         gr.nt, pair_part_.get(), sp);
     TRVG_LAUNCH_CHECK();
   }
 
@@ -491,7 +539,7 @@ class BatchPlan : public BatchPlanBase {
       if (P.l2 == 0) twopt(P, t, P.l1, t.m1, P.bin_a.get());
       if (P.l1 == 0) twopt(P, t, P.l2, t.m2, P.bin_b.get());
       LoadTwoptMocks lt{D_.get(), N00_.get(), nm_, ctx_.tab, d_nmock_.get(), 1. / g.vol};
-      fft3d_batch<double>(ctx_, tfield_.get(), lt, +1, false, d_kcut2_.get(),
+      fft3d_batch<double>(ctx_, tfield_.get(), lt, +1, false, swork_[0].kcut2.get(),
                           wfull_.get(), nfull64_, wfull_.get(), nfull64_,
                           wfull_.get(), nfull64_);
       const long long nQ = (long long)B_ * 2 * nq_;
@@ -632,15 +680,22 @@ class BatchPlan : public BatchPlanBase {
   double* h_pos_ = nullptr;
   double* h_meta_ = nullptr;
   long long* h_offs_ = nullptr;
-  DevArray<double> d_x_, d_y_, d_z_, d_nmock_, d_norm_, d_edges_, d_kcut2_;
+  DevArray<double> d_x_, d_y_, d_z_, d_nmock_, d_norm_, d_edges_;
   DevArray<long long> d_offs_;
   DevArray<double2> D_, N00_, tfield_;
   DevArray<c2<T>> G_;
   DevArray<double> Q_;
-  DevArray<ShellBand> d_bands_;
-  DevArray<int2> d_wshell_, wfull_, wfullT_;
-  int nz_ = 0, ny_ = 0, nx_ = 0, nfull64_ = 0, nfullT_ = 0;
+  // Shell slab work lists: [0] one bin per mesh, [1] packed real pairs.
+  struct SlabWork {
+    DevArray<ShellBand> bands;
+    DevArray<double> kcut2;
+    DevArray<int2> work;   // z, then y, then x blocks
+    int nz = 0, ny = 0, nx = 0;
+  } swork_[2];
+  DevArray<int2> wfull_, wfullT_;
+  int nfull64_ = 0, nfullT_ = 0;
   std::vector<std::pair<int, int>> slabs_;
+  std::vector<bool> slab_packed_;
   std::vector<DevArray<c2<T>>> slab_data_;
   std::vector<std::unique_ptr<MultipolePlan>> mps_;
   size_t pair_width_ = 0;

@@ -261,46 +261,50 @@ void fft3d_batch(Context& ctx, c2<T>* base, const BatchLoader& load, int sign,
         pr, kcut2, wx, nx, 2, 2. * blk * nx)));
 }
 
-// Work lists of a shell slab transform over `ngroups` groups of `nbins`
-// fields (field f = group * nbins + bin): the bands of the bins, each
-// field's pruning radius (+inf when the band covers the whole plane), and
-// the z / y / x blocks.
+// Work lists of a shell slab transform over `ngroups` groups of all
+// `nbins` bins: per group nt transforms (nbins, or (nbins + 1) / 2 when
+// packed, two real bins per transform), transform f = group * nt + t.
+// Returns the bins' bands (two per transform when packed, padded with an
+// empty band), each transform's pruning radius (+inf when the band covers
+// the whole plane), and the z / y / x blocks.
 template <typename T>
 void shell_slab_work(const Geom& g, const trvg_binning& kb, int nbins,
-                     int ngroups, std::vector<ShellBand>& bands,
+                     int ngroups, bool packed, std::vector<ShellBand>& bands,
                      std::vector<double>& kcut2, std::vector<int2>& wz,
                      std::vector<int2>& wy, std::vector<int2>& wx) {
   const int E = FFT_THREADS * FftEpt<T>::v;
   const int nl_z = E / g.n[2], nl_y = E / g.n[1];
   const long long nblk = g.nmesh / E;
-  bands.resize(nbins);
-  std::vector<double> kc(nbins);
-  std::vector<const PruneLists*> pls(nbins);
+  const int per = packed ? 2 : 1;
+  const int nt = (nbins + per - 1) / per;
+  bands.assign((size_t)nt * per, ShellBand{HUGE_VAL, HUGE_VAL});
+  std::vector<double> kc(nt, 0.);
   for (int q = 0; q < nbins; q++) {
     bands[q] = shell_band(kb.bin_edges[q], kb.bin_edges[q + 1]);
     Prune pq = make_prune(g, kb.bin_edges[q + 1]);
-    kc[q] = pq.on ? pq.kcut2 : HUGE_VAL;
-    pls[q] = &prune_lists(g, kc[q], nl_z, nl_y);
+    kc[q / per] = std::max(kc[q / per], pq.on ? pq.kcut2 : HUGE_VAL);
   }
+  std::vector<const PruneLists*> pls(nt);
+  for (int t = 0; t < nt; t++) pls[t] = &prune_lists(g, kc[t], nl_z, nl_y);
   kcut2.clear(); wz.clear(); wy.clear(); wx.clear();
   for (int gi = 0; gi < ngroups; gi++)
-    for (int q = 0; q < nbins; q++) {
-      const int f = gi * nbins + q;
-      kcut2.push_back(kc[q]);
-      for (int w : pls[q]->zblocks) wz.push_back(make_int2(f, w));
-      for (int w : pls[q]->yblocks) wy.push_back(make_int2(f, w));
+    for (int t = 0; t < nt; t++) {
+      const int f = gi * nt + t;
+      kcut2.push_back(kc[t]);
+      for (int w : pls[t]->zblocks) wz.push_back(make_int2(f, w));
+      for (int w : pls[t]->yblocks) wy.push_back(make_int2(f, w));
       for (long long w = 0; w < nblk; w++) wx.push_back(make_int2(f, int(w)));
     }
 }
 
 template void shell_slab_work<double>(const Geom&, const trvg_binning&, int, int,
-                                      std::vector<ShellBand>&, std::vector<double>&,
-                                      std::vector<int2>&, std::vector<int2>&,
-                                      std::vector<int2>&);
+                                      bool, std::vector<ShellBand>&,
+                                      std::vector<double>&, std::vector<int2>&,
+                                      std::vector<int2>&, std::vector<int2>&);
 template void shell_slab_work<float>(const Geom&, const trvg_binning&, int, int,
-                                     std::vector<ShellBand>&, std::vector<double>&,
-                                     std::vector<int2>&, std::vector<int2>&,
-                                     std::vector<int2>&);
+                                     bool, std::vector<ShellBand>&,
+                                     std::vector<double>&, std::vector<int2>&,
+                                     std::vector<int2>&, std::vector<int2>&);
 #define TRVG_FFT3D_BATCH_INST(T, L)                                             \
   template void fft3d_batch<T, L>(Context&, c2<T>*, const L&, int, bool,        \
                                   const double*, const int2*, int, const int2*, \
@@ -309,6 +313,8 @@ TRVG_FFT3D_BATCH_INST(double, LoadScaleShiftMocks)
 TRVG_FFT3D_BATCH_INST(double, LoadTwoptMocks)
 TRVG_FFT3D_BATCH_INST(double, LoadShellMocks<double>)
 TRVG_FFT3D_BATCH_INST(float, LoadShellMocks<float>)
+TRVG_FFT3D_BATCH_INST(double, LoadShellPairMocks<double>)
+TRVG_FFT3D_BATCH_INST(float, LoadShellPairMocks<float>)
 TRVG_FFT3D_BATCH_INST(double, LoadDeconvMocks<double>)
 TRVG_FFT3D_BATCH_INST(float, LoadDeconvMocks<float>)
 #undef TRVG_FFT3D_BATCH_INST
