@@ -196,14 +196,15 @@ k_radial_hist_oct_batch(const double2* __restrict__ t, long long stride, Geom g,
   k_radial_hist_oct_body<double>(t + b * stride, g, la, ma, lb, mb, Q + b * 2 * nq);
 }
 
-static __global__ void __launch_bounds__(RED_THREADS)
-k_shot_pairs_hist_batch(const double* __restrict__ Q, long long nq, double dr,
-                        SplineDev sa, SplineDev sb, const double* __restrict__ ka,
-                        const double* __restrict__ kb, double vol_cell,
-                        double2* __restrict__ out, int P) {
+static __global__ void __launch_bounds__(256)
+k_shot_pairs_tab_batch(const double* __restrict__ Q, long long nq, double dr,
+                       SplineDev sa, SplineDev sb, const double* __restrict__ ka,
+                       int nka, const double* __restrict__ kb, int nkb,
+                       const int2* __restrict__ pidx, int P, int nch, double vol_cell,
+                       double2* __restrict__ part) {
   const long long b = blockIdx.y;
-  k_shot_pairs_hist_body(Q + b * 2 * nq, nq, dr, sa, sb, ka, kb, vol_cell,
-                         out + b * P);
+  k_shot_pairs_tab_body(Q + b * 2 * nq, nq, dr, sa, sb, ka, nka, kb, nkb, pidx, P, nch,
+                        vol_cell, part + b * gridDim.x * P);
 }
 
 // ---- epilogues: per-mock accumulation of the estimator terms ------------

@@ -85,7 +85,13 @@ struct MultipolePlan {
   int l1, l2, L, shape, idx_bin, dim;
   std::vector<int> ra, cb;
   std::vector<TermPlan> terms;
-  DevArray<double> n1, n2, ka, kb;   // per entry: mode counts, k_eff
+  DevArray<double> n1, n2;            // per entry: mode counts
+  // Shot-noise tables: k_eff of the distinct bins of each side, the entries
+  // as (side-a index, side-b index), radii per table block.
+  DevArray<double> ka, kb;
+  DevArray<int2> pidx;
+  int nka = 0, nkb = 0, nch = 0, nchunks = 0;
+  size_t tab_smem = 0;
   DevArray<int> bin_a, bin_b;       // per entry: row / column bin
   DevArray<double2> raw, shot;      // [B][dim]
   double2 phase;
@@ -167,14 +173,34 @@ class BatchPlan : public BatchPlanBase {
       P->shape = mp.shape; P->idx_bin = mp.idx_bin;
       pair_bins(mp.shape, nb_, mp.idx_bin, P->ra, P->cb);
       P->dim = int(P->ra.size());
-      std::vector<double> n1(P->dim), n2(P->dim), e1(P->dim), e2(P->dim);
+      std::vector<double> n1(P->dim), n2(P->dim);
       for (int dv = 0; dv < P->dim; dv++) {
         n1[dv] = ss_.nmodes[P->ra[dv]];
         n2[dv] = ss_.nmodes[P->cb[dv]];
-        e1[dv] = ss_.keff[P->ra[dv]];
-        e2[dv] = ss_.keff[P->cb[dv]];
       }
-      upload(P->n1, n1); upload(P->n2, n2); upload(P->ka, e1); upload(P->kb, e2);
+      upload(P->n1, n1); upload(P->n2, n2);
+      {
+        std::vector<int> ua(P->ra), ub(P->cb);
+        for (auto* u : {&ua, &ub}) {
+          std::sort(u->begin(), u->end());
+          u->erase(std::unique(u->begin(), u->end()), u->end());
+        }
+        std::vector<double> ka, kbv;
+        for (int b : ua) ka.push_back(ss_.keff[b]);
+        for (int b : ub) kbv.push_back(ss_.keff[b]);
+        std::vector<int2> pidx(P->dim);
+        for (int dv = 0; dv < P->dim; dv++)
+          pidx[dv] = make_int2(
+              int(std::lower_bound(ua.begin(), ua.end(), P->ra[dv]) - ua.begin()),
+              int(std::lower_bound(ub.begin(), ub.end(), P->cb[dv]) - ub.begin()));
+        upload(P->ka, ka); upload(P->kb, kbv); upload(P->pidx, pidx);
+        P->nka = int(ka.size()); P->nkb = int(kbv.size());
+        P->nch = shot_tab_chunk(P->nka, P->nkb);
+        P->tab_smem = shot_tab_smem(P->nka, P->nkb, P->nch);
+        if (P->tab_smem > 160 * 1024)
+          throw Error(TRVG_ERR_PARAM, "Too many wavenumber bins for the shot-noise tables.");
+        P->nchunks = int((nq_ + P->nch - 1) / P->nch);
+      }
       upload(P->bin_a, P->ra); upload(P->bin_b, P->cb);
       P->raw.reset((size_t)B_ * P->dim);
       P->shot.reset((size_t)B_ * P->dim);
@@ -197,8 +223,13 @@ class BatchPlan : public BatchPlanBase {
       mps_.push_back(std::move(P));
     }
     int maxdim = 0;
-    for (auto& P : mps_) maxdim = std::max(maxdim, P->dim);
+    size_t maxtab = 0;
+    for (auto& P : mps_) {
+      maxdim = std::max(maxdim, P->dim);
+      maxtab = std::max(maxtab, (size_t)P->nchunks * P->dim);
+    }
     sij_.reset((size_t)B_ * maxdim);
+    tab_part_.reset((size_t)B_ * maxtab);
     counts_.reset((size_t)B_ * nb_);
     tsum_.reset((size_t)B_ * nb_ * 4);
     build_pair_groups();
@@ -536,8 +567,10 @@ class BatchPlan : public BatchPlanBase {
                            P.dim, d_nmock_.get(), t.coupling);
         TRVG_LAUNCH_CHECK();
       }
-      if (P.l2 == 0) twopt(P, t, P.l1, t.m1, P.bin_a.get());
-      if (P.l1 == 0) twopt(P, t, P.l2, t.m2, P.bin_b.get());
+      if (P.l2 == 0) twopt(P, t, P.l1, t.m1, P.bin_a.get(), true);
+      // Both sides monopoles: reuse the first side's statistics.
+      if (P.l1 == 0)
+        twopt(P, t, P.l2, t.m2, P.bin_b.get(), !(P.l2 == 0 && t.m1 == t.m2));
       LoadTwoptMocks lt{D_.get(), N00_.get(), nm_, ctx_.tab, d_nmock_.get(), 1. / g.vol};
       fft3d_batch<double>(ctx_, tfield_.get(), lt, +1, false, swork_[0].kcut2.get(),
                           wfull_.get(), nfull64_, wfull_.get(), nfull64_,
@@ -550,9 +583,13 @@ class BatchPlan : public BatchPlanBase {
                                                        (nrep + 255) / 256));
       hipLaunchKernelGGL(k_radial_hist_oct_batch, dim3(rg, B_), dim3(RED_THREADS), 0, s,
                          tfield_.get(), nm_, g, P.l1, t.m1, P.l2, t.m2, Q_.get(), nq_);
-      hipLaunchKernelGGL(k_shot_pairs_hist_batch, dim3(P.dim, B_), dim3(RED_THREADS), 0,
-                         s, Q_.get(), nq_, g.dr[0], sja.dev(), sjb.dev(), P.ka.get(),
-                         P.kb.get(), g.vol_cell, sij_.get(), P.dim);
+      hipLaunchKernelGGL(k_shot_pairs_tab_batch, dim3(P.nchunks, B_), dim3(256),
+                         P.tab_smem, s, Q_.get(), nq_, g.dr[0], sja.dev(), sjb.dev(),
+                         P.ka.get(), P.nka, P.kb.get(), P.nkb, P.pidx.get(), P.dim,
+                         P.nch, g.vol_cell, tab_part_.get());
+      hipLaunchKernelGGL((k_reduce_cols_batch<double2>),
+                         dim3((P.dim + RC_COLS - 1) / RC_COLS, B_), dim3(RC_COLS * RC_ROWS),
+                         0, s, tab_part_.get(), P.nchunks, P.dim, sij_.get());
       hipLaunchKernelGGL(k_epi_sij, dim3(eg, B_), dim3(256), 0, s, P.shot.get(), P.dim,
                          sij_.get(), t.coupling, t.f_mirror_w3j, P.phase);
       TRVG_LAUNCH_CHECK();
@@ -562,7 +599,19 @@ class BatchPlan : public BatchPlanBase {
     TRVG_LAUNCH_CHECK();
   }
 
-  void twopt(MultipolePlan& P, const Term& t, int ell, int m, const int* bin_of) {
+  // Two-point shot-noise term of one side; compute = false reuses the
+  // statistics left in counts_ / tsum_ by the previous call.
+  void twopt(MultipolePlan& P, const Term& t, int ell, int m, const int* bin_of,
+             bool compute) {
+    hipStream_t s = ctx_.stream;
+    if (compute) twopt_stats(ell, m);
+    const unsigned eg = unsigned(std::max(1, std::min(64, (P.dim + 255) / 256)));
+    hipLaunchKernelGGL(k_epi_twopt, dim3(eg, B_), dim3(256), 0, s, P.shot.get(), P.dim,
+                       tsum_.get(), counts_.get(), nb_, bin_of, t.coupling, t.f_mirror);
+    TRVG_LAUNCH_CHECK();
+  }
+
+  void twopt_stats(int ell, int m) {
     const Geom& g = ctx_.g;
     hipStream_t s = ctx_.stream;
     hipLaunchKernelGGL(k_fill_zero_u64, dim3(1), dim3(256), 0, s, counts_.get(),
@@ -576,9 +625,6 @@ class BatchPlan : public BatchPlanBase {
     hipLaunchKernelGGL((k_reduce_cols_batch<double>), dim3((w + RC_COLS - 1) / RC_COLS, B_),
                        dim3(RC_COLS * RC_ROWS), 0, s, tpart_.get(), tune_.stat_blocks, w,
                        tsum_.get());
-    const unsigned eg = unsigned(std::max(1, std::min(64, (P.dim + 255) / 256)));
-    hipLaunchKernelGGL(k_epi_twopt, dim3(eg, B_), dim3(256), 0, s, P.shot.get(), P.dim,
-                       tsum_.get(), counts_.get(), nb_, bin_of, t.coupling, t.f_mirror);
     TRVG_LAUNCH_CHECK();
   }
 
@@ -699,7 +745,7 @@ class BatchPlan : public BatchPlanBase {
   std::vector<DevArray<c2<T>>> slab_data_;
   std::vector<std::unique_ptr<MultipolePlan>> mps_;
   size_t pair_width_ = 0;
-  DevArray<double2> pair_part_, pair_sum_, sij_;
+  DevArray<double2> pair_part_, pair_sum_, sij_, tab_part_;
   DevArray<double> tpart_, tsum_;
   DevArray<unsigned long long> counts_;
   hipGraph_t graph_ = nullptr;

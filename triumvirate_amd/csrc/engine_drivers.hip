@@ -818,41 +818,85 @@ static ShotBins3pcf shot_bins_3pcf(Context& ctx, const c2<T>* t, int la, int ma,
 
 // S_ij_k for each (k_a, k_b) (field.cpp:3198-3396) from the shot-noise
 // field t(x).
+// Sorted distinct values (bins of a side).
+static std::vector<int> distinct(const std::vector<int>& v) {
+  std::vector<int> u(v);
+  std::sort(u.begin(), u.end());
+  u.erase(std::unique(u.begin(), u.end()), u.end());
+  return u;
+}
+
+// Distinct bins of each side of a pair list and the pairs re-indexed into
+// them: wavenumber lists (k_eff of the bins) for the shot-noise tables.
+struct PairBinLists {
+  std::vector<double> ka, kb;
+  std::vector<int2> pidx;
+};
+static PairBinLists pair_bin_lists(const std::vector<double>& keff,
+                                   const std::vector<int>& ra,
+                                   const std::vector<int>& cb) {
+  PairBinLists L;
+  std::vector<int> ua = distinct(ra), ub = distinct(cb);
+  for (int b : ua) L.ka.push_back(keff[b]);
+  for (int b : ub) L.kb.push_back(keff[b]);
+  L.pidx.resize(ra.size());
+  for (size_t d = 0; d < ra.size(); d++)
+    L.pidx[d] = make_int2(int(std::lower_bound(ua.begin(), ua.end(), ra[d]) - ua.begin()),
+                          int(std::lower_bound(ub.begin(), ub.end(), cb[d]) - ub.begin()));
+  return L;
+}
+
 template <typename T>
 static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
                                            int ma, int lb, int mb,
                                            const Spline& sa, const Spline& sb,
-                                           const std::vector<double>& ka,
-                                           const std::vector<double>& kb) {
-  const int P = int(ka.size());
+                                           const std::vector<double>& keff,
+                                           const std::vector<int>& ra,
+                                           const std::vector<int>& cb) {
+  const int P = int(ra.size());
   const Geom& g = ctx.g;
   std::vector<cplx> out(P);
-  DevArray<double> dka(P), dkb(P);
-  TRVG_HIP(hipMemcpyAsync(dka.get(), ka.data(), P * 8, hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dkb.get(), kb.data(), P * 8, hipMemcpyHostToDevice, ctx.stream));
   const bool cubic = (g.dr[0] == g.dr[1] && g.dr[1] == g.dr[2]);
   if (cubic) {
     long long nq = (long long)(g.n[0] / 2) * (g.n[0] / 2)
                    + (long long)(g.n[1] / 2) * (g.n[1] / 2)
                    + (long long)(g.n[2] / 2) * (g.n[2] / 2) + 1;
+    const PairBinLists L = pair_bin_lists(keff, ra, cb);
+    const int nka = int(L.ka.size()), nkb = int(L.kb.size());
+    const int nch = shot_tab_chunk(nka, nkb);
+    const size_t smem = shot_tab_smem(nka, nkb, nch);
+    if (smem > 160 * 1024)
+      throw Error(TRVG_ERR_PARAM, "Too many wavenumber bins for the shot-noise tables.");
+    DevArray<double> dka(nka), dkb(nkb);
+    DevArray<int2> dp(P);
+    TRVG_HIP(hipMemcpyAsync(dka.get(), L.ka.data(), nka * 8, hipMemcpyHostToDevice, ctx.stream));
+    TRVG_HIP(hipMemcpyAsync(dkb.get(), L.kb.data(), nkb * 8, hipMemcpyHostToDevice, ctx.stream));
+    TRVG_HIP(hipMemcpyAsync(dp.get(), L.pidx.data(), P * sizeof(int2),
+                            hipMemcpyHostToDevice, ctx.stream));
+    const long long nchunks = (nq + nch - 1) / nch;
     DevArray<double> Q(2 * nq);
+    DevArray<double2> part((size_t)nchunks * P), res(P);
     hipLaunchKernelGGL(k_zero_d, dim3(grid_for(ctx, 2 * nq)), dim3(RED_THREADS), 0,
                        ctx.stream, Q.get(), 2 * nq);
     TRVG_LAUNCH_CHECK();
-    ProfScope ps1(ctx, KC_RADIAL, double(g.nmesh) * sizeof(c2<T>));
-    const long long nrep = (long long)(g.n[0] / 2 + 1) * (g.n[1] / 2 + 1) *
-                           (g.n[2] / 2 + 1);
-    hipLaunchKernelGGL((k_radial_hist_oct<T>), dim3(grid_for(ctx, nrep, 8)),
-                       dim3(RED_THREADS), 0, ctx.stream, t, g, la, ma, lb, mb,
-                       Q.get());
-    TRVG_LAUNCH_CHECK();
-    DevArray<double2> res(P);
-    // The histogram is read once from HBM (then from L2 by every pair).
-    ProfScope ps2(ctx, KC_SHOT_PAIRS, double(nq) * 16. + double(P) * 16.);
-    hipLaunchKernelGGL(k_shot_pairs_hist, dim3(P), dim3(RED_THREADS), 0,
-                       ctx.stream, Q.get(), nq, g.dr[0], sa.dev(), sb.dev(),
-                       dka.get(), dkb.get(), g.vol_cell, res.get());
-    TRVG_LAUNCH_CHECK();
+    {
+      ProfScope ps(ctx, KC_RADIAL, double(g.nmesh) * sizeof(c2<T>));
+      const long long nrep = (long long)(g.n[0] / 2 + 1) * (g.n[1] / 2 + 1) *
+                             (g.n[2] / 2 + 1);
+      hipLaunchKernelGGL((k_radial_hist_oct<T>), dim3(grid_for(ctx, nrep, 8)),
+                         dim3(RED_THREADS), 0, ctx.stream, t, g, la, ma, lb, mb,
+                         Q.get());
+      TRVG_LAUNCH_CHECK();
+    }
+    {
+      // The histogram is read once; partials written and reduced once.
+      ProfScope ps(ctx, KC_SHOT_PAIRS, double(nq) * 16. + 2. * nchunks * P * 16.);
+      hipLaunchKernelGGL(k_shot_pairs_tab, dim3(unsigned(nchunks)), dim3(256), smem,
+                         ctx.stream, Q.get(), nq, g.dr[0], sa.dev(), sb.dev(), dka.get(),
+                         nka, dkb.get(), nkb, dp.get(), P, nch, g.vol_cell, part.get());
+      TRVG_LAUNCH_CHECK();
+      reduce_cols(ctx, part.get(), int(nchunks), P, res.get());
+    }
     std::vector<double2> h(P);
     TRVG_HIP(hipMemcpyAsync(h.data(), res.get(), P * sizeof(double2),
                             hipMemcpyDeviceToHost, ctx.stream));
@@ -860,6 +904,11 @@ static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
     for (int p = 0; p < P; p++) out[p] = cplx(h[p].x, h[p].y);
     return out;
   }
+  std::vector<double> ka(P), kb(P);
+  for (int d = 0; d < P; d++) { ka[d] = keff[ra[d]]; kb[d] = keff[cb[d]]; }
+  DevArray<double> dka(P), dkb(P);
+  TRVG_HIP(hipMemcpyAsync(dka.get(), ka.data(), P * 8, hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(dkb.get(), kb.data(), P * 8, hipMemcpyHostToDevice, ctx.stream));
   const int GP = 8;
   unsigned grid = grid_for(ctx, g.nmesh, 4);
   for (int p0 = 0; p0 < P; p0 += GP) {
@@ -940,14 +989,6 @@ static std::vector<Term> enumerate_terms(const trvg_params& p, bool survey) {
 static cplx ipow(int n) {
   static const cplx I(0., 1.);
   return std::pow(I, n);  // threept.cpp:271-272
-}
-
-// Distinct bins of a side and the slab index of each bin.
-static std::vector<int> distinct(const std::vector<int>& v) {
-  std::vector<int> u(v);
-  std::sort(u.begin(), u.end());
-  u.erase(std::unique(u.begin(), u.end()), u.end());
-  return u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1128,18 +1169,22 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
         cplx sijk = tm.coupling * Sbar;
         for (int dv = 0; dv < dim; dv++) out.shot[dv] += sijk;
       }
+      TwoPt st1;
       if (p.ell2 == 0) {
-        TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb,
-                                         modes_in_range);
+        st1 = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb,
+                                    modes_in_range);
         for (int dv = 0; dv < dim; dv++) {
           int bb = ra[dv];
-          cplx sv = tm.coupling * (st.pk[bb] - st.sn[bb]);
+          cplx sv = tm.coupling * (st1.pk[bb] - st1.sn[bb]);
           out.shot[dv] += sv + tm.f_mirror * std::conj(sv);
         }
       }
       if (p.ell1 == 0) {
-        TwoPt st = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell2, tm.m2, kb,
-                                         modes_in_range);
+        // Both sides monopoles: the same statistics as the first side.
+        TwoPt st = (p.ell2 == 0 && tm.m1 == tm.m2)
+                       ? st1
+                       : twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell2,
+                                               tm.m2, kb, modes_in_range);
         for (int dv = 0; dv < dim; dv++) {
           int bb = cb[dv];
           cplx sv = tm.coupling * (st.pk[bb] - st.sn[bb]);
@@ -1150,7 +1195,7 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
                            dmk(Sbar.real(), Sbar.imag()), 1. / g.vol};
       fft3d<double>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
       std::vector<cplx> sij = shot_pairs_bispec<double>(
-          ctx, tfield.get(), p.ell1, tm.m1, p.ell2, tm.m2, sja, sjb, out.e1, out.e2);
+          ctx, tfield.get(), p.ell1, tm.m1, p.ell2, tm.m2, sja, sjb, ss.keff, ra, cb);
       for (int dv = 0; dv < dim; dv++) {
         cplx sv = tm.coupling * sij[dv];
         out.shot[dv] += factor_phase * (sv + tm.f_mirror_w3j * std::conj(sv));

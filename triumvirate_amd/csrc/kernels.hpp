@@ -504,42 +504,71 @@ k_3pcf_shot_bins(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
 // histogrammed into Q[n] and each pair then costs one pass over n.
 // ---------------------------------------------------------------------------
 
-// One block per pair: S_p = V_cell sum_n j_a(ka_p r_n) j_b(kb_p r_n) Q[n].
-__device__ __forceinline__ void k_shot_pairs_hist_body(const double* __restrict__ Q, long long nq, double dr,
-                  SplineDev sa, SplineDev sb, const double* __restrict__ ka,
-                  const double* __restrict__ kb, double vol_cell,
-                  double2* __restrict__ out) {
-  __shared__ double rs[RED_THREADS], is[RED_THREADS];
-  const int p = blockIdx.x;
-  const double kap = ka[p], kbp = kb[p];
-  double sx = 0., sy = 0.;
-  for (long long n = threadIdx.x; n < nq; n += blockDim.x) {
-    double qx = Q[2 * n], qy = Q[2 * n + 1];
-    if (qx == 0. && qy == 0.) continue;
-    double r = sqrt(double(n)) * dr;
-    double jj = sa.eval(kap * r) * sb.eval(kbp * r);
-    sx += jj * qx;
-    sy += jj * qy;
+// S_p = V_cell sum_n j_a(ka[p.x] r_n) j_b(kb[p.y] r_n) Q[n], r_n = sqrt(n) dr,
+// over entries with Q[n] != 0.  The pairs index two short lists of distinct
+// wavenumbers (one per bin of each side), so a block evaluates the Bessel
+// splines once per (wavenumber, n) of its chunk of `nch` radii into LDS and
+// each thread then reduces pairs from the tables.  Output: per-block
+// partials [gridDim.x][P] for a column reduction.
+__device__ __forceinline__ void k_shot_pairs_tab_body(
+    const double* __restrict__ Q, long long nq, double dr, SplineDev sa,
+    SplineDev sb, const double* __restrict__ ka, int nka,
+    const double* __restrict__ kb, int nkb, const int2* __restrict__ pidx, int P,
+    int nch, double vol_cell, double2* __restrict__ part) {
+  extern __shared__ double sp_lds[];
+  const int ld = nch + 1;                   // odd row stride: rows on distinct banks
+  double* ja = sp_lds;                      // [nka][ld]
+  double* jb = ja + (size_t)nka * ld;       // [nkb][ld]
+  double* qs = jb + (size_t)nkb * ld;       // [nch][2]
+  const long long n0 = (long long)blockIdx.x * nch;
+  const int cnt = int(min((long long)nch, nq - n0));
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+    qs[2 * i] = Q[2 * (n0 + i)];
+    qs[2 * i + 1] = Q[2 * (n0 + i) + 1];
   }
-  rs[threadIdx.x] = sx;
-  is[threadIdx.x] = sy;
+  const int nev = (nka + nkb) * cnt;
+  for (int e = threadIdx.x; e < nev; e += blockDim.x) {
+    const int row = e / cnt, i = e - row * cnt;
+    const double r = sqrt(double(n0 + i)) * dr;
+    if (row < nka) ja[row * ld + i] = sa.eval(ka[row] * r);
+    else jb[(row - nka) * ld + i] = sb.eval(kb[row - nka] * r);
+  }
   __syncthreads();
-  for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      rs[threadIdx.x] += rs[threadIdx.x + s];
-      is[threadIdx.x] += is[threadIdx.x + s];
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    const int2 q = pidx[p];
+    const double* a = ja + q.x * ld;
+    const double* c = jb + q.y * ld;
+    double sx = 0., sy = 0.;
+    for (int i = 0; i < cnt; i++) {
+      const double qx = qs[2 * i], qy = qs[2 * i + 1];
+      if (qx == 0. && qy == 0.) continue;
+      const double jj = a[i] * c[i];
+      sx += jj * qx;
+      sy += jj * qy;
     }
-    __syncthreads();
+    part[(long long)blockIdx.x * P + p] = dmk(sx * vol_cell, sy * vol_cell);
   }
-  if (threadIdx.x == 0) out[p] = dmk(rs[0] * vol_cell, is[0] * vol_cell);
 }
 
-static __global__ void __launch_bounds__(RED_THREADS)
-k_shot_pairs_hist(const double* __restrict__ Q, long long nq, double dr,
-                  SplineDev sa, SplineDev sb, const double* __restrict__ ka,
-                  const double* __restrict__ kb, double vol_cell,
-                  double2* __restrict__ out) {
-  k_shot_pairs_hist_body(Q, nq, dr, sa, sb, ka, kb, vol_cell, out);
+static __global__ void __launch_bounds__(256)
+k_shot_pairs_tab(const double* __restrict__ Q, long long nq, double dr,
+                 SplineDev sa, SplineDev sb, const double* __restrict__ ka, int nka,
+                 const double* __restrict__ kb, int nkb,
+                 const int2* __restrict__ pidx, int P, int nch, double vol_cell,
+                 double2* __restrict__ part) {
+  k_shot_pairs_tab_body(Q, nq, dr, sa, sb, ka, nka, kb, nkb, pidx, P, nch, vol_cell,
+                        part);
+}
+
+// Radii per block of k_shot_pairs_tab (at most 128, LDS within 64 KiB) and
+// its dynamic LDS bytes.
+inline int shot_tab_chunk(int nka, int nkb) {
+  const int rows = nka + nkb;
+  const int fit = (8192 - rows) / (rows + 2);
+  return fit < 1 ? 1 : (fit > 128 ? 128 : fit);
+}
+inline size_t shot_tab_smem(int nka, int nkb, int nch) {
+  return ((size_t)(nka + nkb) * (nch + 1) + 2 * (size_t)nch) * sizeof(double);
 }
 
 // General cells: direct sum over the mesh for a group of pairs.
