@@ -152,18 +152,14 @@ struct LoadTwoptMocks {
 
 template <typename T, int MODE>
 static __global__ void __launch_bounds__(256)
-k_pair_tiles3_batch(const c2<T>* __restrict__ FA, int na, long long sa,
-                    const c2<T>* __restrict__ FB, int nbf, long long sb,
-                    const c2<T>* __restrict__ G, long long sg, long long nmesh,
-                    const int2* __restrict__ tiles, int ntiles,
-                    double2* __restrict__ part, long long sp) {
+k_pair_lds_batch(const c2<T>* __restrict__ FA, int na, long long sa,
+                 const c2<T>* __restrict__ FB, int nbf, long long sb,
+                 const c2<T>* __restrict__ G, long long sg, long long nmesh,
+                 const int2* __restrict__ tiles, int ntiles,
+                 double2* __restrict__ part, long long sp) {
   const long long b = blockIdx.y;
-  if constexpr (MODE == 2)
-    k_pair_real_body<T>(FA + b * sa, na, FB + b * sb, nbf, G + b * sg, nmesh, tiles,
-                        ntiles, part + b * sp);
-  else
-    k_pair_tiles3_body<T, MODE>(FA + b * sa, na, FB + b * sb, nbf, G + b * sg, nmesh,
-                                tiles, ntiles, part + b * sp);
+  k_pair_lds_body<T, MODE>(FA + b * sa, na, FB + b * sb, nbf, G + b * sg, nmesh, tiles,
+                           ntiles, part + b * sp);
 }
 
 template <typename V>

@@ -507,6 +507,19 @@ class BatchPlan : public BatchPlanBase {
     TRVG_LAUNCH_CHECK();
   }
 
+  template <int MODE>
+  void launch_pairs_mode(const c2<T>* A, int nas, long long sa, const c2<T>* Bs, int nbs,
+                         long long sb, PairGroup& gr, size_t smem) {
+    if (smem > 64 * 1024)
+      TRVG_HIP(hipFuncSetAttribute((const void*)k_pair_lds_batch<T, MODE>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(smem)));
+    hipLaunchKernelGGL((k_pair_lds_batch<T, MODE>), dim3(tune_.pair_blocks, B_), dim3(256),
+                       smem, ctx_.stream, A, nas, sa, Bs, nbs, sb, G_.get(), nm_, nm_,
+                       gr.tiles.get(), gr.nt, pair_part_.get(),
+                       (long long)tune_.pair_blocks * gr.nt * 16);
+    TRVG_LAUNCH_CHECK();
+  }
+
   void launch_pairs(TermPlan& tp, PairGroup& gr) {
     const int ia = tp.swap ? tp.slab_b : tp.slab_a;
     const int ib = tp.swap ? tp.slab_a : tp.slab_b;
@@ -517,23 +530,16 @@ class BatchPlan : public BatchPlanBase {
     const c2<T>* Bs = slab_data_[ib].get() + (long long)b0 * nm_;
     const long long sa = (long long)slab_meshes(ia) * nm_;
     const long long sb = (long long)slab_meshes(ib) * nm_;
-    const size_t smem = pair_smem<T>(tp.mode, gr.nas, gr.nbs);
-    const dim3 grid(tune_.pair_blocks, B_);
-    const long long sp = (long long)tune_.pair_blocks * gr.nt * 16;
-    if (tp.mode == 2)
-      hipLaunchKernelGGL((k_pair_tiles3_batch<T, 2>), grid, dim3(256), smem, ctx_.stream,
-                         A, gr.nas, sa, Bs, gr.nbs, sb, G_.get(), nm_, nm_, gr.tiles.get(),
-                         gr.nt, pair_part_.get(), sp);
-    else if (tp.mode == 1)
-      hipLaunchKernelGGL((k_pair_tiles3_batch<T, 1>), grid, dim3(256), smem, ctx_.stream,
-                         A, gr.nas, sa, Bs, gr.nbs, sb, G_.get(), nm_, nm_, gr.tiles.get(),
-                         gr.nt, pair_part_.get(), sp);
-    else
-      hipLaunchKernelGGL((k_pair_tiles3_batch<T, 0>), grid, dim3(256), smem, ctx_.stream,
-                         A, gr.nas, sa, Bs, gr.nbs, sb, G_.get(), nm_, nm_, gr.tiles.get(),
-                         gr.nt, pair_part_.get(), sp);
-    TRVG_LAUNCH_CHECK();
+    const int mode = (tp.mode == 2 && A == Bs && gr.nas == gr.nbs && sa == sb) ? 3 : tp.mode;
+    const size_t smem = pair_lds_smem<T>(mode, gr.nas, gr.nbs);
+    switch (mode) {
+      case 3: launch_pairs_mode<3>(A, gr.nas, sa, Bs, gr.nbs, sb, gr, smem); break;
+      case 2: launch_pairs_mode<2>(A, gr.nas, sa, Bs, gr.nbs, sb, gr, smem); break;
+      case 1: launch_pairs_mode<1>(A, gr.nas, sa, Bs, gr.nbs, sb, gr, smem); break;
+      default: launch_pairs_mode<0>(A, gr.nas, sa, Bs, gr.nbs, sb, gr, smem); break;
+    }
   }
+
 
   void run_multipole(MultipolePlan& P) {
     const Geom& g = ctx_.g;

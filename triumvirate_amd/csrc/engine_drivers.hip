@@ -546,7 +546,9 @@ static void assign_order(Context& ctx, c2<T>* mesh, const TiledView& tv,
 template <typename T>
 static void assign(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c,
                    int mode, int ell, int m, double scale) {
-  int cplx_w = ((mode == WM_YLM_W || mode == WM_YLMC_W2) && !(ell == 0 && m == 0));
+  // y_l0 is real (its imaginary part is a signed zero): m = 0 fields take
+  // real parts only.
+  int cplx_w = ((mode == WM_YLM_W || mode == WM_YLMC_W2) && m != 0);
   if ((mode == WM_YLM_W || mode == WM_YLMC_W2) && c.los.get() == nullptr)
     throw Error(TRVG_ERR_DATA, "Lines of sight are required for survey-type fields.");
   const TiledView& tv = tiled_view(ctx, c);
@@ -560,6 +562,59 @@ static void assign(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c,
     case 2: assign_order<T, 2>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
     case 3: assign_order<T, 3>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
     default: assign_order<T, 4>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
+  }
+}
+
+template <typename T, int ORDER>
+static void assign_dual_order(Context& ctx, c2<T>* fa, c2<T>* fb, const TiledView& tv,
+                              int ell, int m, double scale_a, double scale_b) {
+  DevParticles sv;
+  sv.x = tv.x.get(); sv.y = tv.y.get(); sv.z = tv.z.get(); sv.w = tv.w.get();
+  sv.los = tv.los.get();
+  sv.n = tv.nin + tv.nstraggle;
+  if (tv.nwork > 0) {
+    const bool cplx = m != 0;
+    const size_t smem = size_t(cplx ? 4 : 2) * TILE_CELLS * sizeof(double);
+    if (cplx)
+      hipLaunchKernelGGL((k_assign_tiled_dual<T, ORDER, true>), dim3(tv.nwork), dim3(256),
+                         smem, ctx.stream, fa, fb, sv, ctx.g, tv.tg, tv.work.get(), ell, m,
+                         scale_a, scale_b);
+    else
+      hipLaunchKernelGGL((k_assign_tiled_dual<T, ORDER, false>), dim3(tv.nwork), dim3(256),
+                         smem, ctx.stream, fa, fb, sv, ctx.g, tv.tg, tv.work.get(), ell, m,
+                         scale_a, scale_b);
+    TRVG_LAUNCH_CHECK();
+  }
+  if (tv.nstraggle > 0) {
+    DevParticles st = sv;
+    st.x += tv.nin; st.y += tv.nin; st.z += tv.nin; st.w += tv.nin;
+    if (st.los) st.los += 3 * tv.nin;
+    st.n = tv.nstraggle;
+    const int cw = m != 0;
+    hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(grid_for(ctx, st.n, 16)), dim3(256), 0,
+                       ctx.stream, fa, st, ctx.g, WM_YLM_W, ell, m, scale_a, cw, 0);
+    hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(grid_for(ctx, st.n, 16)), dim3(256), 0,
+                       ctx.stream, fb, st, ctx.g, WM_YLMC_W2, ell, m, scale_b, cw, 0);
+    TRVG_LAUNCH_CHECK();
+  }
+}
+
+// fa += scale_a sum y_lm w W / V_cell and fb += scale_b sum y_lm^* w^2 W / V_cell
+// over catalogue c in one pass (survey catalogues: lines of sight needed).
+template <typename T>
+static void assign_dual(Context& ctx, c2<T>* fa, c2<T>* fb, const DeviceCatalogue& c,
+                        int ell, int m, double scale_a, double scale_b) {
+  if (c.los.get() == nullptr)
+    throw Error(TRVG_ERR_DATA, "Lines of sight are required for survey-type fields.");
+  const TiledView& tv = tiled_view(ctx, c);
+  // Algorithmic bytes: position, weight and line of sight read once, both
+  // meshes written once.
+  ProfScope ps(ctx, KC_ASSIGN, c.n * 56. + 2. * double(ctx.g.nmesh) * sizeof(c2<T>));
+  switch (ctx.p.assignment_order) {
+    case 1: assign_dual_order<T, 1>(ctx, fa, fb, tv, ell, m, scale_a, scale_b); break;
+    case 2: assign_dual_order<T, 2>(ctx, fa, fb, tv, ell, m, scale_a, scale_b); break;
+    case 3: assign_dual_order<T, 3>(ctx, fa, fb, tv, ell, m, scale_a, scale_b); break;
+    default: assign_dual_order<T, 4>(ctx, fa, fb, tv, ell, m, scale_a, scale_b); break;
   }
 }
 
@@ -595,6 +650,25 @@ static void build_field_k(Context& ctx, c2<T>* f, const DeviceCatalogue& d,
   fft3d<T>(ctx, f, ld, -1);
 }
 
+// Survey pair of one (l, m): the density contrast
+//   dn = sum_d y w - alpha sum_r y w
+// and its shot-noise field
+//   N = sum_d y^* w^2 + alpha^2 sum_r y^* w^2,
+// both assigned in one pass per catalogue and forward transformed.
+template <typename T>
+static void build_survey_pair_k(Context& ctx, c2<T>* dn, c2<T>* N,
+                                const DeviceCatalogue& d, const DeviceCatalogue& r,
+                                double alpha, int ell, int m) {
+  zero_field<T>(ctx, dn);
+  zero_field<T>(ctx, N);
+  assign_dual<T>(ctx, dn, N, d, ell, m, 1., 1.);
+  assign_dual<T>(ctx, dn, N, r, ell, m, -alpha, alpha * alpha);
+  LoadScaleShift<T> la{dn, T(0), T(ctx.g.vol_cell)};
+  fft3d<T>(ctx, dn, la, -1);
+  LoadScaleShift<T> lb{N, T(0), T(ctx.g.vol_cell)};
+  fft3d<T>(ctx, N, lb, -1);
+}
+
 // ---------------------------------------------------------------------------
 // Reductions
 // ---------------------------------------------------------------------------
@@ -628,6 +702,20 @@ ShellStats shell_stats(Context& ctx, const trvg_binning& kb) {
   return s;
 }
 
+// Launch of the direct-to-LDS pair kernel (opting in to more than 64 KiB of
+// dynamic LDS where a launch needs it).
+template <typename T, int MODE>
+static void launch_pair_lds(Context& ctx, unsigned grid, size_t smem, const c2<T>* fa,
+                            int nas, const c2<T>* fb, int nbs, const c2<T>* G,
+                            long long nm, const int2* tiles, int nt, double2* part) {
+  if (smem > 64 * 1024)
+    TRVG_HIP(hipFuncSetAttribute((const void*)k_pair_lds<T, MODE>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(smem)));
+  hipLaunchKernelGGL((k_pair_lds<T, MODE>), dim3(grid), dim3(256), smem, ctx.stream, fa,
+                     nas, fb, nbs, G, nm, tiles, nt, part);
+  TRVG_LAUNCH_CHECK();
+}
+
 // S_p = sum_x F_a(x) F_b(x) G(x) for pairs p = (a, b) of logical fields.
 // A side / B side packed: real fields stored two per complex mesh (logical
 // field 2q + c is component c of mesh q); g_real: G is real.
@@ -653,12 +741,14 @@ static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
   for (const int2& pr : pairs) active[(size_t)(pr.x / 4) * cbn + pr.y / 4] = 1;
   // Sums for every pair of every active tile, indexed [ia * nbf + ib].
   std::vector<cplx> full((size_t)na * nbf, cplx(0., 0.));
-  // Column blocks per launch (at most 32 fields a side: the v3 kernel
-  // stages PT3_SLOTS * 4 fields per side) and row blocks to match.
+  // Column blocks per launch and row blocks to match (at most 32 fields a
+  // side and PT_MAXT tiles: the double-buffered staging of 65 rows fits the
+  // 160 KiB of LDS).
   const int CB = std::min(cbn, 8);
   const int RB = std::max(1, std::min(std::min((64 - 4 * CB) / 4, PT_MAXT / CB), 8));
-  const long long nchunks = (nm + PT_CELLS - 1) / PT_CELLS;
-  const unsigned grid = unsigned(std::min<long long>(nchunks, (long long)ctx.sm_count * 4));
+  if (nm % PairLds<T>::LDC != 0)
+    throw Error(TRVG_ERR_INTERNAL, "Pair reduction needs whole chunks of the mesh.");
+  const long long nchunks = nm / PairLds<T>::LDC;
   for (int c0b = 0; c0b < cbn; c0b += CB) {
     const int c1b = std::min(cbn, c0b + CB);
     for (int r0b = 0; r0b < rbn; r0b += RB) {
@@ -675,24 +765,26 @@ static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
       DevArray<int2> dt(nt);
       TRVG_HIP(hipMemcpyAsync(dt.get(), tl.data(), nt * sizeof(int2),
                               hipMemcpyHostToDevice, ctx.stream));
+      // Resident blocks only (each loops over chunks): as many per CU as
+      // the double-buffered staging allows.
+      const bool same_side = mode == 2 && FA == FB && ra0 == ca0 && nas == nbs;
+      const int kmode = same_side ? 3 : mode;
+      const size_t smem = pair_lds_smem<T>(kmode, nas, nbs);
+      const int per_cu = std::max(1, std::min(8, int((160 * 1024) / smem)));
+      const unsigned grid = unsigned(std::min<long long>(nchunks, (long long)ctx.sm_count * per_cu));
       DevArray<double2> part((size_t)grid * nt * 16), sum((size_t)nt * 16);
-      const size_t smem = pair_smem<T>(mode, nas, nbs);
       {
-        const double ma = mode == 2 ? (nas + 1) / 2 : nas;
-        const double mb = mode >= 1 ? (nbs + 1) / 2 : nbs;
-        ProfScope ps(ctx, KC_PAIR_REDUCE, (ma + mb + 1.) * nm * sizeof(c2<T>));
         // Packed sides start at mesh ra0 / 2 (group starts are multiples of 4).
         const c2<T>* fa = FA + (long long)(mode == 2 ? ra0 / 2 : ra0) * nm;
         const c2<T>* fb = FB + (long long)(mode >= 1 ? ca0 / 2 : ca0) * nm;
-        if (mode == 2)
-          hipLaunchKernelGGL((k_pair_tiles3<T, 2>), dim3(grid), dim3(256), smem,
-                             ctx.stream, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get());
-        else if (mode == 1)
-          hipLaunchKernelGGL((k_pair_tiles3<T, 1>), dim3(grid), dim3(256), smem,
-                             ctx.stream, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get());
-        else
-          hipLaunchKernelGGL((k_pair_tiles3<T, 0>), dim3(grid), dim3(256), smem,
-                             ctx.stream, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get());
+        const double ma = pair_rows_a(kmode, nas), mb = pair_rows_b(kmode, nbs);
+        ProfScope ps(ctx, KC_PAIR_REDUCE, (ma + mb + 1.) * nm * sizeof(c2<T>));
+        switch (kmode) {
+          case 3: launch_pair_lds<T, 3>(ctx, grid, smem, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get()); break;
+          case 2: launch_pair_lds<T, 2>(ctx, grid, smem, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get()); break;
+          case 1: launch_pair_lds<T, 1>(ctx, grid, smem, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get()); break;
+          default: launch_pair_lds<T, 0>(ctx, grid, smem, fa, nas, fb, nbs, G, nm, dt.get(), nt, part.get()); break;
+        }
         TRVG_LAUNCH_CHECK();
       }
       reduce_cols(ctx, part.get(), int(grid), nt * 16, sum.get());
@@ -1074,8 +1166,12 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
   // Common fields in FP64 whatever the shell precision T: the shot-noise
   // path (two-point statistics, S_ij_k) subtracts nearly equal quantities.
   DevArray<double2> dn00(nm), N00(nm);
-  build_field_k<double>(ctx, dn00.get(), d, r, survey, alpha, 0, 0, false, true);
-  build_field_k<double>(ctx, N00.get(), d, r, survey, alpha, 0, 0, true, false);
+  if (survey) {
+    build_survey_pair_k<double>(ctx, dn00.get(), N00.get(), d, *r, alpha, 0, 0);
+  } else {
+    build_field_k<double>(ctx, dn00.get(), d, r, false, alpha, 0, 0, false, true);
+    build_field_k<double>(ctx, N00.get(), d, r, false, alpha, 0, 0, true, false);
+  }
   ShellStats ss = shell_stats(ctx, kb);
   double modes_in_range = 0.;
   for (int n : ss.nmodes) modes_in_range += n;
@@ -1127,8 +1223,7 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
       const double2* NLMp = N00.get();
       if (!lm_zero) {
         if (!dnLM.get()) { dnLM.reset(nm); NLM.reset(nm); }
-        build_field_k<double>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
-        build_field_k<double>(ctx, NLM.get(), d, r, true, alpha, p.ELL, tm.M, true, false);
+        build_survey_pair_k<double>(ctx, dnLM.get(), NLM.get(), d, *r, alpha, p.ELL, tm.M);
         dnLMp = dnLM.get();
         NLMp = NLM.get();
       }
@@ -1245,8 +1340,12 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
   out.raw.assign(dim, 0.); out.shot.assign(dim, 0.);
 
   DevArray<double2> dn00(nm), N00(nm);
-  build_field_k<double>(ctx, dn00.get(), d, r, survey, alpha, 0, 0, false, true);
-  build_field_k<double>(ctx, N00.get(), d, r, survey, alpha, 0, 0, true, false);
+  if (survey) {
+    build_survey_pair_k<double>(ctx, dn00.get(), N00.get(), d, *r, alpha, 0, 0);
+  } else {
+    build_field_k<double>(ctx, dn00.get(), d, r, false, alpha, 0, 0, false, true);
+    build_field_k<double>(ctx, N00.get(), d, r, false, alpha, 0, 0, true, false);
+  }
 
   const Spline& sja = cached_spline(ctx.dev, p.ell1, ctx.stream);
   const Spline& sjb = cached_spline(ctx.dev, p.ell2, ctx.stream);

@@ -808,6 +808,69 @@ struct TileWork {
   long long start;
 };
 
+// Window weights of a particle along the three axes and the tile-local
+// index (halo origin o0) of the first covered cell per axis.
+template <int ORDER>
+__device__ __forceinline__ void tile_stencil(const Geom& g, const int o0[3],
+                                             double x, double y, double z,
+                                             int lo[3], double win[ORDER][3]) {
+  int idx[3];
+  double s[3];
+  base_cell(g, x, y, z, idx, s);
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const double sa = s[a];
+    int first = idx[a];
+    if (ORDER == 1) {
+      if (sa >= 0.5) first = idx[a] + 1;
+      win[0][a] = 1.;
+    } else if (ORDER == 2) {
+      win[0][a] = 1. - sa;
+      win[1][a] = sa;
+    } else if (ORDER == 3) {
+      if (sa < 0.5) {
+        first = idx[a] - 1;
+        win[0][a] = 0.5 * (0.5 - sa) * (0.5 - sa);
+        win[1][a] = 0.75 - sa * sa;
+        win[2][a] = 0.5 * (0.5 + sa) * (0.5 + sa);
+      } else {
+        double sp = 1. - sa;
+        win[0][a] = 0.5 * (0.5 + sp) * (0.5 + sp);
+        win[1][a] = 0.75 - sp * sp;
+        win[2][a] = 0.5 * (0.5 - sp) * (0.5 - sp);
+      }
+    } else {
+      first = idx[a] - 1;
+      double sm = 1. - sa;
+      win[0][a] = 1. / 6 * sm * sm * sm;
+      win[1][a] = 1. / 6 * (4. - 6. * sa * sa + 3. * sa * sa * sa);
+      win[2][a] = 1. / 6 * (4. - 6. * sm * sm + 3. * sm * sm * sm);
+      win[3][a] = 1. / 6 * sa * sa * sa;
+    }
+    lo[a] = first - o0[a];
+  }
+}
+
+// Tile halo origin of work item wk.
+__device__ __forceinline__ void tile_origin(const TileGeom& tg, int t, int o0[3]) {
+  const int tz = t % tg.nt[2];
+  const int ty = (t / tg.nt[2]) % tg.nt[1];
+  const int tx = t / (tg.nt[2] * tg.nt[1]);
+  o0[0] = tx * TILE - 1;
+  o0[1] = ty * TILE - 1;
+  o0[2] = tz * TILE - 1;
+}
+
+// Periodic mesh cell of tile-local cell c.
+__device__ __forceinline__ long long tile_cell(const Geom& g, const int o0[3], int c) {
+  const int lz = c % TILE_H, ly = (c / TILE_H) % TILE_H, lx = c / (TILE_H * TILE_H);
+  int gx = o0[0] + lx, gy = o0[1] + ly, gz = o0[2] + lz;
+  gx = (gx % g.n[0] + g.n[0]) % g.n[0];
+  gy = (gy % g.n[1] + g.n[1]) % g.n[1];
+  gz = (gz % g.n[2] + g.n[2]) % g.n[2];
+  return ((long long)gx * g.n[1] + gy) * g.n[2] + gz;
+}
+
 template <typename T, int ORDER>
 __global__ void __launch_bounds__(256)
 k_assign_tiled(c2<T>* __restrict__ mesh, DevParticles p, Geom g, TileGeom tg,
@@ -820,52 +883,15 @@ k_assign_tiled(c2<T>* __restrict__ mesh, DevParticles p, Geom g, TileGeom tg,
     acc_re[c] = 0.;
     acc_im[c] = 0.;
   }
-  const int t = wk.tile;
-  const int tz = t % tg.nt[2];
-  const int ty = (t / tg.nt[2]) % tg.nt[1];
-  const int tx = t / (tg.nt[2] * tg.nt[1]);
-  const int o0[3] = {tx * TILE - 1, ty * TILE - 1, tz * TILE - 1};
+  int o0[3];
+  tile_origin(tg, wk.tile, o0);
   const double inv_vol_cell = 1 / g.vol_cell;
   __syncthreads();
   for (int q = threadIdx.x; q < wk.count; q += blockDim.x) {
     const long long pid = wk.start + q;
-    int idx[3];
-    double s[3];
-    base_cell(g, p.x[pid], p.y[pid], p.z[pid], idx, s);
-    int lo[3];            // local LDS index of the first covered cell
+    int lo[3];
     double win[ORDER][3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      const double sa = s[a];
-      int first = idx[a];
-      if (ORDER == 1) {
-        if (sa >= 0.5) first = idx[a] + 1;
-        win[0][a] = 1.;
-      } else if (ORDER == 2) {
-        win[0][a] = 1. - sa;
-        win[1][a] = sa;
-      } else if (ORDER == 3) {
-        if (sa < 0.5) {
-          first = idx[a] - 1;
-          win[0][a] = 0.5 * (0.5 - sa) * (0.5 - sa);
-          win[1][a] = 0.75 - sa * sa;
-          win[2][a] = 0.5 * (0.5 + sa) * (0.5 + sa);
-        } else {
-          double sp = 1. - sa;
-          win[0][a] = 0.5 * (0.5 + sp) * (0.5 + sp);
-          win[1][a] = 0.75 - sp * sp;
-          win[2][a] = 0.5 * (0.5 - sp) * (0.5 - sp);
-        }
-      } else {
-        first = idx[a] - 1;
-        double sm = 1. - sa;
-        win[0][a] = 1. / 6 * sm * sm * sm;
-        win[1][a] = 1. / 6 * (4. - 6. * sa * sa + 3. * sa * sa * sa);
-        win[2][a] = 1. / 6 * (4. - 6. * sm * sm + 3. * sm * sm * sm);
-        win[3][a] = 1. / 6 * sa * sa * sa;
-      }
-      lo[a] = first - o0[a];
-    }
+    tile_stencil<ORDER>(g, o0, p.x[pid], p.y[pid], p.z[pid], lo, win);
     double2 wt = particle_weight(p, pid, mode, ell, m);
     const double wre = inv_vol_cell * wt.x * scale;
     const double wim = inv_vol_cell * wt.y * scale;
@@ -888,14 +914,80 @@ k_assign_tiled(c2<T>* __restrict__ mesh, DevParticles p, Geom g, TileGeom tg,
   for (int c = threadIdx.x; c < TILE_CELLS; c += blockDim.x) {
     const double vr = acc_re[c], vi = acc_im[c];
     if (vr == 0. && vi == 0.) continue;
-    const int lz = c % TILE_H, ly = (c / TILE_H) % TILE_H, lx = c / (TILE_H * TILE_H);
-    int gx = o0[0] + lx, gy = o0[1] + ly, gz = o0[2] + lz;
-    gx = (gx % g.n[0] + g.n[0]) % g.n[0];
-    gy = (gy % g.n[1] + g.n[1]) % g.n[1];
-    gz = (gz % g.n[2] + g.n[2]) % g.n[2];
-    T* cell = reinterpret_cast<T*>(mesh + ((long long)gx * g.n[1] + gy) * g.n[2] + gz);
+    T* cell = reinterpret_cast<T*>(mesh + tile_cell(g, o0, c));
     unsafeAtomicAdd(cell, T(vr));
     if (cplx_w) unsafeAtomicAdd(cell + 1, T(vi));
+  }
+}
+
+// Both survey fields of one (l, m) in one pass over the particles:
+//   mesh_a += scale_a  y_lm w   W / V_cell   (density contrast)
+//   mesh_b += scale_b  y_lm^* w^2 W / V_cell (its shot-noise field)
+// CPLX = false for m = 0 (real weights: real parts only).  Dynamic LDS:
+// (CPLX ? 4 : 2) * TILE_CELLS doubles.
+template <typename T, int ORDER, bool CPLX>
+__global__ void __launch_bounds__(256)
+k_assign_tiled_dual(c2<T>* __restrict__ mesh_a, c2<T>* __restrict__ mesh_b,
+                    DevParticles p, Geom g, TileGeom tg,
+                    const TileWork* __restrict__ work, int ell, int m,
+                    double scale_a, double scale_b) {
+  constexpr int NACC = CPLX ? 4 : 2;
+  extern __shared__ double acc_dual[];   // [NACC][TILE_CELLS]
+  const TileWork wk = work[blockIdx.x];
+  for (int c = threadIdx.x; c < NACC * TILE_CELLS; c += blockDim.x) acc_dual[c] = 0.;
+  int o0[3];
+  tile_origin(tg, wk.tile, o0);
+  const double inv_vol_cell = 1 / g.vol_cell;
+  double* a_re = acc_dual;
+  double* b_re = acc_dual + TILE_CELLS;
+  double* a_im = acc_dual + 2 * TILE_CELLS;
+  double* b_im = acc_dual + 3 * TILE_CELLS;
+  __syncthreads();
+  for (int q = threadIdx.x; q < wk.count; q += blockDim.x) {
+    const long long pid = wk.start + q;
+    int lo[3];
+    double win[ORDER][3];
+    tile_stencil<ORDER>(g, o0, p.x[pid], p.y[pid], p.z[pid], lo, win);
+    const double2 y = ylm_reduced_dev(ell, m, p.los[3 * pid], p.los[3 * pid + 1],
+                                      p.los[3 * pid + 2]);
+    const double w = p.w[pid];
+    const double fa = inv_vol_cell * w * scale_a;
+    const double fb = inv_vol_cell * (w * w) * scale_b;
+    const double are = y.x * fa, aim = y.y * fa;
+    const double bre = y.x * fb, bim = -y.y * fb;
+#pragma unroll
+    for (int il = 0; il < ORDER; il++) {
+#pragma unroll
+      for (int jl = 0; jl < ORDER; jl++) {
+        const int row = ((lo[0] + il) * TILE_H + (lo[1] + jl)) * TILE_H + lo[2];
+        const double wij = win[il][0] * win[jl][1];
+#pragma unroll
+        for (int kl = 0; kl < ORDER; kl++) {
+          const double wv = wij * win[kl][2];
+          atomicAdd(&a_re[row + kl], are * wv);
+          atomicAdd(&b_re[row + kl], bre * wv);
+          if (CPLX) {
+            atomicAdd(&a_im[row + kl], aim * wv);
+            atomicAdd(&b_im[row + kl], bim * wv);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < TILE_CELLS; c += blockDim.x) {
+    const double va = a_re[c], vb = b_re[c];
+    const double wa = CPLX ? a_im[c] : 0., wb = CPLX ? b_im[c] : 0.;
+    if (va == 0. && vb == 0. && wa == 0. && wb == 0.) continue;
+    const long long gc = tile_cell(g, o0, c);
+    T* ca = reinterpret_cast<T*>(mesh_a + gc);
+    T* cb = reinterpret_cast<T*>(mesh_b + gc);
+    unsafeAtomicAdd(ca, T(va));
+    unsafeAtomicAdd(cb, T(vb));
+    if (CPLX) {
+      unsafeAtomicAdd(ca + 1, T(wa));
+      unsafeAtomicAdd(cb + 1, T(wb));
+    }
   }
 }
 
@@ -903,18 +995,14 @@ k_assign_tiled(c2<T>* __restrict__ mesh, DevParticles p, Geom g, TileGeom tg,
 
 namespace trvg {
 
-// ---------------------------------------------------------------------------
-// Register-tiled pair product-reduction.
-//   S[ia][ib] = sum_x FA[ia](x) FB[ib](x) G(x)
-// A launch covers a list of 4x4 tiles of the (ia, ib) grid (up to PT_MAXT
-// tiles).  Each block stages PT_CELLS cells of every field in LDS (one HBM
-// read per field and cell), forms H = FB * G once, and thread (tile, group)
-// accumulates its 16 pair sums in FP64 registers over its cells of every
-// chunk the block visits.  Output: per-block partials [gridDim.x][ntiles*16].
-// ---------------------------------------------------------------------------
-
-constexpr int PT_CELLS = 64;
+// Most 4x4 pair tiles per pair-reduction launch.
 constexpr int PT_MAXT = 32;
+
+// U consecutive LDS elements read as one (16-byte aligned) access.
+template <typename E, int U>
+struct alignas(sizeof(E) * U >= 16 ? 16 : sizeof(E) * U) LdsPack {
+  E v[U];
+};
 
 // Radial histogram over octant representatives: the cells (+-a, +-b, +-c)
 // share n = a^2 + b^2 + c^2, so each thread sums its (up to 8) mirror cells
@@ -973,127 +1061,200 @@ k_radial_hist_oct(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
 
 namespace trvg {
 
-}  // namespace trvg
+// ---------------------------------------------------------------------------
+// Pair product-reduction with direct-to-LDS staging (gfx950
+// global_load_lds_dwordx4), double buffered:
+//   S[ia][ib] = sum_x FA[ia](x) FB[ib](x) G(x)
+// A chunk is LDC cells (1 KiB of c2<T> per mesh row: one wave-wide
+// 16-byte-per-lane load).  Rows: the MA meshes of the A side, the MB meshes
+// of the B side, then G.  While a block reduces chunk c from one buffer, the
+// loads of its next chunk land in the other; no register staging.  Thread
+// (tile, group) forms H = F_b G for its tile's four B fields per cell and
+// accumulates 16 pair sums in T within a chunk, promoted to FP64 once per
+// chunk.  MODE 0: complex sides; 1: B side real, two per mesh; 2: both sides
+// real, two per mesh, G real; 3: as 2 with both sides the same fields (read
+// once).  Requires nmesh % LDC == 0 (meshes are powers of two >= 8^3).
+// Output: per-block partials [gridDim.x][ntiles * 16].
+// ---------------------------------------------------------------------------
 
-namespace trvg {
+template <typename T>
+struct PairLds {
+  static constexpr int LDC = 1024 / int(sizeof(c2<T>));   // cells per chunk
+  static constexpr int ROWB = 1024 + 16;                  // row stride (bytes), bank skew
+  static constexpr int PER_LANE = 16 / int(sizeof(c2<T>)); // cells per lane load
+};
 
-// Register-tiled pair product-reduction
-//   S[ia][ib] = sum_x FA[ia](x) FB[ib](x) G(x).
-// A launch covers a list of 4x4 tiles of the (ia, ib) grid (up to PT_MAXT
-// tiles).  Each block stages PT_CELLS cells of every field in LDS (one HBM
-// read per field and cell), forms H = FB * G once, and thread (tile, group)
-// accumulates its 16 pair sums over its cells of every chunk the block
-// visits: in T within a chunk, promoted to FP64 accumulators once per
-// chunk.  The next chunk's global loads are issued into registers before
-// the current chunk is reduced.  Staging map: thread t owns cell
-// c = t % PT_CELLS of the chunk and fields f = t / PT_CELLS + k * (256 /
-// PT_CELLS), so each thread loads G[c] once.  Output: per-block partials
-// [gridDim.x][ntiles * 16].
-constexpr int PT3_SLOTS = 8;   // per-thread staged values per side (<= 32 fields)
+// Mesh rows of each side for a launch.
+__host__ __device__ inline int pair_rows_a(int mode, int na) {
+  return mode >= 2 ? (na + 1) / 2 : na;
+}
+__host__ __device__ inline int pair_rows_b(int mode, int nbf) {
+  return mode == 3 ? 0 : (mode >= 1 ? (nbf + 1) / 2 : nbf);
+}
 
-// MODE 0: complex fields.  MODE 1: the B fields are real and packed two per
-// complex mesh (logical field 2p + c is component c of mesh p).
+template <typename T>
+inline size_t pair_lds_smem(int mode, int nas, int nbs) {
+  const size_t rows = size_t(pair_rows_a(mode, nas) + pair_rows_b(mode, nbs) + 1);
+  const size_t buf = 2 * rows * PairLds<T>::ROWB;
+  return buf > 256 * sizeof(double2) ? buf : 256 * sizeof(double2);
+}
+
+// Two consecutive cells of one LDS row.
+template <typename T>
+__device__ __forceinline__ LdsPack<c2<T>, 2> lds_cells2(const char* row, int c) {
+  return *reinterpret_cast<const LdsPack<c2<T>, 2>*>(row + (size_t)c * sizeof(c2<T>));
+}
+
 template <typename T, int MODE>
-__device__ __forceinline__ void k_pair_tiles3_body(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
-              int nbf, const c2<T>* __restrict__ G, long long nmesh,
-              const int2* __restrict__ tiles, int ntiles,
-              double2* __restrict__ part) {
-  extern __shared__ char sm3_raw[];
-  c2<T>* sa = reinterpret_cast<c2<T>*>(sm3_raw);           // [na][PT_CELLS]
-  c2<T>* sh = sa + (size_t)na * PT_CELLS;                   // [nbf][PT_CELLS]
-  constexpr int FSTEP = 256 / PT_CELLS;                     // 4
-  const int ngroups = 256 / ntiles;
-  const int group = threadIdx.x % ngroups;
-  const int tile = threadIdx.x / ngroups;
+__device__ __forceinline__ void k_pair_lds_body(
+    const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB, int nbf,
+    const c2<T>* __restrict__ G, long long nmesh, const int2* __restrict__ tiles,
+    int ntiles, double2* __restrict__ part) {
+  using L = PairLds<T>;
+  constexpr bool CPLX = MODE <= 1;
+  extern __shared__ __attribute__((aligned(16))) char smp_raw[];
+  const int MA = pair_rows_a(MODE, na), MB = pair_rows_b(MODE, nbf);
+  const int R = MA + MB + 1;
+  const int ng = 256 / ntiles;
+  const int group = threadIdx.x % ng;
+  const int tile = threadIdx.x / ng;
   const bool active = tile < ntiles;
   const int2 t0 = tiles[active ? tile : 0];
+  // LDS rows read by this thread's tile (clamped to existing meshes; the
+  // host discards pairs past na / nbf).
+  int xr[4], yr[4];
+  if (MODE >= 2) {
+    const int a0 = t0.x / 2, b0 = t0.y / 2;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      xr[q] = min(a0 + q, MA - 1);
+      yr[q] = (MODE == 3) ? min(b0 + q, MA - 1) : MA + min(b0 + q, MB - 1);
+    }
+    xr[2] = xr[3] = yr[2] = yr[3] = 0;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) xr[q] = min(t0.x + q, na - 1);
+    if (MODE == 1) {
+      const int b0 = t0.y / 2;
+      yr[0] = MA + min(b0, MB - 1);
+      yr[1] = MA + min(b0 + 1, MB - 1);
+      yr[2] = yr[3] = 0;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) yr[q] = MA + min(t0.y + q, nbf - 1);
+    }
+  }
+  const int grow = MA + MB;
   double ar[16], ai[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) ar[q] = ai[q] = 0.;
-  int ia[4], ib[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    ia[q] = min(t0.x + q, na - 1) * PT_CELLS;
-    ib[q] = min(t0.y + q, nbf - 1) * PT_CELLS;
-  }
-  const int sc = threadIdx.x % PT_CELLS;   // staged cell
-  const int sf = threadIdx.x / PT_CELLS;   // first staged field
-  c2<T> ra[PT3_SLOTS], rb[PT3_SLOTS], rg;
-  const long long nchunks = (nmesh + PT_CELLS - 1) / PT_CELLS;
-  constexpr bool A_PACKED = false, B_PACKED = (MODE == 1);
-  // Staged meshes per side: packed sides hold two logical fields per mesh.
-  const int ma = A_PACKED ? (na + 1) / 2 : na;
-  const int mb = B_PACKED ? (nbf + 1) / 2 : nbf;
-  auto fetch = [&](long long ch) {
-    const long long gi = ch * PT_CELLS + sc;
-    const bool ok = (ch < nchunks) && (gi < nmesh);
-#pragma unroll
-    for (int k = 0; k < PT3_SLOTS; k++) {
-      const int f = sf + k * FSTEP;
-      ra[k] = (ok && f < ma) ? FA[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
-      rb[k] = (ok && f < mb) ? FB[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
+
+  const long long nchunks = nmesh / L::LDC;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  // Wave w loads rows w, w + 4, ... of chunk ch into buffer b.
+  auto issue = [&](long long ch, int b) {
+    char* base = smp_raw + (size_t)b * R * L::ROWB;
+    const long long off = ch * L::LDC + (long long)lane * L::PER_LANE;
+    for (int r = wave; r < R; r += 4) {
+      const c2<T>* src = r < MA ? FA + (long long)r * nmesh
+                                : (r < MA + MB ? FB + (long long)(r - MA) * nmesh : G);
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(src + off),
+          (__attribute__((address_space(3))) void*)(
+              (__attribute__((address_space(3))) char*)(base + (size_t)r * L::ROWB)),
+          16, 0, 0);
     }
-    rg = ok ? G[gi] : mk<T>(T(0), T(0));
   };
   long long ch = blockIdx.x;
-  fetch(ch);
+  int cur = 0;
+  if (ch < nchunks) issue(ch, 0);
   for (; ch < nchunks; ch += gridDim.x) {
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PT3_SLOTS; k++) {
-      const int f = sf + k * FSTEP;
-      if (A_PACKED) {
-        if (2 * f < na) sa[2 * f * PT_CELLS + sc] = mk<T>(ra[k].x, T(0));
-        if (2 * f + 1 < na) sa[(2 * f + 1) * PT_CELLS + sc] = mk<T>(ra[k].y, T(0));
-      } else if (f < na) {
-        sa[f * PT_CELLS + sc] = ra[k];
-      }
-      if (MODE == 1) {
-        if (2 * f < nbf) sh[2 * f * PT_CELLS + sc] = mk<T>(rb[k].x * rg.x, rb[k].x * rg.y);
-        if (2 * f + 1 < nbf)
-          sh[(2 * f + 1) * PT_CELLS + sc] = mk<T>(rb[k].y * rg.x, rb[k].y * rg.y);
-      } else if (f < nbf) {
-        sh[f * PT_CELLS + sc] = cmul<T>(rb[k], rg);
-      }
-    }
-    __syncthreads();
-    fetch(ch + gridDim.x);
+    if (ch + gridDim.x < nchunks) issue(ch + gridDim.x, cur ^ 1);
     if (active) {
+      const char* buf = smp_raw + (size_t)cur * R * L::ROWB;
       T pr[16], pi[16];
 #pragma unroll
       for (int q = 0; q < 16; q++) pr[q] = pi[q] = T(0);
-      for (int c = group; c < PT_CELLS; c += ngroups) {
-        c2<T> x[4], y[4];
+      for (int cb = group; cb < L::LDC / 2; cb += ng) {
+        const int c = 2 * cb;
+        const LdsPack<c2<T>, 2> gg = lds_cells2<T>(buf + (size_t)grow * L::ROWB, c);
+        if (MODE >= 2) {
+          LdsPack<c2<T>, 2> xa[2], yb[2];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          x[q] = sa[ia[q] + c];
-          y[q] = sh[ib[q] + c];
-        }
+          for (int q = 0; q < 2; q++) {
+            xa[q] = lds_cells2<T>(buf + (size_t)xr[q] * L::ROWB, c);
+            yb[q] = lds_cells2<T>(buf + (size_t)yr[q] * L::ROWB, c);
+          }
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+          for (int u = 0; u < 2; u++) {
+            const T g = gg.v[u].x;
+            T x[4], h[4];
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
-            pr[i * 4 + j] += x[i].x * y[j].x - x[i].y * y[j].y;
-            pi[i * 4 + j] += x[i].x * y[j].y + x[i].y * y[j].x;
+            for (int q = 0; q < 2; q++) {
+              x[2 * q] = xa[q].v[u].x;
+              x[2 * q + 1] = xa[q].v[u].y;
+              h[2 * q] = yb[q].v[u].x * g;
+              h[2 * q + 1] = yb[q].v[u].y * g;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+              for (int j = 0; j < 4; j++) pr[i * 4 + j] += x[i] * h[j];
+          }
+        } else {
+          LdsPack<c2<T>, 2> xa[4], yb[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) xa[q] = lds_cells2<T>(buf + (size_t)xr[q] * L::ROWB, c);
+#pragma unroll
+          for (int q = 0; q < (MODE == 1 ? 2 : 4); q++)
+            yb[q] = lds_cells2<T>(buf + (size_t)yr[q] * L::ROWB, c);
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+            const c2<T> g = gg.v[u];
+            c2<T> h[4];
+            if (MODE == 1) {
+#pragma unroll
+              for (int q = 0; q < 2; q++) {
+                const T b0 = yb[q].v[u].x, b1 = yb[q].v[u].y;
+                h[2 * q] = mk<T>(b0 * g.x, b0 * g.y);
+                h[2 * q + 1] = mk<T>(b1 * g.x, b1 * g.y);
+              }
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; q++) h[q] = cmul<T>(yb[q].v[u], g);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              const c2<T> a = xa[i].v[u];
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                pr[i * 4 + j] += a.x * h[j].x - a.y * h[j].y;
+                pi[i * 4 + j] += a.x * h[j].y + a.y * h[j].x;
+              }
+            }
           }
         }
       }
 #pragma unroll
       for (int q = 0; q < 16; q++) {
         ar[q] += double(pr[q]);
-        ai[q] += double(pi[q]);
+        if (CPLX) ai[q] += double(pi[q]);
       }
     }
+    cur ^= 1;
   }
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  double2* red = reinterpret_cast<double2*>(sm3_raw);   // [256]
+  double2* red = reinterpret_cast<double2*>(smp_raw);   // [256]
   for (int q = 0; q < 16; q++) {
-    red[threadIdx.x] = active ? dmk(ar[q], ai[q]) : dmk(0., 0.);
+    red[threadIdx.x] = active ? dmk(ar[q], CPLX ? ai[q] : 0.) : dmk(0., 0.);
     __syncthreads();
     if (threadIdx.x < ntiles) {
       double sx = 0., sy = 0.;
-      for (int gr = 0; gr < ngroups; gr++) {
-        double2 v = red[threadIdx.x * ngroups + gr];
+      for (int gr = 0; gr < ng; gr++) {
+        const double2 v = red[threadIdx.x * ng + gr];
         sx += v.x;
         sy += v.y;
       }
@@ -1103,120 +1264,12 @@ __device__ __forceinline__ void k_pair_tiles3_body(const c2<T>* __restrict__ FA,
   }
 }
 
-// All-real pair reduction: A and B real and packed two per complex mesh, G
-// real (shells of even l, m = 0 with an M = 0 or periodic-box G).  Same
-// tiles and output as the complex body with imaginary parts zero; values
-// are staged as scalars and chunks are PR2_CELLS cells, since each pair
-// costs one FMA per cell here.
-constexpr int PR2_CELLS = 128;
-
-template <typename T>
-__device__ __forceinline__ void k_pair_real_body(
-    const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB, int nbf,
-    const c2<T>* __restrict__ G, long long nmesh, const int2* __restrict__ tiles,
-    int ntiles, double2* __restrict__ part) {
-  extern __shared__ char smr_raw[];
-  T* sa = reinterpret_cast<T*>(smr_raw);                 // [na][PR2_CELLS]
-  T* sh = sa + (size_t)na * PR2_CELLS;                    // [nbf][PR2_CELLS]
-  constexpr int FSTEP = 256 / PR2_CELLS;                  // packed meshes per pass
-  constexpr int SLOTS = 8;                                // 16 meshes = 32 fields
-  const int ngroups = 256 / ntiles;
-  const int group = threadIdx.x % ngroups;
-  const int tile = threadIdx.x / ngroups;
-  const bool active = tile < ntiles;
-  const int2 t0 = tiles[active ? tile : 0];
-  double ar[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) ar[q] = 0.;
-  int ia[4], ib[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    ia[q] = min(t0.x + q, na - 1) * PR2_CELLS;
-    ib[q] = min(t0.y + q, nbf - 1) * PR2_CELLS;
-  }
-  const int sc = threadIdx.x % PR2_CELLS;
-  const int sf = threadIdx.x / PR2_CELLS;
-  const int ma = (na + 1) / 2, mb = (nbf + 1) / 2;
-  c2<T> ra[SLOTS], rb[SLOTS];
-  T rg;
-  const long long nchunks = (nmesh + PR2_CELLS - 1) / PR2_CELLS;
-  auto fetch = [&](long long ch) {
-    const long long gi = ch * PR2_CELLS + sc;
-    const bool ok = (ch < nchunks) && (gi < nmesh);
-#pragma unroll
-    for (int k = 0; k < SLOTS; k++) {
-      const int f = sf + k * FSTEP;
-      ra[k] = (ok && f < ma) ? FA[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
-      rb[k] = (ok && f < mb) ? FB[(long long)f * nmesh + gi] : mk<T>(T(0), T(0));
-    }
-    rg = ok ? G[gi].x : T(0);
-  };
-  long long ch = blockIdx.x;
-  fetch(ch);
-  for (; ch < nchunks; ch += gridDim.x) {
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < SLOTS; k++) {
-      const int f = sf + k * FSTEP;
-      if (2 * f < na) sa[2 * f * PR2_CELLS + sc] = ra[k].x;
-      if (2 * f + 1 < na) sa[(2 * f + 1) * PR2_CELLS + sc] = ra[k].y;
-      if (2 * f < nbf) sh[2 * f * PR2_CELLS + sc] = rb[k].x * rg;
-      if (2 * f + 1 < nbf) sh[(2 * f + 1) * PR2_CELLS + sc] = rb[k].y * rg;
-    }
-    __syncthreads();
-    fetch(ch + gridDim.x);
-    if (active) {
-      T pr[16];
-#pragma unroll
-      for (int q = 0; q < 16; q++) pr[q] = T(0);
-      for (int c = group; c < PR2_CELLS; c += ngroups) {
-        T x[4], y[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          x[q] = sa[ia[q] + c];
-          y[q] = sh[ib[q] + c];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-          for (int j = 0; j < 4; j++) pr[i * 4 + j] += x[i] * y[j];
-      }
-#pragma unroll
-      for (int q = 0; q < 16; q++) ar[q] += double(pr[q]);
-    }
-  }
-  __syncthreads();
-  double* red = reinterpret_cast<double*>(smr_raw);   // [256]
-  for (int q = 0; q < 16; q++) {
-    red[threadIdx.x] = active ? ar[q] : 0.;
-    __syncthreads();
-    if (threadIdx.x < ntiles) {
-      double sx = 0.;
-      for (int gr = 0; gr < ngroups; gr++) sx += red[threadIdx.x * ngroups + gr];
-      part[(long long)blockIdx.x * ntiles * 16 + threadIdx.x * 16 + q] = dmk(sx, 0.);
-    }
-    __syncthreads();
-  }
-}
-
-// Bytes of dynamic LDS of a pair-reduction launch of a given MODE.
-template <typename T>
-inline size_t pair_smem(int mode, int nas, int nbs) {
-  const size_t staged = mode == 2 ? (size_t)(nas + nbs) * PR2_CELLS * sizeof(T)
-                                  : (size_t)(nas + nbs) * PT_CELLS * sizeof(c2<T>);
-  return staged > 256 * sizeof(double2) ? staged : 256 * sizeof(double2);
-}
-
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
-k_pair_tiles3(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB,
-              int nbf, const c2<T>* __restrict__ G, long long nmesh,
-              const int2* __restrict__ tiles, int ntiles,
-              double2* __restrict__ part) {
-  if constexpr (MODE == 2)
-    k_pair_real_body<T>(FA, na, FB, nbf, G, nmesh, tiles, ntiles, part);
-  else
-    k_pair_tiles3_body<T, MODE>(FA, na, FB, nbf, G, nmesh, tiles, ntiles, part);
+k_pair_lds(const c2<T>* __restrict__ FA, int na, const c2<T>* __restrict__ FB, int nbf,
+           const c2<T>* __restrict__ G, long long nmesh, const int2* __restrict__ tiles,
+           int ntiles, double2* __restrict__ part) {
+  k_pair_lds_body<T, MODE>(FA, na, FB, nbf, G, nmesh, tiles, ntiles, part);
 }
 
 }  // namespace trvg
