@@ -323,6 +323,91 @@ TRVG_FFT3D_BATCH_INST(float, LoadDeconvMocks<float>)
 // spectrum y_l0(k^) f(k) / W(k) is Hermitian).
 static bool real_shells(int ell, int m) { return m == 0 && ell % 2 == 0; }
 
+// ---------------------------------------------------------------------------
+// Shell transforms, x-first: the x pass carries the shell loader (lines
+// along x at fixed (k_y, k_z), pruned on k_y^2 + k_z^2), the y pass is
+// pruned on k_z, and the last pass runs along the contiguous z lines, so
+// the full-size output of every field is written in whole 16 KiB blocks
+// rather than as short strided segments.
+// ---------------------------------------------------------------------------
+
+// Prune over the (z, y) axes: P2(k) tests k_z, P1(k, j) tests (k_z, k_y).
+static Prune prune_zy(const Geom& g) {
+  Prune pr{};
+  pr.on = 1;
+  pr.dk0 = g.dk[2];
+  pr.dk1 = g.dk[1];
+  pr.n0 = g.n[2];
+  pr.n1 = g.n[1];
+  return pr;
+}
+
+struct PruneListsXF {
+  std::vector<int> xblocks, yblocks;
+  double x_lines = 0., y_lines = 0.;   // needed x lines, written y lines
+};
+
+static const PruneListsXF& prune_lists_xfirst(const Geom& g, double kcut2, int nl_x,
+                                              int nl_y) {
+  static std::mutex mu;
+  static std::map<std::vector<double>, std::unique_ptr<PruneListsXF>> cache;
+  std::vector<double> key = {double(g.n[0]), double(g.n[1]), double(g.n[2]),
+                             g.dk[1], g.dk[2], kcut2, double(nl_x), double(nl_y)};
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return *it->second;
+  auto pl = std::make_unique<PruneListsXF>();
+  const double kc = kcut2 * (1. + 1.e-12);
+  const int n1 = g.n[1], n2 = g.n[2];
+  std::vector<char> kz_in(n2), line_in((size_t)n1 * n2);
+  double nkz = 0.;
+  for (int k = 0; k < n2; k++) {
+    const double kz = shifted(k, n2) * g.dk[2];
+    kz_in[k] = kz * kz < kc;
+    nkz += (kz * kz < kcut2);
+    for (int j = 0; j < n1; j++) {
+      const double ky = shifted(j, n1) * g.dk[1];
+      line_in[(size_t)j * n2 + k] = kz * kz + ky * ky < kc;
+      pl->x_lines += (kz * kz + ky * ky < kcut2);
+    }
+  }
+  pl->y_lines = nkz * g.n[0];
+  const long long xl = (long long)n1 * n2;
+  for (long long b = 0; b * nl_x < xl; b++) {
+    bool any = false;
+    for (int l = 0; l < nl_x && !any; l++) any = line_in[b * nl_x + l];
+    if (any) pl->xblocks.push_back(int(b));
+  }
+  const long long yl = (long long)g.n[0] * n2;
+  for (long long b = 0; b * nl_y < yl; b++) {
+    bool any = false;
+    for (int l = 0; l < nl_y && !any; l++) any = kz_in[(b * nl_y + l) % n2];
+    if (any) pl->yblocks.push_back(int(b));
+  }
+  return *(cache[key] = std::move(pl));
+}
+
+template <typename T, int LOGN, class BatchLoader>
+static void launch_x_load_batch(Context& ctx, c2<T>* base, const BatchLoader& load,
+                                const Prune& pr, const double* kcut2, const int2* work,
+                                int nwork, double bytes) {
+  constexpr int EPT = FftEpt<T>::v;
+  constexpr int N = 1 << LOGN;
+  constexpr int E = FFT_THREADS * EPT;
+  const size_t smem = fft_smem<T, LOGN>(false);
+  const c2<T>* tw = twiddles<T>(N, +1);
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
+  if (ctx.g.n[2] % (E / N) == 0)
+    hipLaunchKernelGGL((fft_pass_x_load_batch<T, LOGN, EPT, true, BatchLoader>),
+                       dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream, base,
+                       ctx.g.nmesh, load, tw, ctx.g.n[1], ctx.g.n[2], +1, pr, kcut2, work);
+  else
+    hipLaunchKernelGGL((fft_pass_x_load_batch<T, LOGN, EPT, false, BatchLoader>),
+                       dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream, base,
+                       ctx.g.nmesh, load, tw, ctx.g.n[1], ctx.g.n[2], +1, pr, kcut2, work);
+  TRVG_LAUNCH_CHECK();
+}
+
 // Shell fields of (ell, m) for `bins`: the backward transform of
 // y_lm(k^) src(k) / W(k) on each bin's shell, all bins per pass in one
 // launch.  Unpacked: field q at base + q * nmesh.  Packed (real shells:
@@ -331,81 +416,76 @@ static bool real_shells(int ell, int m) { return m == 0 && ell % 2 == 0; }
 template <typename T>
 void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
                      const trvg_binning& kb, const std::vector<int>& bins,
-                     const std::vector<int>& nmodes, int ell, int m,
-                     bool packed) {
+                     const std::vector<int>& nmodes, int ell, int m, bool packed) {
   const Geom& g = ctx.g;
   const int nf = int(bins.size());
   if (nf == 0) return;
-  const int nt = packed ? (nf + 1) / 2 : nf;    // transforms
-  const int per = packed ? 2 : 1;               // bins per transform
+  const int nt = packed ? (nf + 1) / 2 : nf;
+  const int per = packed ? 2 : 1;
   constexpr int EPT = FftEpt<T>::v;
   const int E = FFT_THREADS * EPT;
-  const int nl_z = E / g.n[2], nl_y = E / g.n[1];
-  const long long nblk = g.nmesh / E;
+  const int nl_x = E / g.n[0], nl_y = E / g.n[1];
   const ShellBand none{HUGE_VAL, HUGE_VAL};
   std::vector<ShellBand> bands((size_t)nt * per, none);
   std::vector<double> kc(nt, 0.);
-  std::vector<int2> wz, wy, wx;
+  const long long nblk = g.nmesh / E;
+  std::vector<int2> wx, wy, wz;
   const double cb = sizeof(c2<T>);
-  double bz = 0., by = 0., bx = 0.;
+  double bx = 0., by = 0., bz = 0.;
   for (int q = 0; q < nf; q++) {
     const int b = bins[q];
     bands[q] = shell_band(kb.bin_edges[b], kb.bin_edges[b + 1]);
     Prune pq = make_prune(g, kb.bin_edges[b + 1]);
     kc[q / per] = std::max(kc[q / per], pq.on ? pq.kcut2 : HUGE_VAL);
-    bz += double(nmodes[b]) * sizeof(double2);
+    bx += double(nmodes[b]) * sizeof(double2);
   }
   for (int f = 0; f < nt; f++) {
-    const PruneLists& pl = prune_lists(g, kc[f], nl_z, nl_y);
-    for (int w : pl.zblocks) wz.push_back(make_int2(f, w));
+    const PruneListsXF& pl = prune_lists_xfirst(g, kc[f], nl_x, nl_y);
+    for (int w : pl.xblocks) wx.push_back(make_int2(f, w));
     for (int w : pl.yblocks) wy.push_back(make_int2(f, w));
-    for (long long w = 0; w < nblk; w++) wx.push_back(make_int2(f, int(w)));
-    const double zw = pl.d_lines * g.n[2] * cb;
-    const double slab_vals = pl.slabs * g.n[1] * g.n[2];
-    bz += zw;
-    by += zw + slab_vals * cb;
-    bx += slab_vals * cb + double(g.nmesh) * cb;
+    for (long long w = 0; w < nblk; w++) wz.push_back(make_int2(f, int(w)));
+    const double xw = pl.x_lines * g.n[0] * cb;
+    const double yw = pl.y_lines * g.n[1] * cb;
+    bx += xw;
+    by += xw + yw;
+    bz += yw + double(g.nmesh) * cb;
   }
   DevArray<ShellBand> dbands(bands.size());
   DevArray<double> dkc(nt);
-  DevArray<int2> dw(wz.size() + wy.size() + wx.size());
+  DevArray<int2> dw(wx.size() + wy.size() + wz.size());
   TRVG_HIP(hipMemcpyAsync(dbands.get(), bands.data(), bands.size() * sizeof(ShellBand),
                           hipMemcpyHostToDevice, ctx.stream));
   TRVG_HIP(hipMemcpyAsync(dkc.get(), kc.data(), nt * sizeof(double),
                           hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dw.get(), wz.data(), wz.size() * sizeof(int2),
+  TRVG_HIP(hipMemcpyAsync(dw.get(), wx.data(), wx.size() * sizeof(int2),
                           hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dw.get() + wz.size(), wy.data(), wy.size() * sizeof(int2),
+  TRVG_HIP(hipMemcpyAsync(dw.get() + wx.size(), wy.data(), wy.size() * sizeof(int2),
                           hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dw.get() + wz.size() + wy.size(), wx.data(),
-                          wx.size() * sizeof(int2), hipMemcpyHostToDevice, ctx.stream));
-  Prune pr{};
-  pr.on = 1;
-  pr.dk0 = g.dk[0];
-  pr.dk1 = g.dk[1];
-  pr.n0 = g.n[0];
-  pr.n1 = g.n[1];
-  const int2* pz = dw.get();
-  const int2* py = pz + wz.size();
-  const int2* px = py + wy.size();
-  if (!wz.empty()) {
+  TRVG_HIP(hipMemcpyAsync(dw.get() + wx.size() + wy.size(), wz.data(),
+                          wz.size() * sizeof(int2), hipMemcpyHostToDevice, ctx.stream));
+  const Prune pr = prune_zy(g);
+  if (!wx.empty()) {
     if (packed) {
       LoadShellPairBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
-      TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
-          ctx, base, ld, +1, pr, dkc.get(), pz, int(wz.size()), bz)));
+      TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_load_batch<T, LG>(
+          ctx, base, ld, pr, dkc.get(), dw.get(), int(wx.size()), bx)));
     } else {
       LoadShellBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
-      TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
-          ctx, base, ld, +1, pr, dkc.get(), pz, int(wz.size()), bz)));
+      TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_load_batch<T, LG>(
+          ctx, base, ld, pr, dkc.get(), dw.get(), int(wx.size()), bx)));
     }
   }
   if (!wy.empty())
     TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
         ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], +1, pr, dkc.get(),
-        py, int(wy.size()), 1, by)));
-  TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_strided_batch<T, LG>(
-      ctx, base, (long long)g.n[1] * g.n[2], 0, (long long)g.n[1] * g.n[2], +1,
-      pr, dkc.get(), px, int(wx.size()), 2, bx)));
+        dw.get() + wx.size(), int(wy.size()), 3, by)));
+  // z pass over every block, reading only the k_z rows the y pass wrote.
+  Prune off{};
+  off.on = 0;
+  LoadMaskedZBatch<T> lz{base, g.nmesh, dkc.get(), g.dk[2], g.n[2]};
+  TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
+      ctx, base, lz, +1, off, dkc.get(), dw.get() + wx.size() + wy.size(), int(wz.size()),
+      bz)));
 }
 
 template <typename T, class Loader>
@@ -1232,18 +1312,16 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
       LoadDeconv<T, double> ldg{dnLMp, ctx.tab, 1. / g.vol};
       fft3d<T>(ctx, G.get(), ldg, +1, double(nm) * sizeof(double2));
 
+      // G_LM is real for M = 0 (real weights Y_L0) and in a periodic box.
+      const bool g_real = !survey || tm.M == 0;
       // Shell fields F_a (l1, m1) for the row bins, F_b (l2, m2) for columns.
       ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr,
-                                  real_shells(p.ell1, tm.m1),
-                                  make_shell(p.ell1, tm.m1));
+                                  real_shells(p.ell1, tm.m1), make_shell(p.ell1, tm.m1));
       ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A,
-                                  real_shells(p.ell2, tm.m2),
-                                  make_shell(p.ell2, tm.m2));
+                                  real_shells(p.ell2, tm.m2), make_shell(p.ell2, tm.m2));
       std::vector<int2> pairs(dim);
       for (int dv = 0; dv < dim; dv++)
         pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
-      // G_LM is real for M = 0 (real weights Y_L0) and in a periodic box.
-      const bool g_real = !survey || tm.M == 0;
       std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields, A->packed,
                                            B->data.get(), B->nfields, B->packed,
                                            G.get(), g_real, pairs);

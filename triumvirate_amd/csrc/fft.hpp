@@ -60,7 +60,8 @@ struct Prune {
 };
 
 // In-LDS Stockham transform of L lines of length N (N = 2^LOGN) stored as
-// buf[line * (N + 1) + n] (one pad element per line against bank conflicts).  W[m] = e^{sign 2 pi i m / N} (m < N) in LDS.
+// buf[line * (N + 1) + n] (one pad element per line against bank
+// conflicts).  W[m] = e^{sign 2 pi i m / N} (m < N) in LDS.
 template <typename T, int LOGN, int EPT>
 __device__ __forceinline__ void lds_fft_lines(c2<T>* buf, const c2<T>* W,
                                               int sign) {
@@ -250,7 +251,9 @@ fft_pass_z_batch(c2<T>* __restrict__ base, long long fstride, BatchLoader load,
 // Each workgroup takes L = E/N consecutive lines.
 //
 // prune_mode (with pr.on): 1 = y pass (slab i = id / inner must satisfy P2,
-// row j read only where P1(i, j)); 2 = x pass (row i read only where P2(i)).
+// row j read only where P1(i, j)); 2 = x pass (row i read only where P2(i));
+// 3 = y pass of the x-first order (Prune over the (z, y) axes: line k =
+// id % inner written only where P2(k), row j read only where P1(k, j)).
 //
 // FAST (inner a multiple of L): the block's lines lie in one slice, so the
 // slice's pruning flags go to LDS once per block and each thread steps a
@@ -273,7 +276,13 @@ __device__ __forceinline__ void fft_strided_block(
   const long long id0 = blk * NLINES;
   if (FAST) {
     const long long slice0 = id0 / inner;
+    const bool m3 = pr.on && prune_mode == 3;
     if (pr.on && prune_mode == 1 && !pr.p2(int(slice0))) return;
+    if (m3) {
+      bool any = false;
+      for (int l = 0; l < NLINES && !any; l++) any = pr.p2(int(id0 % inner) + l);
+      if (!any) return;
+    }
     for (int m = t; m < N; m += FFT_THREADS) {
       W[m] = tw[m];
       bool need = true;
@@ -285,17 +294,20 @@ __device__ __forceinline__ void fft_strided_block(
     }
     __syncthreads();
     const int l = t % NLINES, n0 = t / NLINES;
+    const int kl = int(id0 % inner) + l;   // this thread's line (prune mode 3)
     c2<T>* line = data + (slice0 * outer_stride + id0 % inner + l);
     const long long step = (long long)NSTEP * S;
     c2<T>* p = line + (long long)n0 * S;
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
       const int n = n0 + e * NSTEP;
-      buf[l * LS + n] = row_in[n] ? *p : mk<T>(T(0), T(0));
+      const bool need = m3 ? pr.p1(kl, n) : bool(row_in[n]);
+      buf[l * LS + n] = need ? *p : mk<T>(T(0), T(0));
       p += step;
     }
     __syncthreads();
     lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
+    if (m3 && !pr.p2(kl)) return;
     p = line + (long long)n0 * S;
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
@@ -307,6 +319,11 @@ __device__ __forceinline__ void fft_strided_block(
   if (pr.on && prune_mode == 1) {
     bool any = false;
     for (int l = 0; l < NLINES && !any; l++) any = pr.p2(int((id0 + l) / inner));
+    if (!any) return;
+  }
+  if (pr.on && prune_mode == 3) {
+    bool any = false;
+    for (int l = 0; l < NLINES && !any; l++) any = pr.p2(int((id0 + l) % inner));
     if (!any) return;
   }
   for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
@@ -324,6 +341,8 @@ __device__ __forceinline__ void fft_strided_block(
         need = pr.p2(i) && pr.p1(i, n);
       } else if (prune_mode == 2) {
         need = pr.p2(n);
+      } else if (prune_mode == 3) {
+        need = pr.p1(int(id % inner), n);
       }
     }
     buf[l * LS + n] = need ? data[base + (long long)n * S] : mk<T>(T(0), T(0));
@@ -337,6 +356,7 @@ __device__ __forceinline__ void fft_strided_block(
     const long long id = id0 + l;
     const long long base = (id / inner) * outer_stride + id % inner;
     if (pr.on && prune_mode == 1 && !pr.p2(int(id / inner))) continue;
+    if (pr.on && prune_mode == 3 && !pr.p2(int(id % inner))) continue;
     data[base + (long long)n * S] = buf[l * LS + n];
   }
 }
@@ -363,6 +383,90 @@ fft_pass_strided_batch(c2<T>* __restrict__ base, long long fstride,
   p.kcut2 = kcut2[wk.x];
   fft_strided_block<T, LOGN, EPT, FAST>(base + wk.x * fstride, tw, S, outer_stride,
                                         inner, sign, p, prune_mode, wk.y);
+}
+
+// First pass of the x-first order: lines along x (length N = n0, element
+// stride S = n1 n2), line id = j n2 + k, with a fused loader
+// load(g, i, j, k).  Prune over the (z, y) axes: a block runs only if one of
+// its lines has P1(k, j); it writes all its lines (lines outside the band
+// load as zeros).  FAST (n2 a multiple of the lines per block): the block's
+// lines share j.
+template <typename T, int LOGN, int EPT, bool FAST, class Loader>
+__device__ __forceinline__ void fft_x_load_block(c2<T>* __restrict__ data,
+                                                 const Loader& load,
+                                                 const c2<T>* __restrict__ tw,
+                                                 int n1, int n2, int sign,
+                                                 const Prune& pr, long long blk) {
+  constexpr int N = 1 << LOGN;
+  constexpr int E = FFT_THREADS * EPT;
+  constexpr int NLINES = E / N;
+  constexpr int NSTEP = FFT_THREADS / NLINES;
+  extern __shared__ char smem_raw[];
+  constexpr int LS = N + 1;
+  c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
+  c2<T>* W = buf + NLINES * LS;
+  const int t = threadIdx.x;
+  const long long S = (long long)n1 * n2;
+  const long long id0 = blk * NLINES;
+  if (pr.on) {
+    bool any = false;
+    for (int l = 0; l < NLINES && !any; l++) {
+      const long long id = id0 + l;
+      any = pr.p1(int(id % n2), int(id / n2));
+    }
+    if (!any) return;
+  }
+  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
+  if (FAST) {
+    const int l = t % NLINES, r0 = t / NLINES;
+    const int j = int(id0 / n2), k = int(id0 % n2) + l;
+    const long long g0 = id0 + l;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const int n = r0 + e * NSTEP;
+      buf[l * LS + n] = load(g0 + (long long)n * S, n, j, k);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const int idx = t + e * FFT_THREADS;
+      const int n = idx / NLINES, l = idx % NLINES;
+      const long long id = id0 + l;
+      buf[l * LS + n] = load(id + (long long)n * S, n, int(id / n2), int(id % n2));
+    }
+  }
+  __syncthreads();
+  lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
+  if (FAST) {
+    const int l = t % NLINES, r0 = t / NLINES;
+    c2<T>* p = data + id0 + l + (long long)r0 * S;
+    const long long step = (long long)NSTEP * S;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      *p = buf[l * LS + r0 + e * NSTEP];
+      p += step;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const int idx = t + e * FFT_THREADS;
+      const int n = idx / NLINES, l = idx % NLINES;
+      data[id0 + l + (long long)n * S] = buf[l * LS + n];
+    }
+  }
+}
+
+// Batched x-first loader pass, work[b] = (field, block).
+template <typename T, int LOGN, int EPT, bool FAST, class BatchLoader>
+__global__ void __launch_bounds__(FFT_THREADS)
+fft_pass_x_load_batch(c2<T>* __restrict__ base, long long fstride, BatchLoader load,
+                      const c2<T>* __restrict__ tw, int n1, int n2, int sign, Prune pr,
+                      const double* __restrict__ kcut2, const int2* __restrict__ work) {
+  const int2 wk = work[xcd_block(blockIdx.x, gridDim.x)];
+  Prune p = pr;
+  p.kcut2 = kcut2[wk.x];
+  fft_x_load_block<T, LOGN, EPT, FAST>(base + wk.x * fstride, load.field(wk.x), tw, n1,
+                                       n2, sign, p, wk.y);
 }
 
 // Identity loader (plain in-place transform).

@@ -230,6 +230,32 @@ struct LoadShellPairBatch {
   }
 };
 
+// Final z pass of the x-first shell transform (in place): rows with
+// k_z^2 >= kcut2 were never written by the y pass and read as zero (the
+// same predicate as Prune::p2 over the z axis).
+template <typename T>
+struct LoadMaskedZ {
+  const c2<T>* src;
+  double kcut2, dkz;
+  int nz;
+  __device__ c2<T> operator()(long long g, int, int, int k) const {
+    const double kz = __dmul_rn(double(shifted(k, nz)), dkz);
+    return __dmul_rn(kz, kz) < kcut2 ? src[g] : mk<T>(T(0), T(0));
+  }
+};
+
+template <typename T>
+struct LoadMaskedZBatch {
+  const c2<T>* base;
+  long long fstride;
+  const double* kcut2;
+  double dkz;
+  int nz;
+  __device__ LoadMaskedZ<T> field(int f) const {
+    return LoadMaskedZ<T>{base + (long long)f * fstride, kcut2[f], dkz, nz};
+  }
+};
+
 // LoadShell for field q of a batched slab transform (band of bin q).
 template <typename T, typename S = T>
 struct LoadShellBatch {
