@@ -188,6 +188,20 @@ __host__ __device__ inline double2 ylm_reduced_dev(int ell, int m, double x,
   return dmk(parity * er * p * norm, -parity * ei * p * norm);
 }
 
+// y_lm (as ylm_reduced_dev) at (+-x, +-y, +-z) from its value y at
+// (x, y, z) >= 0: x -> -x maps e^{i m phi} to (-1)^m conj(e^{i m phi}),
+// y -> -y to its conjugate, z -> -z multiplies P_l^|m| by (-1)^(l + |m|).
+// Negation is exact, so the result equals a direct evaluation.
+__host__ __device__ inline double2 ylm_mirror(double2 y, bool fx, bool fy, bool fz,
+                                              int ell, int m) {
+  const int am = m < 0 ? -m : m;
+  double sgn = 1.;
+  if (fx && (am & 1)) sgn = -sgn;
+  if (fz && ((ell + am) & 1)) sgn = -sgn;
+  const bool cj = fx != fy;
+  return dmk(sgn * y.x, cj ? -(sgn * y.y) : sgn * y.y);
+}
+
 // ---------------------------------------------------------------------------
 // Spherical Bessel j_l (FP64), same algorithm as the host node generator:
 // ascending series for x < 1 + l, upward recurrence otherwise.

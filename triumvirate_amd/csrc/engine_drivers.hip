@@ -901,7 +901,9 @@ static TwoPt twopt_fourier(Context& ctx, const c2<T>* fa, const c2<T>* fb,
   DevArray<double> edges(nb + 1);
   TRVG_HIP(hipMemcpyAsync(edges.get(), kb.bin_edges, (nb + 1) * 8,
                           hipMemcpyHostToDevice, ctx.stream));
-  unsigned grid = grid_for(ctx, ctx.g.nmesh, 4);
+  // Latency-bound (dependent table and field loads per cell): eight
+  // workgroups per CU.
+  unsigned grid = grid_for(ctx, ctx.g.nmesh, 8);
   DevArray<unsigned long long> cnt(nb);
   DevArray<double> part((size_t)grid * nb * 4), sum(nb * 4);
   TRVG_HIP(hipMemsetAsync(cnt.get(), 0, nb * 8, ctx.stream));

@@ -426,13 +426,27 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
   __syncthreads();
   const int n_sample = 1000000;
   const double dk_sample = 1.e-5;
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-       c < g.nmesh; c += (long long)gridDim.x * blockDim.x) {
-    int k = int(c % g.n[2]);
-    long long ij = c / g.n[2];
-    int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
-    double kx = grid_coord(i, g.n[0], g.dk[0]);
-    double ky = grid_coord(j, g.n[1], g.dk[1]);
+  // Only cells with |k| below the top edge (plus a fine-bin width) can
+  // enter a bin: each wave takes whole z lines and visits the k_z window
+  // |k_z| <= K of its line (a conservative bound; the exact test below
+  // decides every cell).
+  const double kmax = (edges[nb] + 2. * dk_sample) * (1. + 1.e-12);
+  const double kmax2 = kmax * kmax;
+  const long long nlines = (long long)g.n[0] * g.n[1];
+  const int half = g.n[2] / 2;
+  const int nw = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  for (long long l = (long long)blockIdx.x * nw + wave; l < nlines;
+       l += (long long)gridDim.x * nw) {
+   const int i = int(l / g.n[1]), j = int(l % g.n[1]);
+   const double kx = grid_coord(i, g.n[0], g.dk[0]);
+   const double ky = grid_coord(j, g.n[1], g.dk[1]);
+   const double rem = kmax2 - (kx * kx + ky * ky);
+   if (rem < 0.) continue;
+   const int K = int(sqrt(rem) / g.dk[2]) + 1;
+   const int ncell = K >= half ? g.n[2] : 2 * K + 1;
+   for (int kk = lane; kk < ncell; kk += 64) {
+    const int k = K >= half ? kk : (kk - K + g.n[2]) % g.n[2];
+    const long long c = l * g.n[2] + k;
     double kz = grid_coord(k, g.n[2], g.dk[2]);
     double km = mag3_exact(kx, ky, kz);
     int ik = int(__ddiv_rn(km, dk_sample));
@@ -454,6 +468,7 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
     atomicAdd(&sh2[4 * b + 1], pk.y);
     atomicAdd(&sh2[4 * b + 2], sn.x);
     atomicAdd(&sh2[4 * b + 3], sn.y);
+   }
   }
   __syncthreads();
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -1030,49 +1045,81 @@ struct alignas(sizeof(E) * U >= 16 ? 16 : sizeof(E) * U) LdsPack {
   E v[U];
 };
 
-// Radial histogram over octant representatives: the cells (+-a, +-b, +-c)
-// share n = a^2 + b^2 + c^2, so each thread sums its (up to 8) mirror cells
-// and issues one atomic pair per representative.
+// Sum over the (up to 8) mirror cells (+-a, +-b, +-c) of t y_a y_b, with
+// y_lm evaluated once at (a, b, c) dr and mirrored by sign flips.
+template <typename T>
+__device__ __forceinline__ double2 oct_mirror_sum(const c2<T>* __restrict__ t, const Geom& g,
+                                                  int a, int b, int c, int la, int ma,
+                                                  int lb, int mb, bool trivial) {
+  int is[2], js[2], ks[2];
+  const int ni = (a == 0 || 2 * a == g.n[0]) ? 1 : 2;
+  const int nj = (b == 0 || 2 * b == g.n[1]) ? 1 : 2;
+  const int nk = (c == 0 || 2 * c == g.n[2]) ? 1 : 2;
+  is[0] = a; is[1] = g.n[0] - a;
+  js[0] = b; js[1] = g.n[1] - b;
+  ks[0] = c; ks[1] = g.n[2] - c;
+  double2 ya0 = dmk(1., 0.), yb0 = dmk(1., 0.);
+  if (!trivial) {
+    const double ax = fabs(shifted(is[0], g.n[0]) * g.dr[0]);
+    const double ay = fabs(shifted(js[0], g.n[1]) * g.dr[1]);
+    const double az = fabs(shifted(ks[0], g.n[2]) * g.dr[2]);
+    ya0 = ylm_reduced_dev(la, ma, ax, ay, az);
+    yb0 = ylm_reduced_dev(lb, mb, ax, ay, az);
+  }
+  double sx = 0., sy = 0.;
+  for (int u = 0; u < ni; u++)
+    for (int v = 0; v < nj; v++)
+      for (int w = 0; w < nk; w++) {
+        const int i = is[u], j = js[v], k = ks[w];
+        const c2<T> tv = t[((long long)i * g.n[1] + j) * g.n[2] + k];
+        double2 val = dmk(tv.x, tv.y);
+        if (!trivial) {
+          const bool fx = shifted(i, g.n[0]) < 0, fy = shifted(j, g.n[1]) < 0,
+                     fz = shifted(k, g.n[2]) < 0;
+          val = dmul(val, dmul(ylm_mirror(ya0, fx, fy, fz, la, ma),
+                               ylm_mirror(yb0, fx, fy, fz, lb, mb)));
+        }
+        sx += val.x;
+        sy += val.y;
+      }
+  return dmk(sx, sy);
+}
+
+// Radial histogram Q[n] = sum over cells with i^2 + j^2 + k^2 = n (shifted
+// indices) of t y_a y_b.  Octant representatives (a, b, c) >= 0 carry their
+// mirror cells; with n_x = n_y the rows (a, b) and (b, a) share every n, so
+// rows are taken with a >= b and summed together.  One wave per row, lanes
+// over c: one atomic pair per (row, c).
 template <typename T>
 __device__ __forceinline__ void k_radial_hist_oct_body(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
                   int mb, double* __restrict__ Q) {
   const int h0 = g.n[0] / 2 + 1, h1 = g.n[1] / 2 + 1, h2 = g.n[2] / 2 + 1;
-  const long long nrep = (long long)h0 * h1 * h2;
   const bool trivial = (la == 0 && ma == 0 && lb == 0 && mb == 0);
-  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nrep;
-       r += (long long)gridDim.x * blockDim.x) {
-    const int c = int(r % h2);
-    const int b = int((r / h2) % h1);
-    const int a = int(r / ((long long)h2 * h1));
-    int is[2], js[2], ks[2];
-    const int ni = (a == 0 || 2 * a == g.n[0]) ? 1 : 2;
-    const int nj = (b == 0 || 2 * b == g.n[1]) ? 1 : 2;
-    const int nk = (c == 0 || 2 * c == g.n[2]) ? 1 : 2;
-    is[0] = a; is[1] = g.n[0] - a;
-    js[0] = b; js[1] = g.n[1] - b;
-    ks[0] = c; ks[1] = g.n[2] - c;
-    double sx = 0., sy = 0.;
-    for (int u = 0; u < ni; u++)
-      for (int v = 0; v < nj; v++)
-        for (int w = 0; w < nk; w++) {
-          const int i = is[u], j = js[v], k = ks[w];
-          const long long cell = ((long long)i * g.n[1] + j) * g.n[2] + k;
-          c2<T> tv = t[cell];
-          double2 val = dmk(tv.x, tv.y);
-          if (!trivial) {
-            double rx = shifted(i, g.n[0]) * g.dr[0];
-            double ry = shifted(j, g.n[1]) * g.dr[1];
-            double rz = shifted(k, g.n[2]) * g.dr[2];
-            double2 ya = ylm_reduced_dev(la, ma, rx, ry, rz);
-            double2 yb = ylm_reduced_dev(lb, mb, rx, ry, rz);
-            val = dmul(val, dmul(ya, yb));
-          }
-          sx += val.x;
-          sy += val.y;
-        }
-    const long long nsq = (long long)a * a + (long long)b * b + (long long)c * c;
-    unsafeAtomicAdd(&Q[2 * nsq], sx);
-    unsafeAtomicAdd(&Q[2 * nsq + 1], sy);
+  const bool swap = g.n[0] == g.n[1];
+  const int nrows = swap ? h0 * (h0 + 1) / 2 : h0 * h1;
+  const int nw = blockDim.x / 64, lane = threadIdx.x % 64;
+  for (int r = blockIdx.x * nw + threadIdx.x / 64; r < nrows; r += gridDim.x * nw) {
+    int a, b;
+    if (swap) {
+      a = int((sqrt(8. * r + 1.) - 1.) / 2.);
+      while (a * (a + 1) / 2 > r) a--;
+      while ((a + 1) * (a + 2) / 2 <= r) a++;
+      b = r - a * (a + 1) / 2;
+    } else {
+      a = r / h1;
+      b = r % h1;
+    }
+    for (int c = lane; c < h2; c += 64) {
+      double2 s = oct_mirror_sum<T>(t, g, a, b, c, la, ma, lb, mb, trivial);
+      if (swap && a != b) {
+        const double2 s2 = oct_mirror_sum<T>(t, g, b, a, c, la, ma, lb, mb, trivial);
+        s.x += s2.x;
+        s.y += s2.y;
+      }
+      const long long nsq = (long long)a * a + (long long)b * b + (long long)c * c;
+      unsafeAtomicAdd(&Q[2 * nsq], s.x);
+      unsafeAtomicAdd(&Q[2 * nsq + 1], s.y);
+    }
   }
 }
 
