@@ -646,22 +646,22 @@ static void assign(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c,
 }
 
 template <typename T, int ORDER>
-static void assign_dual_order(Context& ctx, c2<T>* fa, c2<T>* fb, const TiledView& tv,
+static void assign_dual_order(Context& ctx, T* ra, T* rb, const TiledView& tv,
                               int ell, int m, double scale_a, double scale_b) {
   DevParticles sv;
   sv.x = tv.x.get(); sv.y = tv.y.get(); sv.z = tv.z.get(); sv.w = tv.w.get();
   sv.los = tv.los.get();
   sv.n = tv.nin + tv.nstraggle;
+  const bool cplx = m != 0;
   if (tv.nwork > 0) {
-    const bool cplx = m != 0;
     const size_t smem = size_t(cplx ? 4 : 2) * TILE_CELLS * sizeof(double);
     if (cplx)
       hipLaunchKernelGGL((k_assign_tiled_dual<T, ORDER, true>), dim3(tv.nwork), dim3(256),
-                         smem, ctx.stream, fa, fb, sv, ctx.g, tv.tg, tv.work.get(), ell, m,
+                         smem, ctx.stream, ra, rb, sv, ctx.g, tv.tg, tv.work.get(), ell, m,
                          scale_a, scale_b);
     else
       hipLaunchKernelGGL((k_assign_tiled_dual<T, ORDER, false>), dim3(tv.nwork), dim3(256),
-                         smem, ctx.stream, fa, fb, sv, ctx.g, tv.tg, tv.work.get(), ell, m,
+                         smem, ctx.stream, ra, rb, sv, ctx.g, tv.tg, tv.work.get(), ell, m,
                          scale_a, scale_b);
     TRVG_LAUNCH_CHECK();
   }
@@ -670,31 +670,37 @@ static void assign_dual_order(Context& ctx, c2<T>* fa, c2<T>* fb, const TiledVie
     st.x += tv.nin; st.y += tv.nin; st.z += tv.nin; st.w += tv.nin;
     if (st.los) st.los += 3 * tv.nin;
     st.n = tv.nstraggle;
-    const int cw = m != 0;
+    // Real outputs sharing one mesh (rb = ra + 1): B goes to imaginary parts.
+    const bool shared = rb == ra + 1;
+    c2<T>* ma = reinterpret_cast<c2<T>*>(ra);
+    c2<T>* mb = shared ? ma : reinterpret_cast<c2<T>*>(rb);
+    const int cwa = cplx ? 1 : 0, cwb = cplx ? 1 : (shared ? 2 : 0);
     hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(grid_for(ctx, st.n, 16)), dim3(256), 0,
-                       ctx.stream, fa, st, ctx.g, WM_YLM_W, ell, m, scale_a, cw, 0);
+                       ctx.stream, ma, st, ctx.g, WM_YLM_W, ell, m, scale_a, cwa, 0);
     hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(grid_for(ctx, st.n, 16)), dim3(256), 0,
-                       ctx.stream, fb, st, ctx.g, WM_YLMC_W2, ell, m, scale_b, cw, 0);
+                       ctx.stream, mb, st, ctx.g, WM_YLMC_W2, ell, m, scale_b, cwb, 0);
     TRVG_LAUNCH_CHECK();
   }
 }
 
-// fa += scale_a sum y_lm w W / V_cell and fb += scale_b sum y_lm^* w^2 W / V_cell
-// over catalogue c in one pass (survey catalogues: lines of sight needed).
+// A += scale_a sum y_lm w W / V_cell and B += scale_b sum y_lm^* w^2 W /
+// V_cell over catalogue c in one pass (survey catalogues: lines of sight
+// needed); ra, rb point at the real part of cell 0 of each output.
 template <typename T>
-static void assign_dual(Context& ctx, c2<T>* fa, c2<T>* fb, const DeviceCatalogue& c,
-                        int ell, int m, double scale_a, double scale_b) {
+static void assign_dual(Context& ctx, T* ra, T* rb, const DeviceCatalogue& c, int ell,
+                        int m, double scale_a, double scale_b) {
   if (c.los.get() == nullptr)
     throw Error(TRVG_ERR_DATA, "Lines of sight are required for survey-type fields.");
   const TiledView& tv = tiled_view(ctx, c);
-  // Algorithmic bytes: position, weight and line of sight read once, both
-  // meshes written once.
-  ProfScope ps(ctx, KC_ASSIGN, c.n * 56. + 2. * double(ctx.g.nmesh) * sizeof(c2<T>));
+  // Algorithmic bytes: position, weight and line of sight read once, the
+  // output values written once.
+  const double outb = (m != 0 ? 2. : 1.) * double(ctx.g.nmesh) * sizeof(c2<T>);
+  ProfScope ps(ctx, KC_ASSIGN, c.n * 56. + outb);
   switch (ctx.p.assignment_order) {
-    case 1: assign_dual_order<T, 1>(ctx, fa, fb, tv, ell, m, scale_a, scale_b); break;
-    case 2: assign_dual_order<T, 2>(ctx, fa, fb, tv, ell, m, scale_a, scale_b); break;
-    case 3: assign_dual_order<T, 3>(ctx, fa, fb, tv, ell, m, scale_a, scale_b); break;
-    default: assign_dual_order<T, 4>(ctx, fa, fb, tv, ell, m, scale_a, scale_b); break;
+    case 1: assign_dual_order<T, 1>(ctx, ra, rb, tv, ell, m, scale_a, scale_b); break;
+    case 2: assign_dual_order<T, 2>(ctx, ra, rb, tv, ell, m, scale_a, scale_b); break;
+    case 3: assign_dual_order<T, 3>(ctx, ra, rb, tv, ell, m, scale_a, scale_b); break;
+    default: assign_dual_order<T, 4>(ctx, ra, rb, tv, ell, m, scale_a, scale_b); break;
   }
 }
 
@@ -734,19 +740,34 @@ static void build_field_k(Context& ctx, c2<T>* f, const DeviceCatalogue& d,
 //   dn = sum_d y w - alpha sum_r y w
 // and its shot-noise field
 //   N = sum_d y^* w^2 + alpha^2 sum_r y^* w^2,
-// both assigned in one pass per catalogue and forward transformed.
-template <typename T>
-static void build_survey_pair_k(Context& ctx, c2<T>* dn, c2<T>* N,
+// both assigned in one pass per catalogue and forward transformed.  For
+// m = 0 both fields are real: they are assigned as dn + i N into one mesh,
+// transformed once and separated by Hermitian symmetry.
+static void build_survey_pair_k(Context& ctx, double2* dn, double2* N,
                                 const DeviceCatalogue& d, const DeviceCatalogue& r,
                                 double alpha, int ell, int m) {
-  zero_field<T>(ctx, dn);
-  zero_field<T>(ctx, N);
-  assign_dual<T>(ctx, dn, N, d, ell, m, 1., 1.);
-  assign_dual<T>(ctx, dn, N, r, ell, m, -alpha, alpha * alpha);
-  LoadScaleShift<T> la{dn, T(0), T(ctx.g.vol_cell)};
-  fft3d<T>(ctx, dn, la, -1);
-  LoadScaleShift<T> lb{N, T(0), T(ctx.g.vol_cell)};
-  fft3d<T>(ctx, N, lb, -1);
+  double* rdn = reinterpret_cast<double*>(dn);
+  if (m == 0) {
+    zero_field<double>(ctx, dn);
+    assign_dual<double>(ctx, rdn, rdn + 1, d, ell, m, 1., 1.);
+    assign_dual<double>(ctx, rdn, rdn + 1, r, ell, m, -alpha, alpha * alpha);
+    LoadScaleShift<double> la{dn, 0., ctx.g.vol_cell};
+    fft3d<double>(ctx, dn, la, -1);
+    ProfScope ps(ctx, KC_MISC, 3. * double(ctx.g.nmesh) * sizeof(double2));
+    hipLaunchKernelGGL(k_unpack_hermitian, dim3(grid_for(ctx, ctx.g.nmesh)), dim3(256), 0,
+                       ctx.stream, dn, N, ctx.g);
+    TRVG_LAUNCH_CHECK();
+    return;
+  }
+  zero_field<double>(ctx, dn);
+  zero_field<double>(ctx, N);
+  assign_dual<double>(ctx, rdn, reinterpret_cast<double*>(N), d, ell, m, 1., 1.);
+  assign_dual<double>(ctx, rdn, reinterpret_cast<double*>(N), r, ell, m, -alpha,
+                      alpha * alpha);
+  LoadScaleShift<double> la{dn, 0., ctx.g.vol_cell};
+  fft3d<double>(ctx, dn, la, -1);
+  LoadScaleShift<double> lb{N, 0., ctx.g.vol_cell};
+  fft3d<double>(ctx, N, lb, -1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1249,7 +1270,7 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
   // path (two-point statistics, S_ij_k) subtracts nearly equal quantities.
   DevArray<double2> dn00(nm), N00(nm);
   if (survey) {
-    build_survey_pair_k<double>(ctx, dn00.get(), N00.get(), d, *r, alpha, 0, 0);
+    build_survey_pair_k(ctx, dn00.get(), N00.get(), d, *r, alpha, 0, 0);
   } else {
     build_field_k<double>(ctx, dn00.get(), d, r, false, alpha, 0, 0, false, true);
     build_field_k<double>(ctx, N00.get(), d, r, false, alpha, 0, 0, true, false);
@@ -1305,7 +1326,7 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
       const double2* NLMp = N00.get();
       if (!lm_zero) {
         if (!dnLM.get()) { dnLM.reset(nm); NLM.reset(nm); }
-        build_survey_pair_k<double>(ctx, dnLM.get(), NLM.get(), d, *r, alpha, p.ELL, tm.M);
+        build_survey_pair_k(ctx, dnLM.get(), NLM.get(), d, *r, alpha, p.ELL, tm.M);
         dnLMp = dnLM.get();
         NLMp = NLM.get();
       }
@@ -1421,7 +1442,7 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
 
   DevArray<double2> dn00(nm), N00(nm);
   if (survey) {
-    build_survey_pair_k<double>(ctx, dn00.get(), N00.get(), d, *r, alpha, 0, 0);
+    build_survey_pair_k(ctx, dn00.get(), N00.get(), d, *r, alpha, 0, 0);
   } else {
     build_field_k<double>(ctx, dn00.get(), d, r, false, alpha, 0, 0, false, true);
     build_field_k<double>(ctx, N00.get(), d, r, false, alpha, 0, 0, true, false);

@@ -127,6 +127,10 @@ k_assign(c2<T>* __restrict__ mesh, DevParticles p, Geom g, int mode, int ell,
           if (gid < 0 || gid >= g.nmesh) continue;
           const double wv = wij * win[kl][2];
           T* cell = reinterpret_cast<T*>(mesh + gid);
+          if (cplx_w == 2) {           // real weight into the imaginary part
+            unsafeAtomicAdd(cell + 1, T(wre * wv));
+            continue;
+          }
           unsafeAtomicAdd(cell, T(wre * wv));
           if (cplx_w) unsafeAtomicAdd(cell + 1, T(wim * wv));
         }
@@ -962,13 +966,15 @@ k_assign_tiled(c2<T>* __restrict__ mesh, DevParticles p, Geom g, TileGeom tg,
 }
 
 // Both survey fields of one (l, m) in one pass over the particles:
-//   mesh_a += scale_a  y_lm w   W / V_cell   (density contrast)
-//   mesh_b += scale_b  y_lm^* w^2 W / V_cell (its shot-noise field)
-// CPLX = false for m = 0 (real weights: real parts only).  Dynamic LDS:
-// (CPLX ? 4 : 2) * TILE_CELLS doubles.
+//   A += scale_a  y_lm w   W / V_cell   (density contrast)
+//   B += scale_b  y_lm^* w^2 W / V_cell (its shot-noise field)
+// ra, rb: the real part of cell 0 of each output (cells two T apart, the
+// imaginary part following the real part).  CPLX = false for m = 0 (real
+// weights: real parts only, so both fields may share one complex mesh,
+// rb = ra + 1).  Dynamic LDS: (CPLX ? 4 : 2) * TILE_CELLS doubles.
 template <typename T, int ORDER, bool CPLX>
 __global__ void __launch_bounds__(256)
-k_assign_tiled_dual(c2<T>* __restrict__ mesh_a, c2<T>* __restrict__ mesh_b,
+k_assign_tiled_dual(T* __restrict__ ra, T* __restrict__ rb,
                     DevParticles p, Geom g, TileGeom tg,
                     const TileWork* __restrict__ work, int ell, int m,
                     double scale_a, double scale_b) {
@@ -1021,13 +1027,43 @@ k_assign_tiled_dual(c2<T>* __restrict__ mesh_a, c2<T>* __restrict__ mesh_b,
     const double wa = CPLX ? a_im[c] : 0., wb = CPLX ? b_im[c] : 0.;
     if (va == 0. && vb == 0. && wa == 0. && wb == 0.) continue;
     const long long gc = tile_cell(g, o0, c);
-    T* ca = reinterpret_cast<T*>(mesh_a + gc);
-    T* cb = reinterpret_cast<T*>(mesh_b + gc);
+    T* ca = ra + 2 * gc;
+    T* cb = rb + 2 * gc;
     unsafeAtomicAdd(ca, T(va));
     unsafeAtomicAdd(cb, T(vb));
     if (CPLX) {
       unsafeAtomicAdd(ca + 1, T(wa));
       unsafeAtomicAdd(cb + 1, T(wb));
+    }
+  }
+}
+
+// Spectra of two real fields transformed together as P = FFT[a + i b]:
+//   a(k) = (P(k) + P(-k)^*) / 2,  b(k) = (P(k) - P(-k)^*) / (2i).
+// a overwrites P; each thread owns the pair (k, -k), so the update is in
+// place without a race.
+static __global__ void __launch_bounds__(256)
+k_unpack_hermitian(double2* __restrict__ P, double2* __restrict__ B, Geom g) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < g.nmesh;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int k = int(c % g.n[2]);
+    const long long ij = c / g.n[2];
+    const int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
+    const int im = (g.n[0] - i) % g.n[0], jm = (g.n[1] - j) % g.n[1],
+              km = (g.n[2] - k) % g.n[2];
+    const long long cm = ((long long)im * g.n[1] + jm) * g.n[2] + km;
+    if (cm < c) continue;
+    const double2 p = P[c], q = P[cm];
+    // cell c: uses P(-k) = q; cell cm: uses P(k) = p.
+    const double2 a_c = dmk(0.5 * (p.x + q.x), 0.5 * (p.y - q.y));
+    const double2 b_c = dmk(0.5 * (p.y + q.y), -0.5 * (p.x - q.x));
+    const double2 a_m = dmk(0.5 * (q.x + p.x), 0.5 * (q.y - p.y));
+    const double2 b_m = dmk(0.5 * (q.y + p.y), -0.5 * (q.x - p.x));
+    P[c] = a_c;
+    B[c] = b_c;
+    if (cm != c) {
+      P[cm] = a_m;
+      B[cm] = b_m;
     }
   }
 }
@@ -1142,9 +1178,10 @@ namespace trvg {
 // 16-byte-per-lane load).  Rows: the MA meshes of the A side, the MB meshes
 // of the B side, then G.  While a block reduces chunk c from one buffer, the
 // loads of its next chunk land in the other; no register staging.  Thread
-// (tile, group) forms H = F_b G for its tile's four B fields per cell and
-// accumulates 16 pair sums in T within a chunk, promoted to FP64 once per
-// chunk.  MODE 0: complex sides; 1: B side real, two per mesh; 2: both sides
+// (tile, group) folds G into one side per cell (H = F_b G for its tile's
+// four B fields, or F_a G for its four A rows when B is real, so each pair
+// costs two real FMAs) and accumulates 16 pair sums in T within a chunk,
+// promoted to FP64 once per chunk.  MODE 0: complex sides; 1: B side real, two per mesh; 2: both sides
 // real, two per mesh, G real; 3: as 2 with both sides the same fields (read
 // once).  Requires nmesh % LDC == 0 (meshes are powers of two >= 8^3).
 // Output: per-block partials [gridDim.x][ntiles * 16].
@@ -1286,25 +1323,36 @@ __device__ __forceinline__ void k_pair_lds_body(
 #pragma unroll
           for (int u = 0; u < 2; u++) {
             const c2<T> g = gg.v[u];
-            c2<T> h[4];
             if (MODE == 1) {
+              // Real B: form A_a G once per row, then 2 FMAs per pair.
+              c2<T> ag[4];
+              T b[4];
+#pragma unroll
+              for (int i = 0; i < 4; i++) ag[i] = cmul<T>(xa[i].v[u], g);
 #pragma unroll
               for (int q = 0; q < 2; q++) {
-                const T b0 = yb[q].v[u].x, b1 = yb[q].v[u].y;
-                h[2 * q] = mk<T>(b0 * g.x, b0 * g.y);
-                h[2 * q + 1] = mk<T>(b1 * g.x, b1 * g.y);
+                b[2 * q] = yb[q].v[u].x;
+                b[2 * q + 1] = yb[q].v[u].y;
               }
+#pragma unroll
+              for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                  pr[i * 4 + j] += ag[i].x * b[j];
+                  pi[i * 4 + j] += ag[i].y * b[j];
+                }
             } else {
+              c2<T> h[4];
 #pragma unroll
               for (int q = 0; q < 4; q++) h[q] = cmul<T>(yb[q].v[u], g);
-            }
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-              const c2<T> a = xa[i].v[u];
+              for (int i = 0; i < 4; i++) {
+                const c2<T> a = xa[i].v[u];
 #pragma unroll
-              for (int j = 0; j < 4; j++) {
-                pr[i * 4 + j] += a.x * h[j].x - a.y * h[j].y;
-                pi[i * 4 + j] += a.x * h[j].y + a.y * h[j].x;
+                for (int j = 0; j < 4; j++) {
+                  pr[i * 4 + j] += a.x * h[j].x - a.y * h[j].y;
+                  pi[i * 4 + j] += a.x * h[j].y + a.y * h[j].x;
+                }
               }
             }
           }
