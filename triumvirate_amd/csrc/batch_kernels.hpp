@@ -11,41 +11,156 @@
 
 namespace trvg {
 
-// Unit-weight assignment of each mock's particles [offs[b], offs[b+1]) to
-// its mesh (real part), cell += W / V_cell (field.cpp:569-1112 rules, as
-// k_assign with WM_UNIT).
-template <int ORDER>
+// ---- tiled assignment of a batch (all on the device, graph-capturable) ----
+// A counting sort by (mock, tile) key b (ntiles + 1) + t, t = ntiles for
+// particles whose base cell lies outside the mesh (they keep the direct
+// index rule of k_assign), then one workgroup per key accumulates its
+// tile's halo block in LDS and adds it to the mesh (k_assign_tiled's
+// scheme, in FP64).
+
+// Key of every particle and its rank within the key.
 static __global__ void __launch_bounds__(256)
-k_assign_batch(double2* __restrict__ mesh, long long nm,
-               const double* __restrict__ x, const double* __restrict__ y,
-               const double* __restrict__ z, const long long* __restrict__ offs,
-               Geom g) {
+k_tile_key_batch(const double* __restrict__ x, const double* __restrict__ y,
+                 const double* __restrict__ z, const long long* __restrict__ offs,
+                 Geom g, TileGeom tg, unsigned int* __restrict__ counts,
+                 unsigned int* __restrict__ key, unsigned int* __restrict__ rank) {
   const int b = blockIdx.y;
-  double2* out = mesh + (long long)b * nm;
-  const double inv_vol_cell = 1 / g.vol_cell;
+  const long long nk = tg.ntiles + 1;
   const long long p0 = offs[b], p1 = offs[b + 1];
   for (long long p = p0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; p < p1;
        p += (long long)gridDim.x * blockDim.x) {
-    const double pos[3] = {x[p], y[p], z[p]};
-    int ix[3][ORDER];
-    double wn[3][ORDER], dw[3][ORDER];
+    int idx[3];
+    double s[3];
+    base_cell(g, x[p], y[p], z[p], idx, s);
+    long long t = tg.ntiles;
+    if (idx[0] >= 0 && idx[0] < g.n[0] && idx[1] >= 0 && idx[1] < g.n[1] &&
+        idx[2] >= 0 && idx[2] < g.n[2])
+      t = ((long long)(idx[0] / TILE) * tg.nt[1] + idx[1] / TILE) * tg.nt[2] + idx[2] / TILE;
+    const unsigned int k = unsigned(b * nk + t);
+    key[p] = k;
+    rank[p] = atomicAdd(&counts[k], 1u);
+  }
+}
+
+// Exclusive scan of n values in segments of 1024 (segment totals to sums);
+// k_scan_counts then scans the totals and k_scan_add adds them back.
+static __global__ void __launch_bounds__(1024)
+k_scan_seg(const unsigned int* __restrict__ in, long long n, unsigned int* __restrict__ out,
+           unsigned int* __restrict__ sums) {
+  __shared__ unsigned int sh[1024];
+  const long long i = (long long)blockIdx.x * 1024 + threadIdx.x;
+  const unsigned int v = i < n ? in[i] : 0u;
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = sh[threadIdx.x] - v;
+  if (threadIdx.x == 1023) sums[blockIdx.x] = sh[1023];
+}
+
+static __global__ void __launch_bounds__(1024)
+k_scan_add(unsigned int* __restrict__ out, long long n, const unsigned int* __restrict__ soff,
+           int nseg) {
+  const long long i = (long long)blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) out[i] += soff[blockIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = soff[nseg];
+}
+
+// Positions in key order.
+static __global__ void __launch_bounds__(256)
+k_tile_scatter_batch(const double* __restrict__ x, const double* __restrict__ y,
+                     const double* __restrict__ z, const long long* __restrict__ offs,
+                     const unsigned int* __restrict__ key, const unsigned int* __restrict__ rank,
+                     const unsigned int* __restrict__ koff, double* __restrict__ sx,
+                     double* __restrict__ sy, double* __restrict__ sz) {
+  const int b = blockIdx.y;
+  const long long p0 = offs[b], p1 = offs[b + 1];
+  for (long long p = p0 + blockIdx.x * (long long)blockDim.x + threadIdx.x; p < p1;
+       p += (long long)gridDim.x * blockDim.x) {
+    const unsigned int d = koff[key[p]] + rank[p];
+    sx[d] = x[p];
+    sy[d] = y[p];
+    sz[d] = z[p];
+  }
+}
+
+// One workgroup per key (mock b, tile t): unit weights, real FP64 field,
+// cell += W / V_cell (field.cpp:569-1112 rules, as k_assign with WM_UNIT).
+template <int ORDER>
+static __global__ void __launch_bounds__(256)
+k_assign_tiled_batch(double2* __restrict__ mesh, long long nm, const double* __restrict__ sx,
+                     const double* __restrict__ sy, const double* __restrict__ sz,
+                     const unsigned int* __restrict__ koff, Geom g, TileGeom tg) {
+  __shared__ double acc[TILE_CELLS];
+  const long long nk = tg.ntiles + 1;
+  const long long key = blockIdx.x;
+  const int b = int(key / nk);
+  const long long t = key % nk;
+  const unsigned int p0 = koff[key], p1 = koff[key + 1];
+  if (p0 == p1) return;
+  double2* out = mesh + (long long)b * nm;
+  const double inv_vol_cell = 1 / g.vol_cell;
+  if (t == tg.ntiles) {
+    // Base cell outside the mesh: the direct rule (flattened-index guard).
+    for (unsigned int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+      const double pos[3] = {sx[p], sy[p], sz[p]};
+      int ix[3][ORDER];
+      double wn[3][ORDER], dw[3][ORDER];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-      stencil_1d<ORDER>(pos[a], g.n[a], g.L[a], ix[a], wn[a], dw[a]);
+      for (int a = 0; a < 3; a++)
+        stencil_1d<ORDER>(pos[a], g.n[a], g.L[a], ix[a], wn[a], dw[a]);
+#pragma unroll
+      for (int il = 0; il < ORDER; il++)
+#pragma unroll
+        for (int jl = 0; jl < ORDER; jl++) {
+          const long long row = ((long long)ix[0][il] * g.n[1] + ix[1][jl]) * g.n[2];
+          const double wij = inv_vol_cell * wn[0][il] * wn[1][jl];
+#pragma unroll
+          for (int kl = 0; kl < ORDER; kl++) {
+            const long long gid = row + ix[2][kl];
+            if (gid < 0 || gid >= g.nmesh) continue;
+            unsafeAtomicAdd(reinterpret_cast<double*>(out + gid), wij * wn[2][kl]);
+          }
+        }
+    }
+    return;
+  }
+  for (int c = threadIdx.x; c < TILE_CELLS; c += blockDim.x) acc[c] = 0.;
+  int o0[3];
+  tile_origin(tg, int(t), o0);
+  __syncthreads();
+  for (unsigned int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    int lo[3];
+    double win[ORDER][3];
+    tile_stencil<ORDER>(g, o0, sx[p], sy[p], sz[p], lo, win);
 #pragma unroll
     for (int il = 0; il < ORDER; il++)
 #pragma unroll
       for (int jl = 0; jl < ORDER; jl++) {
-        const long long row = ((long long)ix[0][il] * g.n[1] + ix[1][jl]) * g.n[2];
-        const double wij = inv_vol_cell * wn[0][il] * wn[1][jl];
+        const int row = ((lo[0] + il) * TILE_H + (lo[1] + jl)) * TILE_H + lo[2];
+        const double wij = win[il][0] * win[jl][1];
 #pragma unroll
-        for (int kl = 0; kl < ORDER; kl++) {
-          const long long gid = row + ix[2][kl];
-          if (gid < 0 || gid >= g.nmesh) continue;
-          unsafeAtomicAdd(reinterpret_cast<double*>(out + gid), wij * wn[2][kl]);
-        }
+        for (int kl = 0; kl < ORDER; kl++)
+          atomicAdd(&acc[row + kl], inv_vol_cell * (wij * win[kl][2]));
       }
   }
+  __syncthreads();
+  for (int c = threadIdx.x; c < TILE_CELLS; c += blockDim.x) {
+    const double v = acc[c];
+    if (v == 0.) continue;
+    unsafeAtomicAdd(reinterpret_cast<double*>(out + tile_cell(g, o0, c)), v);
+  }
+}
+
+static __global__ void __launch_bounds__(256)
+k_fill_zero_u32(unsigned int* __restrict__ a, long long n) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x)
+    a[c] = 0u;
 }
 
 // Zero a FP64 array (graph-safe replacement of a memset).

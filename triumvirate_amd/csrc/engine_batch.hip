@@ -134,6 +134,18 @@ class BatchPlan : public BatchPlanBase {
     TRVG_HIP(hipHostMalloc((void**)&h_meta_, (3 * B_ + 1) * sizeof(double)));
     d_x_.reset(cap_); d_y_.reset(cap_); d_z_.reset(cap_);
     d_offs_.reset(B_ + 1);
+    // Tiled assignment: keys (mock, tile), their counts and offsets, the
+    // particles in key order.
+    for (int a = 0; a < 3; a++) tg_.nt[a] = (g.n[a] + TILE - 1) / TILE;
+    tg_.ntiles = (long long)tg_.nt[0] * tg_.nt[1] * tg_.nt[2];
+    nkeys_ = (long long)B_ * (tg_.ntiles + 1);
+    nseg_ = int((nkeys_ + 1023) / 1024);
+    d_kcount_.reset(nkeys_);
+    d_koff_.reset(nkeys_ + 1);
+    d_ksum_.reset(nseg_);
+    d_ksoff_.reset(nseg_ + 1);
+    d_key_.reset(cap_); d_rank_.reset(cap_);
+    d_sx_.reset(cap_); d_sy_.reset(cap_); d_sz_.reset(cap_);
     d_nmock_.reset(B_);
     d_norm_.reset(B_);
     TRVG_HIP(hipHostMalloc((void**)&h_offs_, (B_ + 1) * sizeof(long long)));
@@ -489,23 +501,41 @@ class BatchPlan : public BatchPlanBase {
 
   void launch_assign() {
     const Geom& g = ctx_.g;
-    dim3 grid(tune_.assign_blocks, B_);
+    hipStream_t s = ctx_.stream;
+    const dim3 pgrid(tune_.assign_blocks, B_);
+    hipLaunchKernelGGL(k_fill_zero_u32, dim3(grid_for(ctx_, nkeys_)), dim3(256), 0, s,
+                       d_kcount_.get(), nkeys_);
+    hipLaunchKernelGGL(k_tile_key_batch, pgrid, dim3(256), 0, s, d_x_.get(), d_y_.get(),
+                       d_z_.get(), d_offs_.get(), g, tg_, d_kcount_.get(), d_key_.get(),
+                       d_rank_.get());
+    hipLaunchKernelGGL(k_scan_seg, dim3(nseg_), dim3(1024), 0, s, d_kcount_.get(), nkeys_,
+                       d_koff_.get(), d_ksum_.get());
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, d_ksum_.get(),
+                       (long long)nseg_, d_ksoff_.get());
+    hipLaunchKernelGGL(k_scan_add, dim3(nseg_), dim3(1024), 0, s, d_koff_.get(), nkeys_,
+                       d_ksoff_.get(), nseg_);
+    hipLaunchKernelGGL(k_tile_scatter_batch, pgrid, dim3(256), 0, s, d_x_.get(), d_y_.get(),
+                       d_z_.get(), d_offs_.get(), d_key_.get(), d_rank_.get(), d_koff_.get(),
+                       d_sx_.get(), d_sy_.get(), d_sz_.get());
+    TRVG_LAUNCH_CHECK();
+    const dim3 kgrid(static_cast<unsigned>(nkeys_), 1, 1);
     switch (ctx_.p.assignment_order) {
-      case 1: hipLaunchKernelGGL((k_assign_batch<1>), grid, dim3(256), 0, ctx_.stream,
-                                 D_.get(), nm_, d_x_.get(), d_y_.get(), d_z_.get(),
-                                 d_offs_.get(), g); break;
-      case 2: hipLaunchKernelGGL((k_assign_batch<2>), grid, dim3(256), 0, ctx_.stream,
-                                 D_.get(), nm_, d_x_.get(), d_y_.get(), d_z_.get(),
-                                 d_offs_.get(), g); break;
-      case 3: hipLaunchKernelGGL((k_assign_batch<3>), grid, dim3(256), 0, ctx_.stream,
-                                 D_.get(), nm_, d_x_.get(), d_y_.get(), d_z_.get(),
-                                 d_offs_.get(), g); break;
-      default: hipLaunchKernelGGL((k_assign_batch<4>), grid, dim3(256), 0, ctx_.stream,
-                                  D_.get(), nm_, d_x_.get(), d_y_.get(), d_z_.get(),
-                                  d_offs_.get(), g); break;
+      case 1: hipLaunchKernelGGL((k_assign_tiled_batch<1>), kgrid, dim3(256), 0, s, D_.get(),
+                                 nm_, d_sx_.get(), d_sy_.get(), d_sz_.get(), d_koff_.get(), g,
+                                 tg_); break;
+      case 2: hipLaunchKernelGGL((k_assign_tiled_batch<2>), kgrid, dim3(256), 0, s, D_.get(),
+                                 nm_, d_sx_.get(), d_sy_.get(), d_sz_.get(), d_koff_.get(), g,
+                                 tg_); break;
+      case 3: hipLaunchKernelGGL((k_assign_tiled_batch<3>), kgrid, dim3(256), 0, s, D_.get(),
+                                 nm_, d_sx_.get(), d_sy_.get(), d_sz_.get(), d_koff_.get(), g,
+                                 tg_); break;
+      default: hipLaunchKernelGGL((k_assign_tiled_batch<4>), kgrid, dim3(256), 0, s, D_.get(),
+                                  nm_, d_sx_.get(), d_sy_.get(), d_sz_.get(), d_koff_.get(), g,
+                                  tg_); break;
     }
     TRVG_LAUNCH_CHECK();
   }
+
 
   template <int MODE>
   void launch_pairs_mode(const c2<T>* A, int nas, long long sa, const c2<T>* Bs, int nbs,
@@ -734,6 +764,11 @@ class BatchPlan : public BatchPlanBase {
   long long* h_offs_ = nullptr;
   DevArray<double> d_x_, d_y_, d_z_, d_nmock_, d_norm_, d_edges_;
   DevArray<long long> d_offs_;
+  TileGeom tg_{};
+  long long nkeys_ = 0;
+  int nseg_ = 0;
+  DevArray<unsigned int> d_kcount_, d_koff_, d_ksum_, d_ksoff_, d_key_, d_rank_;
+  DevArray<double> d_sx_, d_sy_, d_sz_;
   DevArray<double2> D_, N00_, tfield_;
   DevArray<c2<T>> G_;
   DevArray<double> Q_;
