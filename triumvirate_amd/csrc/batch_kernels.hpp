@@ -178,20 +178,6 @@ k_fill_zero_u64(unsigned long long* __restrict__ a, long long n) {
     a[c] = 0ull;
 }
 
-// N00(k) of every mock from its dn00(k): equal except at k = 0, where the
-// unsubtracted field carries the particle count (V_cell FFT[n] = V_cell
-// FFT[n - N/V] + N delta_k0).
-static __global__ void __launch_bounds__(256)
-k_n00_from_dn(double2* __restrict__ n00, const double2* __restrict__ dn,
-              long long total, long long nm, const double* __restrict__ nmock) {
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < total;
-       c += (long long)gridDim.x * blockDim.x) {
-    double2 v = dn[c];
-    if (c % nm == 0) v.x += nmock[c / nm];
-    n00[c] = v;
-  }
-}
-
 // ---- batched loaders (field(q) gives the per-field loader) ---------------
 
 // (v - N_b / V) V_cell for mock b.
@@ -250,6 +236,7 @@ struct LoadDeconvMocks {
 };
 
 // Shot-noise field of mock b with Sbar = N_b (periodic box).
+// fb: dn00 of each mock; N00 differs from it only by N_b at k = 0.
 struct LoadTwoptMocks {
   const double2* fa;
   const double2* fb;
@@ -259,7 +246,7 @@ struct LoadTwoptMocks {
   double inv_vol;
   __device__ LoadTwopt<double> field(int b) const {
     const long long o = (long long)b * stride;
-    return LoadTwopt<double>{fa + o, fb + o, tab, dmk(nmock[b], 0.), inv_vol};
+    return LoadTwopt<double>{fa + o, fb + o, tab, dmk(nmock[b], 0.), inv_vol, nmock[b]};
   }
 };
 
@@ -285,18 +272,20 @@ k_reduce_cols_batch(const V* __restrict__ part, int nrows, int width,
   k_reduce_cols_body<V>(part + b * nrows * width, nrows, width, out + b * width);
 }
 
-// Two-point statistics of mock b (fa = dn00, fb = N00, Sbar = N_b).
+// Two-point statistics of mock b (fa = dn00, fb = N00, Sbar = N_b); with
+// fb_dc, fb is dn00 and N_b is added at k = 0 to form N00.
 static __global__ void __launch_bounds__(RED_THREADS)
 k_twopt_fourier_batch(const double2* __restrict__ fa,
                       const double2* __restrict__ fb, long long stride, Geom g,
                       AxisTables tab, const double* __restrict__ nmock, int ell,
                       int m, const double* __restrict__ edges, int nb,
                       unsigned long long* __restrict__ counts,
-                      double* __restrict__ part) {
+                      double* __restrict__ part, int fb_dc) {
   const long long b = blockIdx.y;
   k_twopt_fourier_body<double>(fa + b * stride, fb + b * stride, g, tab,
                                dmk(nmock[b], 0.), ell, m, edges, nb,
-                               counts + b * nb, part + b * gridDim.x * nb * 4);
+                               counts + b * nb, part + b * gridDim.x * nb * 4,
+                               fb_dc ? nmock[b] : 0.);
 }
 
 static __global__ void __launch_bounds__(RED_THREADS)

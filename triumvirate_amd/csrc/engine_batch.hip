@@ -152,7 +152,6 @@ class BatchPlan : public BatchPlanBase {
 
     // Common fields.
     D_.reset((size_t)B_ * nm_);
-    N00_.reset((size_t)B_ * nm_);
     tfield_.reset((size_t)B_ * nm_);
     G_.reset((size_t)B_ * nm_);
     Q_.reset((size_t)B_ * 2 * nq_);
@@ -464,7 +463,8 @@ class BatchPlan : public BatchPlanBase {
     hipStream_t s = ctx_.stream;
     const long long total = (long long)B_ * nm_;
     const unsigned zg = grid_for(ctx_, 2 * total);
-    // dn00(k), N00(k) of every mock (FP64).
+    // dn00(k) of every mock (FP64); N00(k) equals it except for N_b at
+    // k = 0, which its two consumers add themselves.
     hipLaunchKernelGGL(k_fill_zero_d, dim3(zg), dim3(256), 0, s,
                        reinterpret_cast<double*>(D_.get()), 2 * total);
     TRVG_LAUNCH_CHECK();
@@ -473,9 +473,6 @@ class BatchPlan : public BatchPlanBase {
     fft3d_batch<double>(ctx_, D_.get(), ls, -1, false, swork_[0].kcut2.get(),
                         wfull_.get(), nfull64_, wfull_.get(), nfull64_,
                         wfull_.get(), nfull64_);
-    hipLaunchKernelGGL(k_n00_from_dn, dim3(grid_for(ctx_, total)), dim3(256), 0, s,
-                       N00_.get(), D_.get(), total, nm_, d_nmock_.get());
-    TRVG_LAUNCH_CHECK();
     // Shell slabs (all bins of every mock per pass).
     for (size_t i = 0; i < slabs_.size(); i++) {
       const SlabWork& sw = swork_[slab_packed_[i] ? 1 : 0];
@@ -496,6 +493,12 @@ class BatchPlan : public BatchPlanBase {
     LoadDeconvMocks<T> ld{D_.get(), nm_, ctx_.tab, 1. / g.vol};
     fft3d_batch<T>(ctx_, G_.get(), ld, +1, false, swork_[0].kcut2.get(), wfullT_.get(),
                    nfullT_, wfullT_.get(), nfullT_, wfullT_.get(), nfullT_);
+    // Shot-noise field t(x) = IFFT[(dn00 N00^* - N) / C1] / V: the same for
+    // every term and multipole of a periodic box (threept.cpp:950-964).
+    LoadTwoptMocks lt{D_.get(), D_.get(), nm_, ctx_.tab, d_nmock_.get(), 1. / g.vol};
+    fft3d_batch<double>(ctx_, tfield_.get(), lt, +1, false, swork_[0].kcut2.get(),
+                        wfull_.get(), nfull64_, wfull_.get(), nfull64_,
+                        wfull_.get(), nfull64_);
     for (auto& P : mps_) run_multipole(*P);
   }
 
@@ -607,10 +610,6 @@ class BatchPlan : public BatchPlanBase {
       // Both sides monopoles: reuse the first side's statistics.
       if (P.l1 == 0)
         twopt(P, t, P.l2, t.m2, P.bin_b.get(), !(P.l2 == 0 && t.m1 == t.m2));
-      LoadTwoptMocks lt{D_.get(), N00_.get(), nm_, ctx_.tab, d_nmock_.get(), 1. / g.vol};
-      fft3d_batch<double>(ctx_, tfield_.get(), lt, +1, false, swork_[0].kcut2.get(),
-                          wfull_.get(), nfull64_, wfull_.get(), nfull64_,
-                          wfull_.get(), nfull64_);
       const long long nQ = (long long)B_ * 2 * nq_;
       hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(ctx_, nQ)), dim3(256), 0, s,
                          Q_.get(), nQ);
@@ -654,9 +653,9 @@ class BatchPlan : public BatchPlanBase {
                        (long long)B_ * nb_);
     const size_t smem = nb_ * (4 * 8 + 4);
     hipLaunchKernelGGL(k_twopt_fourier_batch, dim3(tune_.stat_blocks, B_),
-                       dim3(RED_THREADS), smem, s, D_.get(), N00_.get(), nm_, g,
+                       dim3(RED_THREADS), smem, s, D_.get(), D_.get(), nm_, g,
                        ctx_.tab, d_nmock_.get(), ell, m, d_edges_.get(), nb_,
-                       counts_.get(), tpart_.get());
+                       counts_.get(), tpart_.get(), 1);
     const int w = nb_ * 4;
     hipLaunchKernelGGL((k_reduce_cols_batch<double>), dim3((w + RC_COLS - 1) / RC_COLS, B_),
                        dim3(RC_COLS * RC_ROWS), 0, s, tpart_.get(), tune_.stat_blocks, w,
@@ -725,8 +724,8 @@ class BatchPlan : public BatchPlanBase {
                          counts_.get(), (long long)B_ * nb_);
       hipLaunchKernelGGL(k_twopt_fourier_batch, dim3(tune_.stat_blocks, B_),
                          dim3(RED_THREADS), nb_ * (4 * 8 + 4), ctx_.stream, D_.get(),
-                         N00_.get(), nm_, ctx_.g, ctx_.tab, d_nmock_.get(), 0, 0,
-                         d_edges_.get(), nb_, counts_.get(), tpart_.get());
+                         D_.get(), nm_, ctx_.g, ctx_.tab, d_nmock_.get(), 0, 0,
+                         d_edges_.get(), nb_, counts_.get(), tpart_.get(), 1);
     });
     TRVG_HIP(hipStreamSynchronize(ctx_.stream));
   }
@@ -769,7 +768,7 @@ class BatchPlan : public BatchPlanBase {
   int nseg_ = 0;
   DevArray<unsigned int> d_kcount_, d_koff_, d_ksum_, d_ksoff_, d_key_, d_rank_;
   DevArray<double> d_sx_, d_sy_, d_sz_;
-  DevArray<double2> D_, N00_, tfield_;
+  DevArray<double2> D_, tfield_;
   DevArray<c2<T>> G_;
   DevArray<double> Q_;
   // Shell slab work lists: [0] one bin per mesh, [1] packed real pairs.

@@ -1285,6 +1285,11 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
 
   DevArray<c2<T>> G(nm);
   DevArray<double2> tfield(nm), dnLM, NLM;
+  // Inputs of the shot-noise field held in tfield: (L, M) of its dn field
+  // (-1 for dn00) and Sbar; a periodic box uses one t(x) for every term and
+  // multipole.
+  int t_ell = -2, t_m = 0;
+  cplx t_sbar(0., 0.);
   SlabCache<T> slabs(mps.size() > 1 ? 3 : 2);
   auto make_shell = [&](int l, int mm) {
     return [&, l, mm](c2<T>* base, const std::vector<int>& bins) {
@@ -1387,9 +1392,15 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
           out.shot[dv] += sv + tm.f_mirror * std::conj(sv);
         }
       }
-      LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab,
-                           dmk(Sbar.real(), Sbar.imag()), 1. / g.vol};
-      fft3d<double>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
+      const int key_ell = lm_zero ? -1 : p.ELL, key_m = lm_zero ? 0 : tm.M;
+      if (key_ell != t_ell || key_m != t_m || Sbar != t_sbar) {
+        LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab,
+                             dmk(Sbar.real(), Sbar.imag()), 1. / g.vol};
+        fft3d<double>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
+        t_ell = key_ell;
+        t_m = key_m;
+        t_sbar = Sbar;
+      }
       std::vector<cplx> sij = shot_pairs_bispec<double>(
           ctx, tfield.get(), p.ell1, tm.m1, p.ell2, tm.m2, sja, sjb, ss.keff, ra, cb);
       for (int dv = 0; dv < dim; dv++) {

@@ -306,8 +306,11 @@ struct LoadTwopt {
   AxisTables tab;
   double2 sbar;
   double inv_vol;
+  double fb_dc = 0.;   // added to fb at k = 0 (fb = an unsubtracted field's
+                       // fluctuation spectrum)
   __device__ c2<T> operator()(long long g, int i, int j, int k) const {
     c2<T> a = fa[g], b = fb[g];
+    if (g == 0) b.x += T(fb_dc);
     double2 pk = dmul(dmk(a.x, a.y), dmk(b.x, -double(b.y)));
     double c1 = alias_at(tab, i, j, k);
     double w = tab.interlace ? window_at(tab, i, j, k) * window_at(tab, i, j, k)
@@ -420,7 +423,7 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
                 Geom g, AxisTables tab, double2 sbar, int ell, int m,
                 const double* __restrict__ edges, int nb,
                 unsigned long long* __restrict__ counts,
-                double* __restrict__ part) {
+                double* __restrict__ part, double fb_dc = 0.) {
   extern __shared__ double sh2[];   // [nb][4] then counts
   unsigned int* sh_cnt = reinterpret_cast<unsigned int*>(sh2 + 4 * nb);
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -458,6 +461,7 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
     int b = find_bin(edges, nb, __dmul_rn(double(ik), dk_sample));
     if (b < 0) continue;
     c2<T> a = fa[c], bb = fb[c];
+    if (c == 0) bb.x += T(fb_dc);   // k = 0 offset (as LoadTwopt::fb_dc)
     double2 pk = dmul(dmk(a.x, a.y), dmk(bb.x, -double(bb.y)));
     double c1 = alias_at(tab, i, j, k);
     double w = tab.interlace ? window_at(tab, i, j, k) * window_at(tab, i, j, k)
