@@ -408,6 +408,43 @@ static void launch_x_load_batch(Context& ctx, c2<T>* base, const BatchLoader& lo
   TRVG_LAUNCH_CHECK();
 }
 
+template <typename T>
+static void xfirst_yz(Context& ctx, c2<T>* base, const double* kcut2, const int2* wy, int ny,
+                      const int2* wz, int nz, double by, double bz);
+
+// The three passes of a batched x-first transform: field f of the work
+// lists lives at base + f * nmesh, loads through load.field(f) and is pruned
+// with radius kcut2[f] (+inf: no pruning); wz lists every block.
+template <typename T, class BatchLoader>
+static void xfirst_passes(Context& ctx, c2<T>* base, const BatchLoader& load,
+                          const double* kcut2, const int2* wx, int nx, const int2* wy,
+                          int ny, const int2* wz, int nz, double bx, double by, double bz) {
+  const Geom& g = ctx.g;
+  if (nx > 0)
+    TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_load_batch<T, LG>(
+        ctx, base, load, prune_zy(g), kcut2, wx, nx, bx)));
+  xfirst_yz<T>(ctx, base, kcut2, wy, ny, wz, nz, by, bz);
+}
+
+// The y and z passes of a batched x-first transform.
+template <typename T>
+static void xfirst_yz(Context& ctx, c2<T>* base, const double* kcut2, const int2* wy, int ny,
+                      const int2* wz, int nz, double by, double bz) {
+  const Geom& g = ctx.g;
+  const Prune pr = prune_zy(g);
+  if (ny > 0)
+    TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
+        ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], +1, pr, kcut2, wy, ny, 3,
+        by)));
+  // z pass over every block, reading only the k_z rows the y pass wrote.
+  Prune off{};
+  off.on = 0;
+  LoadMaskedZBatch<T> lz{base, g.nmesh, kcut2, g.dk[2], g.n[2]};
+  if (nz > 0)
+    TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
+        ctx, base, lz, +1, off, kcut2, wz, nz, bz)));
+}
+
 // Shell fields of (ell, m) for `bins`: the backward transform of
 // y_lm(k^) src(k) / W(k) on each bin's shell, all bins per pass in one
 // launch.  Unpacked: field q at base + q * nmesh.  Packed (real shells:
@@ -463,29 +500,82 @@ void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
                           hipMemcpyHostToDevice, ctx.stream));
   TRVG_HIP(hipMemcpyAsync(dw.get() + wx.size() + wy.size(), wz.data(),
                           wz.size() * sizeof(int2), hipMemcpyHostToDevice, ctx.stream));
-  const Prune pr = prune_zy(g);
-  if (!wx.empty()) {
-    if (packed) {
-      LoadShellPairBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
-      TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_load_batch<T, LG>(
-          ctx, base, ld, pr, dkc.get(), dw.get(), int(wx.size()), bx)));
-    } else {
-      LoadShellBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
-      TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_load_batch<T, LG>(
-          ctx, base, ld, pr, dkc.get(), dw.get(), int(wx.size()), bx)));
-    }
+  const int2* px = dw.get();
+  const int2* py = px + wx.size();
+  const int2* pz = py + wy.size();
+  if (packed) {
+    LoadShellPairBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
+    xfirst_passes<T>(ctx, base, ld, dkc.get(), px, int(wx.size()), py, int(wy.size()), pz,
+                     int(wz.size()), bx, by, bz);
+  } else {
+    LoadShellBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
+    xfirst_passes<T>(ctx, base, ld, dkc.get(), px, int(wx.size()), py, int(wy.size()), pz,
+                     int(wz.size()), bx, by, bz);
   }
-  if (!wy.empty())
-    TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
-        ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], +1, pr, dkc.get(),
-        dw.get() + wx.size(), int(wy.size()), 3, by)));
-  // z pass over every block, reading only the k_z rows the y pass wrote.
-  Prune off{};
-  off.on = 0;
-  LoadMaskedZBatch<T> lz{base, g.nmesh, dkc.get(), g.dk[2], g.n[2]};
-  TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
-      ctx, base, lz, +1, off, dkc.get(), dw.get() + wx.size() + wy.size(), int(wz.size()),
-      bz)));
+}
+
+template <typename T, int LOGN>
+static void launch_x_sjl(Context& ctx, c2<T>* base, int nt, const double2* src,
+                         const Spline& sp, const double* r, int ell, int m, bool packed,
+                         double bytes) {
+  constexpr int EPT = FftEpt<T>::v;
+  constexpr int N = 1 << LOGN;
+  constexpr int E = FFT_THREADS * EPT;
+  const Geom& g = ctx.g;
+  const size_t smem = fft_smem<T, LOGN>(false);
+  const unsigned grid = unsigned((long long)g.n[1] * g.n[2] / (E / N));
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
+  if (packed)
+    hipLaunchKernelGGL((fft_x_sjl_multi<T, LOGN, EPT, true>), dim3(grid), dim3(FFT_THREADS),
+                       smem, ctx.stream, base, g.nmesh, nt, src, ctx.tab, g, sp.dev(), r, ell,
+                       m, twiddles<T>(N, +1));
+  else
+    hipLaunchKernelGGL((fft_x_sjl_multi<T, LOGN, EPT, false>), dim3(grid), dim3(FFT_THREADS),
+                       smem, ctx.stream, base, g.nmesh, nt, src, ctx.tab, g, sp.dev(), r, ell,
+                       m, twiddles<T>(N, +1));
+  TRVG_LAUNCH_CHECK();
+}
+
+// j_l-weighted shell fields of (ell, m) for `bins` (3PCF,
+// inv_fourier_transform_sjl_ylm_wgtd_field, field.cpp:1908-2010), r the
+// bins' effective radii: x-first batched transforms over every block (no
+// band limit); layout and packing as shell_slab_ifft.
+template <typename T>
+void sjl_slab_ifft(Context& ctx, c2<T>* base, const double2* src, const Spline& sp,
+                   const std::vector<double>& r, const std::vector<int>& bins, int ell,
+                   int m, bool packed) {
+  const Geom& g = ctx.g;
+  const int nf = int(bins.size());
+  if (nf == 0) return;
+  const int nt = packed ? (nf + 1) / 2 : nf;
+  const int per = packed ? 2 : 1;
+  constexpr int EPT = FftEpt<T>::v;
+  const int E = FFT_THREADS * EPT;
+  const PruneListsXF& pl = prune_lists_xfirst(g, HUGE_VAL, E / g.n[0], E / g.n[1]);
+  const long long nblk = g.nmesh / E;
+  std::vector<double> rr((size_t)nt * per, -1.), kc(nt, HUGE_VAL);
+  for (int q = 0; q < nf; q++) rr[q] = r[bins[q]];
+  std::vector<int2> w;   // y blocks, then z blocks
+  for (int f = 0; f < nt; f++)
+    for (int b : pl.yblocks) w.push_back(make_int2(f, b));
+  const size_t ny = w.size();
+  for (int f = 0; f < nt; f++)
+    for (long long b = 0; b < nblk; b++) w.push_back(make_int2(f, int(b)));
+  const size_t nz = w.size() - ny;
+  DevArray<double> dr(rr.size()), dkc(nt);
+  DevArray<int2> dw(w.size());
+  TRVG_HIP(hipMemcpyAsync(dr.get(), rr.data(), rr.size() * 8, hipMemcpyHostToDevice,
+                          ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(dkc.get(), kc.data(), nt * 8, hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(dw.get(), w.data(), w.size() * sizeof(int2), hipMemcpyHostToDevice,
+                          ctx.stream));
+  // Algorithmic bytes: the source read once, then each pass writes (and the
+  // later ones read) every field.
+  const double fb = double(g.nmesh) * sizeof(c2<T>) * nt;
+  const double bx = double(g.nmesh) * sizeof(double2) + fb, by = 2. * fb, bz = 2. * fb;
+  TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_sjl<T, LG>(ctx, base, nt, src, sp, dr.get(), ell,
+                                                        m, packed, bx)));
+  xfirst_yz<T>(ctx, base, dkc.get(), dw.get(), int(ny), dw.get() + ny, int(nz), by, bz);
 }
 
 template <typename T, class Loader>
@@ -1516,22 +1606,21 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
 
     auto make_sjl = [&](int l, int mm, const Spline& sp) {
       return [&, l, mm](c2<T>* base, const std::vector<int>& bins) {
-        for (size_t q = 0; q < bins.size(); q++) {
-          LoadSjl<T, double> ls{dn00.get(), ctx.tab, g, sp.dev(), reff[bins[q]], l, mm};
-          fft3d<T>(ctx, base + nm * q, ls, +1, double(nm) * sizeof(double2));
-        }
+        sjl_slab_ifft<T>(ctx, base, dn00.get(), sp, reff, bins, l, mm, real_shells(l, mm));
       };
     };
-    ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr, false,
-                                make_sjl(p.ell1, tm.m1, sja));
-    ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A, false,
-                                make_sjl(p.ell2, tm.m2, sjb));
+    ShellSlab<T>* A = slabs.get(ctx, p.ell1, tm.m1, rows, nb, nullptr,
+                                real_shells(p.ell1, tm.m1), make_sjl(p.ell1, tm.m1, sja));
+    ShellSlab<T>* B = slabs.get(ctx, p.ell2, tm.m2, cols, nb, A,
+                                real_shells(p.ell2, tm.m2), make_sjl(p.ell2, tm.m2, sjb));
     std::vector<int2> pairs(dim);
     for (int dv = 0; dv < dim; dv++)
       pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
-    std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields, false,
-                                         B->data.get(), B->nfields, false, G.get(),
-                                         false, pairs);
+    // G_LM is real for M = 0 (real weights Y_L0) and in a periodic box.
+    const bool g_real = !survey || tm.M == 0;
+    std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields, A->packed,
+                                         B->data.get(), B->nfields, B->packed, G.get(),
+                                         g_real, pairs);
     for (int dv = 0; dv < dim; dv++) {
       cplx s = S[dv];
       out.raw[dv] += tm.coupling * g.vol_cell * factor_phase

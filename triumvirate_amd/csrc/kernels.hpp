@@ -273,30 +273,76 @@ struct LoadShellBatch {
   }
 };
 
-// j_l(|k| r) y_lm(k^) src(k) / W(k) / V
-// (inv_fourier_transform_sjl_ylm_wgtd_field, field.cpp:1908-2010).
-template <typename T, typename S = T>
-struct LoadSjl {
-  const c2<S>* src;
-  AxisTables tab;
-  Geom geo;
-  SplineDev sj;
-  double r;
-  int ell, m;
-  __device__ c2<T> operator()(long long g, int i, int j, int k) const {
-    double kx = grid_coord(i, geo.n[0], geo.dk[0]);
-    double ky = grid_coord(j, geo.n[1], geo.dk[1]);
-    double kz = grid_coord(k, geo.n[2], geo.dk[2]);
-    double km = mag3_exact(kx, ky, kz);
-    c2<S> v = src[g];
-    double iw = 1. / window_at(tab, i, j, k);
-    double2 fk = dmk(double(v.x) * iw, double(v.y) * iw);
-    double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
-    double2 yf = dmul(y, fk);
-    double jv = sj.eval(km * r);
-    return mk<T>(T(jv * yf.x / geo.vol), T(jv * yf.y / geo.vol));
+// x pass (x-first order) of every j_l shell field of one (l, m) at once,
+// field f = j_l(|k| r_f) y_lm(k^) src(k) / W(k) / V
+// (inv_fourier_transform_sjl_ylm_wgtd_field, field.cpp:1908-2010):
+// a workgroup takes NLINES x lines, forms y_lm(k^) src(k) / W(k) and |k|
+// of its elements once, then for each transform fills LDS with
+// j_l(|k| r_f) times that factor (PACKED: two radii in the real and
+// imaginary parts, r < 0 absent), transforms and writes field f.
+template <typename T, int LOGN, int EPT, bool PACKED>
+__global__ void __launch_bounds__(FFT_THREADS)
+fft_x_sjl_multi(c2<T>* __restrict__ base, long long fstride, int nt,
+                const double2* __restrict__ src, AxisTables tab, Geom geo, SplineDev sj,
+                const double* __restrict__ r, int ell, int m,
+                const c2<T>* __restrict__ tw) {
+  constexpr int N = 1 << LOGN;
+  constexpr int E = FFT_THREADS * EPT;
+  constexpr int NLINES = E / N;
+  constexpr int LS = N + 1;
+  extern __shared__ char smem_raw[];
+  c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
+  c2<T>* W = buf + NLINES * LS;
+  const int t = threadIdx.x;
+  const int n1 = geo.n[1], n2 = geo.n[2];
+  const long long S = (long long)n1 * n2;
+  const long long id0 = xcd_block(blockIdx.x, gridDim.x) * NLINES;
+  for (int q = t; q < N; q += FFT_THREADS) W[q] = tw[q];
+  double2 fac[EPT];
+  double km[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    const int idx = t + e * FFT_THREADS;
+    const int n = idx / NLINES, l = idx % NLINES;
+    const long long id = id0 + l;
+    const int j = int(id / n2), k = int(id % n2);
+    const double kx = grid_coord(n, geo.n[0], geo.dk[0]);
+    const double ky = grid_coord(j, n1, geo.dk[1]);
+    const double kz = grid_coord(k, n2, geo.dk[2]);
+    km[e] = mag3_exact(kx, ky, kz);
+    const double2 v = src[id + (long long)n * S];
+    const double iw = 1. / window_at(tab, n, j, k);
+    fac[e] = dmul(ylm_reduced_dev(ell, m, kx, ky, kz), dmk(v.x * iw, v.y * iw));
   }
-};
+  for (int f = 0; f < nt; f++) {
+    const double r0 = PACKED ? r[2 * f] : r[f];
+    const double r1 = PACKED ? r[2 * f + 1] : -1.;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const int idx = t + e * FFT_THREADS;
+      const double j0 = sj.eval(km[e] * r0);
+      c2<T> val;
+      if (PACKED) {
+        const double j1 = r1 < 0. ? 0. : sj.eval(km[e] * r1);
+        val = mk<T>(T((j0 * fac[e].x - j1 * fac[e].y) / geo.vol),
+                    T((j0 * fac[e].y + j1 * fac[e].x) / geo.vol));
+      } else {
+        val = mk<T>(T(j0 * fac[e].x / geo.vol), T(j0 * fac[e].y / geo.vol));
+      }
+      buf[(idx % NLINES) * LS + idx / NLINES] = val;
+    }
+    __syncthreads();
+    lds_fft_lines<T, LOGN, EPT>(buf, W, +1);
+    c2<T>* out = base + (long long)f * fstride;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const int idx = t + e * FFT_THREADS;
+      const int n = idx / NLINES, l = idx % NLINES;
+      out[id0 + l + (long long)n * S] = buf[l * LS + n];
+    }
+    __syncthreads();
+  }
+}
 
 // (f_a f_b^* / C1 - S C1 / C1) / V  (field.cpp:3007-3025, 3271-3289).
 template <typename T>
