@@ -411,7 +411,8 @@ class BatchPlan : public BatchPlanBase {
         for (int dv = 0; dv < P->dim; dv++)
           active[(size_t)(rows[dv] / 4) * cbn + cols[dv] / 4] = 1;
         const int CB = std::min(cbn, 8);
-        const int RB = std::max(1, std::min(std::min((64 - 4 * CB) / 4, PT_MAXT / CB),
+        const int RB = std::max(1, std::min(std::min((PT_MAXROWS - 1 - 4 * CB) / 4,
+                                                     PT_MAXT / CB),
                                             tune_.rb_cap));
         for (int c0b = 0; c0b < cbn; c0b += CB) {
           const int c1b = std::min(cbn, c0b + CB);

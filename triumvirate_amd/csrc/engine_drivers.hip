@@ -932,11 +932,10 @@ static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
   for (const int2& pr : pairs) active[(size_t)(pr.x / 4) * cbn + pr.y / 4] = 1;
   // Sums for every pair of every active tile, indexed [ia * nbf + ib].
   std::vector<cplx> full((size_t)na * nbf, cplx(0., 0.));
-  // Column blocks per launch and row blocks to match (at most 32 fields a
-  // side and PT_MAXT tiles: the double-buffered staging of 65 rows fits the
-  // 160 KiB of LDS).
+  // Column blocks per launch and row blocks to match: at most 32 fields a
+  // side, PT_MAXT tiles and PT_MAXROWS staged rows (4 per block a side).
   const int CB = std::min(cbn, 8);
-  const int RB = std::max(1, std::min(std::min((64 - 4 * CB) / 4, PT_MAXT / CB), 8));
+  const int RB = std::max(1, std::min(std::min((PT_MAXROWS - 1 - 4 * CB) / 4, PT_MAXT / CB), 8));
   if (nm % PairLds<T>::LDC != 0)
     throw Error(TRVG_ERR_INTERNAL, "Pair reduction needs whole chunks of the mesh.");
   const long long nchunks = nm / PairLds<T>::LDC;

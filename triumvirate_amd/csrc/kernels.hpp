@@ -1122,8 +1122,11 @@ k_unpack_hermitian(double2* __restrict__ P, double2* __restrict__ B, Geom g) {
 
 namespace trvg {
 
-// Most 4x4 pair tiles per pair-reduction launch.
-constexpr int PT_MAXT = 32;
+// Most 4x4 pair tiles per pair-reduction launch (four thread groups each),
+// and most staged mesh rows (both sides and G), which keeps the double-
+// buffered staging of the pair kernel within the 160 KiB of LDS.
+constexpr int PT_MAXT = 64;
+constexpr int PT_MAXROWS = 76;
 
 // U consecutive LDS elements read as one (16-byte aligned) access.
 template <typename E, int U>
