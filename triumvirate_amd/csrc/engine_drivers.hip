@@ -639,14 +639,16 @@ static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
   tv->tg.ntiles = (long long)tv->tg.nt[0] * tv->tg.nt[1] * tv->tg.nt[2];
   const long long nt = tv->tg.ntiles;
   DevArray<int> tile_of(c.n);
-  DevArray<unsigned int> counts(nt + 1), offs(nt + 2);
+  DevArray<unsigned int> counts(nt + 1), offs(nt + 2), ccount((nt + 1) * TILE_CLASSES),
+      key2(c.n);
   TRVG_HIP(hipMemsetAsync(counts.get(), 0, (nt + 1) * 4, ctx.stream));
+  TRVG_HIP(hipMemsetAsync(ccount.get(), 0, (nt + 1) * TILE_CLASSES * 4, ctx.stream));
   DevParticles pv = c.view();
   unsigned grid = grid_for(ctx, c.n, 16);
   {
     ProfScope ps(ctx, KC_ASSIGN, c.n * 28.);
     hipLaunchKernelGGL(k_tile_count, dim3(grid), dim3(256), 0, ctx.stream, pv,
-                       g, tv->tg, tile_of.get(), counts.get());
+                       g, tv->tg, tile_of.get(), counts.get(), ccount.get(), key2.get());
     TRVG_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx.stream,
@@ -660,8 +662,8 @@ static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
   {
     ProfScope ps(ctx, KC_ASSIGN, c.n * (c.los.get() ? 116. : 68.));
     hipLaunchKernelGGL(k_tile_scatter, dim3(grid), dim3(256), 0, ctx.stream, pv,
-                       tile_of.get(), offs.get(), tv->x.get(), tv->y.get(),
-                       tv->z.get(), tv->w.get(), tv->los.get());
+                       tile_of.get(), offs.get(), ccount.get(), key2.get(), tv->x.get(),
+                       tv->y.get(), tv->z.get(), tv->w.get(), tv->los.get());
     TRVG_LAUNCH_CHECK();
   }
   TRVG_HIP(hipStreamSynchronize(ctx.stream));

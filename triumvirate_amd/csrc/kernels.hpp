@@ -828,10 +828,20 @@ __device__ __forceinline__ void base_cell(const Geom& g, double x, double y,
   }
 }
 
-// Tile id per particle (ntiles for stragglers) and per-tile counts.
+// Bank classes of the tile-local cells (FP64 accumulators: 32 classes of
+// the LDS banks).  Within a tile, particles are ordered by (rank in class,
+// class), so the lanes of one LDS atomic instruction start at distinct
+// classes; every stencil offset shifts all lanes alike, keeping them
+// distinct.
+constexpr int TILE_CLASSES = 32;
+
+// Tile id per particle (ntiles for stragglers), per-tile counts, and the
+// particle's bank class and rank within (tile, class) as key2 = rank * 32 +
+// class.
 static __global__ void __launch_bounds__(256)
 k_tile_count(DevParticles p, Geom g, TileGeom tg, int* __restrict__ tile_of,
-             unsigned int* __restrict__ counts) {
+             unsigned int* __restrict__ counts, unsigned int* __restrict__ ccount,
+             unsigned int* __restrict__ key2) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < p.n;
        i += (long long)gridDim.x * blockDim.x) {
     int idx[3];
@@ -843,8 +853,15 @@ k_tile_count(DevParticles p, Geom g, TileGeom tg, int* __restrict__ tile_of,
       t = ((long long)(idx[0] / TILE) * tg.nt[1] + idx[1] / TILE) * tg.nt[2] +
           idx[2] / TILE;
     }
+    int cls = 0;
+    if (t < tg.ntiles) {
+      const int lx = idx[0] % TILE + 1, ly = idx[1] % TILE + 1, lz = idx[2] % TILE + 1;
+      cls = ((lx * TILE_H + ly) * TILE_H + lz) % TILE_CLASSES;
+    }
     tile_of[i] = int(t);
     atomicAdd(&counts[t], 1u);
+    const unsigned int r = atomicAdd(&ccount[t * TILE_CLASSES + cls], 1u);
+    key2[i] = r * TILE_CLASSES + unsigned(cls);
   }
 }
 
@@ -875,15 +892,28 @@ k_scan_counts(const unsigned int* __restrict__ counts, long long n,
   if (threadIdx.x == 0) offsets[n] = carry;
 }
 
-// Scatter particle columns into tile order.
+// Scatter particle columns into tile order, ordered within a tile by
+// (rank in class, class): position = sum over classes c of min(n_c, rank)
+// plus the classes before the particle's that have more than rank members.
 static __global__ void __launch_bounds__(256)
 k_tile_scatter(DevParticles p, const int* __restrict__ tile_of,
-               unsigned int* __restrict__ cursor, double* __restrict__ sx,
+               const unsigned int* __restrict__ offs, const unsigned int* __restrict__ ccount,
+               const unsigned int* __restrict__ key2, double* __restrict__ sx,
                double* __restrict__ sy, double* __restrict__ sz,
                double* __restrict__ sw, double* __restrict__ slos) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < p.n;
        i += (long long)gridDim.x * blockDim.x) {
-    unsigned int d = atomicAdd(&cursor[tile_of[i]], 1u);
+    const long long t = tile_of[i];
+    const unsigned int r = key2[i] / TILE_CLASSES, cls = key2[i] % TILE_CLASSES;
+    const unsigned int* cc = ccount + t * TILE_CLASSES;
+    unsigned int pos = 0;
+#pragma unroll 8
+    for (unsigned int c = 0; c < TILE_CLASSES; c++) {
+      const unsigned int n = cc[c];
+      pos += n < r ? n : r;
+      if (c < cls && n > r) pos++;
+    }
+    const unsigned int d = offs[t] + pos;
     sx[d] = p.x[i];
     sy[d] = p.y[i];
     sz[d] = p.z[i];
