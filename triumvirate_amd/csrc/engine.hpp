@@ -4,6 +4,8 @@
 #pragma once
 
 #include <complex>
+#include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -186,6 +188,25 @@ class Profiler {
 // Per-call context.
 // ---------------------------------------------------------------------------
 
+// Pinned host memory for the asynchronous transfers of one engine call: a
+// bump allocator over page-locked chunks, one arena per pooled stream
+// (reset when a call takes the stream, after the previous call has
+// synchronised it).
+class PinnedArena {
+ public:
+  void* alloc(size_t bytes);
+  void reset() { chunk_ = 0; used_ = 0; }
+  ~PinnedArena();
+
+ private:
+  struct Chunk {
+    char* p;
+    size_t size;
+  };
+  std::vector<Chunk> chunks_;
+  size_t chunk_ = 0, used_ = 0;
+};
+
 struct Context {
   trvg_params p;
   Geom g;
@@ -194,11 +215,39 @@ struct Context {
   AxisTables tab;               // per-axis sinc, sin^2, u (cached tables)
   int sm_count = 256;
   std::vector<ProfRecord> prof;
+  PinnedArena* pin = nullptr;
+  // Host work waiting for device results, and device buffers those results
+  // are still being copied out of: both released by flush().
+  std::vector<std::function<void()>> deferred;
+  std::vector<std::shared_ptr<void>> keep;
 
   explicit Context(const trvg_params& params);
   ~Context();
   // Resolve pending profiler records (synchronises the stream).
   void flush_profile();
+
+  // Host -> device without blocking the host (staged in pinned memory).
+  template <typename E>
+  void upload(E* dst, const E* src, size_t n) {
+    if (n == 0) return;
+    void* h = pin->alloc(n * sizeof(E));
+    std::memcpy(h, src, n * sizeof(E));
+    TRVG_HIP(hipMemcpyAsync(dst, h, n * sizeof(E), hipMemcpyHostToDevice, stream));
+  }
+  // Device -> host into pinned memory; the values are valid after flush().
+  template <typename E>
+  const E* download(const E* src, size_t n) {
+    E* h = static_cast<E*>(pin->alloc((n ? n : 1) * sizeof(E)));
+    if (n) TRVG_HIP(hipMemcpyAsync(h, src, n * sizeof(E), hipMemcpyDeviceToHost, stream));
+    return h;
+  }
+  template <typename E>
+  void keep_until_flush(DevArray<E>&& a) {
+    keep.push_back(std::make_shared<DevArray<E>>(std::move(a)));
+  }
+  void defer(std::function<void()> f) { deferred.push_back(std::move(f)); }
+  // Synchronise the stream, then run the deferred host work in order.
+  void flush();
 };
 
 // Brackets one kernel launch with profiler events (no-op when disabled).

@@ -241,11 +241,56 @@ static const AxisTableData& axis_tables(int dev, const Geom& g, hipStream_t stre
   return *(g_tabs[key] = std::move(t));
 }
 
+void* PinnedArena::alloc(size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  while (chunk_ < chunks_.size()) {
+    if (used_ + bytes <= chunks_[chunk_].size) {
+      void* p = chunks_[chunk_].p + used_;
+      used_ += bytes;
+      return p;
+    }
+    chunk_++;
+    used_ = 0;
+  }
+  Chunk c;
+  c.size = std::max(bytes, size_t(16) << 20);
+  TRVG_HIP(hipHostMalloc((void**)&c.p, c.size));
+  chunks_.push_back(c);
+  chunk_ = chunks_.size() - 1;
+  used_ = bytes;
+  return c.p;
+}
+
+PinnedArena::~PinnedArena() {
+  for (auto& c : chunks_) (void)hipHostFree(c.p);
+}
+
+// One arena per pooled stream (a stream serves one Context at a time).
+static PinnedArena& pinned_arena(hipStream_t s) {
+  static std::mutex mu;
+  static std::map<hipStream_t, std::unique_ptr<PinnedArena>>* arenas =
+      new std::map<hipStream_t, std::unique_ptr<PinnedArena>>();
+  std::lock_guard<std::mutex> lk(mu);
+  auto& a = (*arenas)[s];
+  if (!a) a = std::make_unique<PinnedArena>();
+  return *a;
+}
+
+void Context::flush() {
+  TRVG_HIP(hipStreamSynchronize(stream));
+  keep.clear();
+  std::vector<std::function<void()>> d;
+  d.swap(deferred);
+  for (auto& f : d) f();
+}
+
 Context::Context(const trvg_params& params) : p(params) {
   if (p.device >= 0) TRVG_HIP(hipSetDevice(p.device));
   g = make_geom(p);
   TRVG_HIP(hipGetDevice(&dev));
   stream = acquire_stream(dev);
+  pin = &pinned_arena(stream);
+  pin->reset();
   TRVG_HIP(hipDeviceGetAttribute(&sm_count, hipDeviceAttributeMultiprocessorCount, dev));
   const AxisTableData& t = axis_tables(dev, g, stream);
   for (int a = 0; a < 3; a++) {
@@ -260,6 +305,7 @@ Context::Context(const trvg_params& params) : p(params) {
 Context::~Context() {
   if (stream) {
     (void)hipStreamSynchronize(stream);
+    keep.clear();
     try {
       flush_profile();
     } catch (...) {

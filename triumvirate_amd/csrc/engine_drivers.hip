@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <set>
 #include <tuple>
 
 #include "batch_kernels.hpp"
@@ -490,16 +491,11 @@ void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
   DevArray<ShellBand> dbands(bands.size());
   DevArray<double> dkc(nt);
   DevArray<int2> dw(wx.size() + wy.size() + wz.size());
-  TRVG_HIP(hipMemcpyAsync(dbands.get(), bands.data(), bands.size() * sizeof(ShellBand),
-                          hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dkc.get(), kc.data(), nt * sizeof(double),
-                          hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dw.get(), wx.data(), wx.size() * sizeof(int2),
-                          hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dw.get() + wx.size(), wy.data(), wy.size() * sizeof(int2),
-                          hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dw.get() + wx.size() + wy.size(), wz.data(),
-                          wz.size() * sizeof(int2), hipMemcpyHostToDevice, ctx.stream));
+  ctx.upload(dbands.get(), bands.data(), bands.size());
+  ctx.upload(dkc.get(), kc.data(), nt);
+  ctx.upload(dw.get(), wx.data(), wx.size());
+  ctx.upload(dw.get() + wx.size(), wy.data(), wy.size());
+  ctx.upload(dw.get() + wx.size() + wy.size(), wz.data(), wz.size());
   const int2* px = dw.get();
   const int2* py = px + wx.size();
   const int2* pz = py + wy.size();
@@ -564,11 +560,9 @@ void sjl_slab_ifft(Context& ctx, c2<T>* base, const double2* src, const Spline& 
   const size_t nz = w.size() - ny;
   DevArray<double> dr(rr.size()), dkc(nt);
   DevArray<int2> dw(w.size());
-  TRVG_HIP(hipMemcpyAsync(dr.get(), rr.data(), rr.size() * 8, hipMemcpyHostToDevice,
-                          ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dkc.get(), kc.data(), nt * 8, hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dw.get(), w.data(), w.size() * sizeof(int2), hipMemcpyHostToDevice,
-                          ctx.stream));
+  ctx.upload(dr.get(), rr.data(), rr.size());
+  ctx.upload(dkc.get(), kc.data(), nt);
+  ctx.upload(dw.get(), w.data(), w.size());
   // Algorithmic bytes: the source read once, then each pass writes (and the
   // later ones read) every field.
   const double fb = double(g.nmesh) * sizeof(c2<T>) * nt;
@@ -681,10 +675,7 @@ static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
   tv->nwork = int(work.size());
   if (!work.empty()) {
     tv->work.reset(work.size());
-    TRVG_HIP(hipMemcpyAsync(tv->work.get(), work.data(),
-                            work.size() * sizeof(TileWork),
-                            hipMemcpyHostToDevice, ctx.stream));
-    TRVG_HIP(hipStreamSynchronize(ctx.stream));
+    ctx.upload(tv->work.get(), work.data(), work.size());
   }
   c.tiled = std::move(tv);
   return *c.tiled;
@@ -909,31 +900,37 @@ static void launch_pair_lds(Context& ctx, unsigned grid, size_t smem, const c2<T
   TRVG_LAUNCH_CHECK();
 }
 
-// S_p = sum_x F_a(x) F_b(x) G(x) for pairs p = (a, b) of logical fields.
+// S_p = sum_x F_a(x) F_b(x) G(x) for pairs p = (a, b) of logical fields,
+// handed to `sink` at the next ctx.flush() (the launches are only queued).
 // A side / B side packed: real fields stored two per complex mesh (logical
 // field 2q + c is component c of mesh q); g_real: G is real.
 template <typename T>
-static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
-                                     bool a_packed, const c2<T>* FB, int nbf,
-                                     bool b_packed, const c2<T>* G, bool g_real,
-                                     const std::vector<int2>& pairs) {
+static void pair_reduce(Context& ctx, const c2<T>* FA, int na, bool a_packed,
+                        const c2<T>* FB, int nbf, bool b_packed, const c2<T>* G,
+                        bool g_real, const std::vector<int2>& pairs,
+                        std::function<void(std::vector<cplx>&&)> sink) {
   if (a_packed && !b_packed) {
     // The product is symmetric in (a, b): put the packed side second.
     std::vector<int2> sw(pairs.size());
     for (size_t d = 0; d < pairs.size(); d++) sw[d] = make_int2(pairs[d].y, pairs[d].x);
-    return pair_reduce<T>(ctx, FB, nbf, false, FA, na, true, G, g_real, sw);
+    pair_reduce<T>(ctx, FB, nbf, false, FA, na, true, G, g_real, sw, std::move(sink));
+    return;
   }
   if (a_packed && !g_real)
     throw Error(TRVG_ERR_INTERNAL, "Real shell pairs need a real G field.");
   const int mode = (a_packed && b_packed) ? 2 : (b_packed ? 1 : 0);
   const long long nm = ctx.g.nmesh;
-  const int P = int(pairs.size());
   const int rbn = (na + 3) / 4, cbn = (nbf + 3) / 4;
   // Active 4x4 tiles of the (ia, ib) grid.
   std::vector<char> active((size_t)rbn * cbn, 0);
   for (const int2& pr : pairs) active[(size_t)(pr.x / 4) * cbn + pr.y / 4] = 1;
-  // Sums for every pair of every active tile, indexed [ia * nbf + ib].
-  std::vector<cplx> full((size_t)na * nbf, cplx(0., 0.));
+  // Per launch group: its tiles, origin and the host copy of its sums.
+  struct Group {
+    std::vector<int2> tl;
+    int ra0, ca0;
+    const double2* h;
+  };
+  std::vector<Group> groups;
   // Column blocks per launch and row blocks to match: at most 32 fields a
   // side, PT_MAXT tiles and PT_MAXROWS staged rows (4 per block a side).
   const int CB = std::min(cbn, 8);
@@ -955,8 +952,7 @@ static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
       const int nbs = std::min(nbf, 4 * c1b) - ca0;
       const int nt = int(tl.size());
       DevArray<int2> dt(nt);
-      TRVG_HIP(hipMemcpyAsync(dt.get(), tl.data(), nt * sizeof(int2),
-                              hipMemcpyHostToDevice, ctx.stream));
+      ctx.upload(dt.get(), tl.data(), nt);
       // Resident blocks only (each loops over chunks): as many per CU as
       // the double-buffered staging allows.
       const bool same_side = mode == 2 && FA == FB && ra0 == ca0 && nas == nbs;
@@ -980,23 +976,30 @@ static std::vector<cplx> pair_reduce(Context& ctx, const c2<T>* FA, int na,
         TRVG_LAUNCH_CHECK();
       }
       reduce_cols(ctx, part.get(), int(grid), nt * 16, sum.get());
-      std::vector<double2> h((size_t)nt * 16);
-      TRVG_HIP(hipMemcpyAsync(h.data(), sum.get(), h.size() * sizeof(double2),
-                              hipMemcpyDeviceToHost, ctx.stream));
-      TRVG_HIP(hipStreamSynchronize(ctx.stream));
-      for (int t = 0; t < nt; t++)
-        for (int i = 0; i < 4; i++)
-          for (int j = 0; j < 4; j++) {
-            int a = ra0 + tl[t].x + i, b = ca0 + tl[t].y + j;
-            if (a < na && b < nbf)
-              full[(size_t)a * nbf + b] = cplx(h[t * 16 + i * 4 + j].x,
-                                               h[t * 16 + i * 4 + j].y);
-          }
+      groups.push_back(Group{tl, ra0, ca0, ctx.download(sum.get(), (size_t)nt * 16)});
+      ctx.keep_until_flush(std::move(dt));
+      ctx.keep_until_flush(std::move(part));
+      ctx.keep_until_flush(std::move(sum));
     }
   }
-  std::vector<cplx> out(P);
-  for (int d = 0; d < P; d++) out[d] = full[(size_t)pairs[d].x * nbf + pairs[d].y];
-  return out;
+  ctx.defer([groups = std::move(groups), pairs, na, nbf, sink = std::move(sink)]() {
+    // Sums for every pair of every active tile, indexed [ia * nbf + ib].
+    std::vector<cplx> full((size_t)na * nbf, cplx(0., 0.));
+    for (const Group& gr : groups)
+      for (size_t t = 0; t < gr.tl.size(); t++)
+        for (int i = 0; i < 4; i++)
+          for (int j = 0; j < 4; j++) {
+            const int a = gr.ra0 + gr.tl[t].x + i, b = gr.ca0 + gr.tl[t].y + j;
+            if (a < na && b < nbf) {
+              const double2 v = gr.h[t * 16 + i * 4 + j];
+              full[(size_t)a * nbf + b] = cplx(v.x, v.y);
+            }
+          }
+    std::vector<cplx> out(pairs.size());
+    for (size_t d = 0; d < pairs.size(); d++)
+      out[d] = full[(size_t)pairs[d].x * nbf + pairs[d].y];
+    sink(std::move(out));
+  });
 }
 
 struct TwoPt {
@@ -1006,13 +1009,12 @@ struct TwoPt {
 
 // FieldStats::compute_ylm_wgtd_2pt_stats_in_fourier (field.cpp:2511-2703).
 template <typename T>
-static TwoPt twopt_fourier(Context& ctx, const c2<T>* fa, const c2<T>* fb,
-                           cplx sbar, int ell, int m, const trvg_binning& kb,
-                           double modes_in_range) {
+static void twopt_fourier(Context& ctx, const c2<T>* fa, const c2<T>* fb, cplx sbar,
+                          int ell, int m, const trvg_binning& kb, double modes_in_range,
+                          std::function<void(TwoPt&&)> sink) {
   const int nb = kb.num_bins;
   DevArray<double> edges(nb + 1);
-  TRVG_HIP(hipMemcpyAsync(edges.get(), kb.bin_edges, (nb + 1) * 8,
-                          hipMemcpyHostToDevice, ctx.stream));
+  ctx.upload(edges.get(), kb.bin_edges, nb + 1);
   // Latency-bound (dependent table and field loads per cell): eight
   // workgroups per CU.
   unsigned grid = grid_for(ctx, ctx.g.nmesh, 8);
@@ -1028,27 +1030,30 @@ static TwoPt twopt_fourier(Context& ctx, const c2<T>* fa, const c2<T>* fb,
                      cnt.get(), part.get());
   TRVG_LAUNCH_CHECK();
   reduce_cols(ctx, part.get(), int(grid), nb * 4, sum.get());
-  std::vector<unsigned long long> hc(nb);
-  std::vector<double> hs(nb * 4);
-  TRVG_HIP(hipMemcpyAsync(hc.data(), cnt.get(), nb * 8, hipMemcpyDeviceToHost, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(hs.data(), sum.get(), nb * 32, hipMemcpyDeviceToHost, ctx.stream));
-  TRVG_HIP(hipStreamSynchronize(ctx.stream));
-  TwoPt r;
-  r.nmodes.resize(nb);
-  r.pk.resize(nb);
-  r.sn.resize(nb);
-  for (int b = 0; b < nb; b++) {
-    r.nmodes[b] = int(hc[b]);
-    if (hc[b] != 0) {
-      double n = double(hc[b]);
-      r.pk[b] = cplx(hs[4 * b], hs[4 * b + 1]) / n;
-      r.sn[b] = cplx(hs[4 * b + 2], hs[4 * b + 3]) / n;
-    } else {
-      r.pk[b] = 0.;
-      r.sn[b] = 0.;
+  const unsigned long long* hc = ctx.download(cnt.get(), nb);
+  const double* hs = ctx.download(sum.get(), (size_t)nb * 4);
+  ctx.keep_until_flush(std::move(edges));
+  ctx.keep_until_flush(std::move(cnt));
+  ctx.keep_until_flush(std::move(part));
+  ctx.keep_until_flush(std::move(sum));
+  ctx.defer([hc, hs, nb, sink = std::move(sink)]() {
+    TwoPt r;
+    r.nmodes.resize(nb);
+    r.pk.resize(nb);
+    r.sn.resize(nb);
+    for (int b = 0; b < nb; b++) {
+      r.nmodes[b] = int(hc[b]);
+      if (hc[b] != 0) {
+        double n = double(hc[b]);
+        r.pk[b] = cplx(hs[4 * b], hs[4 * b + 1]) / n;
+        r.sn[b] = cplx(hs[4 * b + 2], hs[4 * b + 3]) / n;
+      } else {
+        r.pk[b] = 0.;
+        r.sn[b] = 0.;
+      }
     }
-  }
-  return r;
+    sink(std::move(r));
+  });
 }
 
 struct ShotBins3pcf {
@@ -1133,15 +1138,13 @@ static PairBinLists pair_bin_lists(const std::vector<double>& keff,
 }
 
 template <typename T>
-static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
-                                           int ma, int lb, int mb,
-                                           const Spline& sa, const Spline& sb,
-                                           const std::vector<double>& keff,
-                                           const std::vector<int>& ra,
-                                           const std::vector<int>& cb) {
+static void shot_pairs_bispec(Context& ctx, const c2<T>* t, int la, int ma, int lb,
+                              int mb, const Spline& sa, const Spline& sb,
+                              const std::vector<double>& keff, const std::vector<int>& ra,
+                              const std::vector<int>& cb,
+                              std::function<void(std::vector<cplx>&&)> sink) {
   const int P = int(ra.size());
   const Geom& g = ctx.g;
-  std::vector<cplx> out(P);
   const bool cubic = (g.dr[0] == g.dr[1] && g.dr[1] == g.dr[2]);
   if (cubic) {
     long long nq = (long long)(g.n[0] / 2) * (g.n[0] / 2)
@@ -1155,10 +1158,9 @@ static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
       throw Error(TRVG_ERR_PARAM, "Too many wavenumber bins for the shot-noise tables.");
     DevArray<double> dka(nka), dkb(nkb);
     DevArray<int2> dp(P);
-    TRVG_HIP(hipMemcpyAsync(dka.get(), L.ka.data(), nka * 8, hipMemcpyHostToDevice, ctx.stream));
-    TRVG_HIP(hipMemcpyAsync(dkb.get(), L.kb.data(), nkb * 8, hipMemcpyHostToDevice, ctx.stream));
-    TRVG_HIP(hipMemcpyAsync(dp.get(), L.pidx.data(), P * sizeof(int2),
-                            hipMemcpyHostToDevice, ctx.stream));
+    ctx.upload(dka.get(), L.ka.data(), nka);
+    ctx.upload(dkb.get(), L.kb.data(), nkb);
+    ctx.upload(dp.get(), L.pidx.data(), P);
     const long long nchunks = (nq + nch - 1) / nch;
     DevArray<double> Q(2 * nq);
     DevArray<double2> part((size_t)nchunks * P), res(P);
@@ -1183,20 +1185,28 @@ static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
       TRVG_LAUNCH_CHECK();
       reduce_cols(ctx, part.get(), int(nchunks), P, res.get());
     }
-    std::vector<double2> h(P);
-    TRVG_HIP(hipMemcpyAsync(h.data(), res.get(), P * sizeof(double2),
-                            hipMemcpyDeviceToHost, ctx.stream));
-    TRVG_HIP(hipStreamSynchronize(ctx.stream));
-    for (int p = 0; p < P; p++) out[p] = cplx(h[p].x, h[p].y);
-    return out;
+    const double2* h = ctx.download(res.get(), P);
+    ctx.keep_until_flush(std::move(dka));
+    ctx.keep_until_flush(std::move(dkb));
+    ctx.keep_until_flush(std::move(dp));
+    ctx.keep_until_flush(std::move(Q));
+    ctx.keep_until_flush(std::move(part));
+    ctx.keep_until_flush(std::move(res));
+    ctx.defer([h, P, sink = std::move(sink)]() {
+      std::vector<cplx> out(P);
+      for (int p = 0; p < P; p++) out[p] = cplx(h[p].x, h[p].y);
+      sink(std::move(out));
+    });
+    return;
   }
   std::vector<double> ka(P), kb(P);
   for (int d = 0; d < P; d++) { ka[d] = keff[ra[d]]; kb[d] = keff[cb[d]]; }
   DevArray<double> dka(P), dkb(P);
-  TRVG_HIP(hipMemcpyAsync(dka.get(), ka.data(), P * 8, hipMemcpyHostToDevice, ctx.stream));
-  TRVG_HIP(hipMemcpyAsync(dkb.get(), kb.data(), P * 8, hipMemcpyHostToDevice, ctx.stream));
+  ctx.upload(dka.get(), ka.data(), P);
+  ctx.upload(dkb.get(), kb.data(), P);
   const int GP = 8;
   unsigned grid = grid_for(ctx, g.nmesh, 4);
+  std::vector<const double2*> hs;
   for (int p0 = 0; p0 < P; p0 += GP) {
     int np = std::min(GP, P - p0);
     DevArray<double2> part((size_t)grid * np), sum(np);
@@ -1206,28 +1216,63 @@ static std::vector<cplx> shot_pairs_bispec(Context& ctx, const c2<T>* t, int la,
                        dka.get() + p0, dkb.get() + p0, np, part.get());
     TRVG_LAUNCH_CHECK();
     reduce_cols(ctx, part.get(), int(grid), np, sum.get());
-    std::vector<double2> h(np);
-    TRVG_HIP(hipMemcpyAsync(h.data(), sum.get(), np * sizeof(double2),
-                            hipMemcpyDeviceToHost, ctx.stream));
-    TRVG_HIP(hipStreamSynchronize(ctx.stream));
-    for (int q = 0; q < np; q++) out[p0 + q] = cplx(h[q].x, h[q].y) * g.vol_cell;
+    hs.push_back(ctx.download(sum.get(), np));
+    ctx.keep_until_flush(std::move(part));
+    ctx.keep_until_flush(std::move(sum));
   }
-  return out;
+  ctx.keep_until_flush(std::move(dka));
+  ctx.keep_until_flush(std::move(dkb));
+  ctx.defer([hs, P, GP, vol = g.vol_cell, sink = std::move(sink)]() {
+    std::vector<cplx> out(P);
+    for (int p0 = 0, gi = 0; p0 < P; p0 += GP, gi++) {
+      const int np = std::min(GP, P - p0);
+      for (int q = 0; q < np; q++)
+        out[p0 + q] = cplx(hs[gi][q].x, hs[gi][q].y) * vol;
+    }
+    sink(std::move(out));
+  });
 }
 
-static cplx sbar_catalogue(Context& ctx, const DeviceCatalogue& c, int ell, int m) {
+// Per-block partial sums of S-bar_LM of one catalogue, downloaded into
+// pinned memory (valid after the next flush).
+static std::pair<const double2*, unsigned> sbar_partials(Context& ctx,
+                                                         const DeviceCatalogue& c,
+                                                         int ell, int m) {
   unsigned grid = grid_for(ctx, c.n, 4);
   DevArray<double2> part(grid);
   hipLaunchKernelGGL(k_sbar, dim3(grid), dim3(RED_THREADS), 0, ctx.stream,
                      c.view(), ell, m, part.get());
   TRVG_LAUNCH_CHECK();
-  std::vector<double2> h(grid);
-  TRVG_HIP(hipMemcpyAsync(h.data(), part.get(), grid * sizeof(double2),
-                          hipMemcpyDeviceToHost, ctx.stream));
-  TRVG_HIP(hipStreamSynchronize(ctx.stream));
+  const double2* h = ctx.download(part.get(), grid);
+  ctx.keep_until_flush(std::move(part));
+  return {h, grid};
+}
+
+static cplx sum_partials(const std::pair<const double2*, unsigned>& h) {
   double sx = 0., sy = 0.;
-  for (unsigned b = 0; b < grid; b++) { sx += h[b].x; sy += h[b].y; }
+  for (unsigned b = 0; b < h.second; b++) { sx += h.first[b].x; sy += h.first[b].y; }
   return cplx(sx, sy);
+}
+
+// S-bar_LM = S-bar_LM[data] + alpha^3 S-bar_LM[randoms] for every (L, M)
+// in `keys` (threept.cpp:771-800): all reductions queued, one
+// synchronisation.  Call with no deferred work pending on the context.
+static std::map<std::pair<int, int>, cplx> sbar_table(
+    Context& ctx, const DeviceCatalogue& d, const DeviceCatalogue& r, double alpha,
+    const std::set<std::pair<int, int>>& keys) {
+  std::vector<std::pair<const double2*, unsigned>> hd, hr;
+  for (const auto& k : keys) {
+    hd.push_back(sbar_partials(ctx, d, k.first, k.second));
+    hr.push_back(sbar_partials(ctx, r, k.first, k.second));
+  }
+  ctx.flush();
+  std::map<std::pair<int, int>, cplx> out;
+  size_t i = 0;
+  for (const auto& k : keys) {
+    out[k] = sum_partials(hd[i]) + std::pow(alpha, 3) * sum_partials(hr[i]);
+    i++;
+  }
+  return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -1389,6 +1434,18 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
     };
   };
 
+  // S-bar_LM of every term of every multipole, computed up front.
+  std::map<std::pair<int, int>, cplx> sbar;
+  if (survey) {
+    std::set<std::pair<int, int>> keys;
+    for (const Multipole& mp : mps) {
+      trvg_params p = ctx.p;
+      p.ell1 = mp.ell1; p.ell2 = mp.ell2; p.ELL = mp.ELL;
+      for (const Term& tm : enumerate_terms(p, survey)) keys.insert({p.ELL, tm.M});
+    }
+    sbar = sbar_table(ctx, d, *r, alpha, keys);
+  }
+
   std::vector<Result> results;
   for (const Multipole& mp : mps) {
     trvg_params p = ctx.p;
@@ -1441,48 +1498,49 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
       std::vector<int2> pairs(dim);
       for (int dv = 0; dv < dim; dv++)
         pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
-      std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields, A->packed,
-                                           B->data.get(), B->nfields, B->packed,
-                                           G.get(), g_real, pairs);
-      for (int dv = 0; dv < dim; dv++) {
-        cplx sv = S[dv] / (double(ss.nmodes[ra[dv]]) * double(ss.nmodes[cb[dv]]));
-        out.raw[dv] += tm.coupling * g.vol_cell * (sv + tm.f_mirror * std::conj(sv));
-      }
+      // Results arrive at the flush after the last term (the sinks add
+      // into `out`, which outlives them).
+      const double coupling = tm.coupling, f_mirror = tm.f_mirror;
+      pair_reduce<T>(ctx, A->data.get(), A->nfields, A->packed, B->data.get(),
+                     B->nfields, B->packed, G.get(), g_real, pairs,
+                     [&out, &ra, &cb, &ss, dim, coupling, f_mirror,
+                      vc = g.vol_cell](std::vector<cplx>&& S) {
+                       for (int dv = 0; dv < dim; dv++) {
+                         cplx sv = S[dv] / (double(ss.nmodes[ra[dv]]) *
+                                            double(ss.nmodes[cb[dv]]));
+                         out.raw[dv] += coupling * vc * (sv + f_mirror * std::conj(sv));
+                       }
+                     });
 
       // Shot noise (threept.cpp:771-966).
-      cplx Sbar;
-      if (survey) {
-        Sbar = sbar_catalogue(ctx, d, p.ELL, tm.M)
-               + std::pow(alpha, 3) * sbar_catalogue(ctx, *r, p.ELL, tm.M);
-      } else {
-        Sbar = double(d.n);
-      }
+      const cplx Sbar = survey ? sbar.at({p.ELL, tm.M}) : cplx(double(d.n), 0.);
       if (p.ell1 == 0 && p.ell2 == 0) {
         cplx sijk = tm.coupling * Sbar;
         for (int dv = 0; dv < dim; dv++) out.shot[dv] += sijk;
       }
-      TwoPt st1;
-      if (p.ell2 == 0) {
-        st1 = twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb,
-                                    modes_in_range);
-        for (int dv = 0; dv < dim; dv++) {
-          int bb = ra[dv];
-          cplx sv = tm.coupling * (st1.pk[bb] - st1.sn[bb]);
-          out.shot[dv] += sv + tm.f_mirror * std::conj(sv);
-        }
-      }
-      if (p.ell1 == 0) {
-        // Both sides monopoles: the same statistics as the first side.
-        TwoPt st = (p.ell2 == 0 && tm.m1 == tm.m2)
-                       ? st1
-                       : twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell2,
-                                               tm.m2, kb, modes_in_range);
-        for (int dv = 0; dv < dim; dv++) {
-          int bb = cb[dv];
-          cplx sv = tm.coupling * (st.pk[bb] - st.sn[bb]);
-          out.shot[dv] += sv + tm.f_mirror * std::conj(sv);
-        }
-      }
+      // Two-point terms of the row side (ell2 = 0) and/or the column side
+      // (ell1 = 0); with both sides monopoles of one m the statistics are
+      // shared.
+      auto twopt_sink = [&out, &ra, &cb, dim, coupling, f_mirror](bool rows, bool cols) {
+        return [&out, &ra, &cb, dim, coupling, f_mirror, rows, cols](TwoPt&& st) {
+          for (int side = 0; side < 2; side++) {
+            if (!(side == 0 ? rows : cols)) continue;
+            const std::vector<int>& bins = side == 0 ? ra : cb;
+            for (int dv = 0; dv < dim; dv++) {
+              int bb = bins[dv];
+              cplx sv = coupling * (st.pk[bb] - st.sn[bb]);
+              out.shot[dv] += sv + f_mirror * std::conj(sv);
+            }
+          }
+        };
+      };
+      const bool shared = p.ell1 == 0 && p.ell2 == 0 && tm.m1 == tm.m2;
+      if (p.ell2 == 0)
+        twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell1, tm.m1, kb,
+                              modes_in_range, twopt_sink(true, shared));
+      if (p.ell1 == 0 && !shared)
+        twopt_fourier<double>(ctx, dn00.get(), NLMp, Sbar, p.ell2, tm.m2, kb,
+                              modes_in_range, twopt_sink(false, true));
       const int key_ell = lm_zero ? -1 : p.ELL, key_m = lm_zero ? 0 : tm.M;
       if (key_ell != t_ell || key_m != t_m || Sbar != t_sbar) {
         LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab,
@@ -1492,13 +1550,16 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
         t_m = key_m;
         t_sbar = Sbar;
       }
-      std::vector<cplx> sij = shot_pairs_bispec<double>(
-          ctx, tfield.get(), p.ell1, tm.m1, p.ell2, tm.m2, sja, sjb, ss.keff, ra, cb);
-      for (int dv = 0; dv < dim; dv++) {
-        cplx sv = tm.coupling * sij[dv];
-        out.shot[dv] += factor_phase * (sv + tm.f_mirror_w3j * std::conj(sv));
-      }
+      shot_pairs_bispec<double>(
+          ctx, tfield.get(), p.ell1, tm.m1, p.ell2, tm.m2, sja, sjb, ss.keff, ra, cb,
+          [&out, dim, coupling, factor_phase, fmw = tm.f_mirror_w3j](std::vector<cplx>&& sij) {
+            for (int dv = 0; dv < dim; dv++) {
+              cplx sv = coupling * sij[dv];
+              out.shot[dv] += factor_phase * (sv + fmw * std::conj(sv));
+            }
+          });
     }
+    ctx.flush();
     for (int dv = 0; dv < dim; dv++) {
       out.raw[dv] *= mp.norm;
       out.shot[dv] *= mp.norm;
@@ -1559,6 +1620,12 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
   SlabCache<T> slabs;
   std::vector<double> reff(nb, 0.);
   bool first = true;
+  std::map<std::pair<int, int>, cplx> sbar;
+  if (survey) {
+    std::set<std::pair<int, int>> keys;
+    for (const Term& tm : enumerate_terms(p, survey)) keys.insert({p.ELL, tm.M});
+    sbar = sbar_table(ctx, d, *r, alpha, keys);
+  }
 
   for (const Term& tm : enumerate_terms(p, survey)) {
     const bool lm_zero = !survey || (p.ELL == 0 && tm.M == 0);
@@ -1568,13 +1635,7 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
       build_field_k<double>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
       dnLMp = dnLM.get();
     }
-    cplx Sbar;
-    if (survey) {
-      Sbar = sbar_catalogue(ctx, d, p.ELL, tm.M)
-             + std::pow(alpha, 3) * sbar_catalogue(ctx, *r, p.ELL, tm.M);
-    } else {
-      Sbar = double(d.n);
-    }
+    const cplx Sbar = survey ? sbar.at({p.ELL, tm.M}) : cplx(double(d.n), 0.);
 
     // Shot noise first: defines r_eff and n_pairs (threept.cpp:1223-1367).
     LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab, dmk(Sbar.real(), Sbar.imag()),
@@ -1619,15 +1680,15 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
       pairs[dv] = make_int2(A->index[ra[dv]], B->index[cb[dv]]);
     // G_LM is real for M = 0 (real weights Y_L0) and in a periodic box.
     const bool g_real = !survey || tm.M == 0;
-    std::vector<cplx> S = pair_reduce<T>(ctx, A->data.get(), A->nfields, A->packed,
-                                         B->data.get(), B->nfields, B->packed, G.get(),
-                                         g_real, pairs);
-    for (int dv = 0; dv < dim; dv++) {
-      cplx s = S[dv];
-      out.raw[dv] += tm.coupling * g.vol_cell * factor_phase
-                     * (s + tm.f_mirror * std::conj(s));
-    }
+    pair_reduce<T>(ctx, A->data.get(), A->nfields, A->packed, B->data.get(), B->nfields,
+                   B->packed, G.get(), g_real, pairs,
+                   [&out, dim, c = tm.coupling * g.vol_cell * factor_phase,
+                    fm = tm.f_mirror](std::vector<cplx>&& S) {
+                     for (int dv = 0; dv < dim; dv++)
+                       out.raw[dv] += c * (S[dv] + fm * std::conj(S[dv]));
+                   });
   }
+  ctx.flush();
   for (int dv = 0; dv < dim; dv++) {
     out.raw[dv] *= norm_factor;
     out.shot[dv] *= norm_factor;
