@@ -1420,7 +1420,10 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
   };
 
   DevArray<c2<T>> G(nm);
-  DevArray<double2> tfield(nm), dnLM, NLM;
+  // t(x) in the working precision (an FP32 transform in the FP32 path; its
+  // radial histogram and the pair sums over it accumulate in FP64).
+  DevArray<c2<T>> tfield(nm);
+  DevArray<double2> dnLM, NLM;
   // Inputs of the shot-noise field held in tfield: (L, M) of its dn field
   // (-1 for dn00) and Sbar; a periodic box uses one t(x) for every term and
   // multipole.
@@ -1543,14 +1546,14 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
                               modes_in_range, twopt_sink(false, true));
       const int key_ell = lm_zero ? -1 : p.ELL, key_m = lm_zero ? 0 : tm.M;
       if (key_ell != t_ell || key_m != t_m || Sbar != t_sbar) {
-        LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab,
+        LoadTwopt<T, double> lt{dnLMp, N00.get(), ctx.tab,
                              dmk(Sbar.real(), Sbar.imag()), 1. / g.vol};
-        fft3d<double>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
+        fft3d<T>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
         t_ell = key_ell;
         t_m = key_m;
         t_sbar = Sbar;
       }
-      shot_pairs_bispec<double>(
+      shot_pairs_bispec<T>(
           ctx, tfield.get(), p.ell1, tm.m1, p.ell2, tm.m2, sja, sjb, ss.keff, ra, cb,
           [&out, dim, coupling, factor_phase, fmw = tm.f_mirror_w3j](std::vector<cplx>&& sij) {
             for (int dv = 0; dv < dim; dv++) {
@@ -1616,7 +1619,8 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
 
   std::vector<int> rows = distinct(ra), cols = distinct(cb);
   DevArray<c2<T>> G(nm);
-  DevArray<double2> tfield(nm), dnLM;
+  DevArray<c2<T>> tfield(nm);
+  DevArray<double2> dnLM;
   SlabCache<T> slabs;
   std::vector<double> reff(nb, 0.);
   bool first = true;
@@ -1638,10 +1642,10 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
     const cplx Sbar = survey ? sbar.at({p.ELL, tm.M}) : cplx(double(d.n), 0.);
 
     // Shot noise first: defines r_eff and n_pairs (threept.cpp:1223-1367).
-    LoadTwopt<double> lt{dnLMp, N00.get(), ctx.tab, dmk(Sbar.real(), Sbar.imag()),
+    LoadTwopt<T, double> lt{dnLMp, N00.get(), ctx.tab, dmk(Sbar.real(), Sbar.imag()),
                          1. / g.vol};
-    fft3d<double>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
-    ShotBins3pcf sb = shot_bins_3pcf<double>(ctx, tfield.get(), p.ell1, tm.m1, p.ell2,
+    fft3d<T>(ctx, tfield.get(), lt, +1, 2. * double(nm) * sizeof(double2));
+    ShotBins3pcf sb = shot_bins_3pcf<T>(ctx, tfield.get(), p.ell1, tm.m1, p.ell2,
                                         tm.m2, rb);
     for (int dv = 0; dv < dim; dv++) {
       if (ra[dv] != cb[dv]) continue;

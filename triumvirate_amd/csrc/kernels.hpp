@@ -345,18 +345,18 @@ fft_x_sjl_multi(c2<T>* __restrict__ base, long long fstride, int nt,
 }
 
 // (f_a f_b^* / C1 - S C1 / C1) / V  (field.cpp:3007-3025, 3271-3289).
-template <typename T>
+template <typename T, typename S = T>
 struct LoadTwopt {
-  const c2<T>* fa;
-  const c2<T>* fb;
+  const c2<S>* fa;
+  const c2<S>* fb;
   AxisTables tab;
   double2 sbar;
   double inv_vol;
   double fb_dc = 0.;   // added to fb at k = 0 (fb = an unsubtracted field's
                        // fluctuation spectrum)
   __device__ c2<T> operator()(long long g, int i, int j, int k) const {
-    c2<T> a = fa[g], b = fb[g];
-    if (g == 0) b.x += T(fb_dc);
+    c2<S> a = fa[g], b = fb[g];
+    if (g == 0) b.x += S(fb_dc);
     double2 pk = dmul(dmk(a.x, a.y), dmk(b.x, -double(b.y)));
     double c1 = alias_at(tab, i, j, k);
     double w = tab.interlace ? window_at(tab, i, j, k) * window_at(tab, i, j, k)
