@@ -16,6 +16,7 @@
 #include <map>
 #include <set>
 #include <tuple>
+#include <type_traits>
 
 #include "batch_kernels.hpp"
 #include "diff_kernels.hpp"
@@ -58,6 +59,20 @@ static size_t fft_smem(bool row_flags) {
   return size_t(NL * (N + 1) + N) * sizeof(c2<T>) + (row_flags ? N : 0);
 }
 
+// The register-resident z pass for FP32 meshes of 256 and 512 with loaders
+// that only read (the masked rows of shell transforms, plain transforms):
+// measured faster there.  FP64 passes and loaders with per-cell arithmetic
+// keep the LDS-staged pass, whose lower register use keeps more waves
+// resident.
+template <class Loader> struct ZRegLoader : std::false_type {};
+template <> struct ZRegLoader<LoadMaskedZBatch<float>> : std::true_type {};
+template <> struct ZRegLoader<LoadSame<float>> : std::true_type {};
+
+template <typename T, int LOGN, class Loader>
+constexpr bool use_zreg() {
+  return ZRegShape<LOGN>::ok && sizeof(T) == 4 && ZRegLoader<Loader>::value;
+}
+
 template <typename T, int LOGN, class Loader>
 static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign,
                      double read_bytes, double write_bytes, const Prune& pr) {
@@ -68,7 +83,12 @@ static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign,
   const size_t smem = fft_smem<T, LOGN>(false);
   const c2<T>* tw = twiddles<T>(N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, read_bytes + write_bytes);
-  if (ctx.g.n[1] % (E / N) == 0)
+  if constexpr (use_zreg<T, LOGN, Loader>()) {
+    static_assert(E == ZREG_BLOCK, "z pass block size");
+    hipLaunchKernelGGL((fft_pass_z_reg<T, LOGN, Loader>), dim3(unsigned(nblocks)),
+                       dim3(zreg_threads<LOGN>()), (zreg_smem<T, LOGN>()), ctx.stream, data,
+                       load, tw, ctx.g.n[1], sign, pr);
+  } else if (ctx.g.n[1] % (E / N) == 0)
     hipLaunchKernelGGL((fft_pass_z<T, LOGN, EPT, true, Loader>), dim3(unsigned(nblocks)),
                        dim3(FFT_THREADS), smem, ctx.stream, data, load, tw,
                        ctx.g.n[1], sign, pr);
@@ -199,7 +219,12 @@ static void launch_z_batch(Context& ctx, c2<T>* base, const BatchLoader& load,
   const size_t smem = fft_smem<T, LOGN>(false);
   const c2<T>* tw = twiddles<T>(N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, bytes);
-  if (ctx.g.n[1] % (E / N) == 0)
+  if constexpr (use_zreg<T, LOGN, BatchLoader>()) {
+    static_assert(E == ZREG_BLOCK, "z pass block size");
+    hipLaunchKernelGGL((fft_pass_z_batch_reg<T, LOGN, BatchLoader>), dim3(unsigned(nwork)),
+                       dim3(zreg_threads<LOGN>()), (zreg_smem<T, LOGN>()), ctx.stream, base,
+                       ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);
+  } else if (ctx.g.n[1] % (E / N) == 0)
     hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, true, BatchLoader>),
                        dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream,
                        base, ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);

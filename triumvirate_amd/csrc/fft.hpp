@@ -157,6 +157,180 @@ __device__ __forceinline__ void lds_fft_lines(c2<T>* buf, const c2<T>* W,
   (void)NLINES;
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident DFTs for the z pass.
+// ---------------------------------------------------------------------------
+
+// cos(pi k / 16) for k in [0, 16] (k > 8 by symmetry).
+__host__ __device__ constexpr double cos_pi16(int k) {
+  return k > 8   ? -cos_pi16(16 - k)
+         : k == 0 ? 1.0
+         : k == 1 ? 0.9807852804032304
+         : k == 2 ? 0.9238795325112867
+         : k == 3 ? 0.8314696123025452
+         : k == 4 ? 0.7071067811865476
+         : k == 5 ? 0.5555702330196023
+         : k == 6 ? 0.38268343236508984
+         : k == 7 ? 0.19509032201612833
+                  : 0.0;
+}
+
+// d * e^{sgn i pi k / 16} for k in [0, 16); k is a constant after
+// unrolling, so the trivial factors (1, +-i) cost no multiplications.
+template <typename T>
+__device__ __forceinline__ c2<T> rot_pi16(c2<T> d, int k, T sgn) {
+  if (k == 0) return d;
+  if (k == 8) return mk<T>(-sgn * d.y, sgn * d.x);
+  const T c = T(cos_pi16(k));
+  const T sn = sgn * T(cos_pi16(8 - k > 0 ? 8 - k : k - 8));
+  return mk<T>(d.x * c - d.y * sn, d.x * sn + d.y * c);
+}
+
+// One decimation-in-frequency radix-2 stage of length LEN over v[0, R),
+// then the stages of length LEN / 2 ... 2.
+template <typename T, int R, int LEN>
+struct DifStages {
+  static __device__ __forceinline__ void run(c2<T>* v, T sgn) {
+    constexpr int H = LEN / 2;
+#pragma unroll
+    for (int s = 0; s < R; s += LEN) {
+#pragma unroll
+      for (int j = 0; j < H; j++) {
+        const c2<T> a = v[s + j], b = v[s + j + H];
+        v[s + j] = cadd<T>(a, b);
+        v[s + j + H] = rot_pi16<T>(csub<T>(a, b), (32 / LEN) * j, sgn);
+      }
+    }
+    DifStages<T, R, H>::run(v, sgn);
+  }
+};
+template <typename T, int R>
+struct DifStages<T, R, 1> {
+  static __device__ __forceinline__ void run(c2<T>*, T) {}
+};
+
+__host__ __device__ constexpr int bitrev(int k, int bits) {
+  return bits == 0 ? 0 : ((k & 1) << (bits - 1)) | bitrev(k >> 1, bits - 1);
+}
+__host__ __device__ constexpr int ilog2c(int r) { return r <= 1 ? 0 : 1 + ilog2c(r / 2); }
+
+// v <- DFT_R(v): X[k] = sum_n v[n] e^{sgn 2 pi i n k / R}, natural order in
+// and out (R a power of two <= 32, held in registers).
+template <typename T, int R>
+__device__ __forceinline__ void dft_reg(c2<T>* v, T sgn) {
+  static_assert(R >= 1 && R <= 32 && (R & (R - 1)) == 0, "register DFT size");
+  DifStages<T, R, R>::run(v, sgn);
+  c2<T> o[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) o[k] = v[bitrev(k, ilog2c(R))];
+#pragma unroll
+  for (int k = 0; k < R; k++) v[k] = o[k];
+}
+
+// Shape of the register z pass for N = 2^LOGN = A * B: A threads per line,
+// B elements per thread (B a multiple of A), 2048 elements per workgroup.
+template <int LOGN> struct ZRegShape { static constexpr bool ok = false; };
+template <> struct ZRegShape<8> {
+  static constexpr bool ok = true;
+  static constexpr int A = 16, B = 16;
+};
+template <> struct ZRegShape<9> {
+  static constexpr bool ok = true;
+  static constexpr int A = 16, B = 32;
+};
+constexpr int ZREG_BLOCK = 2048;
+template <int LOGN>
+constexpr int zreg_threads() { return ZREG_BLOCK / ZRegShape<LOGN>::B; }
+template <typename T, int LOGN>
+constexpr size_t zreg_smem() {
+  return size_t((ZREG_BLOCK >> LOGN) * ZRegShape<LOGN>::B * (ZRegShape<LOGN>::A + 1) +
+                (1 << LOGN)) * sizeof(c2<T>);
+}
+
+// z pass of block `blk` (2048 elements: L = 2048 / N whole lines) as
+// N = A * B: with n = n1 + A n2 and k = k2 + B k1,
+//   X[k2 + B k1] = sum_n1 w_A^{n1 k1} w_N^{n1 k2} sum_n2 x[n1 + A n2] w_B^{n2 k2}.
+// Thread (line, n1) loads its B elements (lanes of a line read A adjacent
+// elements per load), takes the inner DFT in registers, applies w_N^{n1 k2}
+// and writes to LDS; after one exchange each thread takes B / A outer DFTs
+// of length A in registers and stores A adjacent outputs per lane group.
+// One LDS round trip per element instead of one per radix-4 stage.
+template <typename T, int LOGN, class Loader>
+__device__ __forceinline__ void fft_z_block_reg(c2<T>* __restrict__ data,
+                                                const Loader& load,
+                                                const c2<T>* __restrict__ tw, int n1,
+                                                int sign, const Prune& pr, long long blk) {
+  constexpr int N = 1 << LOGN;
+  constexpr int A = ZRegShape<LOGN>::A, B = ZRegShape<LOGN>::B;
+  constexpr int L = ZREG_BLOCK / N;
+  constexpr int NT = L * A;
+  constexpr int AP = A + 1;        // padded stride between k2 rows in LDS
+  constexpr int LSTR = B * AP;     // LDS elements per line
+  extern __shared__ char smem_raw[];
+  c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
+  c2<T>* W = buf + L * LSTR;
+  const int t = threadIdx.x;
+  const int l = t / A, q = t % A;
+  const long long line0 = blk * L;
+  if (pr.on) {
+    // Skip the workgroup if none of its lines is needed (uniform exit).
+    bool any = false;
+    for (int ll = 0; ll < L && !any; ll++) {
+      const long long ln = line0 + ll;
+      any = pr.p1(int(ln / n1), int(ln % n1));
+    }
+    if (!any) return;
+  }
+  for (int m = t; m < N; m += NT) W[m] = tw[m];
+  const long long line = line0 + l;
+  const int i = int(line / n1), j = int(line % n1);
+  const long long g0 = line * N;
+  const T sgn = T(sign);
+  c2<T> v[B];
+#pragma unroll
+  for (int n2 = 0; n2 < B; n2++) {
+    const int k = q + A * n2;
+    v[n2] = load(g0 + k, i, j, k);
+  }
+  dft_reg<T, B>(v, sgn);
+  __syncthreads();
+  c2<T>* lb = buf + l * LSTR;
+  lb[q] = v[0];
+#pragma unroll
+  for (int k2 = 1; k2 < B; k2++) lb[k2 * AP + q] = cmul<T>(v[k2], W[q * k2]);
+  __syncthreads();
+  c2<T>* out = data + g0;
+#pragma unroll
+  for (int r = 0; r < B / A; r++) {
+    const int k2 = q + A * r;
+    c2<T> u[A];
+#pragma unroll
+    for (int m = 0; m < A; m++) u[m] = lb[k2 * AP + m];
+    dft_reg<T, A>(u, sgn);
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) out[k2 + B * k1] = u[k1];
+  }
+}
+
+template <typename T, int LOGN, class Loader>
+__global__ void __launch_bounds__(zreg_threads<LOGN>())
+fft_pass_z_reg(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
+               int n1, int sign, Prune pr) {
+  fft_z_block_reg<T, LOGN>(data, load, tw, n1, sign, pr, xcd_block(blockIdx.x, gridDim.x));
+}
+
+template <typename T, int LOGN, class BatchLoader>
+__global__ void __launch_bounds__(zreg_threads<LOGN>())
+fft_pass_z_batch_reg(c2<T>* __restrict__ base, long long fstride, BatchLoader load,
+                     const c2<T>* __restrict__ tw, int n1, int sign, Prune pr,
+                     const double* __restrict__ kcut2, const int2* __restrict__ work) {
+  const int2 wk = work[xcd_block(blockIdx.x, gridDim.x)];
+  Prune p = pr;
+  p.kcut2 = kcut2[wk.x];
+  fft_z_block_reg<T, LOGN>(base + wk.x * fstride, load.field(wk.x), tw, n1, sign, p,
+                           wk.y);
+}
+
 // First pass along z (contiguous lines), with a fused loader, for block
 // `blk` of the mesh at `data`.
 // Loader: c2<T> operator()(long long g, int i, int j, int k) const.
