@@ -84,9 +84,9 @@ static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign,
   const c2<T>* tw = twiddles<T>(N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, read_bytes + write_bytes);
   if constexpr (use_zreg<T, LOGN, Loader>()) {
-    static_assert(E == ZREG_BLOCK, "z pass block size");
+    static_assert(E == zreg_block<T>(), "z pass block size");
     hipLaunchKernelGGL((fft_pass_z_reg<T, LOGN, Loader>), dim3(unsigned(nblocks)),
-                       dim3(zreg_threads<LOGN>()), (zreg_smem<T, LOGN>()), ctx.stream, data,
+                       dim3(zreg_threads<T, LOGN>()), (zreg_smem<T, LOGN>()), ctx.stream, data,
                        load, tw, ctx.g.n[1], sign, pr);
   } else if (ctx.g.n[1] % (E / N) == 0)
     hipLaunchKernelGGL((fft_pass_z<T, LOGN, EPT, true, Loader>), dim3(unsigned(nblocks)),
@@ -220,9 +220,9 @@ static void launch_z_batch(Context& ctx, c2<T>* base, const BatchLoader& load,
   const c2<T>* tw = twiddles<T>(N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, bytes);
   if constexpr (use_zreg<T, LOGN, BatchLoader>()) {
-    static_assert(E == ZREG_BLOCK, "z pass block size");
+    static_assert(E == zreg_block<T>(), "z pass block size");
     hipLaunchKernelGGL((fft_pass_z_batch_reg<T, LOGN, BatchLoader>), dim3(unsigned(nwork)),
-                       dim3(zreg_threads<LOGN>()), (zreg_smem<T, LOGN>()), ctx.stream, base,
+                       dim3(zreg_threads<T, LOGN>()), (zreg_smem<T, LOGN>()), ctx.stream, base,
                        ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);
   } else if (ctx.g.n[1] % (E / N) == 0)
     hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, true, BatchLoader>),

@@ -228,7 +228,8 @@ __device__ __forceinline__ void dft_reg(c2<T>* v, T sgn) {
 }
 
 // Shape of the register z pass for N = 2^LOGN = A * B: A threads per line,
-// B elements per thread (B a multiple of A), 2048 elements per workgroup.
+// B elements per thread (B a multiple of A); a workgroup takes the block of
+// every pass, FFT_THREADS * FftEpt<T> elements.
 template <int LOGN> struct ZRegShape { static constexpr bool ok = false; };
 template <> struct ZRegShape<8> {
   static constexpr bool ok = true;
@@ -238,16 +239,18 @@ template <> struct ZRegShape<9> {
   static constexpr bool ok = true;
   static constexpr int A = 16, B = 32;
 };
-constexpr int ZREG_BLOCK = 2048;
-template <int LOGN>
-constexpr int zreg_threads() { return ZREG_BLOCK / ZRegShape<LOGN>::B; }
+template <typename T>
+constexpr int zreg_block() { return FFT_THREADS * FftEpt<T>::v; }
+template <typename T, int LOGN>
+constexpr int zreg_threads() { return zreg_block<T>() / ZRegShape<LOGN>::B; }
 template <typename T, int LOGN>
 constexpr size_t zreg_smem() {
-  return size_t((ZREG_BLOCK >> LOGN) * ZRegShape<LOGN>::B * (ZRegShape<LOGN>::A + 1) +
+  return size_t((zreg_block<T>() >> LOGN) * ZRegShape<LOGN>::B *
+                    (ZRegShape<LOGN>::A + 1) +
                 (1 << LOGN)) * sizeof(c2<T>);
 }
 
-// z pass of block `blk` (2048 elements: L = 2048 / N whole lines) as
+// z pass of block `blk` (L = zreg_block / N whole lines) as
 // N = A * B: with n = n1 + A n2 and k = k2 + B k1,
 //   X[k2 + B k1] = sum_n1 w_A^{n1 k1} w_N^{n1 k2} sum_n2 x[n1 + A n2] w_B^{n2 k2}.
 // Thread (line, n1) loads its B elements (lanes of a line read A adjacent
@@ -262,7 +265,7 @@ __device__ __forceinline__ void fft_z_block_reg(c2<T>* __restrict__ data,
                                                 int sign, const Prune& pr, long long blk) {
   constexpr int N = 1 << LOGN;
   constexpr int A = ZRegShape<LOGN>::A, B = ZRegShape<LOGN>::B;
-  constexpr int L = ZREG_BLOCK / N;
+  constexpr int L = zreg_block<T>() / N;
   constexpr int NT = L * A;
   constexpr int AP = A + 1;        // padded stride between k2 rows in LDS
   constexpr int LSTR = B * AP;     // LDS elements per line
@@ -313,14 +316,14 @@ __device__ __forceinline__ void fft_z_block_reg(c2<T>* __restrict__ data,
 }
 
 template <typename T, int LOGN, class Loader>
-__global__ void __launch_bounds__(zreg_threads<LOGN>())
+__global__ void __launch_bounds__((zreg_threads<T, LOGN>()))
 fft_pass_z_reg(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ tw,
                int n1, int sign, Prune pr) {
   fft_z_block_reg<T, LOGN>(data, load, tw, n1, sign, pr, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <typename T, int LOGN, class BatchLoader>
-__global__ void __launch_bounds__(zreg_threads<LOGN>())
+__global__ void __launch_bounds__((zreg_threads<T, LOGN>()))
 fft_pass_z_batch_reg(c2<T>* __restrict__ base, long long fstride, BatchLoader load,
                      const c2<T>* __restrict__ tw, int n1, int sign, Prune pr,
                      const double* __restrict__ kcut2, const int2* __restrict__ work) {
