@@ -237,28 +237,46 @@ __host__ __device__ inline double sph_bessel_dev(int l, double x) {
 struct SplineDev {
   const double* y;
   const double* c;
+  // Per interval i: (y_i, b_i, c_i, d_i) of the cubic on [x_i, x_i+1)
+  // (b, d as GSL's cspline evaluation forms them), and an FP32 copy.
+  const double4* cf;
+  const float4* cf32;
   int n;          // number of nodes
   int ell;
   double step;    // 0.05
+  double inv_step;
   double split;   // max(1000, l^2)
 
-  __device__ double eval(double x) const {
-    if (x >= split) return sph_bessel_dev(ell, x);
-    // Interval i with x_i <= x < x_{i+1}, x_i = 0 + step * i.
-    int i = int(x / step);
+  // Interval i with x_i <= x < x_{i+1}, x_i = 0 + step * i (exact: the
+  // estimate from x / step is corrected against the node abscissae).
+  __device__ __forceinline__ int interval(double x) const {
+    int i = int(x * inv_step);
     if (i > n - 2) i = n - 2;
     if (i < 0) i = 0;
     if (x < __dmul_rn(step, double(i)) && i > 0) i--;
     else if (i + 1 < n - 1 && x >= __dmul_rn(step, double(i + 1))) i++;
-    double xlo = __dmul_rn(step, double(i));
-    double xhi = __dmul_rn(step, double(i + 1));
-    double dx = xhi - xlo;
-    double dy = y[i + 1] - y[i];
-    double delx = x - xlo;
-    double ci = c[i], cip1 = c[i + 1];
-    double b = dy / dx - dx * (cip1 + 2. * ci) / 3.;
-    double d = (cip1 - ci) / (3. * dx);
-    return y[i] + delx * (b + delx * (ci + delx * d));
+    return i;
+  }
+
+  __device__ double eval(double x) const {
+    if (x >= split) return sph_bessel_dev(ell, x);
+    const int i = interval(x);
+    const double delx = x - __dmul_rn(step, double(i));
+    const double4 q = cf[i];
+    return q.x + delx * (q.y + delx * (q.z + delx * q.w));
+  }
+
+  // FP32 evaluation (FP32 fields): the interval from the FP64 argument, the
+  // cubic in FP32.  At an interval edge the neighbouring cubic is used,
+  // which the spline's continuity makes equivalent to FP32 rounding.
+  __device__ float eval32(double x) const {
+    if (x >= split) return float(sph_bessel_dev(ell, x));
+    int i = int(x * inv_step);
+    if (i > n - 2) i = n - 2;
+    if (i < 0) i = 0;
+    const float dl = float(x - __dmul_rn(step, double(i)));
+    const float4 q = cf32[i];
+    return q.x + dl * (q.y + dl * (q.z + dl * q.w));
   }
 };
 

@@ -119,9 +119,12 @@ struct Spline {
   double step = 0.05;
   int n = 0;
   DevArray<double> y, c;
+  DevArray<double4> cf;
+  DevArray<float4> cf32;
   SplineDev dev() const {
     SplineDev s;
-    s.y = y.get(); s.c = c.get(); s.n = n; s.ell = ell; s.step = step;
+    s.y = y.get(); s.c = c.get(); s.cf = cf.get(); s.cf32 = cf32.get();
+    s.n = n; s.ell = ell; s.step = step; s.inv_step = 1. / step;
     s.split = split;
     return s;
   }

@@ -167,8 +167,28 @@ void build_spline(int ell, Spline& sp, hipStream_t s) {
   sp.n = int(y.size());
   sp.y.reset(sp.n);
   sp.c.reset(sp.n);
+  // Interval coefficients as GSL's cspline evaluation forms them:
+  // b = dy/dx - dx (c_i+1 + 2 c_i) / 3, d = (c_i+1 - c_i) / (3 dx).
+  std::vector<double4> cf(sp.n - 1);
+  std::vector<float4> cf32(sp.n - 1);
+  for (int i = 0; i + 1 < sp.n; i++) {
+    const double xlo = sp.step * double(i), xhi = sp.step * double(i + 1);
+    const double dx = xhi - xlo;
+    const double dy = y[i + 1] - y[i];
+    const double t = dx * (c[i + 1] + 2. * c[i]);
+    const double b = dy / dx - t / 3.;
+    const double d = (c[i + 1] - c[i]) / (3. * dx);
+    cf[i] = make_double4(y[i], b, c[i], d);
+    cf32[i] = make_float4(float(y[i]), float(b), float(c[i]), float(d));
+  }
+  sp.cf.reset(sp.n - 1);
+  sp.cf32.reset(sp.n - 1);
   TRVG_HIP(hipMemcpyAsync(sp.y.get(), y.data(), sp.n * 8, hipMemcpyHostToDevice, s));
   TRVG_HIP(hipMemcpyAsync(sp.c.get(), c.data(), sp.n * 8, hipMemcpyHostToDevice, s));
+  TRVG_HIP(hipMemcpyAsync(sp.cf.get(), cf.data(), (sp.n - 1) * sizeof(double4),
+                          hipMemcpyHostToDevice, s));
+  TRVG_HIP(hipMemcpyAsync(sp.cf32.get(), cf32.data(), (sp.n - 1) * sizeof(float4),
+                          hipMemcpyHostToDevice, s));
   TRVG_HIP(hipStreamSynchronize(s));
 }
 

@@ -298,8 +298,11 @@ fft_x_sjl_multi(c2<T>* __restrict__ base, long long fstride, int nt,
   const long long S = (long long)n1 * n2;
   const long long id0 = xcd_block(blockIdx.x, gridDim.x) * NLINES;
   for (int q = t; q < N; q += FFT_THREADS) W[q] = tw[q];
-  double2 fac[EPT];
+  // Per element: y_lm src / W / V in the working precision (FP32 fields
+  // take j_l and the product in FP32) and |k|.
+  c2<T> fac[EPT];
   double km[EPT];
+  const double inv_vol = 1. / geo.vol;
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
     const int idx = t + e * FFT_THREADS;
@@ -311,8 +314,9 @@ fft_x_sjl_multi(c2<T>* __restrict__ base, long long fstride, int nt,
     const double kz = grid_coord(k, n2, geo.dk[2]);
     km[e] = mag3_exact(kx, ky, kz);
     const double2 v = src[id + (long long)n * S];
-    const double iw = 1. / window_at(tab, n, j, k);
-    fac[e] = dmul(ylm_reduced_dev(ell, m, kx, ky, kz), dmk(v.x * iw, v.y * iw));
+    const double iw = inv_vol / window_at(tab, n, j, k);
+    const double2 fe = dmul(ylm_reduced_dev(ell, m, kx, ky, kz), dmk(v.x * iw, v.y * iw));
+    fac[e] = mk<T>(T(fe.x), T(fe.y));
   }
   for (int f = 0; f < nt; f++) {
     const double r0 = PACKED ? r[2 * f] : r[f];
@@ -320,14 +324,19 @@ fft_x_sjl_multi(c2<T>* __restrict__ base, long long fstride, int nt,
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
       const int idx = t + e * FFT_THREADS;
-      const double j0 = sj.eval(km[e] * r0);
+      T j0, j1 = T(0);
+      if (sizeof(T) == 4) {
+        j0 = T(sj.eval32(km[e] * r0));
+        if (PACKED && r1 >= 0.) j1 = T(sj.eval32(km[e] * r1));
+      } else {
+        j0 = T(sj.eval(km[e] * r0));
+        if (PACKED && r1 >= 0.) j1 = T(sj.eval(km[e] * r1));
+      }
       c2<T> val;
       if (PACKED) {
-        const double j1 = r1 < 0. ? 0. : sj.eval(km[e] * r1);
-        val = mk<T>(T((j0 * fac[e].x - j1 * fac[e].y) / geo.vol),
-                    T((j0 * fac[e].y + j1 * fac[e].x) / geo.vol));
+        val = mk<T>(j0 * fac[e].x - j1 * fac[e].y, j0 * fac[e].y + j1 * fac[e].x);
       } else {
-        val = mk<T>(T(j0 * fac[e].x / geo.vol), T(j0 * fac[e].y / geo.vol));
+        val = mk<T>(j0 * fac[e].x, j0 * fac[e].y);
       }
       buf[(idx % NLINES) * LS + idx / NLINES] = val;
     }
