@@ -476,13 +476,38 @@ static void xfirst_yz(Context& ctx, c2<T>* base, const double* kcut2, const int2
 // launch.  Unpacked: field q at base + q * nmesh.  Packed (real shells:
 // real source, even l, m = 0): bins q = 2p, 2p + 1 are the real and
 // imaginary parts of base + p * nmesh.
+// Launch plan of a shell slab transform (bands, pruning radii, work lists
+// and profiler byte counts), a function of the geometry and the bins only:
+// built once per device and reused by every later call.
+struct ShellPlan {
+  DevArray<ShellBand> bands;
+  DevArray<double> kc;
+  DevArray<int2> w;
+  int nx = 0, ny = 0, nz = 0;
+  double bx = 0., by = 0., bz = 0.;
+};
+
 template <typename T>
-void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
-                     const trvg_binning& kb, const std::vector<int>& bins,
-                     const std::vector<int>& nmodes, int ell, int m, bool packed) {
+static std::shared_ptr<const ShellPlan> shell_plan(Context& ctx, const trvg_binning& kb,
+                                                   const std::vector<int>& bins,
+                                                   const std::vector<int>& nmodes,
+                                                   bool packed) {
   const Geom& g = ctx.g;
   const int nf = int(bins.size());
-  if (nf == 0) return;
+  std::vector<double> key = {double(ctx.dev), double(sizeof(T)), double(packed)};
+  for (int a = 0; a < 3; a++) { key.push_back(g.n[a]); key.push_back(g.dk[a]); }
+  for (int b : bins) {
+    key.push_back(kb.bin_edges[b]);
+    key.push_back(kb.bin_edges[b + 1]);
+    key.push_back(nmodes[b]);
+  }
+  static std::mutex mu;
+  static std::map<std::vector<double>, std::shared_ptr<const ShellPlan>> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
   const int nt = packed ? (nf + 1) / 2 : nf;
   const int per = packed ? 2 : 1;
   constexpr int EPT = FftEpt<T>::v;
@@ -494,44 +519,66 @@ void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
   const long long nblk = g.nmesh / E;
   std::vector<int2> wx, wy, wz;
   const double cb = sizeof(c2<T>);
-  double bx = 0., by = 0., bz = 0.;
+  auto pl = std::make_shared<ShellPlan>();
   for (int q = 0; q < nf; q++) {
     const int b = bins[q];
     bands[q] = shell_band(kb.bin_edges[b], kb.bin_edges[b + 1]);
     Prune pq = make_prune(g, kb.bin_edges[b + 1]);
     kc[q / per] = std::max(kc[q / per], pq.on ? pq.kcut2 : HUGE_VAL);
-    bx += double(nmodes[b]) * sizeof(double2);
+    pl->bx += double(nmodes[b]) * sizeof(double2);
   }
   for (int f = 0; f < nt; f++) {
-    const PruneListsXF& pl = prune_lists_xfirst(g, kc[f], nl_x, nl_y);
-    for (int w : pl.xblocks) wx.push_back(make_int2(f, w));
-    for (int w : pl.yblocks) wy.push_back(make_int2(f, w));
+    const PruneListsXF& px = prune_lists_xfirst(g, kc[f], nl_x, nl_y);
+    for (int w : px.xblocks) wx.push_back(make_int2(f, w));
+    for (int w : px.yblocks) wy.push_back(make_int2(f, w));
     for (long long w = 0; w < nblk; w++) wz.push_back(make_int2(f, int(w)));
-    const double xw = pl.x_lines * g.n[0] * cb;
-    const double yw = pl.y_lines * g.n[1] * cb;
-    bx += xw;
-    by += xw + yw;
-    bz += yw + double(g.nmesh) * cb;
+    const double xw = px.x_lines * g.n[0] * cb;
+    const double yw = px.y_lines * g.n[1] * cb;
+    pl->bx += xw;
+    pl->by += xw + yw;
+    pl->bz += yw + double(g.nmesh) * cb;
   }
-  DevArray<ShellBand> dbands(bands.size());
-  DevArray<double> dkc(nt);
-  DevArray<int2> dw(wx.size() + wy.size() + wz.size());
-  ctx.upload(dbands.get(), bands.data(), bands.size());
-  ctx.upload(dkc.get(), kc.data(), nt);
-  ctx.upload(dw.get(), wx.data(), wx.size());
-  ctx.upload(dw.get() + wx.size(), wy.data(), wy.size());
-  ctx.upload(dw.get() + wx.size() + wy.size(), wz.data(), wz.size());
-  const int2* px = dw.get();
-  const int2* py = px + wx.size();
-  const int2* pz = py + wy.size();
+  pl->nx = int(wx.size());
+  pl->ny = int(wy.size());
+  pl->nz = int(wz.size());
+  pl->bands.reset(bands.size());
+  pl->kc.reset(nt);
+  pl->w.reset(wx.size() + wy.size() + wz.size());
+  wx.insert(wx.end(), wy.begin(), wy.end());
+  wx.insert(wx.end(), wz.begin(), wz.end());
+  TRVG_HIP(hipMemcpyAsync(pl->bands.get(), bands.data(), bands.size() * sizeof(ShellBand),
+                          hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(pl->kc.get(), kc.data(), nt * sizeof(double),
+                          hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(pl->w.get(), wx.data(), wx.size() * sizeof(int2),
+                          hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipStreamSynchronize(ctx.stream));
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache.size() >= 64) cache.clear();   // bounded; plans in use stay alive
+  cache[key] = pl;
+  return pl;
+}
+
+template <typename T>
+void shell_slab_ifft(Context& ctx, c2<T>* base, const double2* src,
+                     const trvg_binning& kb, const std::vector<int>& bins,
+                     const std::vector<int>& nmodes, int ell, int m, bool packed) {
+  const Geom& g = ctx.g;
+  if (bins.empty()) return;
+  const std::shared_ptr<const ShellPlan> pl = shell_plan<T>(ctx, kb, bins, nmodes, packed);
+  // The plan outlives the queued launches even if the cache drops it.
+  ctx.keep.push_back(std::const_pointer_cast<ShellPlan>(pl));
+  const int2* px = pl->w.get();
+  const int2* py = px + pl->nx;
+  const int2* pz = py + pl->ny;
   if (packed) {
-    LoadShellPairBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
-    xfirst_passes<T>(ctx, base, ld, dkc.get(), px, int(wx.size()), py, int(wy.size()), pz,
-                     int(wz.size()), bx, by, bz);
+    LoadShellPairBatch<T, double> ld{src, ctx.tab, g, pl->bands.get(), ell, m};
+    xfirst_passes<T>(ctx, base, ld, pl->kc.get(), px, pl->nx, py, pl->ny, pz, pl->nz,
+                     pl->bx, pl->by, pl->bz);
   } else {
-    LoadShellBatch<T, double> ld{src, ctx.tab, g, dbands.get(), ell, m};
-    xfirst_passes<T>(ctx, base, ld, dkc.get(), px, int(wx.size()), py, int(wy.size()), pz,
-                     int(wz.size()), bx, by, bz);
+    LoadShellBatch<T, double> ld{src, ctx.tab, g, pl->bands.get(), ell, m};
+    xfirst_passes<T>(ctx, base, ld, pl->kc.get(), px, pl->nx, py, pl->ny, pz, pl->nz,
+                     pl->bx, pl->by, pl->bz);
   }
 }
 
@@ -572,29 +619,57 @@ void sjl_slab_ifft(Context& ctx, c2<T>* base, const double2* src, const Spline& 
   const int per = packed ? 2 : 1;
   constexpr int EPT = FftEpt<T>::v;
   const int E = FFT_THREADS * EPT;
-  const PruneListsXF& pl = prune_lists_xfirst(g, HUGE_VAL, E / g.n[0], E / g.n[1]);
-  const long long nblk = g.nmesh / E;
-  std::vector<double> rr((size_t)nt * per, -1.), kc(nt, HUGE_VAL);
+  std::vector<double> rr((size_t)nt * per, -1.);
   for (int q = 0; q < nf; q++) rr[q] = r[bins[q]];
-  std::vector<int2> w;   // y blocks, then z blocks
-  for (int f = 0; f < nt; f++)
-    for (int b : pl.yblocks) w.push_back(make_int2(f, b));
-  const size_t ny = w.size();
-  for (int f = 0; f < nt; f++)
-    for (long long b = 0; b < nblk; b++) w.push_back(make_int2(f, int(b)));
-  const size_t nz = w.size() - ny;
-  DevArray<double> dr(rr.size()), dkc(nt);
-  DevArray<int2> dw(w.size());
+  DevArray<double> dr(rr.size());
   ctx.upload(dr.get(), rr.data(), rr.size());
-  ctx.upload(dkc.get(), kc.data(), nt);
-  ctx.upload(dw.get(), w.data(), w.size());
+  // Work lists (y blocks, then every z block, per transform) and the
+  // unpruned radii depend on the geometry and the transform count only:
+  // built once per device.
+  std::vector<double> key = {double(ctx.dev), double(sizeof(T)), double(nt)};
+  for (int a = 0; a < 3; a++) key.push_back(g.n[a]);
+  static std::mutex mu;
+  static std::map<std::vector<double>, std::shared_ptr<ShellPlan>> cache;
+  std::shared_ptr<ShellPlan> plan;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) plan = it->second;
+  }
+  if (!plan) {
+    const PruneListsXF& pl = prune_lists_xfirst(g, HUGE_VAL, E / g.n[0], E / g.n[1]);
+    const long long nblk = g.nmesh / E;
+    std::vector<double> kc(nt, HUGE_VAL);
+    std::vector<int2> w;   // y blocks, then z blocks
+    for (int f = 0; f < nt; f++)
+      for (int b : pl.yblocks) w.push_back(make_int2(f, b));
+    const size_t ny = w.size();
+    for (int f = 0; f < nt; f++)
+      for (long long b = 0; b < nblk; b++) w.push_back(make_int2(f, int(b)));
+    plan = std::make_shared<ShellPlan>();
+    plan->ny = int(ny);
+    plan->nz = int(w.size() - ny);
+    plan->kc.reset(nt);
+    plan->w.reset(w.size());
+    TRVG_HIP(hipMemcpyAsync(plan->kc.get(), kc.data(), nt * sizeof(double),
+                            hipMemcpyHostToDevice, ctx.stream));
+    TRVG_HIP(hipMemcpyAsync(plan->w.get(), w.data(), w.size() * sizeof(int2),
+                            hipMemcpyHostToDevice, ctx.stream));
+    TRVG_HIP(hipStreamSynchronize(ctx.stream));
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() >= 16) cache.clear();   // bounded; plans in use stay alive
+    cache[key] = plan;
+  }
+  ctx.keep.push_back(plan);
+  const size_t ny = size_t(plan->ny), nz = size_t(plan->nz);
+  const int2* dwp = plan->w.get();
   // Algorithmic bytes: the source read once, then each pass writes (and the
   // later ones read) every field.
   const double fb = double(g.nmesh) * sizeof(c2<T>) * nt;
   const double bx = double(g.nmesh) * sizeof(double2) + fb, by = 2. * fb, bz = 2. * fb;
   TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_sjl<T, LG>(ctx, base, nt, src, sp, dr.get(), ell,
                                                         m, packed, bx)));
-  xfirst_yz<T>(ctx, base, dkc.get(), dw.get(), int(ny), dw.get() + ny, int(nz), by, bz);
+  xfirst_yz<T>(ctx, base, plan->kc.get(), dwp, int(ny), dwp + ny, int(nz), by, bz);
 }
 
 template <typename T, class Loader>
