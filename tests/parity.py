@@ -129,7 +129,11 @@ def survey_pair(nd, nr, L, seed, weights=False):
     nbar = nd / (4. / 3. * np.pi * (0.45 * L) ** 3)
     kw_d = {}
     kw_r = {}
-    if weights:
+    if weights == 'uniform':
+        # One weight per catalogue (not 1): the engine's uniform-weight path.
+        kw_d = dict(ws=np.full(nd, 1.5), wc=np.full(nd, 0.8))
+        kw_r = dict(ws=np.full(nr, 0.7), wc=np.full(nr, 1.1))
+    elif weights:
         kw_d = dict(ws=g.uniform(0.5, 1.5, nd), wc=g.uniform(0.8, 1.2, nd))
         kw_r = dict(ws=g.uniform(0.5, 1.5, nr), wc=g.uniform(0.8, 1.2, nr))
     d = ParticleCatalogue(pd[:, 0], pd[:, 1], pd[:, 2], nz=nbar, **kw_d)

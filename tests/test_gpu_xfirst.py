@@ -48,7 +48,7 @@ CASES = [
 ]
 
 
-def _run(be, deg, form, idx, geo, ng, precision=None):
+def _run(be, deg, form, idx, geo, ng, precision=None, weights=True):
     from triumvirate_amd import threept as T
     from triumvirate_amd.dataobjs import Binning
     cell = 12.5
@@ -61,7 +61,7 @@ def _run(be, deg, form, idx, geo, ng, precision=None):
         d = P.uniform_box_catalogue(4000, cell * ng[0], seed=5, weights=False)
         return T._compute_3pt_stats_sim_like(be, 'bispec', d, paramset=ps, degrees=deg,
                                              binning=b, form=form, idx_bin=idx, **kw)
-    d, r = P.survey_pair(2000, 8000, 0.9 * cell * min(ng), seed=9, weights=True)
+    d, r = P.survey_pair(2000, 8000, 0.9 * cell * min(ng), seed=9, weights=weights)
     return T._compute_3pt_stats_survey_like(be, 'bispec', d, r, paramset=ps, degrees=deg,
                                             binning=b, form=form, idx_bin=idx, **kw)
 
@@ -98,3 +98,15 @@ def test_register_z_pass_fp32(gpu_backend, ng, geo):
         err = np.max(np.abs(a[key] - b[key])) / np.max(np.abs(b[key]))
         print(f"  {key} {err:.2e}", flush=True)
         assert err < 1e-4
+
+
+@pytest.mark.parametrize('deg', [(0, 0, 0), (2, 0, 2), (0, 2, 2)])
+def test_uniform_weight_survey_fields(gpu_backend, oracle_backend, deg):
+    """Catalogues whose particles share one weight (per catalogue) take the
+    engine's single-accumulation path for the survey pair (dn, N): M = 0
+    and M != 0 terms against the oracle at 1e-10."""
+    print(f"uniform weights {deg}", flush=True)
+    case = (deg, 'full', None, 'lpp', (64, 64, 64))
+    a = _run(gpu_backend, *case, weights='uniform')
+    b = _run(oracle_backend, *case, weights='uniform')
+    P.compare_results(a, b, 'bispec', rtol=1e-10)

@@ -95,6 +95,7 @@ struct DeviceCatalogue {
   long long n = 0;
   DevArray<double> x, y, z, w, los;
   double wstotal = 0., wtotal = 0.;
+  double wuni = NAN;   // the weight ws wc of every particle, NaN if they differ
   mutable std::unique_ptr<TiledView> tiled;  // built on first use
   DevParticles view() const {
     DevParticles p;

@@ -86,6 +86,12 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
   out.n = n;
   out.wstotal = wst;
   out.wtotal = wt;
+  out.wuni = hw[0];
+  for (long long i = 1; i < n; i++)
+    if (hw[i] != hw[0]) {
+      out.wuni = NAN;
+      break;
+    }
   out.x.reset(n); out.y.reset(n); out.z.reset(n); out.w.reset(n);
   TRVG_HIP(hipMemcpyAsync(out.x.get(), hx.data(), n * 8, hipMemcpyHostToDevice, s));
   TRVG_HIP(hipMemcpyAsync(out.y.get(), hy.data(), n * 8, hipMemcpyHostToDevice, s));

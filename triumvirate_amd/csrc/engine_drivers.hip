@@ -806,12 +806,18 @@ static void assign_order(Context& ctx, c2<T>* mesh, const TiledView& tv,
   }
 }
 
+// imag: a real weight (m = 0 or a unit/scalar mode) goes into the
+// imaginary parts of the mesh instead of the real parts.
 template <typename T>
 static void assign(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c,
-                   int mode, int ell, int m, double scale) {
+                   int mode, int ell, int m, double scale, bool imag = false) {
   // y_l0 is real (its imaginary part is a signed zero): m = 0 fields take
   // real parts only.
   int cplx_w = ((mode == WM_YLM_W || mode == WM_YLMC_W2) && m != 0);
+  if (imag) {
+    if (cplx_w) throw Error(TRVG_ERR_INTERNAL, "Complex weights have no imaginary slot.");
+    cplx_w = 2;
+  }
   if ((mode == WM_YLM_W || mode == WM_YLMC_W2) && c.los.get() == nullptr)
     throw Error(TRVG_ERR_DATA, "Lines of sight are required for survey-type fields.");
   const TiledView& tv = tiled_view(ctx, c);
@@ -930,15 +936,43 @@ static void build_survey_pair_k(Context& ctx, double2* dn, double2* N,
                                 const DeviceCatalogue& d, const DeviceCatalogue& r,
                                 double alpha, int ell, int m) {
   double* rdn = reinterpret_cast<double*>(dn);
+  // Catalogues of uniform weights: one accumulation per particle, S_c =
+  // sum_c y w W, and the pair from the transforms by linearity (for a
+  // uniform w, y^* w^2 = w (y w)^*).
+  const bool uniform = std::isfinite(d.wuni) && std::isfinite(r.wuni);
   if (m == 0) {
     zero_field<double>(ctx, dn);
-    assign_dual<double>(ctx, rdn, rdn + 1, d, ell, m, 1., 1.);
-    assign_dual<double>(ctx, rdn, rdn + 1, r, ell, m, -alpha, alpha * alpha);
+    if (uniform) {
+      assign<double>(ctx, dn, d, WM_YLM_W, ell, 0, 1.);
+      assign<double>(ctx, dn, r, WM_YLM_W, ell, 0, 1., true);
+    } else {
+      assign_dual<double>(ctx, rdn, rdn + 1, d, ell, m, 1., 1.);
+      assign_dual<double>(ctx, rdn, rdn + 1, r, ell, m, -alpha, alpha * alpha);
+    }
     LoadScaleShift<double> la{dn, 0., ctx.g.vol_cell};
     fft3d<double>(ctx, dn, la, -1);
     ProfScope ps(ctx, KC_MISC, 3. * double(ctx.g.nmesh) * sizeof(double2));
+    // Plain separation (dn + i N), or dn = S_d - alpha S_r and
+    // N = w_d S_d + alpha^2 w_r S_r from S_d + i S_r.
+    const double pa = 1., pb = uniform ? -alpha : 0.;
+    const double qa = uniform ? d.wuni : 0., qb = uniform ? alpha * alpha * r.wuni : 1.;
     hipLaunchKernelGGL(k_unpack_hermitian, dim3(grid_for(ctx, ctx.g.nmesh)), dim3(256), 0,
-                       ctx.stream, dn, N, ctx.g);
+                       ctx.stream, dn, N, ctx.g, pa, pb, qa, qb);
+    TRVG_LAUNCH_CHECK();
+    return;
+  }
+  if (uniform) {
+    zero_field<double>(ctx, dn);
+    zero_field<double>(ctx, N);
+    assign<double>(ctx, dn, d, WM_YLM_W, ell, m, 1.);
+    assign<double>(ctx, N, r, WM_YLM_W, ell, m, 1.);
+    LoadScaleShift<double> la{dn, 0., ctx.g.vol_cell};
+    fft3d<double>(ctx, dn, la, -1);
+    LoadScaleShift<double> lb{N, 0., ctx.g.vol_cell};
+    fft3d<double>(ctx, N, lb, -1);
+    ProfScope ps(ctx, KC_MISC, 4. * double(ctx.g.nmesh) * sizeof(double2));
+    hipLaunchKernelGGL(k_combine_uniform, dim3(grid_for(ctx, ctx.g.nmesh)), dim3(256), 0,
+                       ctx.stream, dn, N, ctx.g, alpha, d.wuni, r.wuni);
     TRVG_LAUNCH_CHECK();
     return;
   }

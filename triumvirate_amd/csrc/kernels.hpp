@@ -1039,7 +1039,7 @@ k_assign_tiled(c2<T>* __restrict__ mesh, DevParticles p, Geom g, TileGeom tg,
         for (int kl = 0; kl < ORDER; kl++) {
           const double wv = wij * win[kl][2];
           atomicAdd(&acc_re[row + kl], wre * wv);
-          if (cplx_w) atomicAdd(&acc_im[row + kl], wim * wv);
+          if (cplx_w == 1) atomicAdd(&acc_im[row + kl], wim * wv);
         }
       }
     }
@@ -1049,6 +1049,10 @@ k_assign_tiled(c2<T>* __restrict__ mesh, DevParticles p, Geom g, TileGeom tg,
     const double vr = acc_re[c], vi = acc_im[c];
     if (vr == 0. && vi == 0.) continue;
     T* cell = reinterpret_cast<T*>(mesh + tile_cell(g, o0, c));
+    if (cplx_w == 2) {             // real weight into the imaginary part
+      unsafeAtomicAdd(cell + 1, T(vr));
+      continue;
+    }
     unsafeAtomicAdd(cell, T(vr));
     if (cplx_w) unsafeAtomicAdd(cell + 1, T(vi));
   }
@@ -1128,11 +1132,13 @@ k_assign_tiled_dual(T* __restrict__ ra, T* __restrict__ rb,
 }
 
 // Spectra of two real fields transformed together as P = FFT[a + i b]:
-//   a(k) = (P(k) + P(-k)^*) / 2,  b(k) = (P(k) - P(-k)^*) / (2i).
-// a overwrites P; each thread owns the pair (k, -k), so the update is in
+//   a(k) = (P(k) + P(-k)^*) / 2,  b(k) = (P(k) - P(-k)^*) / (2i),
+// written as P <- pa a + pb b and B <- qa a + qb b (plain separation:
+// (1, 0, 0, 1)).  Each thread owns the pair (k, -k), so the update is in
 // place without a race.
 static __global__ void __launch_bounds__(256)
-k_unpack_hermitian(double2* __restrict__ P, double2* __restrict__ B, Geom g) {
+k_unpack_hermitian(double2* __restrict__ P, double2* __restrict__ B, Geom g,
+                   double pa, double pb, double qa, double qb) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < g.nmesh;
        c += (long long)gridDim.x * blockDim.x) {
     const int k = int(c % g.n[2]);
@@ -1148,11 +1154,41 @@ k_unpack_hermitian(double2* __restrict__ P, double2* __restrict__ B, Geom g) {
     const double2 b_c = dmk(0.5 * (p.y + q.y), -0.5 * (p.x - q.x));
     const double2 a_m = dmk(0.5 * (q.x + p.x), 0.5 * (q.y - p.y));
     const double2 b_m = dmk(0.5 * (q.y + p.y), -0.5 * (q.x - p.x));
-    P[c] = a_c;
-    B[c] = b_c;
+    P[c] = dmk(pa * a_c.x + pb * b_c.x, pa * a_c.y + pb * b_c.y);
+    B[c] = dmk(qa * a_c.x + qb * b_c.x, qa * a_c.y + qb * b_c.y);
     if (cm != c) {
-      P[cm] = a_m;
-      B[cm] = b_m;
+      P[cm] = dmk(pa * a_m.x + pb * b_m.x, pa * a_m.y + pb * b_m.y);
+      B[cm] = dmk(qa * a_m.x + qb * b_m.x, qa * a_m.y + qb * b_m.y);
+    }
+  }
+}
+
+// Survey pair of one (l, m != 0) from the spectra of the per-catalogue
+// sums S_d = sum_d y w W, S_r = sum_r y w W of catalogues with uniform
+// weights w_d, w_r (in A and B on entry):
+//   dn(k) = S_d(k) - alpha S_r(k)                     -> A
+//   N(k)  = w_d S_d(-k)^* + alpha^2 w_r S_r(-k)^*      -> B
+// (N = sum y^* w^2 W = w conj(S) per catalogue; FFT[f^*](k) = F(-k)^*).
+// Each thread owns the pair (k, -k).
+static __global__ void __launch_bounds__(256)
+k_combine_uniform(double2* __restrict__ A, double2* __restrict__ B, Geom g, double alpha,
+                  double wd, double wr) {
+  const double ca = wd, cb = alpha * alpha * wr;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < g.nmesh;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int k = int(c % g.n[2]);
+    const long long ij = c / g.n[2];
+    const int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
+    const int im = (g.n[0] - i) % g.n[0], jm = (g.n[1] - j) % g.n[1],
+              km = (g.n[2] - k) % g.n[2];
+    const long long cm = ((long long)im * g.n[1] + jm) * g.n[2] + km;
+    if (cm < c) continue;
+    const double2 sd = A[c], sr = B[c], sdm = A[cm], srm = B[cm];
+    A[c] = dmk(sd.x - alpha * sr.x, sd.y - alpha * sr.y);
+    B[c] = dmk(ca * sdm.x + cb * srm.x, -(ca * sdm.y + cb * srm.y));
+    if (cm != c) {
+      A[cm] = dmk(sdm.x - alpha * srm.x, sdm.y - alpha * srm.y);
+      B[cm] = dmk(ca * sd.x + cb * sr.x, -(ca * sd.y + cb * sr.y));
     }
   }
 }
