@@ -962,18 +962,19 @@ static void build_survey_pair_k(Context& ctx, double2* dn, double2* N,
     return;
   }
   if (uniform) {
+    // S_d in dn, S_r in a scratch mesh; N (out of place) first, then dn in
+    // place over S_d, each formed by its transform's first-pass loader.
+    DevArray<double2> sr(ctx.g.nmesh);
     zero_field<double>(ctx, dn);
-    zero_field<double>(ctx, N);
+    zero_field<double>(ctx, sr.get());
     assign<double>(ctx, dn, d, WM_YLM_W, ell, m, 1.);
-    assign<double>(ctx, N, r, WM_YLM_W, ell, m, 1.);
-    LoadScaleShift<double> la{dn, 0., ctx.g.vol_cell};
-    fft3d<double>(ctx, dn, la, -1);
-    LoadScaleShift<double> lb{N, 0., ctx.g.vol_cell};
-    fft3d<double>(ctx, N, lb, -1);
-    ProfScope ps(ctx, KC_MISC, 4. * double(ctx.g.nmesh) * sizeof(double2));
-    hipLaunchKernelGGL(k_combine_uniform, dim3(grid_for(ctx, ctx.g.nmesh)), dim3(256), 0,
-                       ctx.stream, dn, N, ctx.g, alpha, d.wuni, r.wuni);
-    TRVG_LAUNCH_CHECK();
+    assign<double>(ctx, sr.get(), r, WM_YLM_W, ell, m, 1.);
+    const double vc = ctx.g.vol_cell, rb2 = 2. * double(ctx.g.nmesh) * sizeof(double2);
+    LoadLinear2<double> lb{dn, sr.get(), d.wuni * vc, alpha * alpha * r.wuni * vc, 1};
+    fft3d<double>(ctx, N, lb, -1, rb2);
+    LoadLinear2<double> la{dn, sr.get(), vc, -alpha * vc, 0};
+    fft3d<double>(ctx, dn, la, -1, rb2);
+    ctx.keep_until_flush(std::move(sr));
     return;
   }
   zero_field<double>(ctx, dn);
@@ -991,8 +992,20 @@ static void build_survey_pair_k(Context& ctx, double2* dn, double2* N,
 // Reductions
 // ---------------------------------------------------------------------------
 
+// n_modes and k_eff per bin depend on the geometry and the edges only:
+// computed once per device and reused.
 ShellStats shell_stats(Context& ctx, const trvg_binning& kb) {
   const int nb = kb.num_bins;
+  std::vector<double> key = {double(ctx.dev)};
+  for (int a = 0; a < 3; a++) { key.push_back(ctx.g.n[a]); key.push_back(ctx.g.dk[a]); }
+  key.insert(key.end(), kb.bin_edges, kb.bin_edges + nb + 1);
+  static std::mutex mu;
+  static std::map<std::vector<double>, ShellStats> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
   DevArray<double> edges(nb + 1);
   TRVG_HIP(hipMemcpyAsync(edges.get(), kb.bin_edges, (nb + 1) * 8,
                           hipMemcpyHostToDevice, ctx.stream));
@@ -1017,6 +1030,9 @@ ShellStats shell_stats(Context& ctx, const trvg_binning& kb) {
     s.nmodes[b] = int(hc[b]);
     s.keff[b] /= double(s.nmodes[b]);   // NaN for an empty shell, as k_eff /= 0
   }
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache.size() >= 64) cache.clear();
+  cache[key] = s;
   return s;
 }
 

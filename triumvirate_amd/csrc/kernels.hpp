@@ -1163,35 +1163,25 @@ k_unpack_hermitian(double2* __restrict__ P, double2* __restrict__ B, Geom g,
   }
 }
 
-// Survey pair of one (l, m != 0) from the spectra of the per-catalogue
-// sums S_d = sum_d y w W, S_r = sum_r y w W of catalogues with uniform
-// weights w_d, w_r (in A and B on entry):
-//   dn(k) = S_d(k) - alpha S_r(k)                     -> A
-//   N(k)  = w_d S_d(-k)^* + alpha^2 w_r S_r(-k)^*      -> B
-// (N = sum y^* w^2 W = w conj(S) per catalogue; FFT[f^*](k) = F(-k)^*).
-// Each thread owns the pair (k, -k).
-static __global__ void __launch_bounds__(256)
-k_combine_uniform(double2* __restrict__ A, double2* __restrict__ B, Geom g, double alpha,
-                  double wd, double wr) {
-  const double ca = wd, cb = alpha * alpha * wr;
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < g.nmesh;
-       c += (long long)gridDim.x * blockDim.x) {
-    const int k = int(c % g.n[2]);
-    const long long ij = c / g.n[2];
-    const int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
-    const int im = (g.n[0] - i) % g.n[0], jm = (g.n[1] - j) % g.n[1],
-              km = (g.n[2] - k) % g.n[2];
-    const long long cm = ((long long)im * g.n[1] + jm) * g.n[2] + km;
-    if (cm < c) continue;
-    const double2 sd = A[c], sr = B[c], sdm = A[cm], srm = B[cm];
-    A[c] = dmk(sd.x - alpha * sr.x, sd.y - alpha * sr.y);
-    B[c] = dmk(ca * sdm.x + cb * srm.x, -(ca * sdm.y + cb * srm.y));
-    if (cm != c) {
-      A[cm] = dmk(sdm.x - alpha * srm.x, sdm.y - alpha * srm.y);
-      B[cm] = dmk(ca * sd.x + cb * sr.x, -(ca * sd.y + cb * sr.y));
-    }
+// ca a + cb b, or its conjugate, per cell (real coefficients): the survey
+// pair of one (l, m != 0) from the per-catalogue sums S_d = sum_d y w W,
+// S_r = sum_r y w W of catalogues with uniform weights w_d, w_r:
+//   dn = S_d - alpha S_r,   N = w_d S_d^* + alpha^2 w_r S_r^*
+// (N = sum y^* w^2 W = w conj(S) per catalogue), formed as the first pass
+// of each transform reads them.
+template <typename T>
+struct LoadLinear2 {
+  const c2<T>* a;
+  const c2<T>* b;
+  double ca, cb;
+  int conj;
+  __device__ c2<T> operator()(long long g, int, int, int) const {
+    const c2<T> u = a[g], v = b[g];
+    const double re = ca * double(u.x) + cb * double(v.x);
+    const double im = ca * double(u.y) + cb * double(v.y);
+    return mk<T>(T(re), T(conj ? -im : im));
   }
-}
+};
 
 }  // namespace trvg
 
