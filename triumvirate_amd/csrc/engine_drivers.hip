@@ -1383,45 +1383,52 @@ static void shot_pairs_bispec(Context& ctx, const c2<T>* t, int la, int ma, int 
   });
 }
 
-// Per-block partial sums of S-bar_LM of one catalogue, downloaded into
-// pinned memory (valid after the next flush).
-static std::pair<const double2*, unsigned> sbar_partials(Context& ctx,
-                                                         const DeviceCatalogue& c,
-                                                         int ell, int m) {
-  unsigned grid = grid_for(ctx, c.n, 4);
-  DevArray<double2> part(grid);
-  hipLaunchKernelGGL(k_sbar, dim3(grid), dim3(RED_THREADS), 0, ctx.stream,
-                     c.view(), ell, m, part.get());
-  TRVG_LAUNCH_CHECK();
-  const double2* h = ctx.download(part.get(), grid);
-  ctx.keep_until_flush(std::move(part));
-  return {h, grid};
-}
-
-static cplx sum_partials(const std::pair<const double2*, unsigned>& h) {
-  double sx = 0., sy = 0.;
-  for (unsigned b = 0; b < h.second; b++) { sx += h.first[b].x; sy += h.first[b].y; }
-  return cplx(sx, sy);
-}
-
 // S-bar_LM = S-bar_LM[data] + alpha^3 S-bar_LM[randoms] for every (L, M)
-// in `keys` (threept.cpp:771-800): all reductions queued, one
-// synchronisation.  Call with no deferred work pending on the context.
+// in `keys` (threept.cpp:771-800): one pass per catalogue for up to
+// SBAR_MAX_KEYS keys, partials downloaded, one synchronisation.  Call with
+// no deferred work pending on the context.
 static std::map<std::pair<int, int>, cplx> sbar_table(
     Context& ctx, const DeviceCatalogue& d, const DeviceCatalogue& r, double alpha,
     const std::set<std::pair<int, int>>& keys) {
-  std::vector<std::pair<const double2*, unsigned>> hd, hr;
-  for (const auto& k : keys) {
-    hd.push_back(sbar_partials(ctx, d, k.first, k.second));
-    hr.push_back(sbar_partials(ctx, r, k.first, k.second));
+  const std::vector<std::pair<int, int>> kv(keys.begin(), keys.end());
+  struct Chunk {
+    size_t k0;
+    int nk;
+    const double2* hd;
+    const double2* hr;
+    unsigned gd, gr;
+  };
+  std::vector<Chunk> chunks;
+  for (size_t k0 = 0; k0 < kv.size(); k0 += SBAR_MAX_KEYS) {
+    SbarKeys sk{};
+    sk.n = int(std::min<size_t>(SBAR_MAX_KEYS, kv.size() - k0));
+    for (int q = 0; q < sk.n; q++) {
+      sk.ell[q] = kv[k0 + q].first;
+      sk.m[q] = kv[k0 + q].second;
+    }
+    Chunk c{k0, sk.n, nullptr, nullptr, 0u, 0u};
+    for (int side = 0; side < 2; side++) {
+      const DeviceCatalogue& cat = side == 0 ? d : r;
+      const unsigned grid = grid_for(ctx, cat.n, 4);
+      DevArray<double2> part((size_t)grid * sk.n);
+      hipLaunchKernelGGL(k_sbar_multi, dim3(grid), dim3(RED_THREADS), 0, ctx.stream,
+                         cat.view(), sk, part.get());
+      TRVG_LAUNCH_CHECK();
+      const double2* h = ctx.download(part.get(), (size_t)grid * sk.n);
+      ctx.keep_until_flush(std::move(part));
+      if (side == 0) { c.hd = h; c.gd = grid; } else { c.hr = h; c.gr = grid; }
+    }
+    chunks.push_back(c);
   }
   ctx.flush();
   std::map<std::pair<int, int>, cplx> out;
-  size_t i = 0;
-  for (const auto& k : keys) {
-    out[k] = sum_partials(hd[i]) + std::pow(alpha, 3) * sum_partials(hr[i]);
-    i++;
-  }
+  for (const Chunk& c : chunks)
+    for (int q = 0; q < c.nk; q++) {
+      double dx = 0., dy = 0., rx = 0., ry = 0.;
+      for (unsigned b = 0; b < c.gd; b++) { dx += c.hd[b * c.nk + q].x; dy += c.hd[b * c.nk + q].y; }
+      for (unsigned b = 0; b < c.gr; b++) { rx += c.hr[b * c.nk + q].x; ry += c.hr[b * c.nk + q].y; }
+      out[kv[c.k0 + q]] = cplx(dx, dy) + std::pow(alpha, 3) * cplx(rx, ry);
+    }
   return out;
 }
 

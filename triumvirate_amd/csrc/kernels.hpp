@@ -747,33 +747,52 @@ k_cube_sum(const c2<T>* __restrict__ f, long long n, double* __restrict__ part) 
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// Per-block partial of sum_p y_lm(los_p) w_p^3 (threept.cpp:156-236).
+
+// Per-block partials of sum_p y_lm(los_p) w_p^3 (threept.cpp:156-236) for
+// several (l, m) at once, part[block * n + q]; one pass over the particles.
+constexpr int SBAR_MAX_KEYS = 8;
+struct SbarKeys {
+  int n;
+  int ell[SBAR_MAX_KEYS], m[SBAR_MAX_KEYS];
+};
+
 static __global__ void __launch_bounds__(RED_THREADS)
-k_sbar(DevParticles p, int ell, int m, double2* __restrict__ part) {
+k_sbar_multi(DevParticles p, SbarKeys keys, double2* __restrict__ part) {
   __shared__ double rr[RED_THREADS], ri[RED_THREADS];
-  double sx = 0., sy = 0.;
+  double sx[SBAR_MAX_KEYS], sy[SBAR_MAX_KEYS];
+#pragma unroll
+  for (int q = 0; q < SBAR_MAX_KEYS; q++) sx[q] = sy[q] = 0.;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < p.n;
        i += (long long)gridDim.x * blockDim.x) {
-    double w = p.w[i];
-    double w3 = w * w * w;
-    double2 y = (ell == 0 && m == 0)
-                    ? dmk(1., 0.)
-                    : ylm_reduced_dev(ell, m, p.los[3 * i], p.los[3 * i + 1],
-                                      p.los[3 * i + 2]);
-    sx += y.x * w3;
-    sy += y.y * w3;
-  }
-  rr[threadIdx.x] = sx;
-  ri[threadIdx.x] = sy;
-  __syncthreads();
-  for (int q = RED_THREADS / 2; q > 0; q >>= 1) {
-    if (threadIdx.x < q) {
-      rr[threadIdx.x] += rr[threadIdx.x + q];
-      ri[threadIdx.x] += ri[threadIdx.x + q];
+    const double w = p.w[i];
+    const double w3 = w * w * w;
+#pragma unroll
+    for (int q = 0; q < SBAR_MAX_KEYS; q++) {
+      if (q >= keys.n) break;
+      const double2 y = (keys.ell[q] == 0 && keys.m[q] == 0)
+                            ? dmk(1., 0.)
+                            : ylm_reduced_dev(keys.ell[q], keys.m[q], p.los[3 * i],
+                                              p.los[3 * i + 1], p.los[3 * i + 2]);
+      sx[q] += y.x * w3;
+      sy[q] += y.y * w3;
     }
+  }
+#pragma unroll
+  for (int q = 0; q < SBAR_MAX_KEYS; q++) {
+    if (q >= keys.n) break;
+    rr[threadIdx.x] = sx[q];
+    ri[threadIdx.x] = sy[q];
+    __syncthreads();
+    for (int h = RED_THREADS / 2; h > 0; h >>= 1) {
+      if (threadIdx.x < h) {
+        rr[threadIdx.x] += rr[threadIdx.x + h];
+        ri[threadIdx.x] += ri[threadIdx.x + h];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long long)blockIdx.x * keys.n + q] = dmk(rr[0], ri[0]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = dmk(rr[0], ri[0]);
 }
 
 // Elementwise helpers.
