@@ -62,7 +62,8 @@ def _params(ngrid, L, order=3, prec=64):
 # ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize('shape', [(32, 32, 32), (16, 32, 64), (64, 16, 8),
-                                   (8, 8, 256), (8, 8, 512)])
+                                   (8, 8, 256), (8, 8, 512), (8, 256, 8),
+                                   (256, 8, 8)])
 @pytest.mark.parametrize('prec', [64, 32])
 def test_fft3d_matches_numpy(eng, shape, prec):
     """Forward/backward unnormalised FFT vs numpy.fft (FFTW convention).

@@ -13,7 +13,8 @@ periodic-box B_202), half-real x complex (B_022), complex x complex
 Tolerances: FP64 vs the CPU oracle 1e-10 relative to max |B| of the
 multipole; FP32 vs the FP64 engine 1e-4 (as the FP32 bound elsewhere).
 The FP32 cases with 256 and 512 cells along z run the register-resident
-z pass (the FP64 engine and the oracle use the LDS-staged one).
+z pass (the FP64 engine and the oracle use the LDS-staged one); those with
+256 cells along y or x check the FP32 strided passes at that length.
 """
 
 import pytest
@@ -86,12 +87,13 @@ def test_xfirst_fp32_bound(gpu_backend):
         assert err < 1e-4
 
 
-@pytest.mark.parametrize('ng', [(32, 32, 256), (16, 16, 512)])
+@pytest.mark.parametrize('ng', [(32, 32, 256), (16, 16, 512), (16, 256, 16),
+                                (256, 16, 16)])
 @pytest.mark.parametrize('geo', ['lpp', 'gpp'])
-def test_register_z_pass_fp32(gpu_backend, ng, geo):
+def test_fp32_long_axes(gpu_backend, ng, geo):
     import numpy as np
     case = ((2, 0, 2), 'full', None, geo, ng)
-    print(f"register z pass fp32 {geo} {ng}", flush=True)
+    print(f"fp32 long axes {geo} {ng}", flush=True)
     a = _run(gpu_backend, *case, precision='fp32')
     b = _run(gpu_backend, *case, precision='fp64')
     for key in ('bk_raw', 'bk_shot'):
