@@ -1239,22 +1239,34 @@ __device__ __forceinline__ double2 oct_mirror_sum(const c2<T>* __restrict__ t, c
     ya0 = ylm_reduced_dev(la, ma, ax, ay, az);
     yb0 = ylm_reduced_dev(lb, mb, ax, ay, az);
   }
+  // All mirror cells are loaded before any is used (independent loads in
+  // flight together), then summed in the order (u, v, w).
+  // (Absent mirrors index the representative itself, so every address is
+  // inside the mesh.)
+  c2<T> tv8[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int u = q >> 2, v = (q >> 1) & 1, w = q & 1;
+    const int iu = u < ni ? u : 0, jv = v < nj ? v : 0, kw = w < nk ? w : 0;
+    tv8[q] = t[((long long)is[iu] * g.n[1] + js[jv]) * g.n[2] + ks[kw]];
+  }
   double sx = 0., sy = 0.;
-  for (int u = 0; u < ni; u++)
-    for (int v = 0; v < nj; v++)
-      for (int w = 0; w < nk; w++) {
-        const int i = is[u], j = js[v], k = ks[w];
-        const c2<T> tv = t[((long long)i * g.n[1] + j) * g.n[2] + k];
-        double2 val = dmk(tv.x, tv.y);
-        if (!trivial) {
-          const bool fx = shifted(i, g.n[0]) < 0, fy = shifted(j, g.n[1]) < 0,
-                     fz = shifted(k, g.n[2]) < 0;
-          val = dmul(val, dmul(ylm_mirror(ya0, fx, fy, fz, la, ma),
-                               ylm_mirror(yb0, fx, fy, fz, lb, mb)));
-        }
-        sx += val.x;
-        sy += val.y;
-      }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int u = q >> 2, v = (q >> 1) & 1, w = q & 1;
+    if (!(u < ni && v < nj && w < nk)) continue;
+    const int i = is[u], j = js[v], k = ks[w];
+    const c2<T> tv = tv8[q];
+    double2 val = dmk(tv.x, tv.y);
+    if (!trivial) {
+      const bool fx = shifted(i, g.n[0]) < 0, fy = shifted(j, g.n[1]) < 0,
+                 fz = shifted(k, g.n[2]) < 0;
+      val = dmul(val, dmul(ylm_mirror(ya0, fx, fy, fz, la, ma),
+                           ylm_mirror(yb0, fx, fy, fz, lb, mb)));
+    }
+    sx += val.x;
+    sy += val.y;
+  }
   return dmk(sx, sy);
 }
 
