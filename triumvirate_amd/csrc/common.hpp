@@ -235,8 +235,6 @@ __host__ __device__ inline double sph_bessel_dev(int l, double x) {
 // Natural cubic spline of j_l on x_i = 0.05 i (maths.cpp:309-375).
 // y: node values; c: second derivative / 2 at the nodes.
 struct SplineDev {
-  const double* y;
-  const double* c;
   // Per interval i: (y_i, b_i, c_i, d_i) of the cubic on [x_i, x_i+1)
   // (b, d as GSL's cspline evaluation forms them), and an FP32 copy.
   const double4* cf;

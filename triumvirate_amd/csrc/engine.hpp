@@ -119,12 +119,11 @@ struct Spline {
   double split = 1000.;
   double step = 0.05;
   int n = 0;
-  DevArray<double> y, c;
   DevArray<double4> cf;
   DevArray<float4> cf32;
   SplineDev dev() const {
     SplineDev s;
-    s.y = y.get(); s.c = c.get(); s.cf = cf.get(); s.cf32 = cf32.get();
+    s.cf = cf.get(); s.cf32 = cf32.get();
     s.n = n; s.ell = ell; s.step = step; s.inv_step = 1. / step;
     s.split = split;
     return s;

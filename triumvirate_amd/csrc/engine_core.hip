@@ -171,8 +171,6 @@ void build_spline(int ell, Spline& sp, hipStream_t s) {
   sp.ell = ell;
   sp.split = split;
   sp.n = int(y.size());
-  sp.y.reset(sp.n);
-  sp.c.reset(sp.n);
   // Interval coefficients as GSL's cspline evaluation forms them:
   // b = dy/dx - dx (c_i+1 + 2 c_i) / 3, d = (c_i+1 - c_i) / (3 dx).
   std::vector<double4> cf(sp.n - 1);
@@ -189,8 +187,6 @@ void build_spline(int ell, Spline& sp, hipStream_t s) {
   }
   sp.cf.reset(sp.n - 1);
   sp.cf32.reset(sp.n - 1);
-  TRVG_HIP(hipMemcpyAsync(sp.y.get(), y.data(), sp.n * 8, hipMemcpyHostToDevice, s));
-  TRVG_HIP(hipMemcpyAsync(sp.c.get(), c.data(), sp.n * 8, hipMemcpyHostToDevice, s));
   TRVG_HIP(hipMemcpyAsync(sp.cf.get(), cf.data(), (sp.n - 1) * sizeof(double4),
                           hipMemcpyHostToDevice, s));
   TRVG_HIP(hipMemcpyAsync(sp.cf32.get(), cf32.data(), (sp.n - 1) * sizeof(float4),
