@@ -45,6 +45,10 @@ def load():
     lib.trvo_assign.restype = C.c_int
     lib.trvo_fourier_transform.argtypes = [C.POINTER(_abi.Params), dp, dp]
     lib.trvo_fourier_transform.restype = C.c_int
+    lib.trvo_window.argtypes = [C.POINTER(_abi.Params), dp]
+    lib.trvo_window.restype = C.c_int
+    lib.trvo_alias.argtypes = [C.POINTER(_abi.Params), dp]
+    lib.trvo_alias.restype = C.c_int
     lib.trvo_bench_pairs.argtypes = [C.POINTER(_abi.Params),
                                      C.POINTER(_abi.Binning), C.c_int, dp]
     lib.trvo_bench_pairs.restype = C.c_int

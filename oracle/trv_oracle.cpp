@@ -1585,6 +1585,29 @@ int trvo_assign(const trvg_catalogue* cat, const double* weights,
   });
 }
 
+// Fourier-space assignment window W(k) on the mesh of `params` (order from
+// params->assignment_order), nmesh doubles (field.cpp:1114-1201).
+int trvo_window(const trvg_params* params, double* out) {
+  return guarded([&] {
+    Params p = make_params(params);
+    p.interlace = false;
+    Mesh m(p);
+    m.compute_window(p.order);
+    std::memcpy(out, m.window.data(), sizeof(double) * p.nmesh);
+  });
+}
+
+// Aliasing function C1(k) of FieldStats (field.cpp:3403-3564) on the mesh
+// of `params`, with or without the interlaced approximation, nmesh doubles.
+int trvo_alias(const trvg_params* params, double* out) {
+  return guarded([&] {
+    Params p = make_params(params);
+    Stats st(p);
+    st.compute_alias();
+    std::memcpy(out, st.alias.data(), sizeof(double) * p.nmesh);
+  });
+}
+
 // Forward transform of an assigned field as MeshField::fourier_transform,
 // with an optional interlacing shadow (both interleaved complex, in place).
 int trvo_fourier_transform(const trvg_params* params, double* field,

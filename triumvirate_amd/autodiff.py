@@ -48,6 +48,55 @@ def _sync():
     torch.cuda.current_stream().synchronize()
 
 
+def _check_positions(pos):
+    """The engine reads positions as an (n, 3) row-major float64 device
+    array; anything else would be an out-of-bounds device access."""
+    if not isinstance(pos, torch.Tensor):
+        raise ValueError("positions must be a torch tensor.")
+    if not pos.is_cuda:
+        raise ValueError("positions must be a CUDA (HIP) tensor.")
+    if pos.dtype != torch.float64:
+        raise ValueError(f"positions must be float64, not {pos.dtype}.")
+    if pos.ndim != 2 or pos.shape[1] != 3:
+        raise ValueError("positions must have shape (n, 3), not "
+                         f"{tuple(pos.shape)}.")
+
+
+def _check_weights(w, pos):
+    if w is None:
+        return
+    if not isinstance(w, torch.Tensor):
+        raise ValueError("weights must be a torch tensor or None.")
+    if not w.is_cuda or w.device != pos.device:
+        raise ValueError("weights must be a CUDA (HIP) tensor on the device "
+                         "of the positions.")
+    if w.dtype != torch.float64:
+        raise ValueError(f"weights must be float64, not {w.dtype}.")
+    if w.ndim != 1 or w.shape[0] != pos.shape[0]:
+        raise ValueError(f"weights must have shape ({pos.shape[0]},), not "
+                         f"{tuple(w.shape)}.")
+
+
+def _check_mesh(mesh, cfg, what='mesh'):
+    if not isinstance(mesh, torch.Tensor):
+        raise ValueError(f"{what} must be a torch tensor.")
+    if not mesh.is_cuda:
+        raise ValueError(f"{what} must be a CUDA (HIP) tensor.")
+    if tuple(mesh.shape) != tuple(cfg.ngrid):
+        raise ValueError(f"{what} must have shape {tuple(cfg.ngrid)}, not "
+                         f"{tuple(mesh.shape)}.")
+    if mesh.is_complex() or not mesh.is_floating_point():
+        raise ValueError(f"{what} must be a real floating-point tensor.")
+
+
+def _params_on(cfg, t):
+    """The ABI parameters with the device of tensor `t`."""
+    p = _abi.Params.from_buffer_copy(cfg.params)
+    p.device = t.device.index if t.device.index is not None \
+        else torch.cuda.current_device()
+    return p
+
+
 class _Config:
     def __init__(self, boxsize, ngrid, binning, assignment, precision,
                  norm_factor):
@@ -86,13 +135,15 @@ class _Assign(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, pos, w, cfg):
+        _check_positions(pos)
+        _check_weights(w, pos)
         lib = _lib.load()
         pos = pos.detach().contiguous()
         wd = None if w is None else w.detach().contiguous()
         mesh = torch.empty(cfg.ngrid, dtype=cfg.dtype, device=pos.device)
         _sync()
         _check(lib, lib.trvg_diff_assign(
-            C.byref(cfg.params), pos.data_ptr(),
+            C.byref(_params_on(cfg, pos)), pos.data_ptr(),
             None if wd is None else wd.data_ptr(), pos.shape[0],
             mesh.data_ptr(), 0))
         ctx.cfg = cfg
@@ -107,13 +158,15 @@ class _Assign(torch.autograd.Function):
         saved = ctx.saved_tensors
         pos = saved[0]
         wd = saved[1] if ctx.has_w else None
-        gmesh = gmesh.detach().to(cfg.dtype).contiguous()
+        _check_mesh(gmesh, cfg, 'mesh gradient')
+        gmesh = gmesh.detach().to(device=pos.device,
+                                  dtype=cfg.dtype).contiguous()
         gpos = torch.empty_like(pos) if ctx.needs_input_grad[0] else None
         gw = torch.empty_like(wd) if (ctx.has_w and ctx.needs_input_grad[1]) \
             else None
         _sync()
         _check(lib, lib.trvg_diff_assign_backward(
-            C.byref(cfg.params), pos.data_ptr(),
+            C.byref(_params_on(cfg, pos)), pos.data_ptr(),
             None if wd is None else wd.data_ptr(), pos.shape[0],
             gmesh.data_ptr(), None if gpos is None else gpos.data_ptr(),
             None if gw is None else gw.data_ptr()))
@@ -125,6 +178,7 @@ class _BispecDiag(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, mesh, cfg):
+        _check_mesh(mesh, cfg)
         lib = _lib.load()
         mesh = mesh.detach().to(cfg.dtype).contiguous()
         nb = cfg.binning.num_bins
@@ -132,7 +186,7 @@ class _BispecDiag(torch.autograd.Function):
         nmodes = (C.c_int * nb)()
         _sync()
         _check(lib, lib.trvg_diff_bispec_diag(
-            C.byref(cfg.params), C.byref(cfg.bview.struct), cfg.norm,
+            C.byref(_params_on(cfg, mesh)), C.byref(cfg.bview.struct), cfg.norm,
             mesh.data_ptr(), _abi.dptr(bk), nmodes))
         cfg.nmodes = np.array(nmodes[:], dtype=np.int64)
         ctx.cfg = cfg
@@ -148,7 +202,7 @@ class _BispecDiag(torch.autograd.Function):
         grad = torch.empty_like(mesh)
         _sync()
         _check(lib, lib.trvg_diff_bispec_diag_backward(
-            C.byref(cfg.params), C.byref(cfg.bview.struct), cfg.norm,
+            C.byref(_params_on(cfg, mesh)), C.byref(cfg.bview.struct), cfg.norm,
             mesh.data_ptr(), _abi.dptr(g), grad.data_ptr()))
         return grad, None
 
@@ -181,6 +235,8 @@ class DifferentiableBispectrum:
 
     def mesh(self, positions, weights=None):
         """Field fluctuation delta(x) = n(x) - sum(w) / V (differentiable)."""
+        _check_positions(positions)
+        _check_weights(weights, positions)
         n = _Assign.apply(positions, weights, self.cfg)
         total = (weights.sum() if weights is not None
                  else float(positions.shape[0]))
@@ -188,6 +244,7 @@ class DifferentiableBispectrum:
 
     def from_mesh(self, delta):
         """B_000 diag values of a real mesh (differentiable)."""
+        _check_mesh(delta, self.cfg)
         return _BispecDiag.apply(delta, self.cfg)
 
     def __call__(self, positions, weights=None):

@@ -669,6 +669,10 @@ void sjl_slab_ifft(Context& ctx, c2<T>* base, const double2* src, const Spline& 
   const double bx = double(g.nmesh) * sizeof(double2) + fb, by = 2. * fb, bz = 2. * fb;
   TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_sjl<T, LG>(ctx, base, nt, src, sp, dr.get(), ell,
                                                         m, packed, bx)));
+  // The radii stay owned until the stream is synchronised: returned to the
+  // shared pool now, another call could take and overwrite them while the
+  // x pass still reads them.
+  ctx.keep_until_flush(std::move(dr));
   xfirst_yz<T>(ctx, base, plan->kc.get(), dwp, int(ny), dwp + ny, int(nz), by, bz);
 }
 

@@ -191,6 +191,12 @@ class ParameterSet:
                 "Normalisation convention must be 'none', 'particle', "
                 "'mesh' or 'mesh-mixed': "
                 f"`norm_convention` = '{p.get('norm_convention')}'.")
+        # parameters.cpp:870-882
+        if p.get('norm_convention') == 'mesh-mixed' and \
+                p.get('npoint') not in (None, '2pt'):
+            raise InvalidParameterError(
+                "Normalisation convention 'mesh-mixed' only applies to "
+                f"two-point statistics: `npoint` = '{p.get('npoint')}'.")
 
         prec = str(p.get('precision', 'fp64')).lower()
         if prec not in ('fp64', 'fp32'):

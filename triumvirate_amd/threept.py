@@ -244,7 +244,10 @@ def _select_norm(paramset, norm_part, norm_mesh):
         return norm_part
     if conv == 'mesh':
         return norm_mesh
-    return 0.   # 'mesh-mixed' is n/a for three-point statistics
+    # 'mesh-mixed' is rejected for three-point statistics by validation.
+    raise InvalidParameterError(
+        f"Normalisation convention '{conv}' does not apply to three-point "
+        "statistics.")
 
 
 def _multipole_structs(paramset, multipoles):
