@@ -6,8 +6,11 @@ upper triangle, 210 pairs) + B_202 (full, 400 pairs) over 20 linear k-bins
 in [0.005, 0.205] h/Mpc; 1e6 lognormal data points and 5e6 uniform randoms
 in a quarter-sky shell r in [500, 1500] Mpc/h; box L = 3200 Mpc/h centred on
 the randoms; 256^3 mesh, PCS assignment (interlacing off, as the reference
-forces for three-point statistics).  Fields in FP32, sums in FP64.  The
-catalogues are uploaded to HBM once before the timed region.
+forces for three-point statistics).  Fields in FP32, sums in FP64.  Each
+step is one engine call with the catalogues in host memory and the results
+returned to host memory; the value is the median over the steps, beside
+one cold call, FP64 calls, device-resident catalogues and the
+differentiable B_000 forward + backward time.
 
 Prints ONE JSON line with the metric, throughput, per-kernel roofline and a
 CPU baseline (the oracle's reference-schedule per-pair loop on host cores).
@@ -84,7 +87,7 @@ def params_struct(ngrid, degrees, shape, prec):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--ngrid', type=int, default=256)
     ap.add_argument('--ndata', type=int, default=1_000_000)
@@ -99,6 +102,8 @@ def main():
     ap.add_argument('--batch', type=int, default=64)
     ap.add_argument('--skip-fp64', action='store_true',
                     help='skip the FP64 steps (profiler runs)')
+    ap.add_argument('--skip-grad', action='store_true',
+                    help='skip the differentiable fwd+bwd figure')
     ap.add_argument('--cpu-pairs', type=int, default=40,
                     help='bin pairs in the CPU-baseline sample (0: skip)')
     args = ap.parse_args()
@@ -139,31 +144,22 @@ def bench_bispec(args, lib):
         if st != 0:
             raise RuntimeError(lib.trvg_last_error().decode())
 
-    # Device-resident catalogues (upload excluded from the timed region).
-    t_up = time.perf_counter()
-    handles = []
-    for cat, los in ((data, los_d), (rand, los_r)):
-        x, y, z, nz, ws, wc = cat.columns()
-        view = _abi.CatalogueView(x, y, z, nz, ws, wc)
-        h = C.c_void_p()
-        check(lib.trvg_catalogue_upload(C.byref(view.struct), _abi.dptr(los),
-                                        0, C.byref(h)))
-        handles.append(h)
-    t_up = time.perf_counter() - t_up
-    hd, hr = handles
+    dview = _abi.CatalogueView(*data.columns())
+    rview = _abi.CatalogueView(*rand.columns())
 
     # Particle normalisation (reference default convention).
-    x, y, z, nz, ws, wc = rand.columns()
     alpha = data.wstotal / rand.wstotal
     norm = C.c_double()
-    rv = _abi.CatalogueView(x, y, z, nz, ws, wc)
     check(lib.trvg_calc_bispec_normalisation_from_particles(
-        C.byref(rv.struct), alpha, C.byref(norm)))
+        C.byref(rview.struct), alpha, C.byref(norm)))
 
     multipoles = [((0, 0, 0), 'triu'), ((2, 0, 2), 'full')]
+    pairs_per_step = 210 + 400
 
-    def step(prec):
-        # One engine call computes both multipoles on shared fields.
+    def step(prec, handles=None):
+        """One engine call computing both multipoles on shared fields:
+        host-resident catalogues in (the default), results out on the host;
+        with handles, device-resident catalogues."""
         plist = [params_struct(args.ngrid, deg, shape, prec)
                  for deg, shape in multipoles]
         res = [_abi.ResultBuffers(_abi.result_dim(p.shape, 20, 0), 'bispec')
@@ -171,27 +167,35 @@ def bench_bispec(args, lib):
         arr = (_abi.Params * len(plist))(*plist)
         outs = (_abi.BispecResult * len(plist))(*[r.struct for r in res])
         norms = np.full(len(plist), norm.value)
-        check(lib.trvg_compute_bispec_multi_dev(
-            hd, hr, arr, len(plist), C.byref(kbv.struct), _abi.dptr(norms),
-            outs))
+        if handles is None:
+            check(lib.trvg_compute_bispec_multi(
+                C.byref(dview.struct), C.byref(rview.struct), _abi.dptr(los_d),
+                _abi.dptr(los_r), arr, len(plist), C.byref(kbv.struct),
+                _abi.dptr(norms), outs))
+        else:
+            check(lib.trvg_compute_bispec_multi_dev(
+                handles[0], handles[1], arr, len(plist), C.byref(kbv.struct),
+                _abi.dptr(norms), outs))
         out = {}
         for (deg, _), r, o in zip(multipoles, res, outs):
             r.struct.dim = o.dim
             out[deg] = r.as_dict()
         return out
 
-    pairs_per_step = 210 + 400
+    def timed(fn, n):
+        ts = []
+        for _ in range(n):
+            t = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t)
+        return ts, out
 
-    # FP64 (the drop-in precision): one cold call, then one timed warm call.
-    r64, t64 = None, None
-    if not args.skip_fp64:
-        log("FP64 steps ...")
-        step(64)
-        t = time.perf_counter()
-        r64 = step(64)
-        t64 = time.perf_counter() - t
-        log(f"FP64 step {t64 * 1e3:.1f} ms")
-
+    # Cold run: the first engine call of the process (tables, splines, shell
+    # plans, device pool and pinned staging all built inside it).
+    t = time.perf_counter()
+    step(32)
+    t_cold = time.perf_counter() - t
+    log(f"cold FP32 call {t_cold * 1e3:.1f} ms")
     for _ in range(args.warmup):
         step(32)
 
@@ -227,19 +231,37 @@ def bench_bispec(args, lib):
     dom = max((k for k in kernels if kernels[k]['achieved_GBs']),
               key=lambda k: kernels[k]['ms_per_step'])
 
-    # Timed region: K steps, HIP events only around the roofline kernel.
-    log(f"timing {args.steps} FP32 steps (events on {dom}) ...")
+    # Timed region: K host-resident FP32 calls, each timed; HIP events only
+    # around the roofline kernel's launches.
+    log(f"timing {args.steps} FP32 calls, host-resident catalogues "
+        f"(events on {dom}) ...")
     check(lib.trvg_profile_only(kernels[dom]['class']))
     check(lib.trvg_profile_enable(1))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r32 = step(32)
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    t32, r32 = timed(lambda: step(32), args.steps)
     kd = read_profile(args.steps)[dom]
     check(lib.trvg_profile_enable(0))
     check(lib.trvg_profile_only(-1))
-    log(f"FP32 step {ms_per_step:.2f} ms")
+    ms_per_step = float(np.median(t32)) * 1e3
+    log(f"FP32 call median {ms_per_step:.2f} ms "
+        f"(min {min(t32) * 1e3:.2f}, max {max(t32) * 1e3:.2f})")
+
+    # FP64 (the drop-in precision), host-resident: one warm-up, median of >= 5.
+    r64, t64 = None, None
+    if not args.skip_fp64:
+        step(64)
+        t64, r64 = timed(lambda: step(64), max(5, args.steps))
+        log(f"FP64 call median {np.median(t64) * 1e3:.2f} ms")
+
+    # Device-resident catalogues (secondary figure): uploads outside.
+    t = time.perf_counter()
+    handles = [_upload(lib, data, los_d), _upload(lib, rand, los_r)]
+    t_up = time.perf_counter() - t
+    step(32, handles)
+    tdev, _ = timed(lambda: step(32, handles), args.steps)
+    for h in handles:
+        lib.trvg_catalogue_free(h)
+    log(f"device-resident FP32 call median {np.median(tdev) * 1e3:.2f} ms, "
+        f"upload of both catalogues {t_up * 1e3:.1f} ms")
 
     # Plain copy kernel of 1 GiB: the scale for hardware byte counters.
     cal_ms = C.c_double()
@@ -270,33 +292,13 @@ def bench_bispec(args, lib):
         'launches_per_step': kd['launches_per_step'],
     }
 
-    # CPU baseline: oracle reference-schedule per-pair loop on host cores.
-    cpu = None
-    if args.cpu_pairs > 0:
-        import oracle
-        olib = oracle.load()
-        threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
-        olib.trvo_set_threads(threads)
-        p = params_struct(args.ngrid, (0, 0, 0), 'full', 64)
-        sec = C.c_double()
-        log(f"CPU baseline sample: {args.cpu_pairs} pairs on {threads} threads")
-        check_o = olib.trvo_bench_pairs(C.byref(p), C.byref(kbv.struct),
-                                        args.cpu_pairs, C.byref(sec))
-        if check_o == 0:
-            per_pair = sec.value / args.cpu_pairs
-            # Pair-loop work per step: 210 pairs x 1 term (B_000) + 400 pairs
-            # x 3 terms (B_202); fixed per-term costs are not counted.
-            work = 210 * 1 + 400 * 3
-            cpu = {
-                'value': round(pairs_per_step / (per_pair * work), 5),
-                'unit': 'bin-pair evals/s', 'cores': threads, 'kind': 'port',
-                'sample': (f"reference-schedule per-pair loop body (2 shell "
-                           f"IFFTs + triple sum + shot-noise IFFT + spline "
-                           f"pass), FP64, {args.ngrid}^3, {args.cpu_pairs} "
-                           f"pairs, {sec.value:.1f} s; per-term assignment/"
-                           f"FFT costs excluded (favours the CPU)"),
-                'seconds_per_pair': round(per_pair, 3),
-            }
+    # Differentiable B_000 (1e6 particles, 256^3 CIC): forward + backward
+    # time against the plain GPU FP64 forward.
+    fwd_bwd = None
+    if not args.skip_grad:
+        fwd_bwd = grad_timings(lib, args.steps, fd_particles=0)
+
+    cpu = cpu_baseline(args, kbv, pairs_per_step) if args.cpu_pairs > 0 else None
 
     value = pairs_per_step / (ms_per_step / 1e3)
     line = {
@@ -313,27 +315,71 @@ def bench_bispec(args, lib):
             'workload': 'survey-like lpp B_000 (full->triu, 210 pairs) + '
                         'B_202 (full, 400 pairs), 20 lin k-bins '
                         '[0.005,0.205], 1e6 lognormal data + 5e6 randoms, '
-                        'L=3200, PCS, interlace off',
+                        'L=3200, PCS, interlace off; one engine call per '
+                        'step with host-resident catalogues in and results '
+                        'out (median of the steps)',
             'ngrid': args.ngrid, 'pairs_per_step': pairs_per_step,
             'n_data': int(data.ntotal), 'n_rand': int(rand.ntotal),
             'precision': 'fp32 fields, fp64 sums',
         },
         'roofline': roofline,
         'cpu_baseline': cpu,
-        'fp64': {'value': round(pairs_per_step / t64, 2),
-                 'ms_per_step': round(t64 * 1e3, 3)} if t64 else None,
+        'step_ms': [round(t * 1e3, 3) for t in t32],
+        'cold_ms': round(t_cold * 1e3, 3),
+        'fp64': {'value': round(pairs_per_step / float(np.median(t64)), 2),
+                 'ms_per_step': round(float(np.median(t64)) * 1e3, 3),
+                 'step_ms': [round(t * 1e3, 3) for t in t64]} if t64 else None,
         'fp32_rel_err': errs,
+        'device_resident': {
+            'value': round(pairs_per_step / float(np.median(tdev)), 2),
+            'ms_per_step': round(float(np.median(tdev)) * 1e3, 3),
+            'upload_ms': round(t_up * 1e3, 1)},
+        'fwd_bwd': fwd_bwd,
         'ms_breakdown_step': round(ms_breakdown, 3),
         'copy_calibration': {'bytes': 1 << 30, 'ms': round(cal_ms.value, 4),
                              'GBs': round(2 * (1 << 30) / cal_ms.value / 1e6, 1)},
-        'upload_ms': round(t_up * 1e3, 1),
-        'pcie_inclusive_value': round(
-            pairs_per_step / (ms_per_step / 1e3 + t_up), 2),
         'kernels': kernels,
     }
-    for h in handles:
-        lib.trvg_catalogue_free(h)
     return line
+
+
+def cpu_model():
+    try:
+        for ln in open('/proc/cpuinfo'):
+            if ln.startswith('model name'):
+                return ln.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_baseline(args, kbv, pairs_per_step):
+    """The oracle's reference-schedule per-pair loop body on host cores."""
+    import oracle
+    olib = oracle.load()
+    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    olib.trvo_set_threads(threads)
+    p = params_struct(args.ngrid, (0, 0, 0), 'full', 64)
+    sec = C.c_double()
+    log(f"CPU baseline sample: {args.cpu_pairs} pairs on {threads} threads")
+    if olib.trvo_bench_pairs(C.byref(p), C.byref(kbv.struct), args.cpu_pairs,
+                             C.byref(sec)) != 0:
+        return None
+    per_pair = sec.value / args.cpu_pairs
+    # Pair-loop work per step: 210 pairs x 1 term (B_000) + 400 pairs x 3
+    # terms (B_202); fixed per-term costs are not counted.
+    work = 210 * 1 + 400 * 3
+    return {
+        'value': round(pairs_per_step / (per_pair * work), 5),
+        'unit': 'bin-pair evals/s', 'cores': threads, 'kind': 'port',
+        'cpu_model': cpu_model(),
+        'sample': (f"reference-schedule per-pair loop body (2 shell IFFTs + "
+                   f"triple sum + shot-noise IFFT + spline pass), FP64, "
+                   f"{args.ngrid}^3, {args.cpu_pairs} pairs, {sec.value:.1f} "
+                   f"s; per-term assignment/FFT costs excluded (favours the "
+                   f"CPU)"),
+        'seconds_per_pair': round(per_pair, 3),
+    }
 
 
 def _check(lib, st):
@@ -342,6 +388,7 @@ def _check(lib, st):
 
 
 def _upload(lib, cat, los):
+    """Device-resident copy of a catalogue (engine handle)."""
     from triumvirate_amd import _abi
     x, y, z, nz, ws, wc = cat.columns()
     view = _abi.CatalogueView(x, y, z, nz, ws, wc)
@@ -351,21 +398,25 @@ def _upload(lib, cat, los):
     return h
 
 
-def _sync_time(fn, reps):
-    """Mean wall time of fn() over reps calls (each call is synchronous)."""
-    t = time.perf_counter()
+def _median_time(fn, reps):
+    """Median wall time of fn() over reps calls (each call is synchronous)
+    and the last output."""
+    ts = []
     for _ in range(reps):
+        t = time.perf_counter()
         out = fn()
-    return (time.perf_counter() - t) / reps, out
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), out
 
 
-def bench_grad(args, lib):
+def grad_timings(lib, steps, fd_particles=64):
     """Differentiable B_000 diag: 1e6 lognormal particles in a periodic
     L = 1000 Mpc/h box, CIC on a 256^3 FP32 mesh, 15 bins [0.01, 0.16];
     loss = sum_b w_b Re B_000(k_b, k_b), w_b ~ N(0, 1) (seed 5).  Times the
-    forward and forward+backward against the plain GPU FP64 forward of the
-    same statistic, and checks position gradients against central finite
-    differences (eps = 1e-3 Mpc/h, 64 particles, FP64 mesh)."""
+    forward and forward+backward (medians) against the plain GPU FP64
+    forward of the same statistic and, with fd_particles > 0, checks
+    position gradients against central finite differences (eps = 1e-3
+    Mpc/h, FP64 mesh)."""
     import torch
     from triumvirate_amd import _abi, mocks
     from triumvirate_amd import threept as T
@@ -405,7 +456,7 @@ def bench_grad(args, lib):
                                                 C.byref(kbv.struct), norm,
                                                 C.byref(res.struct)))
     plain64()
-    t_plain, _ = _sync_time(plain64, 3)
+    t_plain, _ = _median_time(plain64, max(5, steps))
     lib.trvg_catalogue_free(h)
 
     wb = torch.tensor(np.random.default_rng(5).standard_normal(15), device='cuda')
@@ -413,7 +464,9 @@ def bench_grad(args, lib):
 
     def fwd():
         with torch.no_grad():
-            return B32(pos)
+            out = B32(pos)
+        torch.cuda.synchronize()
+        return out
 
     def fwd_bwd():
         pp = pos.clone().requires_grad_(True)
@@ -423,53 +476,69 @@ def bench_grad(args, lib):
 
     fwd()
     fwd_bwd()
-    t_fwd, bk32 = _sync_time(fwd, args.steps)
-    t_fb, g32 = _sync_time(fwd_bwd, args.steps)
+    t_fwd, bk32 = _median_time(fwd, max(5, steps))
+    t_fb, g32 = _median_time(fwd_bwd, max(5, steps))
     bk32 = bk32.cpu().numpy()
     ref_re = ref['bk_raw'].real
     err_fwd = float(np.max(np.abs(bk32 - ref_re)) / np.max(np.abs(ref_re)))
 
-    # FP64 differentiable path: gradient reference and finite differences.
+    # FP64 differentiable path: the gradient reference.
     B64 = DifferentiableBispectrum(L, ng, kb, 'cic', 'fp64', norm)
     pp = pos.clone().requires_grad_(True)
     (B64(pp) * wb).sum().backward()
     g64 = pp.grad.cpu().numpy()
     err_g32 = float(np.max(np.abs(g32.cpu().numpy() - g64)) / np.max(np.abs(g64)))
-    rng = np.random.default_rng(6)
-    idx = rng.choice(pos.shape[0], 64, replace=False)
-    eps = 1e-3
-    fd_err = 0.
-    gscale = np.max(np.abs(g64[idx]))
-    t = time.perf_counter()
-    with torch.no_grad():
-        for i in idx:
-            for a in range(3):
-                q = pos.clone()
-                q[i, a] += eps
-                lp = float((B64(q) * wb).sum())
-                q[i, a] -= 2 * eps
-                lm = float((B64(q) * wb).sum())
-                fd = (lp - lm) / (2 * eps)
-                fd_err = max(fd_err, abs(fd - g64[i, a]) / gscale)
-    log(f"FD check {time.perf_counter() - t:.1f} s, max err {fd_err:.2e}")
-    return {
-        'metric': 'differentiable B_000 diag fwd+bwd time (1e6 particles, '
-                  '256^3 CIC, FP32 mesh)',
-        'value': round(t_fb * 1e3, 3), 'unit': 'ms', 'n_gpus': 1,
-        'steps': args.steps, 'warmup': 1, 'ms_per_step': round(t_fb * 1e3, 3),
-        'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'f32', 'data': 'synthetic (seeded lognormal box)',
-        'config': {'workload': 'periodic box, 1e6 lognormal particles (seed 4), '
-                               'L=1000, 256^3 CIC, 15 diag k-bins [0.01,0.16], '
-                               'loss = sum_b w_b Re B_000(k_b,k_b)',
-                   'n_particles': int(pos.shape[0])},
+    out = {
+        'workload': 'periodic box, 1e6 lognormal particles (seed 4), L=1000, '
+                    '256^3 CIC, FP32 mesh, 15 diag k-bins [0.01,0.16], loss = '
+                    'sum_b w_b Re B_000(k_b,k_b) (medians of the steps)',
+        'n_particles': int(pos.shape[0]),
+        'fwd_bwd_ms': round(t_fb * 1e3, 3),
         'forward_ms': round(t_fwd * 1e3, 3),
         'plain_fp64_forward_ms': round(t_plain * 1e3, 3),
         'fwd_bwd_over_plain_fp64': round(t_fb / t_plain, 3),
         'fp32_forward_rel_err_vs_fp64_plain': err_fwd,
         'fp32_grad_rel_err_vs_fp64': err_g32,
-        'fd_check': {'particles': 64, 'eps': eps, 'precision': 'fp64',
-                     'max_err_rel_to_max_grad': fd_err},
+    }
+    if fd_particles > 0:
+        rng = np.random.default_rng(6)
+        idx = rng.choice(pos.shape[0], fd_particles, replace=False)
+        eps = 1e-3
+        fd_err = 0.
+        gscale = np.max(np.abs(g64[idx]))
+        t = time.perf_counter()
+        with torch.no_grad():
+            for i in idx:
+                for a in range(3):
+                    q = pos.clone()
+                    q[i, a] += eps
+                    lp = float((B64(q) * wb).sum())
+                    q[i, a] -= 2 * eps
+                    lm = float((B64(q) * wb).sum())
+                    fd = (lp - lm) / (2 * eps)
+                    fd_err = max(fd_err, abs(fd - g64[i, a]) / gscale)
+        log(f"FD check {time.perf_counter() - t:.1f} s, max err {fd_err:.2e}")
+        out['fd_check'] = {'particles': fd_particles, 'eps': eps,
+                           'precision': 'fp64',
+                           'max_err_rel_to_max_grad': fd_err}
+    return out
+
+
+def bench_grad(args, lib):
+    """Differentiable B_000 (config of grad_timings) with the finite-
+    difference check on 64 particles."""
+    g = grad_timings(lib, args.steps, fd_particles=64)
+    return {
+        'metric': 'differentiable B_000 diag fwd+bwd time (1e6 particles, '
+                  '256^3 CIC, FP32 mesh)',
+        'value': g['fwd_bwd_ms'], 'unit': 'ms', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': 1, 'ms_per_step': g['fwd_bwd_ms'],
+        'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic (seeded lognormal box)',
+        'config': {'workload': g['workload'],
+                   'n_particles': g['n_particles']},
+        **{k: v for k, v in g.items() if k not in ('workload', 'n_particles',
+                                                   'fwd_bwd_ms')},
     }
 
 

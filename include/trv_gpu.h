@@ -216,8 +216,16 @@ int trvg_catalogue_upload(const trvg_catalogue* cat, const double* los,
                           int device, trvg_device_catalogue** out);
 int trvg_catalogue_free(trvg_device_catalogue* cat);
 
+/* As trvg_calc_bispec_normalisation_from_mesh on a device-resident
+ * catalogue (the survey drivers compute both normalisations and the
+ * estimator from one upload of the randoms). */
+int trvg_calc_bispec_normalisation_from_mesh_dev(
+  const trvg_device_catalogue* particles, const trvg_params* params,
+  double alpha, double* norm_factor);
+
 /* As trvg_compute_bispec / trvg_compute_bispec_in_gpp_box (rand == NULL)
- * on device-resident catalogues. */
+ * on device-resident catalogues.  A device catalogue must not be used by
+ * two calls at the same time (the engine reorders its columns). */
 int trvg_compute_bispec_dev(const trvg_device_catalogue* data,
                             const trvg_device_catalogue* rand,
                             const trvg_params* params,

@@ -110,3 +110,134 @@ def test_partial_batches_and_reordering():
     for i in range(5):
         assert _rel(a[i][0]['bk_raw'], b[4 - i][0]['bk_raw']) < 1e-12
         assert _rel(a[i][0]['bk_shot'], b[4 - i][0]['bk_shot']) < 1e-12
+
+
+def _oracle_case(xyz, deg, form, ng, binning):
+    """The oracle (deduplicated schedule) on one periodised catalogue."""
+    import oracle
+    from triumvirate_amd import threept as T
+    from triumvirate_amd.catalogue import ParticleCatalogue
+    from triumvirate_amd.parameters import ParameterSet
+    x, y, z = xyz
+    cat = ParticleCatalogue(x.copy(), y.copy(), z.copy())
+    ps = ParameterSet(param_dict={
+        'boxsize': {'x': L, 'y': L, 'z': L},
+        'ngrid': {'x': ng, 'y': ng, 'z': ng}, 'assignment': 'tsc',
+        'norm_convention': 'particle',
+        'degrees': {'ell1': None, 'ell2': None, 'ELL': 0}})
+    r = T._compute_3pt_stats_sim_like(oracle.backend(schedule=1), 'bispec', cat,
+                                      paramset=ps, degrees=deg, binning=binning,
+                                      form=form)
+    r.pop('_norm_factors', None)
+    return r
+
+
+@pytest.mark.parametrize('precision,tol', [('fp64', 1e-10), ('fp32', 1e-5)])
+def test_batch_matches_oracle(precision, tol):
+    """Every catalogue of a batched plan (uneven sizes, a partial last
+    batch) against the oracle: counts bit-exact, k_eff to 1e-12, B_000 and
+    B_202 within tol of the largest |B|."""
+    from triumvirate_amd.batch import BatchedBispectrum
+    print(f"batch vs oracle {precision}", flush=True)
+    cats = _mocks(5)
+    bb = BatchedBispectrum(L, NG, _binning(), MULTIPOLES, 'tsc', precision,
+                           batch=2)
+    got = bb.compute(cats)
+    for i, c in enumerate(cats):
+        for q, (deg, form) in enumerate(MULTIPOLES):
+            ref = _oracle_case(c, deg, form, NG, _binning())
+            res = got[i][q]
+            assert np.array_equal(res['nmodes_1'], ref['nmodes_1'])
+            assert np.array_equal(res['nmodes_2'], ref['nmodes_2'])
+            np.testing.assert_allclose(res['k1_eff'], ref['k1_eff'], rtol=1e-12)
+            np.testing.assert_allclose(res['k2_eff'], ref['k2_eff'], rtol=1e-12)
+            for key in ('bk_raw', 'bk_shot'):
+                err = _rel(res[key], ref[key])
+                print(f"  mock {i} {deg} {key} {err:.2e}", flush=True)
+                assert err < tol, (i, deg, key, err)
+
+
+@pytest.mark.parametrize('precision,tol', [('fp64', 1e-10), ('fp32', 1e-5)])
+def test_batch_matches_golden_cfg3(precision, tol):
+    """The batched plan on the golden periodic lognormal catalogue
+    (cfg3_gpp_b202_full: 32^3 TSC, B_202 full over 5 bins), placed in a
+    batch beside two other catalogues, reproduces the committed vector."""
+    import conftest as cf
+    import golden_cases as G
+    from triumvirate_amd import mocks
+    from triumvirate_amd.batch import BatchedBispectrum
+    from triumvirate_amd.dataobjs import Binning
+    print(f"batch golden cfg3 {precision}", flush=True)
+    ref = G.load_golden(cf.GOLDEN / 'oracle_golden.npz', 'cfg3_gpp_b202_full')
+    golden_cat = mocks.lognormal_box(30000, 1000., 32, seed=0)
+    others = [mocks.lognormal_box(20000, 1000., 32, seed=s) for s in (7, 8)]
+    kb = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.16, num_bins=5)
+    bb = BatchedBispectrum(L, 32, kb, [((2, 0, 2), 'full')], 'tsc', precision,
+                           batch=3)
+    res = bb.compute([others[0], golden_cat, others[1]])[1][0]
+    for k in ('nmodes_1', 'nmodes_2'):
+        assert np.array_equal(res[k], ref[k])
+    for k in ('k1_eff', 'k2_eff', 'k1_bin', 'k2_bin'):
+        np.testing.assert_allclose(res[k], ref[k], rtol=1e-12)
+    for key in ('bk_raw', 'bk_shot'):
+        err = _rel(res[key], ref[key])
+        print(f"  {key} {err:.2e}", flush=True)
+        assert err < tol, (key, err)
+
+
+def test_batch_128_property():
+    """Config-3 shape at reduced count: 16 periodic lognormal mocks of
+    Poisson-mean 3e5 particles at 128^3 (TSC, B_000 + B_202 full, 15 bins
+    [0.01, 0.16]) in batches of 8.  Mode counts equal a brute-force
+    enumeration of the k-grid; three sampled mocks equal FP64 single
+    engine calls within 1e-5 (FP32 fields)."""
+    from triumvirate_amd import mocks
+    from triumvirate_amd.batch import BatchedBispectrum
+    from triumvirate_amd.dataobjs import Binning
+    print("batch 128^3 x 16", flush=True)
+    ng = 128
+    rng = np.random.default_rng(11)
+    cats = [mocks.lognormal_box(int(rng.poisson(300000)), L, ng, seed=s)
+            for s in range(16)]
+    kb = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.16, num_bins=15)
+    bb = BatchedBispectrum(L, ng, kb, MULTIPOLES, 'tsc', 'fp32', batch=8)
+    got = bb.compute(cats)
+    # Brute-force mode counts: k_lo <= |k| < k_hi on the FFT grid.
+    f = np.fft.fftfreq(ng, d=1. / ng) * (2 * np.pi / L)
+    k2 = f[:, None, None] ** 2 + f[None, :, None] ** 2 + f[None, None, :] ** 2
+    kk = np.sqrt(k2).ravel()
+    edges = kb.bin_edges
+    counts = np.array([np.count_nonzero((kk >= edges[i]) & (kk < edges[i + 1]))
+                       for i in range(15)])
+    for i in range(16):
+        res = got[i][1]                  # B_202 keeps the full form
+        n1 = res['nmodes_1'].reshape(15, 15)
+        n2 = res['nmodes_2'].reshape(15, 15)
+        assert np.array_equal(n1[:, 0], counts), (i, n1[:, 0], counts)
+        assert np.array_equal(n2[0, :], counts), (i, n2[0, :], counts)
+    for i in (0, 7, 13):
+        for q, (deg, form) in enumerate(MULTIPOLES):
+            ref = _single_at(cats[i], deg, form, ng, kb)
+            for key in ('bk_raw', 'bk_shot'):
+                err = _rel(got[i][q][key], ref[key])
+                print(f"  mock {i} {deg} {key} {err:.2e}", flush=True)
+                assert err < 1e-5, (i, deg, key, err)
+
+
+def _single_at(xyz, deg, form, ng, binning):
+    from triumvirate_amd import _lib
+    from triumvirate_amd import threept as T
+    from triumvirate_amd.catalogue import ParticleCatalogue
+    from triumvirate_amd.parameters import ParameterSet
+    x, y, z = xyz
+    cat = ParticleCatalogue(x.copy(), y.copy(), z.copy())
+    ps = ParameterSet(param_dict={
+        'boxsize': {'x': L, 'y': L, 'z': L},
+        'ngrid': {'x': ng, 'y': ng, 'z': ng}, 'assignment': 'tsc',
+        'norm_convention': 'particle',
+        'degrees': {'ell1': None, 'ell2': None, 'ELL': 0}})
+    r = T._compute_3pt_stats_sim_like(T.CLibBackend(_lib.load()), 'bispec', cat,
+                                      paramset=ps, degrees=deg, binning=binning,
+                                      form=form, precision='fp64')
+    r.pop('_norm_factors', None)
+    return r

@@ -199,6 +199,9 @@ def declare_extensions(lib):
     lib.trvg_catalogue_upload.restype = C.c_int
     lib.trvg_catalogue_free.argtypes = [vp]
     lib.trvg_catalogue_free.restype = C.c_int
+    lib.trvg_calc_bispec_normalisation_from_mesh_dev.argtypes = [
+        vp, P(Params), C.c_double, _dp]
+    lib.trvg_calc_bispec_normalisation_from_mesh_dev.restype = C.c_int
     lib.trvg_compute_bispec_dev.argtypes = [vp, vp, P(Params), P(Binning),
                                             C.c_double, P(BispecResult)]
     lib.trvg_compute_bispec_dev.restype = C.c_int

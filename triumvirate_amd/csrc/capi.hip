@@ -276,10 +276,18 @@ int trvg_catalogue_upload(const trvg_catalogue* c, const double* los,
     if (out == nullptr) throw Error(TRVG_ERR_PARAM, "Null output handle.");
     if (device >= 0) TRVG_HIP(hipSetDevice(device));
     auto h = new trvg_device_catalogue();
+    hipStream_t s = nullptr;
     try {
       TRVG_HIP(hipGetDevice(&h->device));
-      upload_catalogue(c, los, h->cat, nullptr);
+      s = acquire_stream(h->device);
+      upload_catalogue(c, los, h->cat, s);
+      TRVG_HIP(hipStreamSynchronize(s));
+      release_stream(h->device, s);
     } catch (...) {
+      if (s) {
+        (void)hipStreamSynchronize(s);
+        release_stream(h->device, s);
+      }
       delete h;
       throw;
     }
@@ -289,6 +297,21 @@ int trvg_catalogue_upload(const trvg_catalogue* c, const double* los,
 
 int trvg_catalogue_free(trvg_device_catalogue* h) {
   return guarded([&] { delete h; });
+}
+
+int trvg_calc_bispec_normalisation_from_mesh_dev(const trvg_device_catalogue* cat,
+                                                 const trvg_params* params,
+                                                 double alpha, double* norm_factor) {
+  return guarded([&] {
+    require_device();
+    if (cat == nullptr) throw Error(TRVG_ERR_DATA, "Null catalogue.");
+    trvg_params q = mesh_params(params);
+    if (q.device < 0) q.device = cat->device;
+    Context ctx(q);
+    double nf = (q.precision == TRVG_PREC_FP32) ? run_mesh_norm<float>(ctx, cat->cat)
+                                                : run_mesh_norm<double>(ctx, cat->cat);
+    *norm_factor = nf / std::pow(alpha, 3);
+  });
 }
 
 int trvg_compute_bispec_dev(const trvg_device_catalogue* data,

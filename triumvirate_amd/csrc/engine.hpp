@@ -73,27 +73,35 @@ class DevArray {
 // Device catalogue (owned copy of the columns the kernels read).
 // ---------------------------------------------------------------------------
 
-// Tile-sorted copy of a catalogue for one mesh geometry (see
-// k_assign_tiled): columns in tile order, the chunked work list, and the
-// trailing range of particles outside the mesh.
+// Tile order of a catalogue for one mesh geometry (see k_assign_tiled):
+// the catalogue's columns are permuted in place into tile order, particles
+// inside the mesh first, then the stragglers (positions outside [0, L)).
+// The chunked work list and its length are built on the device, so the
+// sort needs no host round trip; `meta` holds {work items, particles
+// inside the mesh}.  The host learns both after the first call's
+// synchronisation and then launches exact grids.
 struct TiledView {
   int n[3] = {0, 0, 0};
   double L[3] = {0., 0., 0.};
   TileGeom tg;
-  DevArray<double> x, y, z, w, los;
   DevArray<TileWork> work;
-  int nwork = 0;
-  long long nin = 0;        // particles inside the mesh (tile order)
-  long long nstraggle = 0;  // particles [nin, n) outside it
+  DevArray<unsigned int> meta;
+  long long work_cap = 0;   // bound on the work items (grid of the first calls)
+  long long nwork = -1;     // exact count once known on the host, else -1
+  long long nstraggle = -1; // likewise
   bool matches(const Geom& g) const {
     return n[0] == g.n[0] && n[1] == g.n[1] && n[2] == g.n[2] &&
            L[0] == g.L[0] && L[1] == g.L[1] && L[2] == g.L[2];
   }
 };
 
+// Device copy of the columns the kernels read.  The column order is the
+// engine's (tile order of the last mesh geometry used), not the caller's:
+// every use is a sum over particles.  A catalogue is not to be used by two
+// calls at once.
 struct DeviceCatalogue {
   long long n = 0;
-  DevArray<double> x, y, z, w, los;
+  mutable DevArray<double> x, y, z, w, los;
   double wstotal = 0., wtotal = 0.;
   double wuni = NAN;   // the weight ws wc of every particle, NaN if they differ
   mutable std::unique_ptr<TiledView> tiled;  // built on first use
@@ -106,7 +114,13 @@ struct DeviceCatalogue {
   }
 };
 
-// Upload a (possibly strided) host catalogue; `los` optional (n x 3).
+// Streams of the engine (pooled per device, default priority).
+hipStream_t acquire_stream(int dev);
+void release_stream(int dev, hipStream_t s);
+
+// Upload a (possibly strided) host catalogue; `los` optional (n x 3).  The
+// copies are queued on stream s (the host buffers may be reused on
+// return; the device columns are valid in stream order).
 void upload_catalogue(const trvg_catalogue* c, const double* los,
                       DeviceCatalogue& out, hipStream_t s);
 
@@ -223,8 +237,17 @@ struct Context {
   // are still being copied out of: both released by flush().
   std::vector<std::function<void()>> deferred;
   std::vector<std::shared_ptr<void>> keep;
+  // False for a context on borrowed resources (stream, axis tables,
+  // twiddles): it takes nothing from the engine's pools and records no
+  // profiler events.
+  bool pooled = true;
+  // Borrowed twiddle tables by sign (-1: [0], +1: [1]) and log2 length;
+  // null entries use the engine's cached tables.
+  const void* tw[2][11] = {};
 
   explicit Context(const trvg_params& params);
+  // A context on borrowed resources; allocates nothing.
+  Context(const trvg_params& params, hipStream_t s, const AxisTables& t);
   ~Context();
   // Resolve pending profiler records (synchronises the stream).
   void flush_profile();

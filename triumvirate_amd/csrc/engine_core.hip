@@ -60,52 +60,6 @@ void Pool::release_all() {
 }
 
 // ---------------------------------------------------------------------------
-// Catalogue upload
-// ---------------------------------------------------------------------------
-
-void upload_catalogue(const trvg_catalogue* c, const double* los,
-                      DeviceCatalogue& out, hipStream_t s) {
-  if (c == nullptr || c->x == nullptr || c->y == nullptr || c->z == nullptr)
-    throw Error(TRVG_ERR_DATA, "Particle data are uninitialised.");
-  if (c->ntotal <= 0)
-    throw Error(TRVG_ERR_DATA, "Particle catalogue is empty.");
-  const long long n = c->ntotal;
-  const long long st = c->stride > 0 ? c->stride : 1;
-  std::vector<double> hx(n), hy(n), hz(n), hw(n);
-  double wst = 0., wt = 0.;
-  for (long long i = 0; i < n; i++) {
-    hx[i] = c->x[i * st];
-    hy[i] = c->y[i * st];
-    hz[i] = c->z[i * st];
-    double ws = c->ws ? c->ws[i * st] : 1.;
-    double wc = c->wc ? c->wc[i * st] : 1.;
-    hw[i] = ws * wc;   // particles.cpp:553
-    wst += ws;
-    wt += hw[i];
-  }
-  out.n = n;
-  out.wstotal = wst;
-  out.wtotal = wt;
-  out.wuni = hw[0];
-  for (long long i = 1; i < n; i++)
-    if (hw[i] != hw[0]) {
-      out.wuni = NAN;
-      break;
-    }
-  out.x.reset(n); out.y.reset(n); out.z.reset(n); out.w.reset(n);
-  TRVG_HIP(hipMemcpyAsync(out.x.get(), hx.data(), n * 8, hipMemcpyHostToDevice, s));
-  TRVG_HIP(hipMemcpyAsync(out.y.get(), hy.data(), n * 8, hipMemcpyHostToDevice, s));
-  TRVG_HIP(hipMemcpyAsync(out.z.get(), hz.data(), n * 8, hipMemcpyHostToDevice, s));
-  TRVG_HIP(hipMemcpyAsync(out.w.get(), hw.data(), n * 8, hipMemcpyHostToDevice, s));
-  if (los) {
-    out.los.reset(3 * n);
-    TRVG_HIP(hipMemcpyAsync(out.los.get(), los, 3 * n * 8,
-                            hipMemcpyHostToDevice, s));
-  }
-  TRVG_HIP(hipStreamSynchronize(s));
-}
-
-// ---------------------------------------------------------------------------
 // Spherical Bessel spline (maths.cpp:309-375): nodes x_i = 0.05 i up to
 // split = max(1000, l^2); natural boundary; second-derivative/2
 // coefficients from the tridiagonal system (Thomas algorithm).
@@ -203,7 +157,7 @@ void build_spline(int ell, Spline& sp, hipStream_t s) {
 static std::mutex g_stream_mu;
 static std::map<int, std::vector<hipStream_t>> g_free_streams;
 
-static hipStream_t acquire_stream(int dev) {
+hipStream_t acquire_stream(int dev) {
   {
     std::lock_guard<std::mutex> lk(g_stream_mu);
     auto& v = g_free_streams[dev];
@@ -218,7 +172,7 @@ static hipStream_t acquire_stream(int dev) {
   return s;
 }
 
-static void release_stream(int dev, hipStream_t s) {
+void release_stream(int dev, hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_stream_mu);
   g_free_streams[dev].push_back(s);
 }
@@ -324,7 +278,20 @@ Context::Context(const trvg_params& params) : p(params) {
   tab.interlace = p.interlace;
 }
 
+Context::Context(const trvg_params& params, hipStream_t s, const AxisTables& t)
+    : p(params), stream(s), tab(t), pooled(false) {
+  g = make_geom(p);
+  TRVG_HIP(hipGetDevice(&dev));
+  TRVG_HIP(hipDeviceGetAttribute(&sm_count, hipDeviceAttributeMultiprocessorCount, dev));
+  tab.order = p.assignment_order;
+  tab.interlace = p.interlace;
+}
+
 Context::~Context() {
+  if (!pooled) {
+    (void)hipStreamSynchronize(stream);
+    return;
+  }
   if (stream) {
     (void)hipStreamSynchronize(stream);
     keep.clear();
@@ -392,7 +359,7 @@ void Profiler::read(int kc, double* ms, long long* n, double* bytes) {
 
 ProfScope::ProfScope(Context& ctx, int kc, double bytes)
     : ctx_(ctx), kc_(kc), bytes_(bytes) {
-  if (!Profiler::get().enabled_for(kc)) return;
+  if (!ctx.pooled || !Profiler::get().enabled_for(kc)) return;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(ctx_.stream, &cs) != hipSuccess ||
       cs != hipStreamCaptureStatusNone)

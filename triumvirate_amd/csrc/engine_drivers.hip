@@ -29,6 +29,19 @@ namespace trvg {
 // ---------------------------------------------------------------------------
 
 template <typename T>
+static const c2<T>* twiddles(int n, int sign);
+
+// Twiddles of a transform in context ctx: borrowed tables where the context
+// has them, else the engine's cache.
+template <typename T>
+static const c2<T>* ctx_twiddles(const Context& ctx, int n, int sign) {
+  int lg = 0;
+  while ((1 << lg) < n) lg++;
+  const void* b = lg < 11 ? ctx.tw[sign > 0 ? 1 : 0][lg] : nullptr;
+  return b ? static_cast<const c2<T>*>(b) : twiddles<T>(n, sign);
+}
+
+template <typename T>
 static const c2<T>* twiddles(int n, int sign) {
   static std::mutex mu;
   static std::map<std::tuple<int, int, int>, c2<T>*> cache;
@@ -81,7 +94,7 @@ static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign,
   constexpr int E = FFT_THREADS * EPT;
   const long long nblocks = ctx.g.nmesh / E;
   const size_t smem = fft_smem<T, LOGN>(false);
-  const c2<T>* tw = twiddles<T>(N, sign);
+  const c2<T>* tw = ctx_twiddles<T>(ctx, N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, read_bytes + write_bytes);
   if constexpr (use_zreg<T, LOGN, Loader>()) {
     static_assert(E == zreg_block<T>(), "z pass block size");
@@ -108,7 +121,7 @@ static void launch_strided(Context& ctx, c2<T>* data, long long S,
   constexpr int E = FFT_THREADS * EPT;
   const long long nblocks = ctx.g.nmesh / E;
   const size_t smem = fft_smem<T, LOGN>(true);
-  const c2<T>* tw = twiddles<T>(N, sign);
+  const c2<T>* tw = ctx_twiddles<T>(ctx, N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
   if (inner % (E / N) == 0)
     hipLaunchKernelGGL((fft_pass_strided<T, LOGN, EPT, true>), dim3(unsigned(nblocks)),
@@ -217,7 +230,7 @@ static void launch_z_batch(Context& ctx, c2<T>* base, const BatchLoader& load,
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   const size_t smem = fft_smem<T, LOGN>(false);
-  const c2<T>* tw = twiddles<T>(N, sign);
+  const c2<T>* tw = ctx_twiddles<T>(ctx, N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, bytes);
   if constexpr (use_zreg<T, LOGN, BatchLoader>()) {
     static_assert(E == zreg_block<T>(), "z pass block size");
@@ -245,7 +258,7 @@ static void launch_strided_batch(Context& ctx, c2<T>* base, long long S,
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   const size_t smem = fft_smem<T, LOGN>(true);
-  const c2<T>* tw = twiddles<T>(N, sign);
+  const c2<T>* tw = ctx_twiddles<T>(ctx, N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
   if (inner % (E / N) == 0)
     hipLaunchKernelGGL((fft_pass_strided_batch<T, LOGN, EPT, true>), dim3(unsigned(nwork)),
@@ -421,7 +434,7 @@ static void launch_x_load_batch(Context& ctx, c2<T>* base, const BatchLoader& lo
   constexpr int N = 1 << LOGN;
   constexpr int E = FFT_THREADS * EPT;
   const size_t smem = fft_smem<T, LOGN>(false);
-  const c2<T>* tw = twiddles<T>(N, +1);
+  const c2<T>* tw = ctx_twiddles<T>(ctx, N, +1);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
   if (ctx.g.n[2] % (E / N) == 0)
     hipLaunchKernelGGL((fft_pass_x_load_batch<T, LOGN, EPT, true, BatchLoader>),
@@ -596,11 +609,11 @@ static void launch_x_sjl(Context& ctx, c2<T>* base, int nt, const double2* src,
   if (packed)
     hipLaunchKernelGGL((fft_x_sjl_multi<T, LOGN, EPT, true>), dim3(grid), dim3(FFT_THREADS),
                        smem, ctx.stream, base, g.nmesh, nt, src, ctx.tab, g, sp.dev(), r, ell,
-                       m, twiddles<T>(N, +1));
+                       m, ctx_twiddles<T>(ctx, N, +1));
   else
     hipLaunchKernelGGL((fft_x_sjl_multi<T, LOGN, EPT, false>), dim3(grid), dim3(FFT_THREADS),
                        smem, ctx.stream, base, g.nmesh, nt, src, ctx.tab, g, sp.dev(), r, ell,
-                       m, twiddles<T>(N, +1));
+                       m, ctx_twiddles<T>(ctx, N, +1));
   TRVG_LAUNCH_CHECK();
 }
 
@@ -723,8 +736,10 @@ static void zero_field(Context& ctx, c2<T>* f) {
   TRVG_LAUNCH_CHECK();
 }
 
-// Tile-sorted view of `c` for the context's mesh (built once, cached on the
-// catalogue).
+// Tile order of `c` for the context's mesh (built once per geometry and
+// cached on the catalogue): a counting sort by tile with the within-tile
+// bank-class order, the columns permuted in place, and the work list built
+// on the device.  Nothing here waits for the device.
 static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
   if (c.tiled && c.tiled->matches(ctx.g)) return *c.tiled;
   auto tv = std::make_unique<TiledView>();
@@ -738,7 +753,7 @@ static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
   const long long nt = tv->tg.ntiles;
   DevArray<int> tile_of(c.n);
   DevArray<unsigned int> counts(nt + 1), offs(nt + 2), ccount((nt + 1) * TILE_CLASSES),
-      key2(c.n);
+      key2(c.n), nch(nt), woff(nt + 1);
   TRVG_HIP(hipMemsetAsync(counts.get(), 0, (nt + 1) * 4, ctx.stream));
   TRVG_HIP(hipMemsetAsync(ccount.get(), 0, (nt + 1) * TILE_CLASSES * 4, ctx.stream));
   DevParticles pv = c.view();
@@ -752,60 +767,72 @@ static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx.stream,
                      counts.get(), nt + 1, offs.get());
   TRVG_LAUNCH_CHECK();
-  std::vector<unsigned int> hoff(nt + 2);
-  TRVG_HIP(hipMemcpyAsync(hoff.data(), offs.get(), (nt + 2) * 4,
-                          hipMemcpyDeviceToHost, ctx.stream));
-  tv->x.reset(c.n); tv->y.reset(c.n); tv->z.reset(c.n); tv->w.reset(c.n);
-  if (c.los.get()) tv->los.reset(3 * c.n);
+  DevArray<double> sx(c.n), sy(c.n), sz(c.n), sw(c.n), slos(c.los.get() ? 3 * c.n : 0);
   {
     ProfScope ps(ctx, KC_ASSIGN, c.n * (c.los.get() ? 116. : 68.));
     hipLaunchKernelGGL(k_tile_scatter, dim3(grid), dim3(256), 0, ctx.stream, pv,
-                       tile_of.get(), offs.get(), ccount.get(), key2.get(), tv->x.get(),
-                       tv->y.get(), tv->z.get(), tv->w.get(), tv->los.get());
+                       tile_of.get(), offs.get(), ccount.get(), key2.get(), sx.get(),
+                       sy.get(), sz.get(), sw.get(), slos.get());
     TRVG_LAUNCH_CHECK();
   }
-  TRVG_HIP(hipStreamSynchronize(ctx.stream));
-  // Chunked work list (host), one entry per <= CH particles of a tile.
-  const int CH = 2048;
-  std::vector<TileWork> work;
-  for (long long t = 0; t < nt; t++) {
-    long long b = hoff[t], e = hoff[t + 1];
-    for (long long s0 = b; s0 < e; s0 += CH) {
-      work.push_back(TileWork{int(t), int(std::min<long long>(CH, e - s0)), s0});
-    }
-  }
-  tv->nin = hoff[nt];
-  tv->nstraggle = c.n - tv->nin;
-  tv->nwork = int(work.size());
-  if (!work.empty()) {
-    tv->work.reset(work.size());
-    ctx.upload(tv->work.get(), work.data(), work.size());
-  }
+  // Work list: chunks of at most TILE_CHUNK particles per tile.
+  tv->work_cap = nt + (c.n + TILE_CHUNK - 1) / TILE_CHUNK;
+  tv->work.reset(tv->work_cap);
+  tv->meta.reset(2);
+  const unsigned tgrid = grid_for(ctx, nt);
+  hipLaunchKernelGGL(k_tile_chunks, dim3(tgrid), dim3(256), 0, ctx.stream, offs.get(),
+                     nt, nch.get());
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx.stream, nch.get(), nt,
+                     woff.get());
+  hipLaunchKernelGGL(k_tile_work, dim3(tgrid), dim3(256), 0, ctx.stream, offs.get(),
+                     woff.get(), nt, tv->work.get(), tv->meta.get());
+  TRVG_LAUNCH_CHECK();
+  // The permuted columns replace the caller-order ones; the old columns and
+  // the sort's temporaries stay allocated until the stream has passed them.
+  std::swap(c.x, sx); std::swap(c.y, sy); std::swap(c.z, sz); std::swap(c.w, sw);
+  if (c.los.get()) std::swap(c.los, slos);
+  ctx.keep_until_flush(std::move(sx)); ctx.keep_until_flush(std::move(sy));
+  ctx.keep_until_flush(std::move(sz)); ctx.keep_until_flush(std::move(sw));
+  ctx.keep_until_flush(std::move(slos));
+  ctx.keep_until_flush(std::move(tile_of)); ctx.keep_until_flush(std::move(counts));
+  ctx.keep_until_flush(std::move(offs)); ctx.keep_until_flush(std::move(ccount));
+  ctx.keep_until_flush(std::move(key2)); ctx.keep_until_flush(std::move(nch));
+  ctx.keep_until_flush(std::move(woff));
+  // Exact grids for later calls once the host has the counts.
+  const unsigned int* hm = ctx.download(tv->meta.get(), 2);
+  TiledView* tvp = tv.get();
+  const long long n = c.n;
+  ctx.defer([tvp, hm, n] {
+    tvp->nwork = hm[0];
+    tvp->nstraggle = n - (long long)hm[1];
+  });
   c.tiled = std::move(tv);
   return *c.tiled;
 }
 
+// Grid of the tiled kernels: the exact work count once known, else the bound.
+static unsigned tile_grid(const TiledView& tv) {
+  return unsigned(tv.nwork >= 0 ? tv.nwork : tv.work_cap);
+}
+
 template <typename T, int ORDER>
-static void assign_order(Context& ctx, c2<T>* mesh, const TiledView& tv,
-                         int mode, int ell, int m, double scale, int cplx_w) {
-  DevParticles sv;
-  sv.x = tv.x.get(); sv.y = tv.y.get(); sv.z = tv.z.get(); sv.w = tv.w.get();
-  sv.los = tv.los.get();
-  sv.n = tv.nin + tv.nstraggle;
-  if (tv.nwork > 0) {
-    hipLaunchKernelGGL((k_assign_tiled<T, ORDER>), dim3(tv.nwork), dim3(256), 0,
-                       ctx.stream, mesh, sv, ctx.g, tv.tg, tv.work.get(), mode,
-                       ell, m, scale, cplx_w);
+static void assign_order(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c,
+                         const TiledView& tv, int mode, int ell, int m, double scale,
+                         int cplx_w) {
+  DevParticles sv = c.view();
+  if (tile_grid(tv) > 0) {
+    hipLaunchKernelGGL((k_assign_tiled<T, ORDER>), dim3(tile_grid(tv)), dim3(256), 0,
+                       ctx.stream, mesh, sv, ctx.g, tv.tg, tv.work.get(), tv.meta.get(),
+                       mode, ell, m, scale, cplx_w);
     TRVG_LAUNCH_CHECK();
   }
-  if (tv.nstraggle > 0) {
-    DevParticles st = sv;
-    st.x += tv.nin; st.y += tv.nin; st.z += tv.nin; st.w += tv.nin;
-    if (st.los) st.los += 3 * tv.nin;
-    st.n = tv.nstraggle;
-    hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(grid_for(ctx, st.n, 16)),
-                       dim3(256), 0, ctx.stream, mesh, st, ctx.g, mode, ell, m,
-                       scale, cplx_w, 0);
+  // Stragglers [meta[1], n) keep the reference's flattened-index rule.
+  if (tv.nstraggle != 0) {
+    hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(tv.nstraggle > 0
+                                                      ? grid_for(ctx, tv.nstraggle, 16)
+                                                      : 64u),
+                       dim3(256), 0, ctx.stream, mesh, sv, ctx.g, mode, ell, m, scale,
+                       cplx_w, 0, tv.meta.get() + 1);
     TRVG_LAUNCH_CHECK();
   }
 }
@@ -831,47 +858,42 @@ static void assign(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c,
                   ((mode == WM_YLM_W || mode == WM_YLMC_W2) ? 24. : 0.);
   ProfScope ps(ctx, KC_ASSIGN, c.n * pbytes + double(ctx.g.nmesh) * sizeof(c2<T>));
   switch (ctx.p.assignment_order) {
-    case 1: assign_order<T, 1>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
-    case 2: assign_order<T, 2>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
-    case 3: assign_order<T, 3>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
-    default: assign_order<T, 4>(ctx, mesh, tv, mode, ell, m, scale, cplx_w); break;
+    case 1: assign_order<T, 1>(ctx, mesh, c, tv, mode, ell, m, scale, cplx_w); break;
+    case 2: assign_order<T, 2>(ctx, mesh, c, tv, mode, ell, m, scale, cplx_w); break;
+    case 3: assign_order<T, 3>(ctx, mesh, c, tv, mode, ell, m, scale, cplx_w); break;
+    default: assign_order<T, 4>(ctx, mesh, c, tv, mode, ell, m, scale, cplx_w); break;
   }
 }
 
 template <typename T, int ORDER>
-static void assign_dual_order(Context& ctx, T* ra, T* rb, const TiledView& tv,
-                              int ell, int m, double scale_a, double scale_b) {
-  DevParticles sv;
-  sv.x = tv.x.get(); sv.y = tv.y.get(); sv.z = tv.z.get(); sv.w = tv.w.get();
-  sv.los = tv.los.get();
-  sv.n = tv.nin + tv.nstraggle;
+static void assign_dual_order(Context& ctx, T* ra, T* rb, const DeviceCatalogue& c,
+                              const TiledView& tv, int ell, int m, double scale_a,
+                              double scale_b) {
+  DevParticles sv = c.view();
   const bool cplx = m != 0;
-  if (tv.nwork > 0) {
+  if (tile_grid(tv) > 0) {
     const size_t smem = size_t(cplx ? 4 : 2) * TILE_CELLS * sizeof(double);
     if (cplx)
-      hipLaunchKernelGGL((k_assign_tiled_dual<T, ORDER, true>), dim3(tv.nwork), dim3(256),
-                         smem, ctx.stream, ra, rb, sv, ctx.g, tv.tg, tv.work.get(), ell, m,
-                         scale_a, scale_b);
+      hipLaunchKernelGGL((k_assign_tiled_dual<T, ORDER, true>), dim3(tile_grid(tv)),
+                         dim3(256), smem, ctx.stream, ra, rb, sv, ctx.g, tv.tg,
+                         tv.work.get(), tv.meta.get(), ell, m, scale_a, scale_b);
     else
-      hipLaunchKernelGGL((k_assign_tiled_dual<T, ORDER, false>), dim3(tv.nwork), dim3(256),
-                         smem, ctx.stream, ra, rb, sv, ctx.g, tv.tg, tv.work.get(), ell, m,
-                         scale_a, scale_b);
+      hipLaunchKernelGGL((k_assign_tiled_dual<T, ORDER, false>), dim3(tile_grid(tv)),
+                         dim3(256), smem, ctx.stream, ra, rb, sv, ctx.g, tv.tg,
+                         tv.work.get(), tv.meta.get(), ell, m, scale_a, scale_b);
     TRVG_LAUNCH_CHECK();
   }
-  if (tv.nstraggle > 0) {
-    DevParticles st = sv;
-    st.x += tv.nin; st.y += tv.nin; st.z += tv.nin; st.w += tv.nin;
-    if (st.los) st.los += 3 * tv.nin;
-    st.n = tv.nstraggle;
+  if (tv.nstraggle != 0) {
     // Real outputs sharing one mesh (rb = ra + 1): B goes to imaginary parts.
     const bool shared = rb == ra + 1;
     c2<T>* ma = reinterpret_cast<c2<T>*>(ra);
     c2<T>* mb = shared ? ma : reinterpret_cast<c2<T>*>(rb);
     const int cwa = cplx ? 1 : 0, cwb = cplx ? 1 : (shared ? 2 : 0);
-    hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(grid_for(ctx, st.n, 16)), dim3(256), 0,
-                       ctx.stream, ma, st, ctx.g, WM_YLM_W, ell, m, scale_a, cwa, 0);
-    hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(grid_for(ctx, st.n, 16)), dim3(256), 0,
-                       ctx.stream, mb, st, ctx.g, WM_YLMC_W2, ell, m, scale_b, cwb, 0);
+    const unsigned sg = tv.nstraggle > 0 ? grid_for(ctx, tv.nstraggle, 16) : 64u;
+    hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(sg), dim3(256), 0, ctx.stream, ma, sv,
+                       ctx.g, WM_YLM_W, ell, m, scale_a, cwa, 0, tv.meta.get() + 1);
+    hipLaunchKernelGGL((k_assign<T, ORDER>), dim3(sg), dim3(256), 0, ctx.stream, mb, sv,
+                       ctx.g, WM_YLMC_W2, ell, m, scale_b, cwb, 0, tv.meta.get() + 1);
     TRVG_LAUNCH_CHECK();
   }
 }
@@ -890,10 +912,10 @@ static void assign_dual(Context& ctx, T* ra, T* rb, const DeviceCatalogue& c, in
   const double outb = (m != 0 ? 2. : 1.) * double(ctx.g.nmesh) * sizeof(c2<T>);
   ProfScope ps(ctx, KC_ASSIGN, c.n * 56. + outb);
   switch (ctx.p.assignment_order) {
-    case 1: assign_dual_order<T, 1>(ctx, ra, rb, tv, ell, m, scale_a, scale_b); break;
-    case 2: assign_dual_order<T, 2>(ctx, ra, rb, tv, ell, m, scale_a, scale_b); break;
-    case 3: assign_dual_order<T, 3>(ctx, ra, rb, tv, ell, m, scale_a, scale_b); break;
-    default: assign_dual_order<T, 4>(ctx, ra, rb, tv, ell, m, scale_a, scale_b); break;
+    case 1: assign_dual_order<T, 1>(ctx, ra, rb, c, tv, ell, m, scale_a, scale_b); break;
+    case 2: assign_dual_order<T, 2>(ctx, ra, rb, c, tv, ell, m, scale_a, scale_b); break;
+    case 3: assign_dual_order<T, 3>(ctx, ra, rb, c, tv, ell, m, scale_a, scale_b); break;
+    default: assign_dual_order<T, 4>(ctx, ra, rb, c, tv, ell, m, scale_a, scale_b); break;
   }
 }
 

@@ -49,14 +49,17 @@ __device__ __forceinline__ double2 particle_weight(const DevParticles& p,
 // Mesh assignment (field.cpp:569-1112): window weights and periodic index
 // rules per scheme; cell value += w * W_x W_y W_z / V_cell (times `scale`,
 // e.g. -alpha for randoms).  `shadow` selects the half-cell-shifted pass
-// used for interlacing.  Hardware FP64/FP32 atomics on the coarse-grained
+// used for interlacing.  Particles [*first, n) (all of them when `first` is
+// null) are assigned.  Hardware FP64/FP32 atomics on the coarse-grained
 // mesh.
 template <typename T, int ORDER>
 __global__ void __launch_bounds__(256)
 k_assign(c2<T>* __restrict__ mesh, DevParticles p, Geom g, int mode, int ell,
-         int m, double scale, int cplx_w, int shadow) {
+         int m, double scale, int cplx_w, int shadow,
+         const unsigned int* __restrict__ first) {
   const double inv_vol_cell = 1 / g.vol_cell;
-  for (long long pid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long p0 = first ? (long long)*first : 0;
+  for (long long pid = p0 + blockIdx.x * (long long)blockDim.x + threadIdx.x;
        pid < p.n; pid += (long long)gridDim.x * blockDim.x) {
     const double pos[3] = {p.x[pid], p.y[pid], p.z[pid]};
     int ijk[ORDER][3];
@@ -893,31 +896,33 @@ k_tile_count(DevParticles p, Geom g, TileGeom tg, int* __restrict__ tile_of,
   }
 }
 
-// Exclusive scan of n counts by one workgroup (n up to a few 1e6).
+// Exclusive scan of n counts by one workgroup (offsets[n] = total): each
+// thread sums a contiguous segment, the 1024 segment sums are scanned in
+// LDS, and each thread writes its segment's offsets.
 static __global__ void __launch_bounds__(1024)
 k_scan_counts(const unsigned int* __restrict__ counts, long long n,
               unsigned int* __restrict__ offsets) {
   __shared__ unsigned int sh[1024];
-  __shared__ unsigned int carry;
-  if (threadIdx.x == 0) carry = 0u;
+  const long long per = (n + 1023) / 1024;
+  const long long lo = threadIdx.x * per;
+  const long long hi = lo + per < n ? lo + per : n;
+  unsigned int own = 0u;
+  for (long long i = lo; i < hi; i++) own += counts[i];
+  sh[threadIdx.x] = own;
   __syncthreads();
-  for (long long base = 0; base < n; base += 1024) {
-    long long i = base + threadIdx.x;
-    unsigned int v = i < n ? counts[i] : 0u;
-    sh[threadIdx.x] = v;
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      unsigned int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < n) offsets[i] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += sh[1023];
+    sh[threadIdx.x] += t;
     __syncthreads();
   }
-  if (threadIdx.x == 0) offsets[n] = carry;
+  unsigned int run = sh[threadIdx.x] - own;
+  for (long long i = lo; i < hi; i++) {
+    const unsigned int v = counts[i];
+    offsets[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == 1023) offsets[n] = sh[1023];
 }
 
 // Scatter particle columns into tile order, ordered within a tile by
@@ -960,6 +965,39 @@ struct TileWork {
   int count;
   long long start;
 };
+
+// Particles per work item at most (one workgroup each).
+constexpr int TILE_CHUNK = 2048;
+
+// Work items of each tile: ceil(count / TILE_CHUNK) (offs: exclusive scan
+// of the per-tile counts).
+static __global__ void __launch_bounds__(256)
+k_tile_chunks(const unsigned int* __restrict__ offs, long long nt,
+              unsigned int* __restrict__ nch) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nt;
+       t += (long long)gridDim.x * blockDim.x)
+    nch[t] = (offs[t + 1] - offs[t] + TILE_CHUNK - 1) / TILE_CHUNK;
+}
+
+// The work list (woff: exclusive scan of the chunk counts) and
+// meta = {work items, particles inside the mesh}.
+static __global__ void __launch_bounds__(256)
+k_tile_work(const unsigned int* __restrict__ offs, const unsigned int* __restrict__ woff,
+            long long nt, TileWork* __restrict__ work, unsigned int* __restrict__ meta) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nt;
+       t += (long long)gridDim.x * blockDim.x) {
+    const unsigned int b = offs[t], e = offs[t + 1];
+    unsigned int w = woff[t];
+    for (unsigned int s0 = b; s0 < e; s0 += TILE_CHUNK, w++) {
+      const unsigned int c = e - s0 < unsigned(TILE_CHUNK) ? e - s0 : unsigned(TILE_CHUNK);
+      work[w] = TileWork{int(t), int(c), (long long)s0};
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    meta[0] = woff[nt];
+    meta[1] = offs[nt];
+  }
+}
 
 // Window weights of a particle along the three axes and the tile-local
 // index (halo origin o0) of the first covered cell per axis.
@@ -1027,10 +1065,11 @@ __device__ __forceinline__ long long tile_cell(const Geom& g, const int o0[3], i
 template <typename T, int ORDER>
 __global__ void __launch_bounds__(256)
 k_assign_tiled(c2<T>* __restrict__ mesh, DevParticles p, Geom g, TileGeom tg,
-               const TileWork* __restrict__ work, int mode, int ell, int m,
-               double scale, int cplx_w) {
+               const TileWork* __restrict__ work, const unsigned int* __restrict__ meta,
+               int mode, int ell, int m, double scale, int cplx_w) {
   __shared__ double acc_re[TILE_CELLS];
   __shared__ double acc_im[TILE_CELLS];
+  if (blockIdx.x >= meta[0]) return;   // grid sized by a bound
   const TileWork wk = work[blockIdx.x];
   for (int c = threadIdx.x; c < TILE_CELLS; c += blockDim.x) {
     acc_re[c] = 0.;
@@ -1088,10 +1127,12 @@ template <typename T, int ORDER, bool CPLX>
 __global__ void __launch_bounds__(256)
 k_assign_tiled_dual(T* __restrict__ ra, T* __restrict__ rb,
                     DevParticles p, Geom g, TileGeom tg,
-                    const TileWork* __restrict__ work, int ell, int m,
+                    const TileWork* __restrict__ work,
+                    const unsigned int* __restrict__ meta, int ell, int m,
                     double scale_a, double scale_b) {
   constexpr int NACC = CPLX ? 4 : 2;
   extern __shared__ double acc_dual[];   // [NACC][TILE_CELLS]
+  if (blockIdx.x >= meta[0]) return;   // grid sized by a bound
   const TileWork wk = work[blockIdx.x];
   for (int c = threadIdx.x; c < NACC * TILE_CELLS; c += blockDim.x) acc_dual[c] = 0.;
   int o0[3];

@@ -1,0 +1,253 @@
+// upload.hip -- host -> HBM transfer of particle catalogues.
+//
+// The caller's catalogue is host memory of any kind (numpy columns, the
+// 56-byte records of trv::ParticleData viewed with a stride, pageable).  A
+// direct copy from pageable memory runs through the runtime's own staging
+// at a few GB/s, and a single host thread gathering the columns is slower
+// still.  Here a pool of host threads gathers fixed-size chunks of the
+// columns (x, y, z, w = ws * wc and the lines of sight) into a ring of
+// page-locked buffers, and each chunk is copied to the device
+// asynchronously on the call's stream while the next chunk is gathered.
+// The weight column of a catalogue whose particles all carry the same
+// ws * wc is not transferred: the device fills it.  The host returns once
+// every chunk is gathered; the copies complete in stream order before the
+// kernels that read them.
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstdlib>
+#include <map>
+#include <thread>
+
+#include "engine.hpp"
+
+namespace trvg {
+
+// ---------------------------------------------------------------------------
+// Host worker pool
+// ---------------------------------------------------------------------------
+
+namespace {
+
+int host_thread_count() {
+  for (const char* var : {"TRVG_HOST_THREADS", "OMP_NUM_THREADS"}) {
+    const char* v = std::getenv(var);
+    if (v && std::atoi(v) > 0) return std::min(std::atoi(v), 64);
+  }
+  unsigned hw = std::thread::hardware_concurrency();
+  return int(std::max(1u, std::min(hw, 16u)));
+}
+
+// Persistent threads running one job at a time: run(fn) calls fn(t) for
+// t = 0 .. size()-1 (t = 0 on the calling thread) and returns when every
+// call has returned.  Jobs from different host threads are serialised.
+class HostWorkers {
+ public:
+  static HostWorkers& get() {
+    static HostWorkers* w = new HostWorkers(host_thread_count());
+    return *w;
+  }
+  int size() const { return nthreads_; }
+
+  void run(const std::function<void(int)>& fn) {
+    std::lock_guard<std::mutex> serial(run_mu_);
+    if (nthreads_ == 1) {
+      fn(0);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &fn;
+      pending_ = nthreads_ - 1;
+      gen_++;
+    }
+    cv_.notify_all();
+    fn(0);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [&] { return pending_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  explicit HostWorkers(int n) : nthreads_(n) {
+    for (int t = 1; t < n; t++) {
+      std::thread th([this, t] { loop(t); });
+      th.detach();   // lives for the process
+    }
+  }
+  void loop(int t) {
+    long seen = 0;
+    for (;;) {
+      const std::function<void(int)>* job;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+        job = job_;
+      }
+      (*job)(t);
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (--pending_ == 0) done_cv_.notify_one();
+      }
+    }
+  }
+
+  int nthreads_;
+  std::mutex run_mu_, mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)>* job_ = nullptr;
+  long gen_ = 0;
+  int pending_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Page-locked staging ring (one per stream: a stream serves one call at a
+// time)
+// ---------------------------------------------------------------------------
+
+constexpr int RING_SLOTS = 3;
+constexpr size_t RING_BYTES = size_t(32) << 20;   // per slot
+
+struct StagingRing {
+  char* buf[RING_SLOTS] = {};
+  hipEvent_t ev[RING_SLOTS] = {};
+  bool pending[RING_SLOTS] = {};
+  int dev = -1;
+  ~StagingRing() {
+    for (int i = 0; i < RING_SLOTS; i++) {
+      if (buf[i]) (void)hipHostFree(buf[i]);
+      if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+  }
+  // Slot k, once the copies that last read it have completed.
+  char* acquire(int k) {
+    if (buf[k] == nullptr) {
+      TRVG_HIP(hipHostMalloc((void**)&buf[k], RING_BYTES));
+      TRVG_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+    }
+    if (pending[k]) TRVG_HIP(hipEventSynchronize(ev[k]));
+    pending[k] = false;
+    return buf[k];
+  }
+  void release(int k, hipStream_t s) {
+    TRVG_HIP(hipEventRecord(ev[k], s));
+    pending[k] = true;
+  }
+};
+
+StagingRing& staging_ring(hipStream_t s) {
+  static std::mutex mu;
+  static auto* rings = new std::map<hipStream_t, std::unique_ptr<StagingRing>>();
+  std::lock_guard<std::mutex> lk(mu);
+  auto& r = (*rings)[s];
+  if (!r) r = std::make_unique<StagingRing>();
+  return *r;
+}
+
+__global__ void k_fill_d(double* __restrict__ f, long long n, double v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    f[i] = v;
+}
+
+void fill_device(double* f, long long n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  long long blocks = std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_d, dim3(unsigned(blocks)), dim3(256), 0, s, f, n, v);
+  TRVG_LAUNCH_CHECK();
+}
+
+// Per-thread partial results of one chunk (padded to a cache line).
+struct alignas(64) Partial {
+  double wst, wt;
+  int uniform;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Catalogue upload
+// ---------------------------------------------------------------------------
+
+void upload_catalogue(const trvg_catalogue* c, const double* los,
+                      DeviceCatalogue& out, hipStream_t s) {
+  if (c == nullptr || c->x == nullptr || c->y == nullptr || c->z == nullptr)
+    throw Error(TRVG_ERR_DATA, "Particle data are uninitialised.");
+  if (c->ntotal <= 0)
+    throw Error(TRVG_ERR_DATA, "Particle catalogue is empty.");
+  const long long n = c->ntotal;
+  const long long st = c->stride > 0 ? c->stride : 1;
+  out.tiled.reset();
+  out.n = n;
+  out.x.reset(n); out.y.reset(n); out.z.reset(n); out.w.reset(n);
+  if (los) out.los.reset(3 * n);
+  else out.los.reset(0);
+
+  // w = ws * wc (particles.cpp:553); the reference weight of the first
+  // particle decides uniformity.
+  const double w0 = (c->ws ? c->ws[0] : 1.) * (c->wc ? c->wc[0] : 1.);
+  const size_t pbytes = 32 + (los ? 24 : 0);
+  const long long chunk = (long long)(RING_BYTES / pbytes);
+  StagingRing& ring = staging_ring(s);
+  HostWorkers& pool = HostWorkers::get();
+  const int nt = pool.size();
+  std::vector<Partial> part(nt);
+  double wst = 0., wt = 0.;
+  bool uniform = true;   // every chunk so far has w == w0 throughout
+  int slot = 0;
+  for (long long base = 0; base < n; base += chunk, slot = (slot + 1) % RING_SLOTS) {
+    const long long m = std::min(chunk, n - base);
+    double* bx = reinterpret_cast<double*>(ring.acquire(slot));
+    double* by = bx + m;
+    double* bz = by + m;
+    double* bw = bz + m;
+    double* bl = bw + m;
+    pool.run([&](int t) {
+      const long long lo = m * t / nt, hi = m * (t + 1) / nt;
+      double ps = 0., pw = 0.;
+      int uni = 1;
+      for (long long i = lo; i < hi; i++) {
+        const long long g = (base + i) * st;
+        bx[i] = c->x[g];
+        by[i] = c->y[g];
+        bz[i] = c->z[g];
+        const double ws = c->ws ? c->ws[g] : 1.;
+        const double wc = c->wc ? c->wc[g] : 1.;
+        const double w = ws * wc;
+        bw[i] = w;
+        ps += ws;
+        pw += w;
+        uni &= (w == w0);
+      }
+      if (los && hi > lo)
+        std::memcpy(bl + 3 * lo, los + 3 * (base + lo), size_t(hi - lo) * 24);
+      part[t] = Partial{ps, pw, uni};
+    });
+    bool chunk_uniform = true;
+    for (int t = 0; t < nt; t++) {   // fixed order: deterministic sums
+      wst += part[t].wst;
+      wt += part[t].wt;
+      chunk_uniform = chunk_uniform && part[t].uniform;
+    }
+    const size_t cb = size_t(m) * 8;
+    TRVG_HIP(hipMemcpyAsync(out.x.get() + base, bx, cb, hipMemcpyHostToDevice, s));
+    TRVG_HIP(hipMemcpyAsync(out.y.get() + base, by, cb, hipMemcpyHostToDevice, s));
+    TRVG_HIP(hipMemcpyAsync(out.z.get() + base, bz, cb, hipMemcpyHostToDevice, s));
+    if (!(uniform && chunk_uniform)) {
+      if (uniform) fill_device(out.w.get(), base, w0, s);   // the uniform prefix
+      uniform = false;
+      TRVG_HIP(hipMemcpyAsync(out.w.get() + base, bw, cb, hipMemcpyHostToDevice, s));
+    }
+    if (los)
+      TRVG_HIP(hipMemcpyAsync(out.los.get() + 3 * base, bl, 3 * cb,
+                              hipMemcpyHostToDevice, s));
+    ring.release(slot, s);
+  }
+  if (uniform) fill_device(out.w.get(), n, w0, s);
+  out.wstotal = wst;
+  out.wtotal = wt;
+  out.wuni = uniform ? w0 : NAN;
+}
+
+}  // namespace trvg

@@ -22,6 +22,7 @@ reference passes ``threept_algofunc``) so that one host path serves every
 backend.
 """
 
+import contextlib
 import ctypes as C
 import warnings
 
@@ -69,28 +70,56 @@ class CLibBackend:
             C.byref(cat.struct), C.c_double(alpha), C.byref(out)))
         return out.value
 
-    def norm_mesh(self, cat, params, alpha):
+    def norm_mesh(self, cat, params, alpha, handle=None):
         out = C.c_double(0.)
+        if handle is not None:
+            self._check(self._fn('calc_bispec_normalisation_from_mesh_dev')(
+                handle, C.byref(params), C.c_double(alpha), C.byref(out)))
+            return out.value
         self._check(self._fn('calc_bispec_normalisation_from_mesh')(
             C.byref(cat.struct), C.byref(params), C.c_double(alpha),
             C.byref(out)))
         return out.value
+
+    @contextlib.contextmanager
+    def resident(self, cat, los=None):
+        """Device-resident copy of a catalogue for the duration of the block
+        (None for a library without device catalogues), so that the mesh
+        normalisation and the estimator share one upload."""
+        if not hasattr(self.lib, self.prefix + 'catalogue_upload'):
+            yield None
+            return
+        h = C.c_void_p()
+        self._check(self._fn('catalogue_upload')(
+            C.byref(cat.struct), _abi.dptr(los), -1, C.byref(h)))
+        try:
+            yield h
+        finally:
+            self._fn('catalogue_free')(h)
 
     def _result(self, params, binning, kind):
         dim = _abi.result_dim(params.shape, binning.struct.num_bins,
                               params.idx_bin)
         return _abi.ResultBuffers(dim, kind)
 
-    def survey(self, kind, data, rand, los_d, los_r, params, binning, norm):
+    def survey(self, kind, data, rand, los_d, los_r, params, binning, norm,
+               handles=None):
         res = self._result(params, binning, kind)
-        fn = self._fn('compute_bispec' if kind == 'bispec' else 'compute_3pcf')
-        self._check(fn(C.byref(data.struct), C.byref(rand.struct),
-                       _abi.dptr(los_d), _abi.dptr(los_r), C.byref(params),
-                       C.byref(binning.struct), C.c_double(norm),
-                       C.byref(res.struct)))
+        stem = 'compute_bispec' if kind == 'bispec' else 'compute_3pcf'
+        if handles is not None:
+            self._check(self._fn(stem + '_dev')(
+                handles[0], handles[1], C.byref(params),
+                C.byref(binning.struct), C.c_double(norm),
+                C.byref(res.struct)))
+            return res.as_dict()
+        self._check(self._fn(stem)(
+            C.byref(data.struct), C.byref(rand.struct), _abi.dptr(los_d),
+            _abi.dptr(los_r), C.byref(params), C.byref(binning.struct),
+            C.c_double(norm), C.byref(res.struct)))
         return res.as_dict()
 
-    def multi(self, data, rand, los_d, los_r, params_list, binning, norms):
+    def multi(self, data, rand, los_d, los_r, params_list, binning, norms,
+              handles=None):
         """Several bispectrum multipoles in one engine call (engine
         extension trvg_compute_bispec_multi)."""
         n = len(params_list)
@@ -98,16 +127,28 @@ class CLibBackend:
         res = [self._result(p, binning, 'bispec') for p in params_list]
         outs = (_abi.BispecResult * n)(*[r.struct for r in res])
         nrm = np.ascontiguousarray(norms, dtype=np.float64)
-        self._check(self._fn('compute_bispec_multi')(
-            C.byref(data.struct), C.byref(rand.struct) if rand else None,
-            _abi.dptr(los_d), _abi.dptr(los_r), arr, n,
-            C.byref(binning.struct), _abi.dptr(nrm), outs))
+        if handles is not None:
+            self._check(self._fn('compute_bispec_multi_dev')(
+                handles[0], handles[1], arr, n, C.byref(binning.struct),
+                _abi.dptr(nrm), outs))
+        else:
+            self._check(self._fn('compute_bispec_multi')(
+                C.byref(data.struct), C.byref(rand.struct) if rand else None,
+                _abi.dptr(los_d), _abi.dptr(los_r), arr, n,
+                C.byref(binning.struct), _abi.dptr(nrm), outs))
         for r, o in zip(res, outs):
             r.struct.dim = o.dim
         return [r.as_dict() for r in res]
 
-    def sim(self, kind, data, params, binning, norm):
+    def sim(self, kind, data, params, binning, norm, handle=None):
         res = self._result(params, binning, kind)
+        if handle is not None:
+            fn = self._fn('compute_bispec_dev' if kind == 'bispec'
+                          else 'compute_3pcf_dev')
+            self._check(fn(handle, None, C.byref(params),
+                           C.byref(binning.struct), C.c_double(norm),
+                           C.byref(res.struct)))
+            return res.as_dict()
         fn = self._fn('compute_bispec_in_gpp_box' if kind == 'bispec'
                       else 'compute_3pcf_in_gpp_box')
         self._check(fn(C.byref(data.struct), C.byref(params),
@@ -312,16 +353,21 @@ def _compute_3pt_stats_survey_like(backend, kind, catalogue_data,
     alpha = catalogue_data.wstotal / catalogue_rand.wstotal
 
     norm_part = backend.norm_particles(rand_v, alpha)
-    norm_mesh = backend.norm_mesh(rand_v, params, alpha)
-    norm_factor = _select_norm(paramset, norm_part, norm_mesh)
-
-    if multipoles is not None:
-        plist = _multipole_structs(paramset, multipoles)
-        return backend.multi(data_v, rand_v, los_data, los_rand, plist,
-                             _binning_view(binning),
-                             [norm_factor] * len(plist))
-    results = backend.survey(kind, data_v, rand_v, los_data, los_rand, params,
-                             _binning_view(binning), norm_factor)
+    # One upload of each catalogue serves the mesh normalisation and the
+    # estimator.
+    with backend.resident(rand_v, los_rand) as hr, \
+            backend.resident(data_v, los_data) as hd:
+        handles = (hd, hr) if hr is not None else None
+        norm_mesh = backend.norm_mesh(rand_v, params, alpha, handle=hr)
+        norm_factor = _select_norm(paramset, norm_part, norm_mesh)
+        if multipoles is not None:
+            plist = _multipole_structs(paramset, multipoles)
+            return backend.multi(data_v, rand_v, los_data, los_rand, plist,
+                                 _binning_view(binning),
+                                 [norm_factor] * len(plist), handles=handles)
+        results = backend.survey(
+            kind, data_v, rand_v, los_data, los_rand, params,
+            _binning_view(binning), norm_factor, handles=handles)
     results['_norm_factors'] = (norm_part, norm_mesh)
     return results
 
@@ -353,16 +399,17 @@ def _compute_3pt_stats_sim_like(backend, kind, catalogue_data, paramset=None,
     data_v = _view(catalogue_data)
     params = _params_struct(paramset)
     norm_part = backend.norm_particles(data_v, 1.)
-    norm_mesh = backend.norm_mesh(data_v, params, 1.)
-    norm_factor = _select_norm(paramset, norm_part, norm_mesh)
-
-    if multipoles is not None:
-        plist = _multipole_structs(paramset, multipoles)
-        return backend.multi(data_v, None, None, None, plist,
-                             _binning_view(binning),
-                             [norm_factor] * len(plist))
-    results = backend.sim(kind, data_v, params, _binning_view(binning),
-                          norm_factor)
+    with backend.resident(data_v) as hd:
+        norm_mesh = backend.norm_mesh(data_v, params, 1., handle=hd)
+        norm_factor = _select_norm(paramset, norm_part, norm_mesh)
+        if multipoles is not None:
+            plist = _multipole_structs(paramset, multipoles)
+            return backend.multi(data_v, None, None, None, plist,
+                                 _binning_view(binning),
+                                 [norm_factor] * len(plist),
+                                 handles=(hd, None) if hd is not None else None)
+        results = backend.sim(kind, data_v, params, _binning_view(binning),
+                              norm_factor, handle=hd)
     results['_norm_factors'] = (norm_part, norm_mesh)
     return results
 
