@@ -366,6 +366,23 @@ int trvg_diff_bispec_diag(const trvg_params* params,
                           const trvg_binning* kbinning, double norm_factor,
                           const void* mesh, double* bk, int* nmodes);
 
+/* The forward of trvg_diff_bispec_diag without any device allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Every buffer beyond `mesh` and the host outputs lives
+ * in one fixed device array of the library (trvg_noalloc_scratch_bytes(),
+ * 64 MiB, held on each device the library runs on until the process ends):
+ * FP32 meshes up to 2^21 cells and FP64 meshes up to 2^20 cells, at most 256
+ * bins; larger inputs fail with TRVG_ERR_PARAM before any device work.
+ * Runs on the calling thread's default stream and returns with the results
+ * on the host.  Calls that overlap in time on one device share the array
+ * and race. */
+int trvg_diff_bispec_diag_noalloc(const trvg_params* params,
+                                  const trvg_binning* kbinning,
+                                  double norm_factor, const void* mesh,
+                                  double* bk, int* nmodes);
+long long trvg_noalloc_scratch_bytes(void);
+
 /* grad_mesh = sum_b grad_bk[b] dB_b/ddelta (shells recomputed, not
  * stored). */
 int trvg_diff_bispec_diag_backward(const trvg_params* params,

@@ -247,6 +247,12 @@ def declare_extensions(lib):
     lib.trvg_diff_bispec_diag.argtypes = [P(Params), P(Binning), C.c_double,
                                           _vp, _dp, P(C.c_int)]
     lib.trvg_diff_bispec_diag.restype = C.c_int
+    lib.trvg_diff_bispec_diag_noalloc.argtypes = [P(Params), P(Binning),
+                                                  C.c_double, _vp, _dp,
+                                                  P(C.c_int)]
+    lib.trvg_diff_bispec_diag_noalloc.restype = C.c_int
+    lib.trvg_noalloc_scratch_bytes.argtypes = []
+    lib.trvg_noalloc_scratch_bytes.restype = C.c_longlong
     lib.trvg_diff_bispec_diag_backward.argtypes = [P(Params), P(Binning),
                                                    C.c_double, _vp, _dp, _vp]
     lib.trvg_diff_bispec_diag_backward.restype = C.c_int

@@ -629,6 +629,24 @@ int trvg_diff_bispec_diag(const trvg_params* params,
   });
 }
 
+int trvg_diff_bispec_diag_noalloc(const trvg_params* params,
+                                  const trvg_binning* kbinning, double norm_factor,
+                                  const void* mesh, double* bk, int* nmodes) {
+  return guarded([&] {
+    require_device();
+    check_binning(kbinning);
+    trvg_params q = mesh_params(params);
+    if (mesh == nullptr || bk == nullptr)
+      throw Error(TRVG_ERR_PARAM, "Null mesh or output.");
+    if (q.device >= 0) TRVG_HIP(hipSetDevice(q.device));
+    diff_bispec_forward_noalloc(q, *kbinning, norm_factor, mesh, bk, nmodes);
+  });
+}
+
+long long trvg_noalloc_scratch_bytes(void) {
+  return (long long)noalloc_scratch_bytes();
+}
+
 int trvg_diff_bispec_diag_backward(const trvg_params* params,
                                    const trvg_binning* kbinning,
                                    double norm_factor, const void* mesh,

@@ -403,6 +403,14 @@ template <typename T>
 void diff_assign_adjoint(Context& ctx, const T* gmesh, const double* pos,
                          const double* w, long long n, double* gpos, double* gw);
 
+// The forward above without any device allocation: every buffer in the
+// library's fixed scratch array (engine_noalloc.hip); throws
+// TRVG_ERR_PARAM when the mesh or the binning exceeds it.
+void diff_bispec_forward_noalloc(const trvg_params& q, const trvg_binning& kb,
+                                 double norm, const void* mesh, double* bk,
+                                 int* nmodes);
+size_t noalloc_scratch_bytes();
+
 template <typename T>
 Result run_bispec(Context& ctx, const DeviceCatalogue& d,
                   const DeviceCatalogue* r, bool survey,
