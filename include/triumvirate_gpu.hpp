@@ -17,6 +17,7 @@
 #ifndef TRIUMVIRATE_GPU_HPP_INCLUDED
 #define TRIUMVIRATE_GPU_HPP_INCLUDED
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <stdexcept>
@@ -208,19 +209,30 @@ class ParameterSet {
   }
 };
 
-// trv::Binning, linear and logarithmic schemes
-// (src/dataobjs.cpp:134-160 arithmetic).
+// trv::Binning (include/dataobjs.hpp:58-150): linear, logarithmic and the
+// padded schemes (src/dataobjs.cpp:134-250 arithmetic), the grid-scale
+// default binning and custom edges.
 class Binning {
  public:
   std::string space, scheme;
   double bin_min = 0., bin_max = 0.;
   int num_bins = 0;
   std::vector<double> bin_edges, bin_centres, bin_widths;
+  int nbin_pad = 5;                 // padded bins of the *pad schemes
+  double dbin_pad_fourier = 1.e-3;  // their width in Fourier space
+  double dbin_pad_config = 10.;     // and in configuration space
 
   Binning(std::string space_, std::string scheme_)
       : space(std::move(space_)), scheme(std::move(scheme_)) {}
+  // Range from the parameters; padding at the grid scale (dataobjs.cpp:42-56).
   explicit Binning(const ParameterSet& p) : Binning(p.space, p.binning) {
-    set_bins(p.bin_min, p.bin_max, p.num_bins);
+    bin_min = p.bin_min;
+    bin_max = p.bin_max;
+    num_bins = p.num_bins;
+    const double lmax = std::max(p.boxsize[0], std::max(p.boxsize[1], p.boxsize[2]));
+    const int nmin = std::min(p.ngrid[0], std::min(p.ngrid[1], p.ngrid[2]));
+    dbin_pad_config = (1 + 5.e-3) * lmax / double(nmin);
+    dbin_pad_fourier = (1 + 5.e-3) * (2 * M_PI) / lmax;
   }
 
   void set_bins(double coord_min, double coord_max, int nbin) {
@@ -231,28 +243,77 @@ class Binning {
     bin_min = coord_min;
     bin_max = coord_max;
     num_bins = nbin;
+    set_bins();
+  }
+
+  // Recompute the bins from the stored range and scheme.
+  void set_bins() {
     bin_edges.clear();
     bin_centres.clear();
     bin_widths.clear();
+    compute_binning();
+  }
+
+  // Linear bins at the grid scale: half the grid number of bins from zero
+  // (dataobjs.cpp:80-104).
+  void set_bins(double boxsize_max, int ngrid_min) {
+    scheme = "lin";
+    bin_min = 0.;
+    double width = 0.;
+    if (space == "config") {
+      width = boxsize_max / double(ngrid_min);
+      bin_max = boxsize_max / 2.;
+    } else if (space == "fourier") {
+      width = 2 * M_PI / boxsize_max;
+      bin_max = M_PI * double(ngrid_min) / boxsize_max;
+    }
+    bin_max += width / 2.;
+    num_bins = ngrid_min / 2;
+    set_bins();
+  }
+
+  // Custom edges (dataobjs.cpp:106-126).
+  void set_bins(const std::vector<double>& edges) {
+    bin_min = edges.front();
+    bin_max = edges.back();
+    num_bins = int(edges.size()) - 1;
+    scheme = "custom";
+    bin_edges = edges;
+    bin_centres.clear();
+    bin_widths.clear();
+    for (int i = 0; i < num_bins; i++) {
+      bin_centres.push_back((edges[i] + edges[i + 1]) / 2.);
+      bin_widths.push_back(edges[i + 1] - edges[i]);
+    }
+  }
+
+ private:
+  void compute_binning() {
+    const double pad = (space == "fourier") ? dbin_pad_fourier : dbin_pad_config;
     if (scheme == "lin") {
       double dbin = (bin_max - bin_min) / double(num_bins);
-      for (int i = 0; i < num_bins; i++) {
-        double left = bin_min + dbin * i;
-        bin_edges.push_back(left);
-        bin_centres.push_back(left + dbin / 2.);
-        bin_widths.push_back(dbin);
-      }
+      for (int i = 0; i < num_bins; i++) push_bin(bin_min + dbin * i, dbin);
     } else if (scheme == "log") {
       if (bin_min == 0.)
         throw sys::InvalidParameterError(
             "Cannot use logarithmic binning when the lowest edge is zero.");
       double dln = (std::log(bin_max) - std::log(bin_min)) / double(num_bins);
-      for (int i = 0; i < num_bins; i++) {
-        double left = bin_min * std::exp(dln * i);
-        double right = bin_min * std::exp(dln * (i + 1));
-        bin_edges.push_back(left);
-        bin_centres.push_back((left + right) / 2.);
-        bin_widths.push_back(right - left);
+      for (int i = 0; i < num_bins; i++)
+        push_log_bin(bin_min * std::exp(dln * i), bin_min * std::exp(dln * (i + 1)));
+    } else if (scheme == "linpad" || scheme == "logpad") {
+      // nbin_pad linear bins of width `pad` from zero, then the rest of the
+      // range linear or logarithmic.
+      for (int i = 0; i < nbin_pad; i++) push_bin(pad * i, pad);
+      const double lo = pad * nbin_pad;
+      const double nrest = double(num_bins - nbin_pad);
+      if (scheme == "linpad") {
+        double dbin = (bin_max - lo) / nrest;
+        for (int i = nbin_pad; i < num_bins; i++) push_bin(lo + dbin * (i - nbin_pad), dbin);
+      } else {
+        double dln = (std::log(bin_max) - std::log(lo)) / nrest;
+        for (int i = nbin_pad; i < num_bins; i++)
+          push_log_bin(lo * std::exp(dln * (i - nbin_pad)),
+                       lo * std::exp(dln * (i - nbin_pad + 1)));
       }
     } else {
       throw sys::InvalidParameterError(
@@ -260,7 +321,18 @@ class Binning {
     }
     bin_edges.push_back(bin_max);
   }
+  void push_bin(double left, double width) {
+    bin_edges.push_back(left);
+    bin_centres.push_back(left + width / 2.);
+    bin_widths.push_back(width);
+  }
+  void push_log_bin(double left, double right) {
+    bin_edges.push_back(left);
+    bin_centres.push_back((left + right) / 2.);
+    bin_widths.push_back(right - left);
+  }
 
+ public:
   trvg_binning abi_view() const {
     trvg_binning b;
     b.num_bins = num_bins;

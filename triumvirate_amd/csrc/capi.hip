@@ -121,6 +121,9 @@ trvg_params mesh_params(const trvg_params* p) {
   return q;
 }
 
+// FP32-field calls take lines of sight as FP32 across the bus.
+bool fp32(const Context& ctx) { return ctx.p.precision == TRVG_PREC_FP32; }
+
 void require_los(const double* los, const char* which) {
   if (los == nullptr)
     throw Error(TRVG_ERR_DATA, std::string("Lines of sight for the ") + which +
@@ -192,7 +195,7 @@ int trvg_calc_bispec_normalisation_from_mesh(const trvg_catalogue* c,
     trvg_params q = mesh_params(params);
     Context ctx(q);
     DeviceCatalogue dc;
-    upload_catalogue(c, nullptr, dc, ctx.stream);
+    upload_catalogue(c, nullptr, dc, ctx.stream, &ctx);
     double nf = (q.precision == TRVG_PREC_FP32) ? run_mesh_norm<float>(ctx, dc)
                                                 : run_mesh_norm<double>(ctx, dc);
     *norm_factor = nf / std::pow(alpha, 3);
@@ -211,8 +214,8 @@ int trvg_compute_bispec(const trvg_catalogue* data, const trvg_catalogue* rand,
     require_los(los_rand, "random");
     Context ctx(q);
     DeviceCatalogue d, r;
-    upload_catalogue(data, los_data, d, ctx.stream);
-    upload_catalogue(rand, los_rand, r, ctx.stream);
+    upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx));
+    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx));
     write_bispec(dispatch_bispec(ctx, d, &r, true, *kb, norm_factor), out);
   });
 }
@@ -229,8 +232,8 @@ int trvg_compute_3pcf(const trvg_catalogue* data, const trvg_catalogue* rand,
     require_los(los_rand, "random");
     Context ctx(q);
     DeviceCatalogue d, r;
-    upload_catalogue(data, los_data, d, ctx.stream);
-    upload_catalogue(rand, los_rand, r, ctx.stream);
+    upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx));
+    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx));
     write_3pcf(dispatch_3pcf(ctx, d, &r, true, *rb, norm_factor), out);
   });
 }
@@ -245,7 +248,7 @@ int trvg_compute_bispec_in_gpp_box(const trvg_catalogue* data,
     trvg_params q = three_point_params(params, kb->num_bins);
     Context ctx(q);
     DeviceCatalogue d;
-    upload_catalogue(data, nullptr, d, ctx.stream);
+    upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
     write_bispec(dispatch_bispec(ctx, d, nullptr, false, *kb, norm_factor), out);
   });
 }
@@ -260,7 +263,7 @@ int trvg_compute_3pcf_in_gpp_box(const trvg_catalogue* data,
     trvg_params q = three_point_params(params, rb->num_bins);
     Context ctx(q);
     DeviceCatalogue d;
-    upload_catalogue(data, nullptr, d, ctx.stream);
+    upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
     write_3pcf(dispatch_3pcf(ctx, d, nullptr, false, *rb, norm_factor), out);
   });
 }
@@ -456,11 +459,11 @@ int trvg_compute_bispec_multi(const trvg_catalogue* data,
     if (rand) {
       require_los(los_data, "data");
       require_los(los_rand, "random");
-      upload_catalogue(data, los_data, d, ctx.stream);
-      upload_catalogue(rand, los_rand, r, ctx.stream);
+      upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx));
+      upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx));
       run_multi(ctx, d, &r, *kb, mps, outs);
     } else {
-      upload_catalogue(data, nullptr, d, ctx.stream);
+      upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
       run_multi(ctx, d, nullptr, *kb, mps, outs);
     }
   });

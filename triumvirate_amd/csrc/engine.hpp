@@ -114,15 +114,42 @@ struct DeviceCatalogue {
   }
 };
 
+struct Context;
+
+// Tile sort of a device catalogue for the context's mesh in two phases, so
+// that counting can follow an upload chunk by chunk: count() for each range
+// of particles once its columns are on the device (stream order), then
+// finish() (scan, in-place scatter into tile order, work list), which
+// caches the result on the catalogue (c.tiled).  Nothing waits for the
+// device.
+class TileSorter {
+ public:
+  TileSorter(Context& ctx, const DeviceCatalogue& c);
+  void count(long long begin, long long end);
+  void finish();
+
+ private:
+  Context& ctx_;
+  const DeviceCatalogue& c_;
+  std::unique_ptr<TiledView> tv_;
+  DevArray<int> tile_of_;
+  DevArray<unsigned int> ccount_, key2_;
+};
+
 // Streams of the engine (pooled per device, default priority).
 hipStream_t acquire_stream(int dev);
 void release_stream(int dev, hipStream_t s);
 
 // Upload a (possibly strided) host catalogue; `los` optional (n x 3).  The
-// copies are queued on stream s (the host buffers may be reused on
-// return; the device columns are valid in stream order).
+// device columns are valid in the order of stream s (the host buffers may
+// be reused on return).  With a context `sort_for` (s is then its stream),
+// the copies run on a separate stream, chunk by chunk, while the context's
+// stream counts each arrived chunk for the tile sort of the context's mesh,
+// which the upload completes; with los32 the lines of sight cross as FP32
+// (FP32-field calls) and are widened on the device.
 void upload_catalogue(const trvg_catalogue* c, const double* los,
-                      DeviceCatalogue& out, hipStream_t s);
+                      DeviceCatalogue& out, hipStream_t s,
+                      Context* sort_for = nullptr, bool los32 = false);
 
 // ---------------------------------------------------------------------------
 // Spherical Bessel spline (host build, device tables).

@@ -736,56 +736,60 @@ static void zero_field(Context& ctx, c2<T>* f) {
   TRVG_LAUNCH_CHECK();
 }
 
-// Tile order of `c` for the context's mesh (built once per geometry and
-// cached on the catalogue): a counting sort by tile with the within-tile
-// bank-class order, the columns permuted in place, and the work list built
-// on the device.  Nothing here waits for the device.
-static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
-  if (c.tiled && c.tiled->matches(ctx.g)) return *c.tiled;
-  auto tv = std::make_unique<TiledView>();
+TileSorter::TileSorter(Context& ctx, const DeviceCatalogue& c)
+    : ctx_(ctx), c_(c), tv_(std::make_unique<TiledView>()) {
   const Geom& g = ctx.g;
   for (int a = 0; a < 3; a++) {
-    tv->n[a] = g.n[a];
-    tv->L[a] = g.L[a];
-    tv->tg.nt[a] = (g.n[a] + TILE - 1) / TILE;
+    tv_->n[a] = g.n[a];
+    tv_->L[a] = g.L[a];
+    tv_->tg.nt[a] = (g.n[a] + TILE - 1) / TILE;
   }
-  tv->tg.ntiles = (long long)tv->tg.nt[0] * tv->tg.nt[1] * tv->tg.nt[2];
-  const long long nt = tv->tg.ntiles;
-  DevArray<int> tile_of(c.n);
-  DevArray<unsigned int> counts(nt + 1), offs(nt + 2), ccount((nt + 1) * TILE_CLASSES),
-      key2(c.n), nch(nt), woff(nt + 1);
-  TRVG_HIP(hipMemsetAsync(counts.get(), 0, (nt + 1) * 4, ctx.stream));
-  TRVG_HIP(hipMemsetAsync(ccount.get(), 0, (nt + 1) * TILE_CLASSES * 4, ctx.stream));
-  DevParticles pv = c.view();
-  unsigned grid = grid_for(ctx, c.n, 16);
-  {
-    ProfScope ps(ctx, KC_ASSIGN, c.n * 28.);
-    hipLaunchKernelGGL(k_tile_count, dim3(grid), dim3(256), 0, ctx.stream, pv,
-                       g, tv->tg, tile_of.get(), counts.get(), ccount.get(), key2.get());
-    TRVG_LAUNCH_CHECK();
-  }
+  tv_->tg.ntiles = (long long)tv_->tg.nt[0] * tv_->tg.nt[1] * tv_->tg.nt[2];
+  const long long nt = tv_->tg.ntiles;
+  tile_of_.reset(c.n);
+  key2_.reset(c.n);
+  ccount_.reset((nt + 1) * TILE_CLASSES);
+  TRVG_HIP(hipMemsetAsync(ccount_.get(), 0, (nt + 1) * TILE_CLASSES * 4, ctx.stream));
+}
+
+void TileSorter::count(long long begin, long long end) {
+  if (end <= begin) return;
+  ProfScope ps(ctx_, KC_ASSIGN, double(end - begin) * 32.);
+  hipLaunchKernelGGL(k_tile_count, dim3(grid_for(ctx_, end - begin, 16)), dim3(256), 0,
+                     ctx_.stream, c_.view(), begin, end, ctx_.g, tv_->tg, tile_of_.get(),
+                     ccount_.get(), key2_.get());
+  TRVG_LAUNCH_CHECK();
+}
+
+void TileSorter::finish() {
+  Context& ctx = ctx_;
+  const DeviceCatalogue& c = c_;
+  const long long nt = tv_->tg.ntiles;
+  DevArray<unsigned int> counts(nt + 1), offs(nt + 2), nch(nt), woff(nt + 1);
+  hipLaunchKernelGGL(k_tile_sum, dim3(grid_for(ctx, nt + 1)), dim3(256), 0, ctx.stream,
+                     ccount_.get(), nt + 1, counts.get());
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx.stream,
                      counts.get(), nt + 1, offs.get());
   TRVG_LAUNCH_CHECK();
   DevArray<double> sx(c.n), sy(c.n), sz(c.n), sw(c.n), slos(c.los.get() ? 3 * c.n : 0);
   {
     ProfScope ps(ctx, KC_ASSIGN, c.n * (c.los.get() ? 116. : 68.));
-    hipLaunchKernelGGL(k_tile_scatter, dim3(grid), dim3(256), 0, ctx.stream, pv,
-                       tile_of.get(), offs.get(), ccount.get(), key2.get(), sx.get(),
-                       sy.get(), sz.get(), sw.get(), slos.get());
+    hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for(ctx, c.n, 16)), dim3(256), 0,
+                       ctx.stream, c.view(), tile_of_.get(), offs.get(), ccount_.get(),
+                       key2_.get(), sx.get(), sy.get(), sz.get(), sw.get(), slos.get());
     TRVG_LAUNCH_CHECK();
   }
   // Work list: chunks of at most TILE_CHUNK particles per tile.
-  tv->work_cap = nt + (c.n + TILE_CHUNK - 1) / TILE_CHUNK;
-  tv->work.reset(tv->work_cap);
-  tv->meta.reset(2);
+  tv_->work_cap = nt + (c.n + TILE_CHUNK - 1) / TILE_CHUNK;
+  tv_->work.reset(tv_->work_cap);
+  tv_->meta.reset(2);
   const unsigned tgrid = grid_for(ctx, nt);
   hipLaunchKernelGGL(k_tile_chunks, dim3(tgrid), dim3(256), 0, ctx.stream, offs.get(),
                      nt, nch.get());
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx.stream, nch.get(), nt,
                      woff.get());
   hipLaunchKernelGGL(k_tile_work, dim3(tgrid), dim3(256), 0, ctx.stream, offs.get(),
-                     woff.get(), nt, tv->work.get(), tv->meta.get());
+                     woff.get(), nt, tv_->work.get(), tv_->meta.get());
   TRVG_LAUNCH_CHECK();
   // The permuted columns replace the caller-order ones; the old columns and
   // the sort's temporaries stay allocated until the stream has passed them.
@@ -794,19 +798,30 @@ static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
   ctx.keep_until_flush(std::move(sx)); ctx.keep_until_flush(std::move(sy));
   ctx.keep_until_flush(std::move(sz)); ctx.keep_until_flush(std::move(sw));
   ctx.keep_until_flush(std::move(slos));
-  ctx.keep_until_flush(std::move(tile_of)); ctx.keep_until_flush(std::move(counts));
-  ctx.keep_until_flush(std::move(offs)); ctx.keep_until_flush(std::move(ccount));
-  ctx.keep_until_flush(std::move(key2)); ctx.keep_until_flush(std::move(nch));
+  ctx.keep_until_flush(std::move(tile_of_)); ctx.keep_until_flush(std::move(counts));
+  ctx.keep_until_flush(std::move(offs)); ctx.keep_until_flush(std::move(ccount_));
+  ctx.keep_until_flush(std::move(key2_)); ctx.keep_until_flush(std::move(nch));
   ctx.keep_until_flush(std::move(woff));
   // Exact grids for later calls once the host has the counts.
-  const unsigned int* hm = ctx.download(tv->meta.get(), 2);
-  TiledView* tvp = tv.get();
+  const unsigned int* hm = ctx.download(tv_->meta.get(), 2);
+  TiledView* tvp = tv_.get();
   const long long n = c.n;
   ctx.defer([tvp, hm, n] {
     tvp->nwork = hm[0];
     tvp->nstraggle = n - (long long)hm[1];
   });
-  c.tiled = std::move(tv);
+  c.tiled = std::move(tv_);
+}
+
+// Tile order of `c` for the context's mesh (built once per geometry and
+// cached on the catalogue): a counting sort by tile with the within-tile
+// bank-class order, the columns permuted in place, and the work list built
+// on the device.  Nothing here waits for the device.
+static const TiledView& tiled_view(Context& ctx, const DeviceCatalogue& c) {
+  if (c.tiled && c.tiled->matches(ctx.g)) return *c.tiled;
+  TileSorter ts(ctx, c);
+  ts.count(0, c.n);
+  ts.finish();
   return *c.tiled;
 }
 

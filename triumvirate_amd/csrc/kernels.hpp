@@ -499,6 +499,9 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
   const double kmax2 = kmax * kmax;
   const long long nlines = (long long)g.n[0] * g.n[1];
   const int half = g.n[2] / 2;
+  // Each lane takes a contiguous run of the line's window, so consecutive
+  // cells of a lane mostly share a bin: sums stay in registers and go to
+  // LDS (five atomics) only when the bin changes and at the end of the run.
   const int nw = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   for (long long l = (long long)blockIdx.x * nw + wave; l < nlines;
        l += (long long)gridDim.x * nw) {
@@ -508,9 +511,15 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
    const double rem = kmax2 - (kx * kx + ky * ky);
    if (rem < 0.) continue;
    const int K = int(sqrt(rem) / g.dk[2]) + 1;
-   const int ncell = K >= half ? g.n[2] : 2 * K + 1;
-   for (int kk = lane; kk < ncell; kk += 64) {
-    const int k = K >= half ? kk : (kk - K + g.n[2]) % g.n[2];
+   const bool whole = K >= half;
+   const int ncell = whole ? g.n[2] : 2 * K + 1;
+   const int run = (ncell + 63) / 64;
+   const int lo = lane * run, hi = lo + run < ncell ? lo + run : ncell;
+   int cur = -1;
+   unsigned int acc_n = 0u;
+   double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+   for (int kk = lo; kk < hi; kk++) {
+    const int k = whole ? kk : (kk - K + g.n[2]) % g.n[2];
     const long long c = l * g.n[2] + k;
     double kz = grid_coord(k, g.n[2], g.dk[2]);
     double km = mag3_exact(kx, ky, kz);
@@ -529,11 +538,30 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
     double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
     pk = dmul(pk, y);
     sn = dmul(sn, y);
-    atomicAdd(&sh_cnt[b], 1u);
-    atomicAdd(&sh2[4 * b], pk.x);
-    atomicAdd(&sh2[4 * b + 1], pk.y);
-    atomicAdd(&sh2[4 * b + 2], sn.x);
-    atomicAdd(&sh2[4 * b + 3], sn.y);
+    if (b != cur) {
+      if (acc_n) {
+        atomicAdd(&sh_cnt[cur], acc_n);
+        atomicAdd(&sh2[4 * cur], a0);
+        atomicAdd(&sh2[4 * cur + 1], a1);
+        atomicAdd(&sh2[4 * cur + 2], a2);
+        atomicAdd(&sh2[4 * cur + 3], a3);
+      }
+      cur = b;
+      acc_n = 0u;
+      a0 = a1 = a2 = a3 = 0.;
+    }
+    acc_n++;
+    a0 += pk.x;
+    a1 += pk.y;
+    a2 += sn.x;
+    a3 += sn.y;
+   }
+   if (acc_n) {
+     atomicAdd(&sh_cnt[cur], acc_n);
+     atomicAdd(&sh2[4 * cur], a0);
+     atomicAdd(&sh2[4 * cur + 1], a1);
+     atomicAdd(&sh2[4 * cur + 2], a2);
+     atomicAdd(&sh2[4 * cur + 3], a3);
    }
   }
   __syncthreads();
@@ -866,33 +894,71 @@ __device__ __forceinline__ void base_cell(const Geom& g, double x, double y,
 // distinct.
 constexpr int TILE_CLASSES = 32;
 
-// Tile id per particle (ntiles for stragglers), per-tile counts, and the
-// particle's bank class and rank within (tile, class) as key2 = rank * 32 +
-// class.
+// Tile id per particle of [begin, end) (ntiles for stragglers), its bank
+// class, and its rank within (tile, class) as key2 = rank * 32 + class, from
+// one atomic on the (tile, class) counter.  Lanes of a wave that hold the
+// same key (spatially ordered input) take their ranks from one atomic.
 static __global__ void __launch_bounds__(256)
-k_tile_count(DevParticles p, Geom g, TileGeom tg, int* __restrict__ tile_of,
-             unsigned int* __restrict__ counts, unsigned int* __restrict__ ccount,
+k_tile_count(DevParticles p, long long begin, long long end, Geom g, TileGeom tg,
+             int* __restrict__ tile_of, unsigned int* __restrict__ ccount,
              unsigned int* __restrict__ key2) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < p.n;
-       i += (long long)gridDim.x * blockDim.x) {
-    int idx[3];
-    double s[3];
-    base_cell(g, p.x[i], p.y[i], p.z[i], idx, s);
+  const int lane = __lane_id();
+  for (long long i0 = begin + blockIdx.x * (long long)blockDim.x; i0 < end;
+       i0 += (long long)gridDim.x * blockDim.x) {
+    const long long i = i0 + threadIdx.x;
+    const bool valid = i < end;
     long long t = tg.ntiles;
-    if (idx[0] >= 0 && idx[0] < g.n[0] && idx[1] >= 0 && idx[1] < g.n[1] &&
-        idx[2] >= 0 && idx[2] < g.n[2]) {
-      t = ((long long)(idx[0] / TILE) * tg.nt[1] + idx[1] / TILE) * tg.nt[2] +
-          idx[2] / TILE;
-    }
     int cls = 0;
-    if (t < tg.ntiles) {
-      const int lx = idx[0] % TILE + 1, ly = idx[1] % TILE + 1, lz = idx[2] % TILE + 1;
-      cls = ((lx * TILE_H + ly) * TILE_H + lz) % TILE_CLASSES;
+    if (valid) {
+      int idx[3];
+      double s[3];
+      base_cell(g, p.x[i], p.y[i], p.z[i], idx, s);
+      if (idx[0] >= 0 && idx[0] < g.n[0] && idx[1] >= 0 && idx[1] < g.n[1] &&
+          idx[2] >= 0 && idx[2] < g.n[2]) {
+        t = ((long long)(idx[0] / TILE) * tg.nt[1] + idx[1] / TILE) * tg.nt[2] +
+            idx[2] / TILE;
+        const int lx = idx[0] % TILE + 1, ly = idx[1] % TILE + 1, lz = idx[2] % TILE + 1;
+        cls = ((lx * TILE_H + ly) * TILE_H + lz) % TILE_CLASSES;
+      }
     }
-    tile_of[i] = int(t);
-    atomicAdd(&counts[t], 1u);
-    const unsigned int r = atomicAdd(&ccount[t * TILE_CLASSES + cls], 1u);
-    key2[i] = r * TILE_CLASSES + unsigned(cls);
+    const unsigned int key = unsigned(t) * TILE_CLASSES + unsigned(cls);
+    unsigned long long pending = __ballot(valid);
+    unsigned int rank = 0u;
+    bool have = false;
+    for (int round = 0; round < 4 && pending != 0ull; round++) {
+      const int leader = __ffsll((long long)pending) - 1;
+      const unsigned int k0 = __shfl(key, leader);
+      const unsigned long long m = __ballot(valid && key == k0) & pending;
+      const unsigned int nm = unsigned(__popcll(m));
+      if (nm < 4u) break;                 // mostly distinct keys: direct atomics
+      unsigned int base = 0u;
+      if (lane == leader) base = atomicAdd(&ccount[k0], nm);
+      base = __shfl(base, leader);
+      if ((m >> lane) & 1ull) {
+        rank = base + unsigned(__popcll(m & ((1ull << lane) - 1ull)));
+        have = true;
+      }
+      pending &= ~m;
+    }
+    if (valid && !have) rank = atomicAdd(&ccount[key], 1u);
+    if (valid) {
+      tile_of[i] = int(t);
+      key2[i] = rank * TILE_CLASSES + unsigned(cls);
+    }
+  }
+}
+
+// Particles per tile (t < ntiles) and stragglers (t = ntiles) from the
+// (tile, class) counters.
+static __global__ void __launch_bounds__(256)
+k_tile_sum(const unsigned int* __restrict__ ccount, long long nkeys,
+           unsigned int* __restrict__ counts) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nkeys;
+       t += (long long)gridDim.x * blockDim.x) {
+    unsigned int s = 0u;
+#pragma unroll 8
+    for (int c = 0; c < TILE_CLASSES; c++) s += ccount[t * TILE_CLASSES + c];
+    counts[t] = s;
   }
 }
 

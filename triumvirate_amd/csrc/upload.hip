@@ -119,6 +119,7 @@ struct StagingRing {
       if (buf[i]) (void)hipHostFree(buf[i]);
       if (ev[i]) (void)hipEventDestroy(ev[i]);
     }
+    if (tail) (void)hipEventDestroy(tail);
   }
   // Slot k, once the copies that last read it have completed.
   char* acquire(int k) {
@@ -134,6 +135,12 @@ struct StagingRing {
     TRVG_HIP(hipEventRecord(ev[k], s));
     pending[k] = true;
   }
+  // Event for the end of an upload's copies (waited on by other streams).
+  hipEvent_t tail_event() {
+    if (tail == nullptr) TRVG_HIP(hipEventCreateWithFlags(&tail, hipEventDisableTiming));
+    return tail;
+  }
+  hipEvent_t tail = nullptr;
 };
 
 StagingRing& staging_ring(hipStream_t s) {
@@ -149,6 +156,19 @@ __global__ void k_fill_d(double* __restrict__ f, long long n, double v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     f[i] = v;
+}
+
+__global__ void k_widen(double* __restrict__ d, const float* __restrict__ f, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    d[i] = double(f[i]);
+}
+
+void widen_device(double* d, const float* f, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  long long blocks = std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_widen, dim3(unsigned(blocks)), dim3(256), 0, s, d, f, n);
+  TRVG_LAUNCH_CHECK();
 }
 
 void fill_device(double* f, long long n, double v, hipStream_t s) {
@@ -171,7 +191,8 @@ struct alignas(64) Partial {
 // ---------------------------------------------------------------------------
 
 void upload_catalogue(const trvg_catalogue* c, const double* los,
-                      DeviceCatalogue& out, hipStream_t s) {
+                      DeviceCatalogue& out, hipStream_t s, Context* sort_for,
+                      bool los32) {
   if (c == nullptr || c->x == nullptr || c->y == nullptr || c->z == nullptr)
     throw Error(TRVG_ERR_DATA, "Particle data are uninitialised.");
   if (c->ntotal <= 0)
@@ -183,68 +204,115 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
   out.x.reset(n); out.y.reset(n); out.z.reset(n); out.w.reset(n);
   if (los) out.los.reset(3 * n);
   else out.los.reset(0);
+  los32 = los32 && los != nullptr && sort_for != nullptr;
+
+  // Copies on their own stream when the upload feeds a tile sort, so that
+  // the counting of chunk k overlaps the copy of chunk k + 1.
+  hipStream_t cs = s;
+  int dev = 0;
+  if (sort_for) {
+    dev = sort_for->dev;
+    cs = acquire_stream(dev);
+  }
+  std::unique_ptr<TileSorter> sorter;
+  if (sort_for) sorter = std::make_unique<TileSorter>(*sort_for, out);
+  DevArray<float> los_f(los32 ? 3 * n : 0);   // FP32 lines of sight (widened)
 
   // w = ws * wc (particles.cpp:553); the reference weight of the first
   // particle decides uniformity.
   const double w0 = (c->ws ? c->ws[0] : 1.) * (c->wc ? c->wc[0] : 1.);
-  const size_t pbytes = 32 + (los ? 24 : 0);
+  const size_t pbytes = 32 + (los ? (los32 ? 12 : 24) : 0);
   const long long chunk = (long long)(RING_BYTES / pbytes);
-  StagingRing& ring = staging_ring(s);
+  StagingRing& ring = staging_ring(cs);
   HostWorkers& pool = HostWorkers::get();
   const int nt = pool.size();
   std::vector<Partial> part(nt);
   double wst = 0., wt = 0.;
   bool uniform = true;   // every chunk so far has w == w0 throughout
   int slot = 0;
-  for (long long base = 0; base < n; base += chunk, slot = (slot + 1) % RING_SLOTS) {
-    const long long m = std::min(chunk, n - base);
-    double* bx = reinterpret_cast<double*>(ring.acquire(slot));
-    double* by = bx + m;
-    double* bz = by + m;
-    double* bw = bz + m;
-    double* bl = bw + m;
-    pool.run([&](int t) {
-      const long long lo = m * t / nt, hi = m * (t + 1) / nt;
-      double ps = 0., pw = 0.;
-      int uni = 1;
-      for (long long i = lo; i < hi; i++) {
-        const long long g = (base + i) * st;
-        bx[i] = c->x[g];
-        by[i] = c->y[g];
-        bz[i] = c->z[g];
-        const double ws = c->ws ? c->ws[g] : 1.;
-        const double wc = c->wc ? c->wc[g] : 1.;
-        const double w = ws * wc;
-        bw[i] = w;
-        ps += ws;
-        pw += w;
-        uni &= (w == w0);
+  try {
+    for (long long base = 0; base < n; base += chunk, slot = (slot + 1) % RING_SLOTS) {
+      const long long m = std::min(chunk, n - base);
+      double* bx = reinterpret_cast<double*>(ring.acquire(slot));
+      double* by = bx + m;
+      double* bz = by + m;
+      double* bw = bz + m;
+      double* bl = bw + m;
+      float* bl32 = reinterpret_cast<float*>(bl);
+      pool.run([&](int t) {
+        const long long lo = m * t / nt, hi = m * (t + 1) / nt;
+        double ps = 0., pw = 0.;
+        int uni = 1;
+        for (long long i = lo; i < hi; i++) {
+          const long long g = (base + i) * st;
+          bx[i] = c->x[g];
+          by[i] = c->y[g];
+          bz[i] = c->z[g];
+          const double ws = c->ws ? c->ws[g] : 1.;
+          const double wc = c->wc ? c->wc[g] : 1.;
+          const double w = ws * wc;
+          bw[i] = w;
+          ps += ws;
+          pw += w;
+          uni &= (w == w0);
+        }
+        if (los && hi > lo) {
+          if (los32) {
+            const double* src = los + 3 * (base + lo);
+            for (long long q = 0; q < 3 * (hi - lo); q++)
+              bl32[3 * lo + q] = float(src[q]);
+          } else {
+            std::memcpy(bl + 3 * lo, los + 3 * (base + lo), size_t(hi - lo) * 24);
+          }
+        }
+        part[t] = Partial{ps, pw, uni};
+      });
+      bool chunk_uniform = true;
+      for (int t = 0; t < nt; t++) {   // fixed order: deterministic sums
+        wst += part[t].wst;
+        wt += part[t].wt;
+        chunk_uniform = chunk_uniform && part[t].uniform;
       }
-      if (los && hi > lo)
-        std::memcpy(bl + 3 * lo, los + 3 * (base + lo), size_t(hi - lo) * 24);
-      part[t] = Partial{ps, pw, uni};
-    });
-    bool chunk_uniform = true;
-    for (int t = 0; t < nt; t++) {   // fixed order: deterministic sums
-      wst += part[t].wst;
-      wt += part[t].wt;
-      chunk_uniform = chunk_uniform && part[t].uniform;
+      const size_t cb = size_t(m) * 8;
+      TRVG_HIP(hipMemcpyAsync(out.x.get() + base, bx, cb, hipMemcpyHostToDevice, cs));
+      TRVG_HIP(hipMemcpyAsync(out.y.get() + base, by, cb, hipMemcpyHostToDevice, cs));
+      TRVG_HIP(hipMemcpyAsync(out.z.get() + base, bz, cb, hipMemcpyHostToDevice, cs));
+      if (!(uniform && chunk_uniform)) {
+        if (uniform) fill_device(out.w.get(), base, w0, cs);   // the uniform prefix
+        uniform = false;
+        TRVG_HIP(hipMemcpyAsync(out.w.get() + base, bw, cb, hipMemcpyHostToDevice, cs));
+      }
+      if (los32) {
+        TRVG_HIP(hipMemcpyAsync(los_f.get() + 3 * base, bl32, 3 * size_t(m) * 4,
+                                hipMemcpyHostToDevice, cs));
+        widen_device(out.los.get() + 3 * base, los_f.get() + 3 * base, 3 * m, cs);
+      } else if (los) {
+        TRVG_HIP(hipMemcpyAsync(out.los.get() + 3 * base, bl, 3 * cb,
+                                hipMemcpyHostToDevice, cs));
+      }
+      ring.release(slot, cs);
+      if (sorter) {
+        // The positions of this chunk are on the device once the slot's
+        // event has passed (w is not read by the count).
+        TRVG_HIP(hipStreamWaitEvent(s, ring.ev[slot], 0));
+        sorter->count(base, base + m);
+      }
     }
-    const size_t cb = size_t(m) * 8;
-    TRVG_HIP(hipMemcpyAsync(out.x.get() + base, bx, cb, hipMemcpyHostToDevice, s));
-    TRVG_HIP(hipMemcpyAsync(out.y.get() + base, by, cb, hipMemcpyHostToDevice, s));
-    TRVG_HIP(hipMemcpyAsync(out.z.get() + base, bz, cb, hipMemcpyHostToDevice, s));
-    if (!(uniform && chunk_uniform)) {
-      if (uniform) fill_device(out.w.get(), base, w0, s);   // the uniform prefix
-      uniform = false;
-      TRVG_HIP(hipMemcpyAsync(out.w.get() + base, bw, cb, hipMemcpyHostToDevice, s));
+    if (uniform) fill_device(out.w.get(), n, w0, cs);
+    if (sorter) {
+      TRVG_HIP(hipEventRecord(ring.tail_event(), cs));
+      TRVG_HIP(hipStreamWaitEvent(s, ring.tail_event(), 0));
+      sorter->finish();
+      sort_for->keep_until_flush(std::move(los_f));
+      release_stream(dev, cs);
     }
-    if (los)
-      TRVG_HIP(hipMemcpyAsync(out.los.get() + 3 * base, bl, 3 * cb,
-                              hipMemcpyHostToDevice, s));
-    ring.release(slot, s);
+  } catch (...) {
+    if (sort_for) {
+      (void)hipStreamSynchronize(cs);
+      release_stream(dev, cs);
+    }
+    throw;
   }
-  if (uniform) fill_device(out.w.get(), n, w0, s);
   out.wstotal = wst;
   out.wtotal = wt;
   out.wuni = uniform ? w0 : NAN;
