@@ -129,6 +129,27 @@ typedef struct trvg_3pcf_result {
   double* zeta_shot;
 } trvg_3pcf_result;
 
+/* trv::PowspecMeasurements (include/dataobjs.hpp:202-211).  pk_raw and
+ * pk_shot hold 2*dim doubles (real, imaginary). */
+typedef struct trvg_powspec_result {
+  int dim;
+  double* kbin;
+  double* keff;
+  int* nmodes;
+  double* pk_raw;
+  double* pk_shot;
+} trvg_powspec_result;
+
+/* trv::TwoPCFMeasurements and trv::TwoPCFWindowMeasurements
+ * (include/dataobjs.hpp:217-238).  xi holds 2*dim doubles. */
+typedef struct trvg_2pcf_result {
+  int dim;
+  double* rbin;
+  double* reff;
+  int* npairs;
+  double* xi;
+} trvg_2pcf_result;
+
 /* ---------------------------------------------------------------------- */
 /* Library information and errors                                          */
 /* ---------------------------------------------------------------------- */
@@ -199,6 +220,73 @@ int trvg_compute_3pcf_in_gpp_box(
   const trvg_catalogue* data,
   const trvg_params* params, const trvg_binning* rbinning,
   double norm_factor, trvg_3pcf_result* out);
+
+
+/* ---------------------------------------------------------------------- */
+/* Two-point statistics (include/twopt.hpp; src/twopt.cpp).  params->ELL is */
+/* the multipole degree; interlacing follows params->interlace.  Result    */
+/* capacity: `dim` >= num_bins.                                            */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces trv::calc_powspec_normalisation_from_particles
+ * (include/twopt.hpp:89-91; src/twopt.cpp:56-96): 1 / (alpha sum ws nz wc^2). */
+int trvg_calc_powspec_normalisation_from_particles(
+  const trvg_catalogue* particles, double alpha, double* norm_factor);
+
+/* Replaces trv::calc_powspec_normalisation_from_mesh (include/twopt.hpp:
+ * 101-103; src/twopt.cpp:98-109): 1 / (V_cell sum_x n(x)^2) / alpha^2. */
+int trvg_calc_powspec_normalisation_from_mesh(
+  const trvg_catalogue* particles, const trvg_params* params, double alpha,
+  double* norm_factor);
+
+/* Replaces trv::calc_powspec_normalisation_from_meshes (include/twopt.hpp:
+ * 122-125; src/twopt.cpp:111-228) for catalogues already aligned in the
+ * box of `params`: 1 / (alpha V_cell sum_x n_d(x) n_r(x)).  The reference's
+ * re-alignment (centre or pad in that box) is the caller's step; the
+ * Python and C++ layers of this library perform it. */
+int trvg_calc_powspec_normalisation_from_meshes(
+  const trvg_catalogue* data, const trvg_catalogue* rand,
+  const trvg_params* params, double alpha, double* norm_factor);
+
+/* Replaces trv::calc_powspec_shotnoise_from_particles (include/twopt.hpp:
+ * 169-171; src/twopt.cpp:271-293): sum ws^2 wc^2. */
+int trvg_calc_powspec_shotnoise_from_particles(
+  const trvg_catalogue* particles, double alpha, double* shotnoise);
+
+/* Replaces trv::compute_powspec (include/twopt.hpp:246-251;
+ * src/twopt.cpp:388-496). */
+int trvg_compute_powspec(
+  const trvg_catalogue* data, const trvg_catalogue* rand,
+  const double* los_data, const double* los_rand,
+  const trvg_params* params, const trvg_binning* kbinning,
+  double norm_factor, trvg_powspec_result* out);
+
+/* Replaces trv::compute_corrfunc (include/twopt.hpp:266-271;
+ * src/twopt.cpp:498-607). */
+int trvg_compute_corrfunc(
+  const trvg_catalogue* data, const trvg_catalogue* rand,
+  const double* los_data, const double* los_rand,
+  const trvg_params* params, const trvg_binning* rbinning,
+  double norm_factor, trvg_2pcf_result* out);
+
+/* Replaces trv::compute_powspec_in_gpp_box (include/twopt.hpp:283-287;
+ * src/twopt.cpp:609-701). */
+int trvg_compute_powspec_in_gpp_box(
+  const trvg_catalogue* data, const trvg_params* params,
+  const trvg_binning* kbinning, double norm_factor, trvg_powspec_result* out);
+
+/* Replaces trv::compute_corrfunc_in_gpp_box (include/twopt.hpp:299-303;
+ * src/twopt.cpp:703-793). */
+int trvg_compute_corrfunc_in_gpp_box(
+  const trvg_catalogue* data, const trvg_params* params,
+  const trvg_binning* rbinning, double norm_factor, trvg_2pcf_result* out);
+
+/* Replaces trv::compute_corrfunc_window (include/twopt.hpp:317-321;
+ * src/twopt.cpp:795-901). */
+int trvg_compute_corrfunc_window(
+  const trvg_catalogue* rand, const double* los_rand,
+  const trvg_params* params, const trvg_binning* rbinning, double alpha,
+  double norm_factor, trvg_2pcf_result* out);
 
 
 /* ---------------------------------------------------------------------- */

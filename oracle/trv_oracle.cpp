@@ -628,6 +628,15 @@ struct Mesh {
     }
   }
 
+  // field.cpp:1325-1362: one catalogue, Y_lm-weighted, times alpha (the
+  // main field only: the interlacing shadow keeps the unscaled sum).
+  void ylm_wgtd_field_single(const Catalogue& c, const double* los,
+                             double alpha, int ell, int m) {
+    assign(c, ylm_weights(c, los, ell, m, 1, false));
+    for (long long g = 0; g < p.nmesh; g++)
+      f[g] = cplx(f[g].real() * alpha, f[g].imag() * alpha);
+  }
+
   // field.cpp:1496-1655
   void fourier_transform() {
     for (long long g = 0; g < p.nmesh; g++) f[g] *= vol_cell;
@@ -931,6 +940,60 @@ struct Stats {
         }
     fft3d(t.data(), p.ngrid, +1);
     return t;
+  }
+
+  // field.cpp:2705-2945: the shot-noise-subtracted product transformed
+  // back, then 1e6 fine bins of width 0.1 by int(r / 0.1), Y_lm(r^)
+  // weighted, assigned to coarse bins by the fine lower edge, mean per bin
+  // (empty bins: centre and 0).
+  void twopt_config(const Mesh& fa, const Mesh& fb, cplx sn_amp, int ell,
+                    int m, const trvg_binning& rb) {
+    std::vector<cplx> t = twopt_config_field(fa, fb, sn_amp);
+    const int n_sample = 1000000;
+    const double dr_sample = 1.e-1;
+    std::vector<int> np_s(n_sample, 0);
+    std::vector<double> r_s(n_sample, 0.), xr(n_sample, 0.), xim(n_sample, 0.);
+#pragma omp parallel for collapse(3)
+    for (int i = 0; i < p.ngrid[0]; i++)
+      for (int j = 0; j < p.ngrid[1]; j++)
+        for (int k2 = 0; k2 < p.ngrid[2]; k2++) {
+          long long g = geom.gidx(i, j, k2);
+          double rv[3];
+          geom.rvec(i, j, k2, rv);
+          double rm = mag3(rv);
+          int ir = int(rm / dr_sample);
+          if (0 <= ir && ir < n_sample) {
+            cplx xp = t[g] * ylm_reduced(ell, m, rv);
+#pragma omp atomic
+            np_s[ir]++;
+#pragma omp atomic
+            r_s[ir] += rm;
+#pragma omp atomic
+            xr[ir] += xp.real();
+#pragma omp atomic
+            xim[ir] += xp.imag();
+          }
+        }
+    int nb = rb.num_bins;
+    npairs.assign(nb, 0); r.assign(nb, 0.); xi.assign(nb, 0.);
+    for (int b = 0; b < nb; b++) {
+      double lo = rb.bin_edges[b], hi = rb.bin_edges[b + 1];
+      for (int i = 0; i < n_sample; i++) {
+        double rr = i * dr_sample;
+        if (lo <= rr && rr < hi) {
+          npairs[b] += np_s[i];
+          r[b] += r_s[i];
+          xi[b] += cplx(xr[i], xim[i]);
+        }
+      }
+      if (npairs[b] != 0) {
+        r[b] /= double(npairs[b]);
+        xi[b] /= double(npairs[b]);
+      } else {
+        r[b] = rb.bin_centres[b];
+        xi[b] = 0.;
+      }
+    }
   }
 
   // field.cpp:2947-3196: fine bins of width 1 (1e5 bins), mean, then the
@@ -1466,6 +1529,135 @@ Output threepcf_impl(bool survey, const Catalogue& d, const Catalogue* r,
   return out;
 }
 
+// --------------------------------------------------------------------------
+// Two-point statistics (src/twopt.cpp)
+// --------------------------------------------------------------------------
+
+// twopt.cpp:45-49
+double coupling_2pt(int ell, int ELL, int m, int M) {
+  return (2 * ell + 1) * (2 * ELL + 1) * wigner_3j(ell, 0, ELL, 0, 0, 0) *
+         wigner_3j(ell, 0, ELL, m, 0, M);
+}
+
+// twopt.cpp:298-381: sum_p y_lm(los_p) w_p^2 over one catalogue.
+cplx sn_amp_powspec(const Catalogue& c, const double* los, int ell, int m) {
+  double sr = 0., si = 0.;
+#pragma omp parallel for reduction(+ : sr, si)
+  for (long long i = 0; i < c.n; i++) {
+    double l3[3] = {los[3 * i], los[3 * i + 1], los[3 * i + 2]};
+    cplx y = ylm_reduced(ell, m, l3) * std::pow(c.w[i], 2);
+    sr += y.real();
+    si += y.imag();
+  }
+  return cplx(sr, si);
+}
+
+struct Output2 {
+  std::vector<double> bin, eff;
+  std::vector<int> cnt;
+  std::vector<cplx> raw, shot;
+  explicit Output2(int nb) : bin(nb, 0.), eff(nb, 0.), cnt(nb, 0), raw(nb, 0.), shot(nb, 0.) {}
+};
+
+// The survey-type two-point body (twopt.cpp:388-496 in Fourier space,
+// 498-607 in configuration space, 795-901 for the window of one random
+// catalogue): dn_00, then for every M the field dn_LM, its shot-noise
+// amplitude and the coupled statistics of dn_LM against dn_00.
+Output2 twopt_survey_impl(bool fourier, const Catalogue* d, const Catalogue& r,
+                          const double* los_d, const double* los_r,
+                          const Params& p, const trvg_binning& b,
+                          double alpha, double norm_factor) {
+  const int ELL = p.ELL, ell1 = p.ELL, nb = b.num_bins;
+  Output2 out(nb);
+  Mesh dn00(p);
+  if (d) dn00.ylm_wgtd_field(*d, r, los_d, los_r, alpha, 0, 0);
+  else dn00.ylm_wgtd_field_single(r, los_r, alpha, 0, 0);
+  dn00.fourier_transform();
+  Stats stats(p);
+  for (int M = -ELL; M <= ELL; M++) {
+    Mesh dnLM(p);
+    if (d) dnLM.ylm_wgtd_field(*d, r, los_d, los_r, alpha, ELL, M);
+    else dnLM.ylm_wgtd_field_single(r, los_r, alpha, ELL, M);
+    dnLM.fourier_transform();
+    cplx sn_amp = d ? sn_amp_powspec(*d, los_d, ELL, M) +
+                          std::pow(alpha, 2) * sn_amp_powspec(r, los_r, ELL, M)
+                    : std::pow(alpha, 2) * sn_amp_powspec(r, los_r, ELL, M);
+    for (int m1 = -ell1; m1 <= ell1; m1++) {
+      double coupling = coupling_2pt(ell1, ELL, m1, M);
+      if (std::fabs(coupling) < 1.e-10) continue;
+      if (fourier) {
+        stats.twopt_fourier(dnLM, dn00, sn_amp, ell1, m1, b);
+        for (int i = 0; i < nb; i++) {
+          out.raw[i] += coupling * stats.pk[i];
+          out.shot[i] += coupling * stats.sn[i];
+        }
+      } else {
+        stats.twopt_config(dnLM, dn00, sn_amp, ell1, m1, b);
+        for (int i = 0; i < nb; i++) out.raw[i] += coupling * stats.xi[i];
+      }
+      if (M == 0 && m1 == 0) {
+        for (int i = 0; i < nb; i++) {
+          out.cnt[i] = fourier ? stats.nmodes[i] : stats.npairs[i];
+          out.eff[i] = fourier ? stats.k[i] : stats.r[i];
+        }
+      }
+    }
+  }
+  for (int i = 0; i < nb; i++) {
+    out.bin[i] = b.bin_centres[i];
+    out.raw[i] *= norm_factor;
+    out.shot[i] *= norm_factor;
+  }
+  return out;
+}
+
+// twopt.cpp:609-701 (Fourier) and 703-793 (configuration): the unweighted
+// field fluctuation against itself, (2 ELL + 1) times the M = 0 statistic.
+Output2 twopt_box_impl(bool fourier, const Catalogue& d, const Params& p,
+                       const trvg_binning& b, double norm_factor) {
+  const int nb = b.num_bins;
+  Output2 out(nb);
+  Mesh dn(p);
+  dn.unweighted_fluctuations(d);
+  dn.fourier_transform();
+  cplx sn_amp = double(d.n);
+  Stats stats(p);
+  const double f = double(2 * p.ELL + 1);
+  if (fourier) stats.twopt_fourier(dn, dn, sn_amp, p.ELL, 0, b);
+  else stats.twopt_config(dn, dn, sn_amp, p.ELL, 0, b);
+  for (int i = 0; i < nb; i++) {
+    out.bin[i] = b.bin_centres[i];
+    out.cnt[i] = fourier ? stats.nmodes[i] : stats.npairs[i];
+    out.eff[i] = fourier ? stats.k[i] : stats.r[i];
+    out.raw[i] = norm_factor * f * (fourier ? stats.pk[i] : stats.xi[i]);
+    out.shot[i] = fourier ? norm_factor * f * stats.sn[i] : cplx(0., 0.);
+  }
+  return out;
+}
+
+void write_powspec(const Output2& o, trvg_powspec_result* res) {
+  const int nb = int(o.bin.size());
+  if (res->dim < nb)
+    fail(TRVG_ERR_PARAM, "Result capacity %d < data-vector dimension %d.", res->dim, nb);
+  for (int i = 0; i < nb; i++) {
+    res->kbin[i] = o.bin[i]; res->keff[i] = o.eff[i]; res->nmodes[i] = o.cnt[i];
+    res->pk_raw[2 * i] = o.raw[i].real(); res->pk_raw[2 * i + 1] = o.raw[i].imag();
+    res->pk_shot[2 * i] = o.shot[i].real(); res->pk_shot[2 * i + 1] = o.shot[i].imag();
+  }
+  res->dim = nb;
+}
+
+void write_2pcf(const Output2& o, trvg_2pcf_result* res) {
+  const int nb = int(o.bin.size());
+  if (res->dim < nb)
+    fail(TRVG_ERR_PARAM, "Result capacity %d < data-vector dimension %d.", res->dim, nb);
+  for (int i = 0; i < nb; i++) {
+    res->rbin[i] = o.bin[i]; res->reff[i] = o.eff[i]; res->npairs[i] = o.cnt[i];
+    res->xi[2 * i] = o.raw[i].real(); res->xi[2 * i + 1] = o.raw[i].imag();
+  }
+  res->dim = nb;
+}
+
 void write_bispec(const Output& o, trvg_bispec_result* res) {
   if (res->dim < o.dim)
     fail(TRVG_ERR_PARAM, "Result capacity %d < data-vector dimension %d.",
@@ -1747,6 +1939,123 @@ int trvo_compute_3pcf_in_gpp_box(const trvg_catalogue* data,
     p.interlace = false;
     Catalogue d = load_catalogue(data);
     write_3pcf(threepcf_impl(false, d, nullptr, nullptr, nullptr, p, *rb, norm_factor), out);
+  });
+}
+
+int trvo_calc_powspec_normalisation_from_particles(const trvg_catalogue* cat,
+                                                   double alpha, double* norm) {
+  return guarded([&] {
+    Catalogue c = load_catalogue(cat);
+    double s = 0.;
+    for (long long i = 0; i < c.n; i++) s += c.ws[i] * c.nz[i] * std::pow(c.wc[i], 2);
+    if (s == 0.)
+      fail(TRVG_ERR_DATA,
+           "Particle 'nz' values appear to be all zeros. "
+           "Check the input catalogue contains valid 'nz' field.");
+    *norm = 1. / (alpha * s);
+  });
+}
+
+int trvo_calc_powspec_normalisation_from_mesh(const trvg_catalogue* cat,
+                                              const trvg_params* params,
+                                              double alpha, double* norm) {
+  return guarded([&] {
+    Params p = make_params(params);
+    Catalogue c = load_catalogue(cat);
+    Mesh m(p);
+    *norm = m.grid_powlaw_norm(c, 2) / std::pow(alpha, 2);
+  });
+}
+
+// twopt.cpp:111-228 after the re-alignment: both catalogues assigned with
+// their weights w, 1 / (alpha V_cell sum_x n_d n_r).
+int trvo_calc_powspec_normalisation_from_meshes(const trvg_catalogue* data,
+                                                const trvg_catalogue* rand,
+                                                const trvg_params* params,
+                                                double alpha, double* norm) {
+  return guarded([&] {
+    Params p = make_params(params);
+    Catalogue d = load_catalogue(data), r = load_catalogue(rand);
+    Mesh md(p), mr(p);
+    std::vector<cplx> wd(d.n), wr(r.n);
+    for (long long i = 0; i < d.n; i++) wd[i] = cplx(d.w[i], 0.);
+    for (long long i = 0; i < r.n; i++) wr[i] = cplx(r.w[i], 0.);
+    md.assign(d, wd);
+    mr.assign(r, wr);
+    double s = 0.;
+    for (long long g = 0; g < p.nmesh; g++) s += md.f[g].real() * mr.f[g].real();
+    *norm = 1. / (alpha * md.vol_cell * s);
+  });
+}
+
+int trvo_calc_powspec_shotnoise_from_particles(const trvg_catalogue* cat,
+                                               double alpha, double* sn) {
+  return guarded([&] {
+    (void)alpha;
+    Catalogue c = load_catalogue(cat);
+    double s = 0.;
+    for (long long i = 0; i < c.n; i++) s += std::pow(c.ws[i], 2) * std::pow(c.wc[i], 2);
+    *sn = s;
+  });
+}
+
+int trvo_compute_powspec(const trvg_catalogue* data, const trvg_catalogue* rand,
+                         const double* los_data, const double* los_rand,
+                         const trvg_params* params, const trvg_binning* kb,
+                         double norm_factor, trvg_powspec_result* out) {
+  return guarded([&] {
+    Params p = make_params(params);
+    Catalogue d = load_catalogue(data), r = load_catalogue(rand);
+    double alpha = d.wstotal / r.wstotal;
+    write_powspec(twopt_survey_impl(true, &d, r, los_data, los_rand, p, *kb, alpha,
+                                    norm_factor), out);
+  });
+}
+
+int trvo_compute_corrfunc(const trvg_catalogue* data, const trvg_catalogue* rand,
+                          const double* los_data, const double* los_rand,
+                          const trvg_params* params, const trvg_binning* rb,
+                          double norm_factor, trvg_2pcf_result* out) {
+  return guarded([&] {
+    Params p = make_params(params);
+    Catalogue d = load_catalogue(data), r = load_catalogue(rand);
+    double alpha = d.wstotal / r.wstotal;
+    write_2pcf(twopt_survey_impl(false, &d, r, los_data, los_rand, p, *rb, alpha,
+                                 norm_factor), out);
+  });
+}
+
+int trvo_compute_powspec_in_gpp_box(const trvg_catalogue* data,
+                                    const trvg_params* params,
+                                    const trvg_binning* kb, double norm_factor,
+                                    trvg_powspec_result* out) {
+  return guarded([&] {
+    Params p = make_params(params);
+    Catalogue d = load_catalogue(data);
+    write_powspec(twopt_box_impl(true, d, p, *kb, norm_factor), out);
+  });
+}
+
+int trvo_compute_corrfunc_in_gpp_box(const trvg_catalogue* data,
+                                     const trvg_params* params,
+                                     const trvg_binning* rb, double norm_factor,
+                                     trvg_2pcf_result* out) {
+  return guarded([&] {
+    Params p = make_params(params);
+    Catalogue d = load_catalogue(data);
+    write_2pcf(twopt_box_impl(false, d, p, *rb, norm_factor), out);
+  });
+}
+
+int trvo_compute_corrfunc_window(const trvg_catalogue* rand, const double* los_rand,
+                                 const trvg_params* params, const trvg_binning* rb,
+                                 double alpha, double norm_factor,
+                                 trvg_2pcf_result* out) {
+  return guarded([&] {
+    Params p = make_params(params);
+    Catalogue r = load_catalogue(rand);
+    write_2pcf(twopt_survey_impl(false, nullptr, r, nullptr, los_rand, p, *rb, alpha,
+                                 norm_factor), out);
   });
 }
 

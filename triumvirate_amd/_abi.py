@@ -64,6 +64,16 @@ class ThreePCFResult(C.Structure):
     ]
 
 
+class PowspecResult(C.Structure):
+    _fields_ = [('dim', C.c_int), ('kbin', _dp), ('keff', _dp),
+                ('nmodes', _ip), ('pk_raw', _dp), ('pk_shot', _dp)]
+
+
+class TwoPCFResult(C.Structure):
+    _fields_ = [('dim', C.c_int), ('rbin', _dp), ('reff', _dp),
+                ('npairs', _ip), ('xi', _dp)]
+
+
 def dptr(a):
     """Pointer to a C-contiguous float64 array (or NULL for None)."""
     if a is None:
@@ -144,6 +154,36 @@ class ResultBuffers:
         }
 
 
+class TwoPtBuffers:
+    """Caller-owned output arrays for one power spectrum ('powspec') or
+    correlation function ('2pcf') call."""
+
+    def __init__(self, num_bins, kind):
+        self.kind = kind
+        self.bin = np.zeros(num_bins)
+        self.eff = np.zeros(num_bins)
+        self.cnt = np.zeros(num_bins, dtype=np.intc)
+        self.raw = np.zeros(2 * num_bins)
+        self.shot = np.zeros(2 * num_bins)
+        if kind == 'powspec':
+            self.struct = PowspecResult(num_bins, dptr(self.bin), dptr(self.eff),
+                                        iptr(self.cnt), dptr(self.raw),
+                                        dptr(self.shot))
+        else:
+            self.struct = TwoPCFResult(num_bins, dptr(self.bin), dptr(self.eff),
+                                       iptr(self.cnt), dptr(self.raw))
+
+    def as_dict(self):
+        n = self.struct.dim
+        raw = self.raw[:2 * n].view(np.complex128).copy()
+        if self.kind == 'powspec':
+            return {'kbin': self.bin[:n].copy(), 'keff': self.eff[:n].copy(),
+                    'nmodes': self.cnt[:n].astype(np.int64), 'pk_raw': raw,
+                    'pk_shot': self.shot[:2 * n].view(np.complex128).copy()}
+        return {'rbin': self.bin[:n].copy(), 'reff': self.eff[:n].copy(),
+                'npairs': self.cnt[:n].astype(np.int64), 'xi': raw}
+
+
 def result_dim(shape_code, num_bins, idx_bin):
     """Data-vector length per shape (compute_bispec dv_dim switch)."""
     if shape_code in (0, 2):
@@ -181,6 +221,37 @@ def declare_common(lib, prefix):
     f.restype = C.c_int
     f = getattr(lib, prefix + 'calc_bispec_normalisation_from_mesh')
     f.argtypes = [P(Catalogue), P(Params), C.c_double, _dp]
+    f.restype = C.c_int
+    for name in ('calc_powspec_normalisation_from_particles',
+                 'calc_powspec_shotnoise_from_particles'):
+        f = getattr(lib, prefix + name)
+        f.argtypes = [P(Catalogue), C.c_double, _dp]
+        f.restype = C.c_int
+    f = getattr(lib, prefix + 'calc_powspec_normalisation_from_mesh')
+    f.argtypes = [P(Catalogue), P(Params), C.c_double, _dp]
+    f.restype = C.c_int
+    f = getattr(lib, prefix + 'calc_powspec_normalisation_from_meshes')
+    f.argtypes = [P(Catalogue), P(Catalogue), P(Params), C.c_double, _dp]
+    f.restype = C.c_int
+    f = getattr(lib, prefix + 'compute_powspec')
+    f.argtypes = [P(Catalogue), P(Catalogue), _dp, _dp, P(Params), P(Binning),
+                  C.c_double, P(PowspecResult)]
+    f.restype = C.c_int
+    f = getattr(lib, prefix + 'compute_corrfunc')
+    f.argtypes = [P(Catalogue), P(Catalogue), _dp, _dp, P(Params), P(Binning),
+                  C.c_double, P(TwoPCFResult)]
+    f.restype = C.c_int
+    f = getattr(lib, prefix + 'compute_powspec_in_gpp_box')
+    f.argtypes = [P(Catalogue), P(Params), P(Binning), C.c_double,
+                  P(PowspecResult)]
+    f.restype = C.c_int
+    f = getattr(lib, prefix + 'compute_corrfunc_in_gpp_box')
+    f.argtypes = [P(Catalogue), P(Params), P(Binning), C.c_double,
+                  P(TwoPCFResult)]
+    f.restype = C.c_int
+    f = getattr(lib, prefix + 'compute_corrfunc_window')
+    f.argtypes = [P(Catalogue), _dp, P(Params), P(Binning), C.c_double,
+                  C.c_double, P(TwoPCFResult)]
     f.restype = C.c_int
     f = getattr(lib, prefix + 'result_dim')
     f.argtypes = [C.c_int, C.c_int, C.c_int, _ip]

@@ -121,6 +121,93 @@ trvg_params mesh_params(const trvg_params* p) {
   return q;
 }
 
+// Parameters of an assignment-only operation (no transform): any positive
+// mesh numbers, as the reference's mixed-mesh normalisation grids are
+// (an even number of cells of a fixed size).
+trvg_params assign_only_params(const trvg_params* p) {
+  if (p == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
+  trvg_params q = *p;
+  q.interlace = 0;
+  q.shape = TRVG_SHAPE_DIAG;
+  q.idx_bin = 0;
+  q.ell1 = q.ell2 = q.ELL = 0;
+  for (int a = 0; a < 3; a++) {
+    if (!(q.boxsize[a] > 0.)) throw Error(TRVG_ERR_PARAM, "Box size must be positive.");
+    if (q.ngrid[a] < 1 || q.ngrid[a] > 4096)
+      throw Error(TRVG_ERR_PARAM, "Mesh number per dimension must be in [1, 4096].");
+  }
+  if (q.assignment_order < 1 || q.assignment_order > 4)
+    throw Error(TRVG_ERR_PARAM,
+                "Mesh assignment scheme must be 'ngp', 'cic', 'tsc' or 'pcs'.");
+  return q;
+}
+
+// Parameters of a two-point call: the degree is ELL, interlacing is kept;
+// the three-point shape fields play no part.
+trvg_params two_point_params(const trvg_params* p, int num_bins) {
+  if (p == nullptr) throw Error(TRVG_ERR_PARAM, "Null parameters.");
+  trvg_params q = *p;
+  if (q.ELL < 0) throw Error(TRVG_ERR_PARAM, "Multipole degree must be non-negative.");
+  q.shape = TRVG_SHAPE_DIAG;
+  q.idx_bin = 0;
+  q.ell1 = q.ELL;
+  q.ell2 = 0;
+  q.precision = TRVG_PREC_FP64;   // two-point fields are FP64
+  validate_params(q, num_bins);
+  return q;
+}
+
+void write_powspec(const TwoPtResult& r, trvg_powspec_result* out) {
+  if (out == nullptr) throw Error(TRVG_ERR_PARAM, "Null result.");
+  const int nb = int(r.bin.size());
+  if (out->dim < nb)
+    throw Error(TRVG_ERR_PARAM, "Result capacity " + std::to_string(out->dim) +
+                                    " < data-vector dimension " + std::to_string(nb) + ".");
+  for (int i = 0; i < nb; i++) {
+    out->kbin[i] = r.bin[i];
+    out->keff[i] = r.eff[i];
+    out->nmodes[i] = r.count[i];
+    out->pk_raw[2 * i] = r.raw[i].real(); out->pk_raw[2 * i + 1] = r.raw[i].imag();
+    out->pk_shot[2 * i] = r.shot[i].real(); out->pk_shot[2 * i + 1] = r.shot[i].imag();
+  }
+  out->dim = nb;
+}
+
+void write_2pcf(const TwoPtResult& r, trvg_2pcf_result* out) {
+  if (out == nullptr) throw Error(TRVG_ERR_PARAM, "Null result.");
+  const int nb = int(r.bin.size());
+  if (out->dim < nb)
+    throw Error(TRVG_ERR_PARAM, "Result capacity " + std::to_string(out->dim) +
+                                    " < data-vector dimension " + std::to_string(nb) + ".");
+  for (int i = 0; i < nb; i++) {
+    out->rbin[i] = r.bin[i];
+    out->reff[i] = r.eff[i];
+    out->npairs[i] = r.count[i];
+    out->xi[2 * i] = r.raw[i].real(); out->xi[2 * i + 1] = r.raw[i].imag();
+  }
+  out->dim = nb;
+}
+
+// sum_p f(ws_p, wc_p, nz_p) over a host catalogue.
+template <class F>
+double particle_sum(const trvg_catalogue* c, F f) {
+  if (c == nullptr || c->x == nullptr)
+    throw Error(TRVG_ERR_DATA, "Particle data are uninitialised.");
+  const long long st = c->stride > 0 ? c->stride : 1;
+  double s = 0.;
+  for (long long i = 0; i < c->ntotal; i++) {
+    const double ws = c->ws ? c->ws[i * st] : 1.;
+    const double wc = c->wc ? c->wc[i * st] : 1.;
+    const double nz = c->nz ? c->nz[i * st] : 0.;
+    s += f(ws, wc, nz);
+  }
+  return s;
+}
+
+double alpha_of(const DeviceCatalogue& d, const DeviceCatalogue& r) {
+  return d.wstotal / r.wstotal;
+}
+
 // FP32-field calls take lines of sight as FP32 across the bus.
 bool fp32(const Context& ctx) { return ctx.p.precision == TRVG_PREC_FP32; }
 
@@ -265,6 +352,142 @@ int trvg_compute_3pcf_in_gpp_box(const trvg_catalogue* data,
     DeviceCatalogue d;
     upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
     write_3pcf(dispatch_3pcf(ctx, d, nullptr, false, *rb, norm_factor), out);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Two-point statistics (include/twopt.hpp).
+// ---------------------------------------------------------------------------
+
+int trvg_calc_powspec_normalisation_from_particles(const trvg_catalogue* c, double alpha,
+                                                   double* norm_factor) {
+  return guarded([&] {
+    // twopt.cpp:56-96: 1 / (alpha sum ws nz wc^2).
+    const double s = particle_sum(c, [](double ws, double wc, double nz) {
+      return ws * nz * std::pow(wc, 2);
+    });
+    if (s == 0.)
+      throw Error(TRVG_ERR_DATA,
+                  "Particle 'nz' values appear to be all zeros. "
+                  "Check the input catalogue contains valid 'nz' field.");
+    *norm_factor = 1. / (alpha * s);
+  });
+}
+
+int trvg_calc_powspec_normalisation_from_mesh(const trvg_catalogue* c,
+                                              const trvg_params* params, double alpha,
+                                              double* norm_factor) {
+  return guarded([&] {
+    require_device();
+    trvg_params q = assign_only_params(params);
+    Context ctx(q);
+    DeviceCatalogue dc;
+    upload_catalogue(c, nullptr, dc, ctx.stream, &ctx);
+    *norm_factor = 1. / mesh_square_integral(ctx, dc) / std::pow(alpha, 2);
+  });
+}
+
+int trvg_calc_powspec_normalisation_from_meshes(const trvg_catalogue* data,
+                                                const trvg_catalogue* rand,
+                                                const trvg_params* params, double alpha,
+                                                double* norm_factor) {
+  return guarded([&] {
+    require_device();
+    trvg_params q = assign_only_params(params);
+    Context ctx(q);
+    DeviceCatalogue d, r;
+    upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
+    upload_catalogue(rand, nullptr, r, ctx.stream, &ctx);
+    *norm_factor = 1. / (alpha * mesh_cross_integral(ctx, d, r));
+  });
+}
+
+int trvg_calc_powspec_shotnoise_from_particles(const trvg_catalogue* c, double alpha,
+                                               double* shotnoise) {
+  return guarded([&] {
+    (void)alpha;   // twopt.cpp:271-293 does not use it
+    *shotnoise = particle_sum(c, [](double ws, double wc, double) {
+      return std::pow(ws, 2) * std::pow(wc, 2);
+    });
+  });
+}
+
+int trvg_compute_powspec(const trvg_catalogue* data, const trvg_catalogue* rand,
+                         const double* los_data, const double* los_rand,
+                         const trvg_params* params, const trvg_binning* kb,
+                         double norm_factor, trvg_powspec_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(kb);
+    trvg_params q = two_point_params(params, kb->num_bins);
+    require_los(los_data, "data");
+    require_los(los_rand, "random");
+    Context ctx(q);
+    DeviceCatalogue d, r;
+    upload_catalogue(data, los_data, d, ctx.stream, &ctx);
+    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx);
+    write_powspec(run_twopt_survey(ctx, &d, r, true, *kb, alpha_of(d, r), norm_factor), out);
+  });
+}
+
+int trvg_compute_corrfunc(const trvg_catalogue* data, const trvg_catalogue* rand,
+                          const double* los_data, const double* los_rand,
+                          const trvg_params* params, const trvg_binning* rb,
+                          double norm_factor, trvg_2pcf_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(rb);
+    trvg_params q = two_point_params(params, rb->num_bins);
+    require_los(los_data, "data");
+    require_los(los_rand, "random");
+    Context ctx(q);
+    DeviceCatalogue d, r;
+    upload_catalogue(data, los_data, d, ctx.stream, &ctx);
+    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx);
+    write_2pcf(run_twopt_survey(ctx, &d, r, false, *rb, alpha_of(d, r), norm_factor), out);
+  });
+}
+
+int trvg_compute_powspec_in_gpp_box(const trvg_catalogue* data, const trvg_params* params,
+                                    const trvg_binning* kb, double norm_factor,
+                                    trvg_powspec_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(kb);
+    trvg_params q = two_point_params(params, kb->num_bins);
+    Context ctx(q);
+    DeviceCatalogue d;
+    upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
+    write_powspec(run_twopt_box(ctx, d, true, *kb, norm_factor), out);
+  });
+}
+
+int trvg_compute_corrfunc_in_gpp_box(const trvg_catalogue* data, const trvg_params* params,
+                                     const trvg_binning* rb, double norm_factor,
+                                     trvg_2pcf_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(rb);
+    trvg_params q = two_point_params(params, rb->num_bins);
+    Context ctx(q);
+    DeviceCatalogue d;
+    upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
+    write_2pcf(run_twopt_box(ctx, d, false, *rb, norm_factor), out);
+  });
+}
+
+int trvg_compute_corrfunc_window(const trvg_catalogue* rand, const double* los_rand,
+                                 const trvg_params* params, const trvg_binning* rb,
+                                 double alpha, double norm_factor, trvg_2pcf_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(rb);
+    trvg_params q = two_point_params(params, rb->num_bins);
+    require_los(los_rand, "random");
+    Context ctx(q);
+    DeviceCatalogue r;
+    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx);
+    write_2pcf(run_twopt_survey(ctx, nullptr, r, false, *rb, alpha, norm_factor), out);
   });
 }
 

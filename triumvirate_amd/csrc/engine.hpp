@@ -374,6 +374,32 @@ template <typename T>
 void assign_field(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c, int mode,
                   int ell, int m, double scale);
 
+// Add catalogue `c` with weight mode and scale into `mesh` (no zeroing).
+template <typename T>
+void assign_add(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c, int mode,
+                int ell, int m, double scale);
+
+// Two-point statistics (engine_twopt.hip).  Binned values of one
+// power-spectrum or correlation-function multipole.
+struct TwoPtResult {
+  std::vector<double> bin, eff;
+  std::vector<int> count;
+  std::vector<cplx> raw, shot;
+};
+// Survey pair (d, r) or, with d null, the window of one random catalogue;
+// fourier selects the power spectrum, else the correlation function.
+TwoPtResult run_twopt_survey(Context& ctx, const DeviceCatalogue* d,
+                             const DeviceCatalogue& r, bool fourier,
+                             const trvg_binning& b, double alpha, double norm);
+// Periodic box (global plane-parallel).
+TwoPtResult run_twopt_box(Context& ctx, const DeviceCatalogue& d, bool fourier,
+                          const trvg_binning& b, double norm);
+// sum_x n(x)^2 V_cell of the w-weighted mesh, and sum_x n_d(x) n_r(x)
+// V_cell of two catalogues (the mesh normalisations of the power spectrum).
+double mesh_square_integral(Context& ctx, const DeviceCatalogue& c);
+double mesh_cross_integral(Context& ctx, const DeviceCatalogue& d,
+                           const DeviceCatalogue& r);
+
 // Estimators.
 struct Result {
   int dim = 0;

@@ -935,6 +935,14 @@ static void assign_dual(Context& ctx, T* ra, T* rb, const DeviceCatalogue& c, in
 }
 
 template <typename T>
+void assign_add(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c, int mode,
+                int ell, int m, double scale) {
+  assign<T>(ctx, mesh, c, mode, ell, m, scale);
+}
+template void assign_add<double>(Context&, double2*, const DeviceCatalogue&, int, int,
+                                 int, double);
+
+template <typename T>
 void assign_field(Context& ctx, c2<T>* mesh, const DeviceCatalogue& c, int mode,
                   int ell, int m, double scale) {
   zero_field<T>(ctx, mesh);
@@ -1944,6 +1952,14 @@ template void fft3d<double, LoadSame<double>>(Context&, double2*,
 template void fft3d<float, LoadSame<float>>(Context&, float2*,
                                             const LoadSame<float>&, int, double,
                                             const Prune*);
+
+// Loaders of the two-point path (engine_twopt.hip).
+template void fft3d<double, LoadScaleShift<double>>(Context&, double2*,
+                                                    const LoadScaleShift<double>&, int,
+                                                    double, const Prune*);
+template void fft3d<double, LoadTwopt<double, double>>(Context&, double2*,
+                                                       const LoadTwopt<double, double>&,
+                                                       int, double, const Prune*);
 
 // Loaders of the differentiable path (engine_diff.hip).
 #define TRVG_FFT3D_INST(T, L)                                                   \

@@ -91,10 +91,20 @@ k_assign(c2<T>* __restrict__ mesh, DevParticles p, Geom g, int mode, int ell,
           win[0][a] = 0.5 * (0.5 - s) * (0.5 - s);
           win[1][a] = 0.75 - s * s;
           win[2][a] = 0.5 * (0.5 + s) * (0.5 + s);
-        } else {
+        } else if (!shadow) {
           ijk[0][a] = idx;
           ijk[1][a] = (idx == ng - 1) ? 0 : idx + 1;
           ijk[2][a] = (ijk[1][a] == ng - 1) ? 0 : ijk[1][a] + 1;
+          double sp = 1. - s;
+          win[0][a] = 0.5 * (0.5 + sp) * (0.5 + sp);
+          win[1][a] = 0.75 - sp * sp;
+          win[2][a] = 0.5 * (0.5 - sp) * (0.5 - sp);
+        } else {
+          // The reference's shadow pass tests the base index for the last
+          // cell too (field.cpp:944-950): kept for identical fields.
+          ijk[0][a] = idx;
+          ijk[1][a] = (idx == ng - 1) ? 0 : idx + 1;
+          ijk[2][a] = (idx == ng - 1) ? 0 : ijk[1][a] + 1;
           double sp = 1. - s;
           win[0][a] = 0.5 * (0.5 + sp) * (0.5 + sp);
           win[1][a] = 0.75 - sp * sp;
@@ -476,16 +486,18 @@ k_reduce_cols(const V* __restrict__ part, int nrows, int width, V* __restrict__ 
 // Output per block: [nb] counts (global atomics) and [nb][4] doubles.
 // ---------------------------------------------------------------------------
 
-template <typename T>
+template <typename T, int NV = 4>
 __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
                 Geom g, AxisTables tab, double2 sbar, int ell, int m,
                 const double* __restrict__ edges, int nb,
                 unsigned long long* __restrict__ counts,
                 double* __restrict__ part, double fb_dc = 0.) {
-  extern __shared__ double sh2[];   // [nb][4] then counts
-  unsigned int* sh_cnt = reinterpret_cast<unsigned int*>(sh2 + 4 * nb);
+  // NV = 4: sums of pk (re, im) and sn (re, im) per bin; NV = 5 adds the
+  // sum of |k| (the power spectrum's effective wavenumber).
+  extern __shared__ double sh2[];   // [nb][NV] then counts
+  unsigned int* sh_cnt = reinterpret_cast<unsigned int*>(sh2 + NV * nb);
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    sh2[4 * b] = sh2[4 * b + 1] = sh2[4 * b + 2] = sh2[4 * b + 3] = 0.;
+    for (int q = 0; q < NV; q++) sh2[NV * b + q] = 0.;
     sh_cnt[b] = 0u;
   }
   __syncthreads();
@@ -517,7 +529,7 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
    const int lo = lane * run, hi = lo + run < ncell ? lo + run : ncell;
    int cur = -1;
    unsigned int acc_n = 0u;
-   double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+   double a0 = 0., a1 = 0., a2 = 0., a3 = 0., a4 = 0.;
    for (int kk = lo; kk < hi; kk++) {
     const int k = whole ? kk : (kk - K + g.n[2]) % g.n[2];
     const long long c = l * g.n[2] + k;
@@ -541,45 +553,48 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
     if (b != cur) {
       if (acc_n) {
         atomicAdd(&sh_cnt[cur], acc_n);
-        atomicAdd(&sh2[4 * cur], a0);
-        atomicAdd(&sh2[4 * cur + 1], a1);
-        atomicAdd(&sh2[4 * cur + 2], a2);
-        atomicAdd(&sh2[4 * cur + 3], a3);
+        atomicAdd(&sh2[NV * cur], a0);
+        atomicAdd(&sh2[NV * cur + 1], a1);
+        atomicAdd(&sh2[NV * cur + 2], a2);
+        atomicAdd(&sh2[NV * cur + 3], a3);
+        if (NV == 5) atomicAdd(&sh2[NV * cur + 4], a4);
       }
       cur = b;
       acc_n = 0u;
-      a0 = a1 = a2 = a3 = 0.;
+      a0 = a1 = a2 = a3 = a4 = 0.;
     }
     acc_n++;
     a0 += pk.x;
     a1 += pk.y;
     a2 += sn.x;
     a3 += sn.y;
+    a4 += km;
    }
    if (acc_n) {
      atomicAdd(&sh_cnt[cur], acc_n);
-     atomicAdd(&sh2[4 * cur], a0);
-     atomicAdd(&sh2[4 * cur + 1], a1);
-     atomicAdd(&sh2[4 * cur + 2], a2);
-     atomicAdd(&sh2[4 * cur + 3], a3);
+     atomicAdd(&sh2[NV * cur], a0);
+     atomicAdd(&sh2[NV * cur + 1], a1);
+     atomicAdd(&sh2[NV * cur + 2], a2);
+     atomicAdd(&sh2[NV * cur + 3], a3);
+     if (NV == 5) atomicAdd(&sh2[NV * cur + 4], a4);
    }
   }
   __syncthreads();
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
     atomicAdd(&counts[b], (unsigned long long)sh_cnt[b]);
-    for (int q = 0; q < 4; q++)
-      part[((long long)blockIdx.x * nb + b) * 4 + q] = sh2[4 * b + q];
+    for (int q = 0; q < NV; q++)
+      part[((long long)blockIdx.x * nb + b) * NV + q] = sh2[NV * b + q];
   }
 }
 
-template <typename T>
+template <typename T, int NV = 4>
 static __global__ void __launch_bounds__(RED_THREADS)
 k_twopt_fourier(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
                 Geom g, AxisTables tab, double2 sbar, int ell, int m,
                 const double* __restrict__ edges, int nb,
                 unsigned long long* __restrict__ counts,
                 double* __restrict__ part) {
-  k_twopt_fourier_body<T>(fa, fb, g, tab, sbar, ell, m, edges, nb, counts, part);
+  k_twopt_fourier_body<T, NV>(fa, fb, g, tab, sbar, ell, m, edges, nb, counts, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -587,7 +602,7 @@ k_twopt_fourier(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
 // (field.cpp:3097-3170): n_pairs, sum r, sum t(x) y_a(x^) y_b(x^).
 // ---------------------------------------------------------------------------
 
-template <typename T>
+template <typename T, bool TENTHS = false>
 static __global__ void __launch_bounds__(RED_THREADS)
 k_3pcf_shot_bins(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
                  int mb, const double* __restrict__ edges, int nb,
@@ -600,7 +615,10 @@ k_3pcf_shot_bins(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
     sh_cnt[b] = 0u;
   }
   __syncthreads();
-  const int n_sample = 100000;
+  // Fine bins of width 1 (1e5 of them) for the three-point shot noise, of
+  // width 0.1 (1e6) for the two-point correlation function
+  // (field.cpp:2875-2877).
+  const int n_sample = TENTHS ? 1000000 : 100000;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
        c < g.nmesh; c += (long long)gridDim.x * blockDim.x) {
     int k = int(c % g.n[2]);
@@ -610,9 +628,11 @@ k_3pcf_shot_bins(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
     double ry = grid_coord(j, g.n[1], g.dr[1]);
     double rz = grid_coord(k, g.n[2], g.dr[2]);
     double rm = mag3_exact(rx, ry, rz);
-    int ir = int(rm);   // r / dr_sample with dr_sample = 1
+    // r / dr_sample, truncated; the fine bin enters a coarse bin by its
+    // lower edge ir * dr_sample.
+    int ir = TENTHS ? int(__ddiv_rn(rm, 0.1)) : int(rm);
     if (ir < 0 || ir >= n_sample) continue;
-    int b = find_bin(edges, nb, double(ir));
+    int b = find_bin(edges, nb, TENTHS ? __dmul_rn(double(ir), 0.1) : double(ir));
     if (b < 0) continue;
     c2<T> tv = t[c];
     double2 ya = ylm_reduced_dev(la, ma, rx, ry, rz);
@@ -779,12 +799,14 @@ k_cube_sum(const c2<T>* __restrict__ f, long long n, double* __restrict__ part) 
 }
 
 
-// Per-block partials of sum_p y_lm(los_p) w_p^3 (threept.cpp:156-236) for
-// several (l, m) at once, part[block * n + q]; one pass over the particles.
+// Per-block partials of sum_p y_lm(los_p) w_p^3 (threept.cpp:156-236), or
+// of sum_p y_lm(los_p) w_p^2 (twopt.cpp:298-381), for several (l, m) at
+// once, part[block * n + q]; one pass over the particles.
 constexpr int SBAR_MAX_KEYS = 8;
 struct SbarKeys {
   int n;
   int ell[SBAR_MAX_KEYS], m[SBAR_MAX_KEYS];
+  int wpow;   // power of the weight: 2 (power spectrum), else 3
 };
 
 static __global__ void __launch_bounds__(RED_THREADS)
@@ -796,7 +818,7 @@ k_sbar_multi(DevParticles p, SbarKeys keys, double2* __restrict__ part) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < p.n;
        i += (long long)gridDim.x * blockDim.x) {
     const double w = p.w[i];
-    const double w3 = w * w * w;
+    const double w3 = keys.wpow == 2 ? w * w : w * w * w;
 #pragma unroll
     for (int q = 0; q < SBAR_MAX_KEYS; q++) {
       if (q >= keys.n) break;

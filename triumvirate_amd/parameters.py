@@ -159,15 +159,36 @@ class ParameterSet:
                 "Interlacing must be 'true'/'on' or 'false'/'off': "
                 f"`interlace` = '{interlace}'.")
 
+        # parameters.cpp:704-828: statistic type -> npoint and space, with
+        # the catalogue type each statistic requires.
         stype = p.get('statistic_type')
-        if stype in ('powspec', 'modes'):
-            p['npoint'], p['space'] = '2pt', 'fourier'
-        elif stype in ('2pcf', 'pairs'):
-            p['npoint'], p['space'] = '2pt', 'config'
-        elif stype in ('bispec',):
-            p['npoint'], p['space'] = '3pt', 'fourier'
-        elif stype in ('3pcf',):
-            p['npoint'], p['space'] = '3pt', 'config'
+        ctype = p.get('catalogue_type')
+        needs = {'powspec': ('2pt', 'fourier', 'obs'),
+                 '2pcf': ('2pt', 'config', 'obs'),
+                 '2pcf-win': ('2pt', 'config', 'random'),
+                 'bispec': ('3pt', 'fourier', 'obs'),
+                 '3pcf': ('3pt', 'config', 'obs'),
+                 '3pcf-win': ('3pt', 'config', 'random'),
+                 '3pcf-win-wa': ('3pt', 'config', 'random')}
+        if stype in needs:
+            p['npoint'], p['space'], need = needs[stype]
+            names = {'powspec': 'Power spectrum',
+                     '2pcf': 'Two-point correlation function',
+                     'bispec': 'Bispectrum',
+                     '3pcf': 'Three-point correlation function'}
+            if need == 'obs' and ctype in ('random', 'none'):
+                raise InvalidParameterError(
+                    f"{names[stype]} requires 'sim' or 'survey' "
+                    f"catalogue(s): `catalogue_type` = '{ctype}'.")
+            if need == 'random' and ctype is not None and ctype != 'random':
+                what = ('Two-point' if stype == '2pcf-win' else 'Three-point')
+                raise InvalidParameterError(
+                    f"{what} correlation function window requires 'random' "
+                    f"catalogue: `catalogue_type` = '{ctype}'.")
+        elif stype == 'modes':
+            p['npoint'], p['space'] = 'none', 'fourier'
+        elif stype == 'pairs':
+            p['npoint'], p['space'] = 'none', 'config'
         elif stype is None:
             p['npoint'], p['space'] = None, None
         else:
