@@ -115,6 +115,16 @@ class ParticleCatalogue {
     return 0;
   }
 
+  // ParticleCatalogue::offset_coords (src/particles.cpp:658-676).
+  void offset_coords(const double dvec[3]) {
+    for (int i = 0; i < ntotal; i++)
+      for (int a = 0; a < 3; a++) pdata[i].pos[a] -= dvec[a];
+    for (int a = 0; a < 3; a++) {
+      pos_min[a] -= dvec[a];
+      pos_max[a] -= dvec[a];
+    }
+  }
+
   // C-ABI view of the 56-byte records in place (stride 7 doubles).
   trvg_catalogue abi_view() const {
     if (pdata == nullptr)
@@ -155,6 +165,9 @@ class ParameterSet {
   double bin_min = 0., bin_max = 0.;
   std::string binning = "lin";
   std::string norm_convention = "particle";
+  std::string alignment = "centre";
+  std::string padscale = "box";
+  double padfactor = 0.;
   int verbose = 20;
   int precision = TRVG_PREC_FP64;   // engine extension: 64 or 32
 
@@ -174,6 +187,11 @@ class ParameterSet {
           "Interlacing must be 'true'/'on' or 'false'/'off'.");
     if (statistic_type == "bispec") { npoint = "3pt"; space = "fourier"; }
     else if (statistic_type == "3pcf") { npoint = "3pt"; space = "config"; }
+    else if (statistic_type == "powspec") { npoint = "2pt"; space = "fourier"; }
+    else if (statistic_type == "2pcf" || statistic_type == "2pcf-win") {
+      npoint = "2pt";
+      space = "config";
+    }
     if (!(form == "full" || form == "diag" || form == "off-diag" || form == "row"))
       throw sys::InvalidParameterError(
           "Three-point statistic form is not recognised: `form` = '" + form + "'.");
@@ -342,6 +360,27 @@ class Binning {
   }
 };
 
+struct PowspecMeasurements {
+  int dim = 0;
+  std::vector<double> kbin, keff;
+  std::vector<int> nmodes;
+  std::vector<std::complex<double>> pk_raw, pk_shot;
+};
+
+struct TwoPCFMeasurements {
+  int dim = 0;
+  std::vector<double> rbin, reff;
+  std::vector<int> npairs;
+  std::vector<std::complex<double>> xi;
+};
+
+struct TwoPCFWindowMeasurements {
+  int dim = 0;
+  std::vector<double> rbin, reff;
+  std::vector<int> npairs;
+  std::vector<std::complex<double>> xi;
+};
+
 struct BispecMeasurements {
   int dim = 0;
   std::vector<double> k1_bin, k2_bin, k1_eff, k2_eff;
@@ -395,6 +434,32 @@ ThreePCFMeasurements run_3pcf(const trvg_params& p, const Binning& b,
                      m.r2_eff.data(), m.npairs_1.data(), m.npairs_2.data(),
                      reinterpret_cast<double*>(m.zeta_raw.data()),
                      reinterpret_cast<double*>(m.zeta_shot.data())};
+  sys::check_status(call(&r));
+  m.dim = r.dim;
+  return m;
+}
+
+template <class Call>
+PowspecMeasurements run_powspec(const Binning& b, Call&& call) {
+  PowspecMeasurements m;
+  const int n = b.num_bins;
+  m.kbin.resize(n); m.keff.resize(n); m.nmodes.resize(n);
+  m.pk_raw.resize(n); m.pk_shot.resize(n);
+  trvg_powspec_result r{n, m.kbin.data(), m.keff.data(), m.nmodes.data(),
+                        reinterpret_cast<double*>(m.pk_raw.data()),
+                        reinterpret_cast<double*>(m.pk_shot.data())};
+  sys::check_status(call(&r));
+  m.dim = r.dim;
+  return m;
+}
+
+template <class M, class Call>
+M run_2pcf(const Binning& b, Call&& call) {
+  M m;
+  const int n = b.num_bins;
+  m.rbin.resize(n); m.reff.resize(n); m.npairs.resize(n); m.xi.resize(n);
+  trvg_2pcf_result r{n, m.rbin.data(), m.reff.data(), m.npairs.data(),
+                     reinterpret_cast<double*>(m.xi.data())};
   sys::check_status(call(&r));
   m.dim = r.dim;
   return m;
@@ -465,6 +530,151 @@ inline ThreePCFMeasurements compute_3pcf_in_gpp_box(
   trvg_binning b = rbinning.abi_view();
   return detail::run_3pcf(p, rbinning, [&](trvg_3pcf_result* out) {
     return trvg_compute_3pcf_in_gpp_box(&d, &p, &b, norm_factor, out);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Two-point statistics (include/twopt.hpp).
+// ---------------------------------------------------------------------------
+
+inline double calc_powspec_normalisation_from_particles(ParticleCatalogue& particles,
+                                                        double alpha = 1.) {
+  trvg_catalogue c = particles.abi_view();
+  double nf = 0.;
+  sys::check_status(trvg_calc_powspec_normalisation_from_particles(&c, alpha, &nf));
+  return nf;
+}
+
+inline double calc_powspec_normalisation_from_mesh(ParticleCatalogue& particles,
+                                                   ParameterSet& params,
+                                                   double alpha = 1.) {
+  trvg_catalogue c = particles.abi_view();
+  trvg_params p = params.abi_view();
+  double nf = 0.;
+  sys::check_status(trvg_calc_powspec_normalisation_from_mesh(&c, &p, alpha, &nf));
+  return nf;
+}
+
+// twopt.cpp:111-228: both catalogues aligned in the box of `params` as the
+// measurement would be (centre, or pad by box or grid), the mixed-mesh
+// normalisation computed, and the original positions restored.
+inline double calc_powspec_normalisation_from_meshes(ParticleCatalogue& particles_data,
+                                                     ParticleCatalogue& particles_rand,
+                                                     ParameterSet& params, double alpha) {
+  double dvec[3];
+  for (int a = 0; a < 3; a++) {
+    if (params.alignment == "pad") {
+      const double pad = (params.padscale == "grid")
+                             ? params.padfactor * params.boxsize[a] / params.ngrid[a]
+                             : params.padfactor * params.boxsize[a];
+      dvec[a] = particles_rand.pos_min[a] - pad;
+    } else {
+      dvec[a] = (particles_rand.pos_min[a] + particles_rand.pos_max[a]) / 2. -
+                params.boxsize[a] / 2.;
+    }
+  }
+  particles_data.offset_coords(dvec);
+  particles_rand.offset_coords(dvec);
+  trvg_catalogue d = particles_data.abi_view(), r = particles_rand.abi_view();
+  trvg_params p = params.abi_view();
+  double nf = 0.;
+  const int st = trvg_calc_powspec_normalisation_from_meshes(&d, &r, &p, alpha, &nf);
+  const double back[3] = {-dvec[0], -dvec[1], -dvec[2]};
+  particles_data.offset_coords(back);
+  particles_rand.offset_coords(back);
+  sys::check_status(st);
+  return nf;
+}
+
+// twopt.cpp:230-268: the mixed-mesh geometry from the catalogue spans, a
+// cell size and a scheme.
+inline double calc_powspec_normalisation_from_meshes(ParticleCatalogue& particles_data,
+                                                     ParticleCatalogue& particles_rand,
+                                                     ParameterSet& params, double alpha,
+                                                     double padding, double cellsize,
+                                                     const std::string& assignment) {
+  ParameterSet pn = params;
+  double span = 0.;
+  for (int a = 0; a < 3; a++)
+    span = std::max(span, std::max(particles_data.pos_span[a], particles_rand.pos_span[a]));
+  double L = (1. + padding) * span;
+  int n = int(std::ceil(L / cellsize));
+  n += n % 2;
+  L = n * cellsize;
+  for (int a = 0; a < 3; a++) {
+    pn.boxsize[a] = L;
+    pn.ngrid[a] = n;
+  }
+  pn.assignment = assignment;
+  pn.validate();
+  return calc_powspec_normalisation_from_meshes(particles_data, particles_rand, pn, alpha);
+}
+
+inline double calc_powspec_shotnoise_from_particles(ParticleCatalogue& particles,
+                                                    double alpha = 1.) {
+  trvg_catalogue c = particles.abi_view();
+  double sn = 0.;
+  sys::check_status(trvg_calc_powspec_shotnoise_from_particles(&c, alpha, &sn));
+  return sn;
+}
+
+inline PowspecMeasurements compute_powspec(
+    ParticleCatalogue& catalogue_data, ParticleCatalogue& catalogue_rand,
+    LineOfSight* los_data, LineOfSight* los_rand, ParameterSet& params,
+    Binning& kbinning, double norm_factor) {
+  trvg_catalogue d = catalogue_data.abi_view(), r = catalogue_rand.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = kbinning.abi_view();
+  return detail::run_powspec(kbinning, [&](trvg_powspec_result* out) {
+    return trvg_compute_powspec(&d, &r, &los_data[0].pos[0], &los_rand[0].pos[0], &p,
+                                &b, norm_factor, out);
+  });
+}
+
+inline TwoPCFMeasurements compute_corrfunc(
+    ParticleCatalogue& catalogue_data, ParticleCatalogue& catalogue_rand,
+    LineOfSight* los_data, LineOfSight* los_rand, ParameterSet& params,
+    Binning& rbinning, double norm_factor) {
+  trvg_catalogue d = catalogue_data.abi_view(), r = catalogue_rand.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = rbinning.abi_view();
+  return detail::run_2pcf<TwoPCFMeasurements>(rbinning, [&](trvg_2pcf_result* out) {
+    return trvg_compute_corrfunc(&d, &r, &los_data[0].pos[0], &los_rand[0].pos[0], &p,
+                                 &b, norm_factor, out);
+  });
+}
+
+inline PowspecMeasurements compute_powspec_in_gpp_box(
+    ParticleCatalogue& catalogue_data, ParameterSet& params, Binning kbinning,
+    double norm_factor) {
+  trvg_catalogue d = catalogue_data.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = kbinning.abi_view();
+  return detail::run_powspec(kbinning, [&](trvg_powspec_result* out) {
+    return trvg_compute_powspec_in_gpp_box(&d, &p, &b, norm_factor, out);
+  });
+}
+
+inline TwoPCFMeasurements compute_corrfunc_in_gpp_box(
+    ParticleCatalogue& catalogue_data, ParameterSet& params, Binning& rbinning,
+    double norm_factor) {
+  trvg_catalogue d = catalogue_data.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = rbinning.abi_view();
+  return detail::run_2pcf<TwoPCFMeasurements>(rbinning, [&](trvg_2pcf_result* out) {
+    return trvg_compute_corrfunc_in_gpp_box(&d, &p, &b, norm_factor, out);
+  });
+}
+
+inline TwoPCFWindowMeasurements compute_corrfunc_window(
+    ParticleCatalogue& catalogue_rand, LineOfSight* los_rand, ParameterSet& params,
+    Binning rbinning, double alpha, double norm_factor) {
+  trvg_catalogue r = catalogue_rand.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = rbinning.abi_view();
+  return detail::run_2pcf<TwoPCFWindowMeasurements>(rbinning, [&](trvg_2pcf_result* out) {
+    return trvg_compute_corrfunc_window(&r, &los_rand[0].pos[0], &p, &b, alpha,
+                                        norm_factor, out);
   });
 }
 

@@ -1,7 +1,8 @@
 // Drop-in check: calls the trv:: C++ API of include/triumvirate_gpu.hpp the
 // way the reference CLI calls its own (main/triumvirate.cpp:711-726) on the
-// reference's periodic-box test catalogue, and prints the data vector.
-// Usage: dropin_main <data_catalogue.txt>
+// reference's periodic-box test catalogue, and prints the data vector: the
+// bispectrum B_000 (default) or, with "powspec", the power spectrum P_0.
+// Usage: dropin_main <data_catalogue.txt> [powspec]
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -55,6 +56,23 @@ int main(int argc, char** argv) {
   trv::Binning kbin("fourier", "lin");
   kbin.set_bins(0.005, 0.105, 4);
 
+  if (argc > 2 && std::string(argv[2]) == "powspec") {
+    params.statistic_type = "powspec";
+    params.validate();
+    try {
+      double norm = trv::calc_powspec_normalisation_from_particles(cat, 1.);
+      trv::PowspecMeasurements m =
+          trv::compute_powspec_in_gpp_box(cat, params, kbin, norm);
+      for (int i = 0; i < m.dim; i++)
+        std::printf("%.9e %.9e %d %.12e %.12e %.12e %.12e\n", m.kbin[i], m.keff[i],
+                    m.nmodes[i], m.pk_raw[i].real(), m.pk_raw[i].imag(),
+                    m.pk_shot[i].real(), m.pk_shot[i].imag());
+    } catch (const std::exception& e) {
+      std::cerr << "error: " << e.what() << "\n";
+      return 1;
+    }
+    return 0;
+  }
   try {
     double norm = trv::calc_bispec_normalisation_from_particles(cat, 1.);
     trv::BispecMeasurements m =

@@ -43,6 +43,21 @@ def test_dropin_reproduces_fixture(tmp_path):
         assert err < 2e-9, (col, err)
 
 
+@pytest.mark.gpu
+def test_dropin_powspec_reproduces_fixture(tmp_path):
+    exe = _build(tmp_path)
+    out = subprocess.run([str(exe), str(cf.REF_STATS / 'test_data_catalogue.txt'),
+                          'powspec'], check=True, capture_output=True, text=True,
+                         timeout=120).stdout
+    got = np.array([[float(v) for v in ln.split()] for ln in out.splitlines()]).T
+    ext = cf.load_ref_stats('pk0_gpp.txt')
+    assert np.array_equal(got[2], ext[2])
+    np.testing.assert_allclose(got[1], ext[1], rtol=1e-9)
+    for col in (3, 5):
+        err = np.max(np.abs(got[col] - ext[col])) / np.max(np.abs(ext[col]))
+        assert err < 2e-9, (col, err)
+
+
 def test_dropin_binning_schemes_match_python(tmp_path):
     """trv::Binning of the C++ drop-in (every scheme, the grid-scale padding
     of the ParameterSet constructor and the grid-scale default binning)

@@ -1,5 +1,6 @@
-"""MI355X (gfx950) engine for Triumvirate's three-point clustering
-estimators: bispectrum and three-point correlation function.
+"""MI355X (gfx950) engine for Triumvirate's clustering estimators:
+bispectrum and three-point correlation function, and the two-point
+statistics (power spectrum, correlation function, window).
 
 The estimators run in hand-written HIP kernels (``libtrvgpu.so``) behind a
 C ABI (``include/trv_gpu.h``); this package is the Python host layer with
@@ -17,4 +18,11 @@ from .threept import (  # noqa: F401
     compute_bispec,
     compute_bispec_in_gpp_box,
     compute_bispec_multipoles,
+)
+from .twopt import (  # noqa: F401
+    compute_corrfunc,
+    compute_corrfunc_in_gpp_box,
+    compute_corrfunc_window,
+    compute_powspec,
+    compute_powspec_in_gpp_box,
 )
