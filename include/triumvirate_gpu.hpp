@@ -165,6 +165,7 @@ class ParameterSet {
   double bin_min = 0., bin_max = 0.;
   std::string binning = "lin";
   std::string norm_convention = "particle";
+  int i_wa = 0, j_wa = 0;   // wide-angle orders of the 3PCF window
   std::string alignment = "centre";
   std::string padscale = "box";
   double padfactor = 0.;
@@ -186,8 +187,11 @@ class ParameterSet {
       throw sys::InvalidParameterError(
           "Interlacing must be 'true'/'on' or 'false'/'off'.");
     if (statistic_type == "bispec") { npoint = "3pt"; space = "fourier"; }
-    else if (statistic_type == "3pcf") { npoint = "3pt"; space = "config"; }
-    else if (statistic_type == "powspec") { npoint = "2pt"; space = "fourier"; }
+    else if (statistic_type == "3pcf" || statistic_type == "3pcf-win" ||
+             statistic_type == "3pcf-win-wa") {
+      npoint = "3pt";
+      space = "config";
+    } else if (statistic_type == "powspec") { npoint = "2pt"; space = "fourier"; }
     else if (statistic_type == "2pcf" || statistic_type == "2pcf-win") {
       npoint = "2pt";
       space = "config";
@@ -395,6 +399,10 @@ struct ThreePCFMeasurements {
   std::vector<std::complex<double>> zeta_raw, zeta_shot;
 };
 
+// trv::ThreePCFWindowMeasurements (include/dataobjs.hpp:288): the fields
+// of ThreePCFMeasurements.
+struct ThreePCFWindowMeasurements : ThreePCFMeasurements {};
+
 namespace detail {
 
 inline int dim_for(const trvg_params& p, const Binning& b) {
@@ -531,6 +539,23 @@ inline ThreePCFMeasurements compute_3pcf_in_gpp_box(
   return detail::run_3pcf(p, rbinning, [&](trvg_3pcf_result* out) {
     return trvg_compute_3pcf_in_gpp_box(&d, &p, &b, norm_factor, out);
   });
+}
+
+// threept.cpp:2621-3077 (include/threept.hpp:310-314).
+inline ThreePCFWindowMeasurements compute_3pcf_window(
+    ParticleCatalogue& catalogue_rand, LineOfSight* los_rand, ParameterSet& params,
+    Binning& rbinning, double alpha, double norm_factor, bool wide_angle = false) {
+  trvg_catalogue r = catalogue_rand.abi_view();
+  trvg_params p = params.abi_view();
+  trvg_binning b = rbinning.abi_view();
+  ThreePCFWindowMeasurements w;
+  static_cast<ThreePCFMeasurements&>(w) =
+      detail::run_3pcf(p, rbinning, [&](trvg_3pcf_result* out) {
+        return trvg_compute_3pcf_window(&r, &los_rand[0].pos[0], &p, &b, alpha,
+                                        norm_factor, wide_angle ? 1 : 0, params.i_wa,
+                                        params.j_wa, out);
+      });
+  return w;
 }
 
 // ---------------------------------------------------------------------------

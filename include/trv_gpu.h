@@ -222,6 +222,18 @@ int trvg_compute_3pcf_in_gpp_box(
   double norm_factor, trvg_3pcf_result* out);
 
 
+/* Replaces trv::compute_3pcf_window (include/threept.hpp:310-314;
+ * src/threept.cpp:2621-3077): the three-point correlation function window
+ * of one random catalogue (fields of that catalogue alone scaled by
+ * `alpha`), with the wide-angle power-law kernel |x|^-(i_wa + j_wa) on the
+ * G field when wide_angle is non-zero (field.cpp:1727-1762; the reference
+ * reads the orders from ParameterSet::i_wa, j_wa). */
+int trvg_compute_3pcf_window(
+  const trvg_catalogue* rand, const double* los_rand,
+  const trvg_params* params, const trvg_binning* rbinning, double alpha,
+  double norm_factor, int wide_angle, int i_wa, int j_wa,
+  trvg_3pcf_result* out);
+
 /* ---------------------------------------------------------------------- */
 /* Two-point statistics (include/twopt.hpp; src/twopt.cpp).  params->ELL is */
 /* the multipole degree; interlacing follows params->interlace.  Result    */

@@ -637,6 +637,28 @@ struct Mesh {
       f[g] = cplx(f[g].real() * alpha, f[g].imag() * alpha);
   }
 
+  // field.cpp:1449-1489: one catalogue, conj(Y_lm) w^2, times alpha^2.
+  void ylm_wgtd_quad_field_single(const Catalogue& c, const double* los,
+                                  double alpha, int ell, int m) {
+    assign(c, ylm_weights(c, los, ell, m, 2, true));
+    const double a2 = std::pow(alpha, 2);
+    for (long long g = 0; g < p.nmesh; g++)
+      f[g] = cplx(f[g].real() * a2, f[g].imag() * a2);
+  }
+
+  // field.cpp:1727-1762: f(x) |x|^-(i + j) where |x| >= 1e-6.
+  void wide_angle_kernel(int power) {
+    for (int i = 0; i < p.ngrid[0]; i++)
+      for (int j = 0; j < p.ngrid[1]; j++)
+        for (int k = 0; k < p.ngrid[2]; k++) {
+          double rv[3];
+          rvec(i, j, k, rv);
+          double r = mag3(rv);
+          if (r < 1.e-6) continue;
+          f[gidx(i, j, k)] *= std::pow(r, -power);
+        }
+  }
+
   // field.cpp:1496-1655
   void fourier_transform() {
     for (long long g = 0; g < p.nmesh; g++) f[g] *= vol_cell;
@@ -1378,11 +1400,22 @@ Output bispec_impl(bool survey, const Catalogue& d, const Catalogue* r,
 
 // Three-point correlation function body: survey (threept.cpp:1014-1471) or
 // periodic box (threept.cpp:2190-2619).
+// Window of a random catalogue (threept.cpp:2621-3077): with `window` the
+// catalogue is *r alone (single-catalogue fields and shot-noise amplitude
+// scaled by alpha), `alpha_win` its contrast and `wa_power` = i + j of the
+// wide-angle kernel on G (negative: none).
+struct Window3 {
+  bool on = false;
+  double alpha = 1.;
+  int wa_power = -1;
+};
+
 Output threepcf_impl(bool survey, const Catalogue& d, const Catalogue* r,
                      const double* los_d, const double* los_r, const Params& p,
-                     const trvg_binning& rb, double norm_factor) {
+                     const trvg_binning& rb, double norm_factor,
+                     const Window3& win = Window3()) {
   validate_coupling(p);
-  double alpha = survey ? d.wstotal / r->wstotal : 1.;
+  double alpha = win.on ? win.alpha : (survey ? d.wstotal / r->wstotal : 1.);
   cplx factor_phase = ipow(p.ell1 + p.ell2);
   double factor_parity = std::pow(-1., p.ell1 + p.ell2);
   int nb = rb.num_bins;
@@ -1392,13 +1425,15 @@ Output threepcf_impl(bool survey, const Catalogue& d, const Catalogue* r,
   Output out(dim);
 
   Mesh dn00(p);
-  if (survey) dn00.ylm_wgtd_field(d, *r, los_d, los_r, alpha, 0, 0);
+  if (win.on) dn00.ylm_wgtd_field_single(*r, los_r, alpha, 0, 0);
+  else if (survey) dn00.ylm_wgtd_field(d, *r, los_d, los_r, alpha, 0, 0);
   else dn00.unweighted_fluctuations(d);
   dn00.fourier_transform();
   double vol_cell = dn00.vol_cell;
 
   Mesh N00(p);
-  if (survey) N00.ylm_wgtd_quad_field(d, *r, los_d, los_r, alpha, 0, 0);
+  if (win.on) N00.ylm_wgtd_quad_field_single(*r, los_r, alpha, 0, 0);
+  else if (survey) N00.ylm_wgtd_quad_field(d, *r, los_d, los_r, alpha, 0, 0);
   else N00.unweighted_field(d);
   N00.fourier_transform();
 
@@ -1442,7 +1477,12 @@ Output threepcf_impl(bool survey, const Catalogue& d, const Catalogue* r,
         Mesh dnLM(p);
         Mesh* dn_for_sn = &dn00;
         cplx Sbar;
-        if (survey) {
+        if (win.on) {
+          dnLM.ylm_wgtd_field_single(*r, los_r, alpha, p.ELL, M);
+          dnLM.fourier_transform();
+          Sbar = std::pow(alpha, 3) * sbar_lm(*r, los_r, p.ELL, M);
+          dn_for_sn = &dnLM;
+        } else if (survey) {
           dnLM.ylm_wgtd_field(d, *r, los_d, los_r, alpha, p.ELL, M);
           dnLM.fourier_transform();
           Sbar = sbar_lm(d, los_d, p.ELL, M)
@@ -1472,11 +1512,13 @@ Output threepcf_impl(bool survey, const Catalogue& d, const Catalogue* r,
 
         // Raw 3PCF.
         Mesh G(p);
-        if (survey) G.ylm_wgtd_field(d, *r, los_d, los_r, alpha, p.ELL, M);
+        if (win.on) G.ylm_wgtd_field_single(*r, los_r, alpha, p.ELL, M);
+        else if (survey) G.ylm_wgtd_field(d, *r, los_d, los_r, alpha, p.ELL, M);
         else G.unweighted_fluctuations(d);
         G.fourier_transform();
         G.apply_compensation();
         G.inv_fourier_transform();
+        if (win.on && win.wa_power >= 0) G.wide_angle_kernel(win.wa_power);
 
         if (g_schedule == 0) {
           Mesh Fa(p), Fb(p);
@@ -1939,6 +1981,22 @@ int trvo_compute_3pcf_in_gpp_box(const trvg_catalogue* data,
     p.interlace = false;
     Catalogue d = load_catalogue(data);
     write_3pcf(threepcf_impl(false, d, nullptr, nullptr, nullptr, p, *rb, norm_factor), out);
+  });
+}
+
+int trvo_compute_3pcf_window(const trvg_catalogue* rand, const double* los_rand,
+                             const trvg_params* params, const trvg_binning* rb,
+                             double alpha, double norm_factor, int wide_angle,
+                             int i_wa, int j_wa, trvg_3pcf_result* out) {
+  return guarded([&] {
+    Params p = make_params(params);
+    p.interlace = false;
+    Catalogue r = load_catalogue(rand);
+    Window3 w;
+    w.on = true;
+    w.alpha = alpha;
+    w.wa_power = wide_angle ? i_wa + j_wa : -1;
+    write_3pcf(threepcf_impl(true, r, &r, los_rand, los_rand, p, *rb, norm_factor, w), out);
   });
 }
 

@@ -15,6 +15,7 @@ from .parameters import ParameterSet, fetch_paramset_template  # noqa: F401
 from .threept import (  # noqa: F401
     compute_3pcf,
     compute_3pcf_in_gpp_box,
+    compute_3pcf_window,
     compute_bispec,
     compute_bispec_in_gpp_box,
     compute_bispec_multipoles,

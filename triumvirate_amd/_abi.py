@@ -222,6 +222,10 @@ def declare_common(lib, prefix):
     f = getattr(lib, prefix + 'calc_bispec_normalisation_from_mesh')
     f.argtypes = [P(Catalogue), P(Params), C.c_double, _dp]
     f.restype = C.c_int
+    f = getattr(lib, prefix + 'compute_3pcf_window')
+    f.argtypes = [P(Catalogue), _dp, P(Params), P(Binning), C.c_double,
+                  C.c_double, C.c_int, C.c_int, C.c_int, P(ThreePCFResult)]
+    f.restype = C.c_int
     for name in ('calc_powspec_normalisation_from_particles',
                  'calc_powspec_shotnoise_from_particles'):
         f = getattr(lib, prefix + name)

@@ -359,6 +359,30 @@ int trvg_compute_3pcf_in_gpp_box(const trvg_catalogue* data,
 // Two-point statistics (include/twopt.hpp).
 // ---------------------------------------------------------------------------
 
+int trvg_compute_3pcf_window(const trvg_catalogue* rand, const double* los_rand,
+                             const trvg_params* params, const trvg_binning* rb,
+                             double alpha, double norm_factor, int wide_angle,
+                             int i_wa, int j_wa, trvg_3pcf_result* out) {
+  return guarded([&] {
+    require_device();
+    check_binning(rb);
+    trvg_params q = three_point_params(params, rb->num_bins);
+    require_los(los_rand, "random");
+    if (wide_angle && (i_wa < 0 || j_wa < 0))
+      throw Error(TRVG_ERR_PARAM, "Wide-angle orders must be non-negative.");
+    Context ctx(q);
+    DeviceCatalogue r;
+    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx));
+    Window3pcf w;
+    w.alpha = alpha;
+    w.wa_power = wide_angle ? i_wa + j_wa : -1;
+    Result res = (q.precision == TRVG_PREC_FP32)
+                     ? run_3pcf<float>(ctx, r, &r, true, *rb, norm_factor, &w)
+                     : run_3pcf<double>(ctx, r, &r, true, *rb, norm_factor, &w);
+    write_3pcf(res, out);
+  });
+}
+
 int trvg_calc_powspec_normalisation_from_particles(const trvg_catalogue* c, double alpha,
                                                    double* norm_factor) {
   return guarded([&] {

@@ -469,10 +469,19 @@ Result run_bispec(Context& ctx, const DeviceCatalogue& d,
                   const DeviceCatalogue* r, bool survey,
                   const trvg_binning& kb, double norm_factor);
 
+// The window of one random catalogue (threept.cpp:2621-3077): fields of
+// the catalogue alone scaled by alpha, and the wide-angle kernel
+// |x|^-wa_power on G when wa_power >= 0.
+struct Window3pcf {
+  double alpha = 1.;
+  int wa_power = -1;
+};
+
 template <typename T>
 Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
                 const DeviceCatalogue* r, bool survey,
-                const trvg_binning& rb, double norm_factor);
+                const trvg_binning& rb, double norm_factor,
+                const Window3pcf* win = nullptr);
 
 template <typename T>
 double run_mesh_norm(Context& ctx, const DeviceCatalogue& c);

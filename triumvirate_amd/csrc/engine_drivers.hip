@@ -974,6 +974,18 @@ static void build_field_k(Context& ctx, c2<T>* f, const DeviceCatalogue& d,
   fft3d<T>(ctx, f, ld, -1);
 }
 
+// One catalogue's field alpha sum_c y w (quad: alpha^2 sum_c y^* w^2),
+// forward transformed with the V_cell factor (field.cpp:1325-1362,
+// 1449-1489).
+static void build_single_field_k(Context& ctx, double2* f, const DeviceCatalogue& c,
+                                 double alpha, int ell, int m, bool quad) {
+  zero_field<double>(ctx, f);
+  assign<double>(ctx, f, c, quad ? WM_YLMC_W2 : WM_YLM_W, ell, m,
+                 quad ? alpha * alpha : alpha);
+  LoadScaleShift<double> ld{f, 0., ctx.g.vol_cell};
+  fft3d<double>(ctx, f, ld, -1);
+}
+
 // Survey pair of one (l, m): the density contrast
 //   dn = sum_d y w - alpha sum_r y w
 // and its shot-noise field
@@ -1433,11 +1445,12 @@ static void shot_pairs_bispec(Context& ctx, const c2<T>* t, int la, int ma, int 
 }
 
 // S-bar_LM = S-bar_LM[data] + alpha^3 S-bar_LM[randoms] for every (L, M)
-// in `keys` (threept.cpp:771-800): one pass per catalogue for up to
+// in `keys` (threept.cpp:771-800; with no data catalogue, alpha^3
+// S-bar_LM[randoms], threept.cpp:209-237): one pass per catalogue for up to
 // SBAR_MAX_KEYS keys, partials downloaded, one synchronisation.  Call with
 // no deferred work pending on the context.
 static std::map<std::pair<int, int>, cplx> sbar_table(
-    Context& ctx, const DeviceCatalogue& d, const DeviceCatalogue& r, double alpha,
+    Context& ctx, const DeviceCatalogue* d, const DeviceCatalogue& r, double alpha,
     const std::set<std::pair<int, int>>& keys) {
   const std::vector<std::pair<int, int>> kv(keys.begin(), keys.end());
   struct Chunk {
@@ -1456,8 +1469,8 @@ static std::map<std::pair<int, int>, cplx> sbar_table(
       sk.m[q] = kv[k0 + q].second;
     }
     Chunk c{k0, sk.n, nullptr, nullptr, 0u, 0u};
-    for (int side = 0; side < 2; side++) {
-      const DeviceCatalogue& cat = side == 0 ? d : r;
+    for (int side = d ? 0 : 1; side < 2; side++) {
+      const DeviceCatalogue& cat = side == 0 ? *d : r;
       const unsigned grid = grid_for(ctx, cat.n, 4);
       DevArray<double2> part((size_t)grid * sk.n);
       hipLaunchKernelGGL(k_sbar_multi, dim3(grid), dim3(RED_THREADS), 0, ctx.stream,
@@ -1474,7 +1487,7 @@ static std::map<std::pair<int, int>, cplx> sbar_table(
   for (const Chunk& c : chunks)
     for (int q = 0; q < c.nk; q++) {
       double dx = 0., dy = 0., rx = 0., ry = 0.;
-      for (unsigned b = 0; b < c.gd; b++) { dx += c.hd[b * c.nk + q].x; dy += c.hd[b * c.nk + q].y; }
+      for (unsigned b = 0; b < c.gd && c.hd; b++) { dx += c.hd[b * c.nk + q].x; dy += c.hd[b * c.nk + q].y; }
       for (unsigned b = 0; b < c.gr; b++) { rx += c.hr[b * c.nk + q].x; ry += c.hr[b * c.nk + q].y; }
       out[kv[c.k0 + q]] = cplx(dx, dy) + std::pow(alpha, 3) * cplx(rx, ry);
     }
@@ -1652,7 +1665,7 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
       p.ell1 = mp.ell1; p.ell2 = mp.ell2; p.ELL = mp.ELL;
       for (const Term& tm : enumerate_terms(p, survey)) keys.insert({p.ELL, tm.M});
     }
-    sbar = sbar_table(ctx, d, *r, alpha, keys);
+    sbar = sbar_table(ctx, &d, *r, alpha, keys);
   }
 
   std::vector<Result> results;
@@ -1794,11 +1807,12 @@ Result run_bispec(Context& ctx, const DeviceCatalogue& d,
 template <typename T>
 Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
                 const DeviceCatalogue* r, bool survey,
-                const trvg_binning& rb, double norm_factor) {
+                const trvg_binning& rb, double norm_factor,
+                const Window3pcf* win) {
   const trvg_params& p = ctx.p;
   const Geom& g = ctx.g;
   const long long nm = g.nmesh;
-  const double alpha = survey ? d.wstotal / r->wstotal : 1.;
+  const double alpha = win ? win->alpha : (survey ? d.wstotal / r->wstotal : 1.);
   const cplx factor_phase = ipow(p.ell1 + p.ell2);
   const double factor_parity = ((p.ell1 + p.ell2) % 2) ? -1. : 1.;
   const int nb = rb.num_bins;
@@ -1813,7 +1827,10 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
   out.raw.assign(dim, 0.); out.shot.assign(dim, 0.);
 
   DevArray<double2> dn00(nm), N00(nm);
-  if (survey) {
+  if (win) {
+    build_single_field_k(ctx, dn00.get(), d, alpha, 0, 0, false);
+    build_single_field_k(ctx, N00.get(), d, alpha, 0, 0, true);
+  } else if (survey) {
     build_survey_pair_k(ctx, dn00.get(), N00.get(), d, *r, alpha, 0, 0);
   } else {
     build_field_k<double>(ctx, dn00.get(), d, r, false, alpha, 0, 0, false, true);
@@ -1834,7 +1851,8 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
   if (survey) {
     std::set<std::pair<int, int>> keys;
     for (const Term& tm : enumerate_terms(p, survey)) keys.insert({p.ELL, tm.M});
-    sbar = sbar_table(ctx, d, *r, alpha, keys);
+    sbar = win ? sbar_table(ctx, nullptr, d, alpha, keys)
+               : sbar_table(ctx, &d, *r, alpha, keys);
   }
 
   for (const Term& tm : enumerate_terms(p, survey)) {
@@ -1842,7 +1860,10 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
     const double2* dnLMp = dn00.get();
     if (!lm_zero) {
       if (!dnLM.get()) dnLM.reset(nm);
-      build_field_k<double>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
+      if (win)
+        build_single_field_k(ctx, dnLM.get(), d, alpha, p.ELL, tm.M, false);
+      else
+        build_field_k<double>(ctx, dnLM.get(), d, r, true, alpha, p.ELL, tm.M, false, false);
       dnLMp = dnLM.get();
     }
     const cplx Sbar = survey ? sbar.at({p.ELL, tm.M}) : cplx(double(d.n), 0.);
@@ -1875,6 +1896,13 @@ Result run_3pcf(Context& ctx, const DeviceCatalogue& d,
 
     LoadDeconv<T, double> ldg{dnLMp, ctx.tab, 1. / g.vol};
     fft3d<T>(ctx, G.get(), ldg, +1, double(nm) * sizeof(double2));
+    if (win && win->wa_power >= 0) {
+      // Wide-angle power-law kernel (field.cpp:1727-1762).
+      ProfScope ps(ctx, KC_MISC, 2. * double(nm) * sizeof(c2<T>));
+      hipLaunchKernelGGL((k_wide_angle<T>), dim3(grid_for(ctx, nm)), dim3(RED_THREADS), 0,
+                         ctx.stream, G.get(), g, win->wa_power);
+      TRVG_LAUNCH_CHECK();
+    }
 
     auto make_sjl = [&](int l, int mm, const Spline& sp) {
       return [&, l, mm](c2<T>* base, const std::vector<int>& bins) {
@@ -1940,10 +1968,10 @@ template Result run_bispec<float>(Context&, const DeviceCatalogue&,
                                   const trvg_binning&, double);
 template Result run_3pcf<double>(Context&, const DeviceCatalogue&,
                                  const DeviceCatalogue*, bool,
-                                 const trvg_binning&, double);
+                                 const trvg_binning&, double, const Window3pcf*);
 template Result run_3pcf<float>(Context&, const DeviceCatalogue&,
                                 const DeviceCatalogue*, bool,
-                                const trvg_binning&, double);
+                                const trvg_binning&, double, const Window3pcf*);
 template double run_mesh_norm<double>(Context&, const DeviceCatalogue&);
 template double run_mesh_norm<float>(Context&, const DeviceCatalogue&);
 template void fft3d<double, LoadSame<double>>(Context&, double2*,

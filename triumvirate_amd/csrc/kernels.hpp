@@ -799,6 +799,25 @@ k_cube_sum(const c2<T>* __restrict__ f, long long n, double* __restrict__ part) 
 }
 
 
+// G(x) |x|^-power for |x| >= 1e-6 (wide-angle power-law kernel,
+// field.cpp:1727-1762).
+template <typename T>
+__global__ void k_wide_angle(c2<T>* __restrict__ G, Geom g, int power) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < g.nmesh;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int k = int(c % g.n[2]);
+    const long long ij = c / g.n[2];
+    const int j = int(ij % g.n[1]), i = int(ij / g.n[1]);
+    const double r = mag3_exact(grid_coord(i, g.n[0], g.dr[0]),
+                                grid_coord(j, g.n[1], g.dr[1]),
+                                grid_coord(k, g.n[2], g.dr[2]));
+    if (r < 1.e-6) continue;
+    const double f = pow(r, -double(power));
+    const c2<T> v = G[c];
+    G[c] = mk<T>(T(double(v.x) * f), T(double(v.y) * f));
+  }
+}
+
 // Per-block partials of sum_p y_lm(los_p) w_p^3 (threept.cpp:156-236), or
 // of sum_p y_lm(los_p) w_p^2 (twopt.cpp:298-381), for several (l, m) at
 // once, part[block * n + q]; one pass over the particles.

@@ -6,6 +6,7 @@ Public surface mirrors ``triumvirate.threept``:
 * :func:`compute_3pcf` (threept.py:868),
 * :func:`compute_bispec_in_gpp_box` (threept.py:1624),
 * :func:`compute_3pcf_in_gpp_box` (threept.py:1726),
+* :func:`compute_3pcf_window` (threept.py:1832),
 
 with the same arguments and result dictionaries.  The catalogue preparation
 follows the reference drivers ``_compute_3pt_stats_survey_like``
@@ -482,6 +483,62 @@ def compute_3pcf_in_gpp_box(catalogue_data, degrees=None, binning=None,
         _engine_backend(), '3pcf', catalogue_data, paramset=paramset,
         params_sampling=sampling_params, degrees=degrees, binning=binning,
         form=form, idx_bin=idx_bin, precision=precision, logger=logger)
+    return _finish(res, save)
+
+
+def _compute_3pcf_window(backend, catalogue_rand, los_rand=None, paramset=None,
+                         params_sampling=None, degrees=None, wa_orders=None,
+                         binning=None, form=None, idx_bin=None, precision=None,
+                         logger=None):
+    """Driver of the 3PCF window (reference threept.py:1832-2105): the random
+    catalogue centred and periodised in the box, both normalisations with
+    alpha = 1, the wide-angle kernel when `wa_orders` is given."""
+    paramset = _amalgamate_parameters(
+        paramset=paramset, params_sampling=params_sampling, degrees=degrees,
+        binning=binning, form=form, idx_bin=idx_bin, precision=precision,
+        catalogue_type='random', statistic_type='3pcf-win')
+    wide_angle = wa_orders is not None
+    if wide_angle:
+        i_wa, j_wa = (int(v) for v in wa_orders)
+        paramset['wa_orders'] = {'i': i_wa, 'j': j_wa}
+    else:
+        i_wa = j_wa = 0
+    if binning is None:
+        binning = Binning.from_parameter_set(paramset)
+    if los_rand is None:
+        los_rand = catalogue_rand.compute_los()
+    los_rand = np.ascontiguousarray(los_rand, dtype=np.float64)
+    boxsize = [paramset['boxsize'][a] for a in 'xyz']
+    catalogue_rand.centre(boxsize)
+    catalogue_rand.periodise(boxsize)
+    rand_v = _view(catalogue_rand)
+    params = _params_struct(paramset)
+    norm_part = backend.norm_particles(rand_v, 1.)
+    norm_mesh = backend.norm_mesh(rand_v, params, 1.)
+    norm_factor = _select_norm(paramset, norm_part, norm_mesh)
+    bv = _binning_view(binning)
+    res = backend._result(params, bv, '3pcf')
+    backend._check(backend._fn('compute_3pcf_window')(
+        C.byref(rand_v.struct), _abi.dptr(los_rand), C.byref(params),
+        C.byref(bv.struct), C.c_double(1.), C.c_double(norm_factor),
+        int(wide_angle), i_wa, j_wa, C.byref(res.struct)))
+    results = res.as_dict()
+    results['_norm_factors'] = (norm_part, norm_mesh)
+    return results
+
+
+def compute_3pcf_window(catalogue_rand, los_rand=None, degrees=None,
+                        wa_orders=None, binning=None, form=None, idx_bin=None,
+                        sampling_params=None, paramset=None, save=False,
+                        logger=None, precision=None):
+    """Three-point correlation function window from a random catalogue, with
+    wide-angle corrections of orders `wa_orders` = (i, j) when given
+    (reference threept.py:1832-2105)."""
+    res = _compute_3pcf_window(
+        _engine_backend(), catalogue_rand, los_rand=los_rand,
+        paramset=paramset, params_sampling=sampling_params, degrees=degrees,
+        wa_orders=wa_orders, binning=binning, form=form, idx_bin=idx_bin,
+        precision=precision, logger=logger)
     return _finish(res, save)
 
 
