@@ -1,0 +1,204 @@
+"""On-disk measurement formats (reference threept.py:158-384,
+twopt.py:158-320, catalogue.py:803-865) against the reference's own saved
+files.
+
+Each fixture under ``tests/golden/reference_stats`` is a file the reference
+wrote with ``save='.txt'``.  The same configuration, run here through the
+oracle with ``save='.txt'`` into a temporary directory, must produce:
+
+* the same file name;
+* the same header, line for line: text lines (catalogue sources, sizes,
+  extents, box, mesh, assignment, column names) identical, normalisation
+  lines identical up to the last printed digit of each number;
+* the same table layout: identical bin and count columns as text, the same
+  field widths in every column, values within the tolerance of the
+  fixture tests.
+
+The '.npz' form must round-trip the result arrays and hold the header.
+"""
+
+import re
+
+import numpy as np
+import pytest
+
+import conftest as cf
+from triumvirate_amd import io as tio
+
+RAND_SOURCE = 'extfile:tests/test_input/ctlgs/test_rand_catalogue.txt'
+DATA_SOURCE = 'extfile:tests/test_input/ctlgs/test_data_catalogue.txt'
+
+NUM = re.compile(r'[-+]?\d+\.\d+e[-+]\d+')
+
+
+def _oracle():
+    import oracle
+    return oracle.backend(schedule=0)
+
+
+def _catalogues():
+    d = cf.make_test_data_catalogue()
+    r = cf.make_test_rand_catalogue()
+    d._source, r._source = DATA_SOURCE, RAND_SOURCE
+    return d, r
+
+
+def _paramset(tmp_path, geo, **kw):
+    # The reference's fixtures carry the geometry as output tag.
+    tag = f'_{geo}' if geo in ('gpp', 'lpp') else None
+    return cf.make_test_paramset(
+        directories={'catalogues': None, 'measurements': str(tmp_path)},
+        tags={'output': tag}, **kw)
+
+
+def _run(kind, geo, deg, form, idx, tmp_path, save):
+    from triumvirate_amd import threept as T
+    from triumvirate_amd import twopt as P
+    d, r = _catalogues()
+    be = _oracle()
+    if kind in ('powspec', '2pcf'):
+        ps = _paramset(tmp_path, geo, degrees={'ell1': None, 'ell2': None,
+                                          'ELL': deg})
+        b = cf.binning_fourier() if kind == 'powspec' else cf.binning_config()
+        if geo == 'gpp':
+            return P._compute_2pt_stats_sim_like(
+                be, kind, d, paramset=ps, degree=deg, binning=b, save=save)
+        if geo == 'lpp':
+            return P._compute_2pt_stats_survey_like(
+                be, kind, d, r, paramset=ps, degree=deg, binning=b, save=save)
+        return P._compute_corrfunc_window(be, r, paramset=ps, degree=deg,
+                                          binning=b, save=save)
+    ps = _paramset(tmp_path, geo)
+    b = cf.binning_fourier() if kind == 'bispec' else cf.binning_config()
+    if geo == 'gpp':
+        return T._compute_3pt_stats_sim_like(
+            be, kind, d, paramset=ps, degrees=deg, binning=b, form=form,
+            idx_bin=idx, save=save)
+    if geo == 'lpp':
+        return T._compute_3pt_stats_survey_like(
+            be, kind, d, r, paramset=ps, degrees=deg, binning=b, form=form,
+            idx_bin=idx, save=save)
+    return T._compute_3pcf_window(be, r, paramset=ps, degrees=deg,
+                                  binning=b, form=form, idx_bin=idx,
+                                  save=save)
+
+
+CASES = [
+    ('powspec', 'lpp', 0, None, None, 'pk0_lpp.txt'),
+    ('powspec', 'gpp', 2, None, None, 'pk2_gpp.txt'),
+    ('2pcf', 'gpp', 0, None, None, 'xi0_gpp.txt'),
+    ('2pcf', 'win', 2, None, None, 'xiw2.txt'),
+    ('bispec', 'gpp', (0, 0, 0), 'diag', None, 'bk000_diag_gpp.txt'),
+    ('bispec', 'lpp', (2, 0, 2), 'diag', None, 'bk202_diag_lpp.txt'),
+    ('3pcf', 'lpp', (0, 0, 0), 'row', 0, 'zeta000_row0_lpp.txt'),
+    ('3pcf', 'win', (0, 0, 0), 'diag', None, 'zetaw000_diag.txt'),
+]
+
+
+def _split(text):
+    head = [ln[2:] for ln in text.splitlines() if ln.startswith('# ')]
+    body = [ln for ln in text.splitlines() if not ln.startswith('#')]
+    return head, body
+
+
+def _same_numbers(a, b, rtol):
+    na, nb = NUM.findall(a), NUM.findall(b)
+    assert NUM.sub('#', a) == NUM.sub('#', b), (a, b)
+    for x, y in zip(na, nb):
+        assert abs(float(x) - float(y)) <= rtol * max(abs(float(y)), 1e-300), \
+            (a, b)
+
+
+@pytest.mark.parametrize('kind,geo,deg,form,idx,name', CASES)
+def test_saved_text_matches_reference_file(tmp_path, kind, geo, deg, form,
+                                           idx, name):
+    _run(kind, geo, deg, form, idx, tmp_path, '.txt')
+    out = tmp_path / name
+    assert out.exists(), sorted(p.name for p in tmp_path.iterdir())
+    got_head, got_body = _split(out.read_text())
+    ref_head, ref_body = _split((cf.REF_STATS / name).read_text())
+    assert len(got_head) == len(ref_head)
+    for g, r in zip(got_head, ref_head):
+        if r.startswith('Normalisation'):
+            _same_numbers(g, r, 2e-9)
+        else:
+            assert g == r
+    assert len(got_body) == len(ref_body)
+    ncount = 1 if kind in ('powspec', '2pcf') else 2
+    for g, r in zip(got_body, ref_body):
+        gc, rc = g.split('\t'), r.split('\t')
+        assert [len(c) for c in gc] == [len(c) for c in rc], (g, r)
+        # Bin centres and counts print identically.
+        for j in range(ncount):
+            assert gc[3 * j] == rc[3 * j] and gc[3 * j + 2] == rc[3 * j + 2]
+    # Values: within the tolerances of the fixture tests.
+    ext = np.loadtxt(cf.REF_STATS / name, unpack=True)
+    got = np.loadtxt(out, unpack=True)
+    nlead = 3 * ncount
+    np.testing.assert_allclose(got[:nlead], ext[:nlead], rtol=1e-9, atol=0)
+    for col in range(nlead, len(ext), 2):      # real parts
+        scale = max(np.max(np.abs(ext[col])), 1e-300)
+        atol = 1e-6 if kind == 'powspec' and col == nlead + 2 else 0.
+        assert np.max(np.abs(got[col] - ext[col])) <= max(2e-9 * scale, atol)
+
+
+def test_saved_npz_round_trips(tmp_path):
+    res = _run('bispec', 'gpp', (0, 0, 0), 'diag', None, tmp_path, '.npz')
+    out = tmp_path / 'bk000_diag_gpp.npz'
+    with np.load(out) as z:
+        assert set(z.files) == {k for k in res if not k.startswith('_')}
+        for k in z.files:
+            if k == 'header':
+                assert str(z[k]) == res['header']
+            else:
+                np.testing.assert_array_equal(z[k], res[k])
+    head = res['header'].splitlines()
+    assert head[0] == f'Catalogue source: {DATA_SOURCE}'
+    assert head[-1].startswith('[0] k1_cen, [1] k1_eff, [2] nmodes_1')
+
+
+def test_measurement_filenames():
+    base = {'tags': {'output': '_t'}, 'form': 'off-diag', 'idx_bin': 3,
+            'degrees': {'ell1': 1, 'ell2': 1, 'ELL': 0}}
+    cases = [
+        (dict(base, statistic_type='bispec'), 'bk110_offdiag3_t'),
+        (dict(base, statistic_type='3pcf', form='row'), 'zeta110_row3_t'),
+        (dict(base, statistic_type='3pcf-win', form='full'),
+         'zetaw110_full_t'),
+        (dict(base, statistic_type='3pcf-win-wa', form='diag',
+              wa_orders={'i': 1, 'j': 2}), 'zetaw110_diag_wa12_t'),
+        (dict(base, statistic_type='powspec',
+              degrees={'ell1': None, 'ell2': None, 'ELL': 4}), 'pk4_t'),
+        (dict(base, statistic_type='2pcf-win', tags={'output': None},
+              degrees={'ell1': None, 'ell2': None, 'ELL': 2}), 'xiw2'),
+    ]
+    for ps, stem in cases:
+        assert tio.measurement_filename(ps) == stem
+    with pytest.raises(ValueError):
+        tio.measurement_filename(dict(base, statistic_type='modes'))
+
+
+def test_unknown_save_format_raises(tmp_path):
+    with pytest.raises(ValueError, match='Unrecognised save format'):
+        _run('2pcf', 'gpp', 0, None, None, tmp_path, '.csv')
+
+
+def test_read_from_file_sets_source_and_columns(tmp_path):
+    from triumvirate_amd.catalogue import ParticleCatalogue
+    path = tmp_path / 'cat.txt'
+    data = np.array([[1., 2., 3., 4e-4, 0.5], [4., 5., 6., 5e-4, 2.]])
+    np.savetxt(path, data)
+    cat = ParticleCatalogue.read_from_file(
+        path, names=['x', 'y', 'z', 'nz', 'wsys'],
+        name_mapping={'ws': 'wsys'})
+    assert str(cat) == f'ParticleCatalogue(source=extfile:{path})'
+    np.testing.assert_array_equal(cat['ws'], [0.5, 2.])
+    np.testing.assert_array_equal(cat['wc'], [1., 1.])
+    assert cat.write_attrs_as_header().splitlines() == [
+        f'Catalogue source: extfile:{path}',
+        'Catalogue size: ntotal = 2, wtotal = 2.500, wstotal = 2.500',
+        'Catalogue particle extents: ([1.000, 4.000], [2.000, 5.000], '
+        '[3.000, 6.000])',
+    ]
+    with pytest.raises(ValueError):
+        ParticleCatalogue.read_from_file(path, format='fits')

@@ -18,6 +18,19 @@ import warnings
 import numpy as np
 
 
+def _header_names(filepath, format):
+    """Column names from a first line '# x y z ...' of a plain-text file
+    whose format is not 'ascii.no_header'."""
+    if format == 'ascii.no_header':
+        return None
+    with open(filepath) as f:
+        first = f.readline().strip()
+    if not first.startswith('#'):
+        return None
+    names = first.lstrip('#').replace(',', ' ').split()
+    return names or None
+
+
 class ParticleCatalogue:
     """Particle catalogue.
 
@@ -63,14 +76,81 @@ class ParticleCatalogue:
         self.wstotal = self._pdata['ws'].sum()
 
     @classmethod
-    def read_from_file(cls, filepath, names=('x', 'y', 'z', 'nz'),
-                       logger=None):
-        """Read a whitespace-separated plain-text catalogue."""
-        data = np.loadtxt(filepath, ndmin=2)
+    def read_from_file(cls, filepath, reader='astropy', names=None,
+                       format='ascii.no_header', name_mapping=None,
+                       table_kwargs=None, file_kwargs=None, logger=None):
+        """Read a plain-text catalogue (catalogue.py:150-388).
+
+        Plain-text formats ('ascii...', 'text') are read with numpy:
+        `names` names the columns in file order (default x, y, z, nz,
+        ws, wc for as many columns as the file holds; a header line
+        starting with '#' may name them instead) and `name_mapping`
+        maps the default names to file names.  Binary formats (FITS,
+        HDF5) need astropy or nbodykit, which this package does not
+        depend on, and raise.  The catalogue source becomes
+        ``extfile:<filepath>``.
+        """
+        if reader.lower() not in ('astropy', 'nbodykit'):
+            raise ValueError(
+                f"Unsupported catalogue file reader: {reader}. "
+                "Possible options: {'astropy', 'nbodykit'}.")
+        if not (format.startswith('ascii') or format == 'text'):
+            raise ValueError(
+                f"Unsupported catalogue file format without astropy: {format}.")
+        data = np.loadtxt(filepath, ndmin=2, **(table_kwargs or {}))
+        if names is None:
+            names = _header_names(filepath, format) or \
+                ('x', 'y', 'z', 'nz', 'ws', 'wc')[:data.shape[1]]
+        names = list(names)
+        if len(names) != data.shape[1]:
+            raise ValueError(
+                f"{len(names)} column names for {data.shape[1]} columns "
+                f"in {filepath}.")
         cols = {name: data[:, i] for i, name in enumerate(names)}
-        return cls(cols['x'], cols['y'], cols['z'], nz=cols.get('nz'),
+        for name, alt in (name_mapping or {}).items():
+            cols[name] = cols.pop(alt)
+        for ax in ('x', 'y', 'z'):
+            if ax not in cols:
+                raise ValueError(f"Mandatory field {ax} is missing.")
+        if 'nz' not in cols:
+            warnings.warn(
+                "Catalogue 'nz' field is not provided and thus set to zero, "
+                "which may raise errors in some computations.")
+            cols['nz'] = 0.
+        for wname in ('ws', 'wc'):
+            if wname not in cols:
+                warnings.warn(
+                    f"Catalogue '{wname}' field is not provided, "
+                    "so is set to unity.")
+        self = cls(cols['x'], cols['y'], cols['z'], nz=cols['nz'],
                    ws=cols.get('ws', 1.), wc=cols.get('wc', 1.),
                    logger=logger)
+        self._source = f'extfile:{filepath}'
+        return self
+
+    def write_attrs_as_header(self, catalogue_ref=None):
+        """Catalogue source, size and extents as header lines
+        (catalogue.py:803-865); with `catalogue_ref` (the random-source
+        catalogue) both are written, as data and random."""
+        def lines(cat, source_label, size_label, extent_label):
+            b = cat.bounds
+            return [
+                f"{source_label}: {cat._source}",
+                "{}: ntotal = {:d}, wtotal = {:.3f}, wstotal = {:.3f}".format(
+                    size_label, cat.ntotal, cat.wtotal, cat.wstotal),
+                "{}: ([{:.3f}, {:.3f}], [{:.3f}, {:.3f}], [{:.3f}, {:.3f}])"
+                .format(extent_label, *b['x'], *b['y'], *b['z']),
+            ]
+        if catalogue_ref is None:
+            text = lines(self, "Catalogue source", "Catalogue size",
+                         "Catalogue particle extents")
+        else:
+            text = lines(self, "Data catalogue source", "Data catalogue size",
+                         "Data-source particle extents") + \
+                lines(catalogue_ref, "Random catalogue source",
+                      "Random catalogue size",
+                      "Random-source particle extents")
+        return "\n".join(text)
 
     def __len__(self):
         return self.ntotal

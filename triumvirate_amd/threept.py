@@ -25,11 +25,11 @@ backend.
 
 import contextlib
 import ctypes as C
-import warnings
 
 import numpy as np
 
 from . import _abi
+from . import io as _io
 from .dataobjs import Binning
 from .parameters import (InvalidParameterError, ParameterSet,
                          fetch_paramset_template)
@@ -314,7 +314,7 @@ def _compute_3pt_stats_survey_like(backend, kind, catalogue_data,
                                    params_sampling=None, degrees=None,
                                    binning=None, form=None, idx_bin=None,
                                    precision=None, logger=None,
-                                   multipoles=None):
+                                   multipoles=None, save=False):
     stype = 'bispec' if kind == 'bispec' else '3pcf'
     paramset = _amalgamate_parameters(
         paramset=paramset, params_sampling=params_sampling, degrees=degrees,
@@ -370,6 +370,10 @@ def _compute_3pt_stats_survey_like(backend, kind, catalogue_data,
             kind, data_v, rand_v, los_data, los_rand, params,
             _binning_view(binning), norm_factor, handles=handles)
     results['_norm_factors'] = (norm_part, norm_mesh)
+    _io.record(results, paramset,
+               catalogue_data.write_attrs_as_header(
+                   catalogue_ref=catalogue_rand),
+               results['_norm_factors'], save, logger)
     return results
 
 
@@ -377,7 +381,7 @@ def _compute_3pt_stats_sim_like(backend, kind, catalogue_data, paramset=None,
                                 params_sampling=None, degrees=None,
                                 binning=None, form=None, idx_bin=None,
                                 precision=None, logger=None,
-                                multipoles=None):
+                                multipoles=None, save=False):
     stype = 'bispec' if kind == 'bispec' else '3pcf'
     paramset = _amalgamate_parameters(
         paramset=paramset, params_sampling=params_sampling, degrees=degrees,
@@ -412,14 +416,13 @@ def _compute_3pt_stats_sim_like(backend, kind, catalogue_data, paramset=None,
         results = backend.sim(kind, data_v, params, _binning_view(binning),
                               norm_factor, handle=hd)
     results['_norm_factors'] = (norm_part, norm_mesh)
+    _io.record(results, paramset, catalogue_data.write_attrs_as_header(),
+               results['_norm_factors'], save, logger)
     return results
 
 
-def _finish(results, save):
+def _finish(results):
     results.pop('_norm_factors', None)
-    if save:
-        warnings.warn("Saving measurements is not provided by this engine; "
-                      "the results are returned only.")
     return results
 
 
@@ -442,8 +445,9 @@ def compute_bispec(catalogue_data, catalogue_rand, los_data=None,
         _engine_backend(), 'bispec', catalogue_data, catalogue_rand,
         los_data=los_data, los_rand=los_rand, paramset=paramset,
         params_sampling=sampling_params, degrees=degrees, binning=binning,
-        form=form, idx_bin=idx_bin, precision=precision, logger=logger)
-    return _finish(res, save)
+        form=form, idx_bin=idx_bin, precision=precision, logger=logger,
+        save=save)
+    return _finish(res)
 
 
 def compute_3pcf(catalogue_data, catalogue_rand, los_data=None,
@@ -456,8 +460,9 @@ def compute_3pcf(catalogue_data, catalogue_rand, los_data=None,
         _engine_backend(), '3pcf', catalogue_data, catalogue_rand,
         los_data=los_data, los_rand=los_rand, paramset=paramset,
         params_sampling=sampling_params, degrees=degrees, binning=binning,
-        form=form, idx_bin=idx_bin, precision=precision, logger=logger)
-    return _finish(res, save)
+        form=form, idx_bin=idx_bin, precision=precision, logger=logger,
+        save=save)
+    return _finish(res)
 
 
 def compute_bispec_in_gpp_box(catalogue_data, degrees=None, binning=None,
@@ -469,8 +474,9 @@ def compute_bispec_in_gpp_box(catalogue_data, degrees=None, binning=None,
     res = _compute_3pt_stats_sim_like(
         _engine_backend(), 'bispec', catalogue_data, paramset=paramset,
         params_sampling=sampling_params, degrees=degrees, binning=binning,
-        form=form, idx_bin=idx_bin, precision=precision, logger=logger)
-    return _finish(res, save)
+        form=form, idx_bin=idx_bin, precision=precision, logger=logger,
+        save=save)
+    return _finish(res)
 
 
 def compute_3pcf_in_gpp_box(catalogue_data, degrees=None, binning=None,
@@ -482,14 +488,15 @@ def compute_3pcf_in_gpp_box(catalogue_data, degrees=None, binning=None,
     res = _compute_3pt_stats_sim_like(
         _engine_backend(), '3pcf', catalogue_data, paramset=paramset,
         params_sampling=sampling_params, degrees=degrees, binning=binning,
-        form=form, idx_bin=idx_bin, precision=precision, logger=logger)
-    return _finish(res, save)
+        form=form, idx_bin=idx_bin, precision=precision, logger=logger,
+        save=save)
+    return _finish(res)
 
 
 def _compute_3pcf_window(backend, catalogue_rand, los_rand=None, paramset=None,
                          params_sampling=None, degrees=None, wa_orders=None,
                          binning=None, form=None, idx_bin=None, precision=None,
-                         logger=None):
+                         logger=None, save=False):
     """Driver of the 3PCF window (reference threept.py:1832-2105): the random
     catalogue centred and periodised in the box, both normalisations with
     alpha = 1, the wide-angle kernel when `wa_orders` is given."""
@@ -524,6 +531,8 @@ def _compute_3pcf_window(backend, catalogue_rand, los_rand=None, paramset=None,
         int(wide_angle), i_wa, j_wa, C.byref(res.struct)))
     results = res.as_dict()
     results['_norm_factors'] = (norm_part, norm_mesh)
+    _io.record(results, paramset, catalogue_rand.write_attrs_as_header(),
+               results['_norm_factors'], save, logger)
     return results
 
 
@@ -538,8 +547,8 @@ def compute_3pcf_window(catalogue_rand, los_rand=None, degrees=None,
         _engine_backend(), catalogue_rand, los_rand=los_rand,
         paramset=paramset, params_sampling=sampling_params, degrees=degrees,
         wa_orders=wa_orders, binning=binning, form=form, idx_bin=idx_bin,
-        precision=precision, logger=logger)
-    return _finish(res, save)
+        precision=precision, logger=logger, save=save)
+    return _finish(res)
 
 
 def compute_bispec_multipoles(catalogue_data, catalogue_rand=None,

@@ -28,6 +28,7 @@ import math
 import numpy as np
 
 from . import _abi
+from . import io as _io
 from .catalogue import ParticleCatalogue
 from .dataobjs import Binning
 from .parameters import (InvalidParameterError, ParameterSet,
@@ -232,7 +233,7 @@ def _compute_2pt_stats_survey_like(backend, kind, catalogue_data,
                                    catalogue_rand, los_data=None,
                                    los_rand=None, paramset=None,
                                    params_sampling=None, degree=None,
-                                   binning=None, logger=None):
+                                   binning=None, logger=None, save=False):
     stype = 'powspec' if kind == 'powspec' else '2pcf'
     paramset = _amalgamate_parameters(
         paramset=paramset, params_sampling=params_sampling, degree=degree,
@@ -272,12 +273,16 @@ def _compute_2pt_stats_survey_like(backend, kind, catalogue_data,
                _abi.dptr(los_rand), C.byref(params), C.byref(bv.struct),
                C.c_double(norm))
     res['_norm_factors'] = (norm_part, norm_mesh, norm_meshes)
+    _io.record(res, paramset,
+               catalogue_data.write_attrs_as_header(
+                   catalogue_ref=catalogue_rand),
+               res['_norm_factors'], save, logger)
     return res
 
 
 def _compute_2pt_stats_sim_like(backend, kind, catalogue_data, paramset=None,
                                 params_sampling=None, degree=None,
-                                binning=None, logger=None):
+                                binning=None, logger=None, save=False):
     stype = 'powspec' if kind == 'powspec' else '2pcf'
     paramset = _amalgamate_parameters(
         paramset=paramset, params_sampling=params_sampling, degree=degree,
@@ -304,12 +309,14 @@ def _compute_2pt_stats_sim_like(backend, kind, catalogue_data, paramset=None,
     res = _run(backend, name, kind, binning.num_bins, C.byref(data_v.struct),
                C.byref(params), C.byref(bv.struct), C.c_double(norm))
     res['_norm_factors'] = (norm_part, norm_mesh, 0.)
+    _io.record(res, paramset, catalogue_data.write_attrs_as_header(),
+               res['_norm_factors'], save, logger)
     return res
 
 
 def _compute_corrfunc_window(backend, catalogue_rand, los_rand=None,
                              paramset=None, params_sampling=None, degree=None,
-                             binning=None, logger=None):
+                             binning=None, logger=None, save=False):
     paramset = _amalgamate_parameters(
         paramset=paramset, params_sampling=params_sampling, degree=degree,
         binning=binning, catalogue_type='random', statistic_type='2pcf-win')
@@ -336,6 +343,8 @@ def _compute_corrfunc_window(backend, catalogue_rand, los_rand=None,
                C.byref(rand_v.struct), _abi.dptr(los_rand), C.byref(params),
                C.byref(bv.struct), C.c_double(1.), C.c_double(norm))
     res['_norm_factors'] = (norm_part, norm_mesh, 0.)
+    _io.record(res, paramset, catalogue_rand.write_attrs_as_header(),
+               res['_norm_factors'], save, logger)
     return res
 
 
@@ -354,8 +363,8 @@ def compute_powspec(catalogue_data, catalogue_rand, los_data=None,
         _engine_backend(), 'powspec', catalogue_data, catalogue_rand,
         los_data=los_data, los_rand=los_rand, paramset=paramset,
         params_sampling=sampling_params, degree=degree, binning=binning,
-        logger=logger)
-    return _finish(res, save)
+        logger=logger, save=save)
+    return _finish(res)
 
 
 def compute_corrfunc(catalogue_data, catalogue_rand, los_data=None,
@@ -368,8 +377,8 @@ def compute_corrfunc(catalogue_data, catalogue_rand, los_data=None,
         _engine_backend(), '2pcf', catalogue_data, catalogue_rand,
         los_data=los_data, los_rand=los_rand, paramset=paramset,
         params_sampling=sampling_params, degree=degree, binning=binning,
-        logger=logger)
-    return _finish(res, save)
+        logger=logger, save=save)
+    return _finish(res)
 
 
 def compute_powspec_in_gpp_box(catalogue_data, degree=None, binning=None,
@@ -381,8 +390,8 @@ def compute_powspec_in_gpp_box(catalogue_data, degree=None, binning=None,
     res = _compute_2pt_stats_sim_like(
         _engine_backend(), 'powspec', catalogue_data, paramset=paramset,
         params_sampling=sampling_params, degree=degree, binning=binning,
-        logger=logger)
-    return _finish(res, save)
+        logger=logger, save=save)
+    return _finish(res)
 
 
 def compute_corrfunc_in_gpp_box(catalogue_data, degree=None, binning=None,
@@ -393,8 +402,8 @@ def compute_corrfunc_in_gpp_box(catalogue_data, degree=None, binning=None,
     res = _compute_2pt_stats_sim_like(
         _engine_backend(), '2pcf', catalogue_data, paramset=paramset,
         params_sampling=sampling_params, degree=degree, binning=binning,
-        logger=logger)
-    return _finish(res, save)
+        logger=logger, save=save)
+    return _finish(res)
 
 
 def compute_corrfunc_window(catalogue_rand, los_rand=None, degree=None,
@@ -405,8 +414,8 @@ def compute_corrfunc_window(catalogue_rand, los_rand=None, degree=None,
     res = _compute_corrfunc_window(
         _engine_backend(), catalogue_rand, los_rand=los_rand,
         paramset=paramset, params_sampling=sampling_params, degree=degree,
-        binning=binning, logger=logger)
-    return _finish(res, save)
+        binning=binning, logger=logger, save=save)
+    return _finish(res)
 
 
 __all__ = ['compute_powspec', 'compute_corrfunc', 'compute_powspec_in_gpp_box',
