@@ -30,9 +30,9 @@ def test_form_full_becomes_triu_when_degrees_equal():
 def test_interlacing_is_switched_off_for_three_point():
     """parameters.cpp:1240-1249."""
     ps = cf.make_test_paramset(interlace=True, statistic_type='bispec')
-    assert ps['interlace'] == 'false'
+    assert ps['interlace'] is False
     ps = cf.make_test_paramset(interlace='on', statistic_type='powspec')
-    assert ps['interlace'] == 'true'
+    assert ps['interlace'] is True
 
 
 @pytest.mark.parametrize('scheme,order', [('ngp', 1), ('cic', 2), ('tsc', 3),

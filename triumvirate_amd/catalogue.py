@@ -79,34 +79,45 @@ class ParticleCatalogue:
     def read_from_file(cls, filepath, reader='astropy', names=None,
                        format='ascii.no_header', name_mapping=None,
                        table_kwargs=None, file_kwargs=None, logger=None):
-        """Read a plain-text catalogue (catalogue.py:150-388).
+        """Read a catalogue file (catalogue.py:150-388).
 
         Plain-text formats ('ascii...', 'text') are read with numpy:
         `names` names the columns in file order (default x, y, z, nz,
         ws, wc for as many columns as the file holds; a header line
-        starting with '#' may name them instead) and `name_mapping`
-        maps the default names to file names.  Binary formats (FITS,
-        HDF5) need astropy or nbodykit, which this package does not
-        depend on, and raise.  The catalogue source becomes
-        ``extfile:<filepath>``.
+        starting with '#' may name them instead).  The 'binary' format
+        reads fixed-size records whose field names and types `names`
+        gives as a structured dtype (a list of (name, type) pairs), as
+        nbodykit's binary reader does.  `name_mapping` maps the default
+        names to file names.  FITS and HDF5 need astropy or nbodykit,
+        which this package does not depend on, and raise.  The catalogue
+        source becomes ``extfile:<filepath>``.
         """
         if reader.lower() not in ('astropy', 'nbodykit'):
             raise ValueError(
                 f"Unsupported catalogue file reader: {reader}. "
                 "Possible options: {'astropy', 'nbodykit'}.")
-        if not (format.startswith('ascii') or format == 'text'):
+        if format == 'binary':
+            if names is None:
+                raise ValueError(
+                    "Binary catalogues need `names` as a structured dtype.")
+            rec = np.fromfile(filepath, dtype=np.dtype(list(names)),
+                              **(file_kwargs or {}))
+            cols = {name: np.asarray(rec[name], dtype=np.float64)
+                    for name in rec.dtype.names}
+        elif format.startswith('ascii') or format == 'text':
+            data = np.loadtxt(filepath, ndmin=2, **(table_kwargs or {}))
+            if names is None:
+                names = _header_names(filepath, format) or \
+                    ('x', 'y', 'z', 'nz', 'ws', 'wc')[:data.shape[1]]
+            names = list(names)
+            if len(names) != data.shape[1]:
+                raise ValueError(
+                    f"{len(names)} column names for {data.shape[1]} columns "
+                    f"in {filepath}.")
+            cols = {name: data[:, i] for i, name in enumerate(names)}
+        else:
             raise ValueError(
                 f"Unsupported catalogue file format without astropy: {format}.")
-        data = np.loadtxt(filepath, ndmin=2, **(table_kwargs or {}))
-        if names is None:
-            names = _header_names(filepath, format) or \
-                ('x', 'y', 'z', 'nz', 'ws', 'wc')[:data.shape[1]]
-        names = list(names)
-        if len(names) != data.shape[1]:
-            raise ValueError(
-                f"{len(names)} column names for {data.shape[1]} columns "
-                f"in {filepath}.")
-        cols = {name: data[:, i] for i, name in enumerate(names)}
         for name, alt in (name_mapping or {}).items():
             cols[name] = cols.pop(alt)
         for ax in ('x', 'y', 'z'):

@@ -16,8 +16,8 @@ import numpy as np
 
 
 def _interlace_text(paramset):
-    # The reference prints the transmuted boolean of its parameter set.
-    return 'True' if paramset['interlace'] in ('true', True) else 'False'
+    # The validated parameter set holds interlacing as a bool.
+    return str(bool(paramset['interlace']))
 
 
 def _three_point_form_tag(paramset):

@@ -6,7 +6,7 @@ the three-point path reads, and restates the validation/derivation rules of
 
 * ``assignment`` -> ``assignment_order`` (parameters.cpp:659-670);
 * ``interlace`` accepts True/False/'on'/'off'/'true'/'false' and is
-  transmuted to 'true'/'false' (parameters.cpp:672-690);
+  transmuted to a bool (parameters.cpp:672-690, parameters.pyx:576-585);
 * ``statistic_type`` -> ``npoint``/``space`` (parameters.cpp:700-815);
 * ``form`` 'full' with ``ell1 == ell2`` -> ``shape`` 'triu', otherwise
   ``shape = form`` (parameters.cpp:830-850);
@@ -20,6 +20,8 @@ Defaults follow the reference template
 
 import copy
 import logging
+from pathlib import Path
+from pprint import pformat, pprint
 
 import yaml
 
@@ -98,8 +100,13 @@ class ParameterSet:
             with open(param_filepath) as f:
                 loaded = yaml.safe_load(f) or {}
             params.update(loaded)
+            self._source = f'extfile:{param_filepath}'
         elif param_dict is not None:
             params.update(copy.deepcopy(param_dict))
+            self._source = 'extdict'
+        else:
+            self._source = 'template'
+        self._original = True
         self._params = params
         self._validate()
 
@@ -109,6 +116,7 @@ class ParameterSet:
 
     def __setitem__(self, key, value):
         self._params[key] = value
+        self._original = False
 
     def __contains__(self, key):
         return key in self._params
@@ -122,12 +130,41 @@ class ParameterSet:
     def keys(self):
         return self._params.keys()
 
+    def names(self):
+        return self._params.keys()
+
     def update(self, *args, **kwargs):
         self._params.update(*args, **kwargs)
         self._validate()
+        self._original = False
 
     def __repr__(self):
         return f"ParameterSet({self._params!r})"
+
+    def __str__(self):
+        return "\n".join([
+            f"ParameterSet(source={self._source}, original={self._original})",
+            pformat(self._params, sort_dicts=False)])
+
+    def print(self):
+        """Print the parameters as a dictionary (parameters.pyx:265-270)."""
+        pprint(self._params, sort_dicts=False)
+
+    def save(self, filepath=None):
+        """Write the validated parameters to a YAML file; by default
+        ``parameters_used{output tag}.yml`` in the measurement directory
+        (parameters.pyx:272-296)."""
+        if filepath is None:
+            filepath = Path(
+                self._params['directories']['measurements'] or '',
+                "parameters_used{}.yml".format(
+                    self._params['tags']['output'] or ''))
+        with open(filepath, 'w') as f:
+            yaml.safe_dump(self._params, f, sort_keys=False,
+                           default_flow_style=False)
+        if self._logger:
+            self._logger.info("Saved parameters to %s.", filepath)
+        return filepath
 
     # validation -------------------------------------------------------------
     def _validate(self):
@@ -151,9 +188,9 @@ class ParameterSet:
 
         interlace = p.get('interlace')
         if interlace in (True, 'true', 'on'):
-            p['interlace'] = 'true'
+            p['interlace'] = True
         elif interlace in (False, 'false', 'off', None):
-            p['interlace'] = 'false'
+            p['interlace'] = False
         else:
             raise InvalidParameterError(
                 "Interlacing must be 'true'/'on' or 'false'/'off': "
@@ -245,8 +282,8 @@ class ParameterSet:
                     "Bin index `idx_bin` must be < `num_bins` in absolute "
                     "value.")
 
-        if p.get('npoint') == '3pt' and p['interlace'] == 'true':
-            p['interlace'] = 'false'
+        if p.get('npoint') == '3pt' and p['interlace'] is True:
+            p['interlace'] = False
             self._logger.info(
                 "Interlacing is unsupported for three-point measurements. "
                 "`interlace` is set to 'false'.")

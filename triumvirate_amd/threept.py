@@ -255,7 +255,7 @@ def _params_struct(paramset):
         p.boxsize[i] = float(box[a])
         p.ngrid[i] = int(ngrid[a])
     p.assignment_order = int(paramset['assignment_order'])
-    p.interlace = 1 if paramset['interlace'] == 'true' else 0
+    p.interlace = 1 if paramset['interlace'] is True else 0
     p.ell1, p.ell2, p.ELL = int(deg['ell1']), int(deg['ell2']), int(deg['ELL'])
     p.shape = _abi.SHAPE_CODES[shape]
     p.idx_bin = int(paramset['idx_bin'] or 0)

@@ -151,7 +151,7 @@ def _params_struct(paramset):
         p.boxsize[i] = float(box[a])
         p.ngrid[i] = int(ngrid[a])
     p.assignment_order = int(paramset['assignment_order'])
-    p.interlace = 1 if paramset['interlace'] == 'true' else 0
+    p.interlace = 1 if paramset['interlace'] is True else 0
     p.ell1, p.ell2, p.ELL = 0, 0, int(ELL)
     p.shape = 0
     p.idx_bin = 0
