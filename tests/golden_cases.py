@@ -2,6 +2,8 @@
 golden-vector generator (tests/golden/make_golden.py) and the tests.
 
   cfg1_gpp_b000_diag : periodic box, uniform positions, TSC, 10 diag bins
+  cfg1_full_gpp_b000_diag : the same at the stated size (64^3, 2e5
+                       particles in L = 1000 Mpc/h, bins [0.01, 0.11])
   cfg2_lpp_b000_full : survey quadrant shell, lognormal data + randoms, PCS,
                        B_000 full (-> upper triangle)
   cfg2_lpp_b202_full : same inputs, B_202 full
@@ -11,8 +13,9 @@ golden-vector generator (tests/golden/make_golden.py) and the tests.
 
 import numpy as np
 
-CASES = ['cfg1_gpp_b000_diag', 'cfg2_lpp_b000_full', 'cfg2_lpp_b202_full',
-         'cfg3_gpp_b202_full', 'cfg5_lpp_z110_full']
+CASES = ['cfg1_gpp_b000_diag', 'cfg1_full_gpp_b000_diag',
+         'cfg2_lpp_b000_full', 'cfg2_lpp_b202_full', 'cfg3_gpp_b202_full',
+         'cfg5_lpp_z110_full']
 
 
 def _paramset(ngrid, L, scheme):
@@ -46,6 +49,14 @@ def run_case(backend, name, precision=None):
         b = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.11, num_bins=10)
         return T._compute_3pt_stats_sim_like(
             backend, 'bispec', d, paramset=_paramset(32, 1000., 'tsc'),
+            degrees=(0, 0, 0), binning=b, form='diag', precision=precision)
+    if name == 'cfg1_full_gpp_b000_diag':
+        g = np.random.default_rng(11)
+        x, y, z = g.uniform(0., 1000., (3, 200000))
+        d = ParticleCatalogue(x, y, z, nz=200000 / 1e9)
+        b = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.11, num_bins=10)
+        return T._compute_3pt_stats_sim_like(
+            backend, 'bispec', d, paramset=_paramset(64, 1000., 'tsc'),
             degrees=(0, 0, 0), binning=b, form='diag', precision=precision)
     if name in ('cfg2_lpp_b000_full', 'cfg2_lpp_b202_full'):
         d, r = _survey_inputs(20000, 100000, seed=2)

@@ -39,7 +39,7 @@ def gaussian_field(ngrid, boxsize, pk=default_pk, seed=0):
     amp[0, 0, 0] = 0.
     wk *= amp
     del amp, k
-    return np.fft.irfftn(wk, s=(n, n, n))
+    return np.fft.irfftn(wk, s=(n, n, n), axes=(0, 1, 2))
 
 
 def _lognormal(delta_g):
