@@ -156,10 +156,16 @@ def bench_bispec(args, lib):
     multipoles = [((0, 0, 0), 'triu'), ((2, 0, 2), 'full')]
     pairs_per_step = 210 + 400
 
-    def step(prec, handles=None):
+    # Observer position in the box frame: the catalogues were centred by
+    # an offset d after their lines of sight were taken (positions r - d).
+    observer = -np.asarray(data._offset, dtype=np.float64)
+
+    def step(prec, handles=None, los_arrays=False):
         """One engine call computing both multipoles on shared fields:
-        host-resident catalogues in (the default), results out on the host;
-        with handles, device-resident catalogues."""
+        host-resident catalogues in (the default; lines of sight formed on
+        the device from the observer position, or with los_arrays passed
+        as n x 3 host arrays), results out on the host; with handles,
+        device-resident catalogues."""
         plist = [params_struct(args.ngrid, deg, shape, prec)
                  for deg, shape in multipoles]
         res = [_abi.ResultBuffers(_abi.result_dim(p.shape, 20, 0), 'bispec')
@@ -167,11 +173,15 @@ def bench_bispec(args, lib):
         arr = (_abi.Params * len(plist))(*plist)
         outs = (_abi.BispecResult * len(plist))(*[r.struct for r in res])
         norms = np.full(len(plist), norm.value)
-        if handles is None:
+        if handles is None and los_arrays:
             check(lib.trvg_compute_bispec_multi(
                 C.byref(dview.struct), C.byref(rview.struct), _abi.dptr(los_d),
                 _abi.dptr(los_r), arr, len(plist), C.byref(kbv.struct),
                 _abi.dptr(norms), outs))
+        elif handles is None:
+            check(lib.trvg_compute_bispec_multi_observer(
+                C.byref(dview.struct), C.byref(rview.struct), _abi.dptr(observer),
+                arr, len(plist), C.byref(kbv.struct), _abi.dptr(norms), outs))
         else:
             check(lib.trvg_compute_bispec_multi_dev(
                 handles[0], handles[1], arr, len(plist), C.byref(kbv.struct),
@@ -252,6 +262,17 @@ def bench_bispec(args, lib):
         t64, r64 = timed(lambda: step(64), max(5, args.steps))
         log(f"FP64 call median {np.median(t64) * 1e3:.2f} ms")
 
+    # Host-resident with the lines of sight as n x 3 host arrays (the
+    # reference's calling convention): secondary figure, and the check that
+    # both forms agree.
+    step(32, los_arrays=True)
+    tlos, rlos = timed(lambda: step(32, los_arrays=True), args.steps)
+    los_agree = max(
+        float(np.max(np.abs(rlos[d][k] - r32[d][k])) / np.max(np.abs(r32[d][k])))
+        for d, _ in multipoles for k in ('bk_raw', 'bk_shot'))
+    log(f"host LOS arrays: FP32 call median {np.median(tlos) * 1e3:.2f} ms, "
+        f"max rel diff vs device LOS {los_agree:.1e}")
+
     # Device-resident catalogues (secondary figure): uploads outside.
     t = time.perf_counter()
     handles = [_upload(lib, data, los_d), _upload(lib, rand, los_r)]
@@ -316,8 +337,9 @@ def bench_bispec(args, lib):
                         'B_202 (full, 400 pairs), 20 lin k-bins '
                         '[0.005,0.205], 1e6 lognormal data + 5e6 randoms, '
                         'L=3200, PCS, interlace off; one engine call per '
-                        'step with host-resident catalogues in and results '
-                        'out (median of the steps)',
+                        'step with host-resident catalogues in (lines of '
+                        'sight formed on the device from the observer '
+                        'position) and results out (median of the steps)',
             'ngrid': args.ngrid, 'pairs_per_step': pairs_per_step,
             'n_data': int(data.ntotal), 'n_rand': int(rand.ntotal),
             'precision': 'fp32 fields, fp64 sums',
@@ -330,6 +352,10 @@ def bench_bispec(args, lib):
                  'ms_per_step': round(float(np.median(t64)) * 1e3, 3),
                  'step_ms': [round(t * 1e3, 3) for t in t64]} if t64 else None,
         'fp32_rel_err': errs,
+        'host_los_arrays': {
+            'value': round(pairs_per_step / float(np.median(tlos)), 2),
+            'ms_per_step': round(float(np.median(tlos)) * 1e3, 3),
+            'max_rel_diff_vs_device_los': los_agree},
         'device_resident': {
             'value': round(pairs_per_step / float(np.median(tdev)), 2),
             'ms_per_step': round(float(np.median(tdev)) * 1e3, 3),

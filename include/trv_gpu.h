@@ -314,6 +314,17 @@ typedef struct trvg_device_catalogue trvg_device_catalogue;
 
 int trvg_catalogue_upload(const trvg_catalogue* cat, const double* los,
                           int device, trvg_device_catalogue** out);
+
+/* As trvg_catalogue_upload with the lines of sight formed on the device
+ * from the positions: los_i = (r_i - observer) / |r_i - observer| (a zero
+ * norm taken as 1), observer[3] being the observer's position in the
+ * coordinates of `cat`.  For a catalogue aligned in the box by an offset
+ * (ParticleCatalogue::offset_coords, positions r - d), observer = -d
+ * gives the lines of sight the reference's driver computes before
+ * alignment (threept.py:525-528, catalogue.py:437-458), up to the
+ * rounding of the shift; no n x 3 array crosses the bus. */
+int trvg_catalogue_upload_observer(const trvg_catalogue* cat, const double* observer,
+                                   int device, trvg_device_catalogue** out);
 int trvg_catalogue_free(trvg_device_catalogue* cat);
 
 /* As trvg_calc_bispec_normalisation_from_mesh on a device-resident
@@ -361,6 +372,17 @@ int trvg_compute_bispec_multi(const trvg_catalogue* data,
                               const trvg_binning* kbinning,
                               const double* norm_factors,
                               trvg_bispec_result* outs);
+
+/* trvg_compute_bispec_multi for survey catalogues with the lines of sight
+ * formed on the device from an observer position (as
+ * trvg_catalogue_upload_observer). */
+int trvg_compute_bispec_multi_observer(const trvg_catalogue* data,
+                                       const trvg_catalogue* rand,
+                                       const double* observer,
+                                       const trvg_params* params, int nmultipoles,
+                                       const trvg_binning* kbinning,
+                                       const double* norm_factors,
+                                       trvg_bispec_result* outs);
 
 /* Frees the cached device work buffers (they are otherwise kept for reuse
  * by later calls). */
@@ -491,8 +513,10 @@ int trvg_diff_bispec_diag_backward(const trvg_params* params,
                                    const double* grad_bk, void* grad_mesh);
 
 /* Streams `bytes` (rounded down to 16-byte units) from one device buffer
- * to another with a plain copy kernel and returns its duration in
- * milliseconds: the reference point for scaling hardware byte counters
+ * to another with a streaming copy kernel (four 16-byte loads in flight
+ * per thread, nontemporal loads and stores, one pass) and returns its
+ * duration in milliseconds (best of three): the device's practical HBM
+ * copy rate, and the reference point for scaling hardware byte counters
  * (a copy moves exactly `bytes` in and `bytes` out). */
 int trvg_copy_calibration(long long bytes, double* ms);
 

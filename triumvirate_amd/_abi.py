@@ -272,6 +272,13 @@ def declare_extensions(lib):
     vp = C.c_void_p
     lib.trvg_catalogue_upload.argtypes = [P(Catalogue), _dp, C.c_int, P(vp)]
     lib.trvg_catalogue_upload.restype = C.c_int
+    lib.trvg_catalogue_upload_observer.argtypes = [P(Catalogue), _dp, C.c_int,
+                                                   P(vp)]
+    lib.trvg_catalogue_upload_observer.restype = C.c_int
+    lib.trvg_compute_bispec_multi_observer.argtypes = [
+        P(Catalogue), P(Catalogue), _dp, P(Params), C.c_int, P(Binning), _dp,
+        P(BispecResult)]
+    lib.trvg_compute_bispec_multi_observer.restype = C.c_int
     lib.trvg_catalogue_free.argtypes = [vp]
     lib.trvg_catalogue_free.restype = C.c_int
     lib.trvg_calc_bispec_normalisation_from_mesh_dev.argtypes = [

@@ -70,6 +70,7 @@ class ParticleCatalogue:
             'nz': col(nz), 'ws': col(ws), 'wc': col(wc),
         }
         self._source = f'extdata:{id(self._pdata)}'
+        self._offset = np.zeros(3)
         self._calc_bounds()
         self.ntotal = n
         self.wtotal = (self._pdata['ws'] * self._pdata['wc']).sum()
@@ -212,6 +213,9 @@ class ParticleCatalogue:
         for ax, c in zip('xyz', origin):
             self._pdata[ax] = self._pdata[ax] - c
         self._calc_bounds()
+        # Total displacement applied since construction: the original
+        # coordinates are the current ones plus this vector.
+        self._offset = self._offset + np.asarray(origin, dtype=np.float64)
 
     def centre(self, boxsize, catalogue_ref=None):
         if np.isscalar(boxsize):

@@ -146,10 +146,14 @@ void release_stream(int dev, hipStream_t s);
 // the copies run on a separate stream, chunk by chunk, while the context's
 // stream counts each arrived chunk for the tile sort of the context's mesh,
 // which the upload completes; with los32 the lines of sight cross as FP32
-// (FP32-field calls) and are widened on the device.
+// (FP32-field calls) and are widened on the device.  Without `los` and
+// with `observer` (3 coordinates in the frame of the positions), the lines
+// of sight are formed on the device from the positions relative to the
+// observer, as the chunks arrive.
 void upload_catalogue(const trvg_catalogue* c, const double* los,
                       DeviceCatalogue& out, hipStream_t s,
-                      Context* sort_for = nullptr, bool los32 = false);
+                      Context* sort_for = nullptr, bool los32 = false,
+                      const double* observer = nullptr);
 
 // ---------------------------------------------------------------------------
 // Spherical Bessel spline (host build, device tables).

@@ -232,19 +232,26 @@ static void launch_z_batch(Context& ctx, c2<T>* base, const BatchLoader& load,
   const size_t smem = fft_smem<T, LOGN>(false);
   const c2<T>* tw = ctx_twiddles<T>(ctx, N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, bytes);
+  bool reg = false;
   if constexpr (use_zreg<T, LOGN, BatchLoader>()) {
     static_assert(E == zreg_block<T>(), "z pass block size");
-    hipLaunchKernelGGL((fft_pass_z_batch_reg<T, LOGN, BatchLoader>), dim3(unsigned(nwork)),
-                       dim3(zreg_threads<T, LOGN>()), (zreg_smem<T, LOGN>()), ctx.stream, base,
-                       ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);
-  } else if (ctx.g.n[1] % (E / N) == 0)
-    hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, true, BatchLoader>),
-                       dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream,
-                       base, ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);
-  else
-    hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, false, BatchLoader>),
-                       dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream,
-                       base, ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);
+    reg = !pr.on;   // the register pass takes every listed block
+    if (reg)
+      hipLaunchKernelGGL((fft_pass_z_batch_reg<T, LOGN, BatchLoader>),
+                         dim3(unsigned((nwork + 1) / 2)), dim3(zreg_threads<T, LOGN>()),
+                         (zreg_smem<T, LOGN>()), ctx.stream, base, ctx.g.nmesh, load, tw,
+                         ctx.g.n[1], sign, work, nwork);
+  }
+  if (!reg) {
+    if (ctx.g.n[1] % (E / N) == 0)
+      hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, true, BatchLoader>),
+                         dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream,
+                         base, ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);
+    else
+      hipLaunchKernelGGL((fft_pass_z_batch<T, LOGN, EPT, false, BatchLoader>),
+                         dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream,
+                         base, ctx.g.nmesh, load, tw, ctx.g.n[1], sign, pr, kcut2, work);
+  }
   TRVG_LAUNCH_CHECK();
 }
 

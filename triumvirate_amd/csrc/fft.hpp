@@ -21,6 +21,15 @@ namespace trvg {
 
 constexpr int FFT_THREADS = 256;
 
+// Global stores and loads of the passes' mesh data (one place for their
+// cache policy; plain accesses measured best: streaming hints gained on
+// some passes and lost on others).
+template <typename T>
+__device__ __forceinline__ void st_mesh(c2<T>* p, c2<T> v) { *p = v; }
+
+template <typename T>
+__device__ __forceinline__ c2<T> ld_mesh(const c2<T>* p) { return *p; }
+
 // Elements per thread of a pass (a workgroup holds 256 * EPT elements).
 // Eight for both precisions: the FP32 passes are latency-bound, and the
 // smaller LDS footprint admits twice the workgroups per CU.
@@ -258,6 +267,64 @@ constexpr size_t zreg_smem() {
 // and writes to LDS; after one exchange each thread takes B / A outer DFTs
 // of length A in registers and stores A adjacent outputs per lane group.
 // One LDS round trip per element instead of one per radix-4 stage.
+// Loads of block `blk`: thread (l, q) takes elements k = q + A n2 of line
+// l of the block (lanes of a line read A adjacent elements per load).
+template <typename T, int LOGN, class Loader>
+__device__ __forceinline__ void zreg_load(c2<T>* v, const Loader& load, int n1,
+                                          long long blk) {
+  constexpr int N = 1 << LOGN;
+  constexpr int A = ZRegShape<LOGN>::A, B = ZRegShape<LOGN>::B;
+  constexpr int L = zreg_block<T>() / N;
+  const int l = threadIdx.x / A, q = threadIdx.x % A;
+  const long long line = blk * L + l;
+  const int i = int(line / n1), j = int(line % n1);
+  const long long g0 = line * N;
+#pragma unroll
+  for (int n2 = 0; n2 < B; n2++) {
+    const int k = q + A * n2;
+    v[n2] = load(g0 + k, i, j, k);
+  }
+}
+
+// Transform of the loaded block and its store: the inner DFT in registers,
+// w_N^{n1 k2} and one LDS exchange, the outer DFTs in registers, A adjacent
+// outputs per lane group.  W (the N twiddles) is in LDS; the LDS line
+// buffer may be reused by the next block of the workgroup.
+template <typename T, int LOGN>
+__device__ __forceinline__ void zreg_transform_store(c2<T>* v, c2<T>* __restrict__ data,
+                                                     const c2<T>* W, c2<T>* buf,
+                                                     int sign, long long blk) {
+  constexpr int N = 1 << LOGN;
+  constexpr int A = ZRegShape<LOGN>::A, B = ZRegShape<LOGN>::B;
+  constexpr int L = zreg_block<T>() / N;
+  constexpr int AP = A + 1;        // padded stride between k2 rows in LDS
+  constexpr int LSTR = B * AP;     // LDS elements per line
+  const int l = threadIdx.x / A, q = threadIdx.x % A;
+  const T sgn = T(sign);
+  dft_reg<T, B>(v, sgn);
+  __syncthreads();
+  c2<T>* lb = buf + l * LSTR;
+  lb[q] = v[0];
+#pragma unroll
+  for (int k2 = 1; k2 < B; k2++) lb[k2 * AP + q] = cmul<T>(v[k2], W[q * k2]);
+  __syncthreads();
+  c2<T>* out = data + (blk * L + l) * N;
+#pragma unroll
+  for (int r = 0; r < B / A; r++) {
+    const int k2 = q + A * r;
+    c2<T> u[A];
+#pragma unroll
+    for (int m = 0; m < A; m++) u[m] = lb[k2 * AP + m];
+    dft_reg<T, A>(u, sgn);
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) st_mesh<T>(out + k2 + B * k1, u[k1]);
+  }
+}
+
+// z pass of block `blk` (L = zreg_block / N whole lines) as
+// N = A * B: with n = n1 + A n2 and k = k2 + B k1,
+//   X[k2 + B k1] = sum_n1 w_A^{n1 k1} w_N^{n1 k2} sum_n2 x[n1 + A n2] w_B^{n2 k2}.
+// One LDS round trip per element instead of one per radix-4 stage.
 template <typename T, int LOGN, class Loader>
 __device__ __forceinline__ void fft_z_block_reg(c2<T>* __restrict__ data,
                                                 const Loader& load,
@@ -267,52 +334,23 @@ __device__ __forceinline__ void fft_z_block_reg(c2<T>* __restrict__ data,
   constexpr int A = ZRegShape<LOGN>::A, B = ZRegShape<LOGN>::B;
   constexpr int L = zreg_block<T>() / N;
   constexpr int NT = L * A;
-  constexpr int AP = A + 1;        // padded stride between k2 rows in LDS
-  constexpr int LSTR = B * AP;     // LDS elements per line
   extern __shared__ char smem_raw[];
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
-  c2<T>* W = buf + L * LSTR;
+  c2<T>* W = buf + L * B * (A + 1);
   const int t = threadIdx.x;
-  const int l = t / A, q = t % A;
-  const long long line0 = blk * L;
   if (pr.on) {
     // Skip the workgroup if none of its lines is needed (uniform exit).
     bool any = false;
     for (int ll = 0; ll < L && !any; ll++) {
-      const long long ln = line0 + ll;
+      const long long ln = blk * L + ll;
       any = pr.p1(int(ln / n1), int(ln % n1));
     }
     if (!any) return;
   }
   for (int m = t; m < N; m += NT) W[m] = tw[m];
-  const long long line = line0 + l;
-  const int i = int(line / n1), j = int(line % n1);
-  const long long g0 = line * N;
-  const T sgn = T(sign);
   c2<T> v[B];
-#pragma unroll
-  for (int n2 = 0; n2 < B; n2++) {
-    const int k = q + A * n2;
-    v[n2] = load(g0 + k, i, j, k);
-  }
-  dft_reg<T, B>(v, sgn);
-  __syncthreads();
-  c2<T>* lb = buf + l * LSTR;
-  lb[q] = v[0];
-#pragma unroll
-  for (int k2 = 1; k2 < B; k2++) lb[k2 * AP + q] = cmul<T>(v[k2], W[q * k2]);
-  __syncthreads();
-  c2<T>* out = data + g0;
-#pragma unroll
-  for (int r = 0; r < B / A; r++) {
-    const int k2 = q + A * r;
-    c2<T> u[A];
-#pragma unroll
-    for (int m = 0; m < A; m++) u[m] = lb[k2 * AP + m];
-    dft_reg<T, A>(u, sgn);
-#pragma unroll
-    for (int k1 = 0; k1 < A; k1++) out[k2 + B * k1] = u[k1];
-  }
+  zreg_load<T, LOGN>(v, load, n1, blk);
+  zreg_transform_store<T, LOGN>(v, data, W, buf, sign, blk);
 }
 
 template <typename T, int LOGN, class Loader>
@@ -322,16 +360,34 @@ fft_pass_z_reg(c2<T>* __restrict__ data, Loader load, const c2<T>* __restrict__ 
   fft_z_block_reg<T, LOGN>(data, load, tw, n1, sign, pr, xcd_block(blockIdx.x, gridDim.x));
 }
 
+// Batched unpruned register z pass, two work items (field, block) per
+// workgroup: the loads of the second are issued before the first is
+// transformed, so each wave keeps twice the bytes in flight.
 template <typename T, int LOGN, class BatchLoader>
 __global__ void __launch_bounds__((zreg_threads<T, LOGN>()))
 fft_pass_z_batch_reg(c2<T>* __restrict__ base, long long fstride, BatchLoader load,
-                     const c2<T>* __restrict__ tw, int n1, int sign, Prune pr,
-                     const double* __restrict__ kcut2, const int2* __restrict__ work) {
-  const int2 wk = work[xcd_block(blockIdx.x, gridDim.x)];
-  Prune p = pr;
-  p.kcut2 = kcut2[wk.x];
-  fft_z_block_reg<T, LOGN>(base + wk.x * fstride, load.field(wk.x), tw, n1, sign, p,
-                           wk.y);
+                     const c2<T>* __restrict__ tw, int n1, int sign,
+                     const int2* __restrict__ work, int nwork) {
+  constexpr int N = 1 << LOGN;
+  constexpr int A = ZRegShape<LOGN>::A, B = ZRegShape<LOGN>::B;
+  constexpr int L = zreg_block<T>() / N;
+  constexpr int NT = L * A;
+  extern __shared__ char smem_raw[];
+  c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
+  c2<T>* W = buf + L * B * (A + 1);
+  for (int m = threadIdx.x; m < N; m += NT) W[m] = tw[m];
+  const long long b0 = 2LL * xcd_block(blockIdx.x, gridDim.x);
+  const bool two = b0 + 1 < nwork;   // uniform over the workgroup
+  const int2 w0 = work[b0];
+  c2<T> v[B], vn[B];
+  zreg_load<T, LOGN>(v, load.field(w0.x), n1, w0.y);
+  int2 w1 = w0;
+  if (two) {
+    w1 = work[b0 + 1];
+    zreg_load<T, LOGN>(vn, load.field(w1.x), n1, w1.y);
+  }
+  zreg_transform_store<T, LOGN>(v, base + w0.x * fstride, W, buf, sign, w0.y);
+  if (two) zreg_transform_store<T, LOGN>(vn, base + w1.x * fstride, W, buf, sign, w1.y);
 }
 
 // First pass along z (contiguous lines), with a fused loader, for block
@@ -392,7 +448,7 @@ __device__ __forceinline__ void fft_z_block(c2<T>* __restrict__ data,
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
     const int idx = t + e * FFT_THREADS;
-    out[idx] = buf[(idx / N) * LS + idx % N];
+    st_mesh<T>(out + idx, buf[(idx / N) * LS + idx % N]);
   }
 }
 
@@ -479,7 +535,7 @@ __device__ __forceinline__ void fft_strided_block(
     for (int e = 0; e < EPT; e++) {
       const int n = n0 + e * NSTEP;
       const bool need = m3 ? pr.p1(kl, n) : bool(row_in[n]);
-      buf[l * LS + n] = need ? *p : mk<T>(T(0), T(0));
+      buf[l * LS + n] = need ? ld_mesh<T>(p) : mk<T>(T(0), T(0));
       p += step;
     }
     __syncthreads();
@@ -488,7 +544,7 @@ __device__ __forceinline__ void fft_strided_block(
     p = line + (long long)n0 * S;
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
-      *p = buf[l * LS + n0 + e * NSTEP];
+      st_mesh<T>(p, buf[l * LS + n0 + e * NSTEP]);
       p += step;
     }
     return;
@@ -522,7 +578,7 @@ __device__ __forceinline__ void fft_strided_block(
         need = pr.p1(int(id % inner), n);
       }
     }
-    buf[l * LS + n] = need ? data[base + (long long)n * S] : mk<T>(T(0), T(0));
+    buf[l * LS + n] = need ? ld_mesh<T>(data + base + (long long)n * S) : mk<T>(T(0), T(0));
   }
   __syncthreads();
   lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
@@ -534,7 +590,7 @@ __device__ __forceinline__ void fft_strided_block(
     const long long base = (id / inner) * outer_stride + id % inner;
     if (pr.on && prune_mode == 1 && !pr.p2(int(id / inner))) continue;
     if (pr.on && prune_mode == 3 && !pr.p2(int(id % inner))) continue;
-    data[base + (long long)n * S] = buf[l * LS + n];
+    st_mesh<T>(data + base + (long long)n * S, buf[l * LS + n]);
   }
 }
 
@@ -620,7 +676,7 @@ __device__ __forceinline__ void fft_x_load_block(c2<T>* __restrict__ data,
     const long long step = (long long)NSTEP * S;
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
-      *p = buf[l * LS + r0 + e * NSTEP];
+      st_mesh<T>(p, buf[l * LS + r0 + e * NSTEP]);
       p += step;
     }
   } else {
@@ -628,7 +684,7 @@ __device__ __forceinline__ void fft_x_load_block(c2<T>* __restrict__ data,
     for (int e = 0; e < EPT; e++) {
       const int idx = t + e * FFT_THREADS;
       const int n = idx / NLINES, l = idx % NLINES;
-      data[id0 + l + (long long)n * S] = buf[l * LS + n];
+      st_mesh<T>(data + id0 + l + (long long)n * S, buf[l * LS + n]);
     }
   }
 }
@@ -651,7 +707,7 @@ template <typename T>
 struct LoadSame {
   const c2<T>* src;
   __device__ c2<T> operator()(long long g, int, int, int) const {
-    return src[g];
+    return ld_mesh<T>(src + g);
   }
 };
 

@@ -257,7 +257,7 @@ struct LoadMaskedZ {
   int nz;
   __device__ c2<T> operator()(long long g, int, int, int k) const {
     const double kz = __dmul_rn(double(shifted(k, nz)), dkz);
-    return __dmul_rn(kz, kz) < kcut2 ? src[g] : mk<T>(T(0), T(0));
+    return __dmul_rn(kz, kz) < kcut2 ? ld_mesh<T>(src + g) : mk<T>(T(0), T(0));
   }
 };
 
@@ -881,13 +881,6 @@ static __global__ void k_zero_d(double* __restrict__ f, long long n) {
     f[c] = 0.;
 }
 
-template <typename T>
-__global__ void k_copy(c2<T>* __restrict__ dst, const c2<T>* __restrict__ src,
-                       long long n) {
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
-       c += (long long)gridDim.x * blockDim.x)
-    dst[c] = src[c];
-}
 
 }  // namespace trvg
 

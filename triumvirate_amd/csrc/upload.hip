@@ -178,6 +178,35 @@ void fill_device(double* f, long long n, double v, hipStream_t s) {
   TRVG_LAUNCH_CHECK();
 }
 
+// Lines of sight of particles [lo, hi) from their positions relative to the
+// observer o: (r - o) / |r - o|, a zero norm taken as 1, in the operation
+// order of ParticleCatalogue.compute_los (catalogue.py:437-458:
+// sqrt(x^2 + y^2 + z^2), then each component divided by it), every
+// operation rounded as on the host.
+__global__ void k_los_observer(const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ z, double* __restrict__ los,
+                               long long lo, long long hi, double ox, double oy, double oz) {
+  for (long long i = lo + blockIdx.x * (long long)blockDim.x + threadIdx.x; i < hi;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double a = __dsub_rn(x[i], ox), b = __dsub_rn(y[i], oy), c = __dsub_rn(z[i], oz);
+    double r = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)),
+                                    __dmul_rn(c, c)));
+    if (r == 0.) r = 1.;
+    los[3 * i] = __ddiv_rn(a, r);
+    los[3 * i + 1] = __ddiv_rn(b, r);
+    los[3 * i + 2] = __ddiv_rn(c, r);
+  }
+}
+
+void los_from_observer(DeviceCatalogue& d, const double* o, long long lo, long long hi,
+                       hipStream_t s) {
+  if (hi <= lo) return;
+  long long blocks = std::min<long long>((hi - lo + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_los_observer, dim3(unsigned(blocks)), dim3(256), 0, s, d.x.get(),
+                     d.y.get(), d.z.get(), d.los.get(), lo, hi, o[0], o[1], o[2]);
+  TRVG_LAUNCH_CHECK();
+}
+
 // Per-thread partial results of one chunk (padded to a cache line).
 struct alignas(64) Partial {
   double wst, wt;
@@ -192,7 +221,7 @@ struct alignas(64) Partial {
 
 void upload_catalogue(const trvg_catalogue* c, const double* los,
                       DeviceCatalogue& out, hipStream_t s, Context* sort_for,
-                      bool los32) {
+                      bool los32, const double* observer) {
   if (c == nullptr || c->x == nullptr || c->y == nullptr || c->z == nullptr)
     throw Error(TRVG_ERR_DATA, "Particle data are uninitialised.");
   if (c->ntotal <= 0)
@@ -202,7 +231,11 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
   out.tiled.reset();
   out.n = n;
   out.x.reset(n); out.y.reset(n); out.z.reset(n); out.w.reset(n);
-  if (los) out.los.reset(3 * n);
+  const bool derive = los == nullptr && observer != nullptr;
+  if (derive && !(std::isfinite(observer[0]) && std::isfinite(observer[1]) &&
+                  std::isfinite(observer[2])))
+    throw Error(TRVG_ERR_PARAM, "Observer position must be finite.");
+  if (los || derive) out.los.reset(3 * n);
   else out.los.reset(0);
   los32 = los32 && los != nullptr && sort_for != nullptr;
 
@@ -290,6 +323,7 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
         TRVG_HIP(hipMemcpyAsync(out.los.get() + 3 * base, bl, 3 * cb,
                                 hipMemcpyHostToDevice, cs));
       }
+      if (derive) los_from_observer(out, observer, base, base + m, cs);
       ring.release(slot, cs);
       if (sorter) {
         // The positions of this chunk are on the device once the slot's

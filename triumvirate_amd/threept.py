@@ -82,17 +82,28 @@ class CLibBackend:
             C.byref(out)))
         return out.value
 
+    def derives_los(self):
+        """Whether the library forms lines of sight on the device from an
+        observer position (trvg_catalogue_upload_observer)."""
+        return hasattr(self.lib, self.prefix + 'catalogue_upload_observer')
+
     @contextlib.contextmanager
-    def resident(self, cat, los=None):
+    def resident(self, cat, los=None, observer=None):
         """Device-resident copy of a catalogue for the duration of the block
         (None for a library without device catalogues), so that the mesh
-        normalisation and the estimator share one upload."""
+        normalisation and the estimator share one upload.  With `observer`
+        and no `los`, the lines of sight are formed on the device."""
         if not hasattr(self.lib, self.prefix + 'catalogue_upload'):
             yield None
             return
         h = C.c_void_p()
-        self._check(self._fn('catalogue_upload')(
-            C.byref(cat.struct), _abi.dptr(los), -1, C.byref(h)))
+        if los is None and observer is not None:
+            obs = np.ascontiguousarray(observer, dtype=np.float64)
+            self._check(self._fn('catalogue_upload_observer')(
+                C.byref(cat.struct), _abi.dptr(obs), -1, C.byref(h)))
+        else:
+            self._check(self._fn('catalogue_upload')(
+                C.byref(cat.struct), _abi.dptr(los), -1, C.byref(h)))
         try:
             yield h
         finally:
@@ -324,13 +335,19 @@ def _compute_3pt_stats_survey_like(backend, kind, catalogue_data,
         binning = Binning.from_parameter_set(paramset)
 
     # Lines of sight in the observer frame, before alignment
-    # (threept.py:525-528 precede :566-582).
-    if los_data is None:
-        los_data = catalogue_data.compute_los()
-    if los_rand is None:
-        los_rand = catalogue_rand.compute_los()
-    los_data = np.ascontiguousarray(los_data, dtype=np.float64)
-    los_rand = np.ascontiguousarray(los_rand, dtype=np.float64)
+    # (threept.py:525-528 precede :566-582).  A library that forms them on
+    # the device from an observer position is given that position instead
+    # of two n x 3 arrays (the alignment below only shifts coordinates).
+    derive = (los_data is None and los_rand is None and
+              getattr(backend, 'derives_los', lambda: False)())
+    if not derive:
+        if los_data is None:
+            los_data = catalogue_data.compute_los()
+        if los_rand is None:
+            los_rand = catalogue_rand.compute_los()
+        los_data = np.ascontiguousarray(los_data, dtype=np.float64)
+        los_rand = np.ascontiguousarray(los_rand, dtype=np.float64)
+    offset0 = catalogue_data._offset.copy()
 
     if any(paramset['boxsize'][a] is None for a in 'xyz'):
         for a in 'xyz':
@@ -352,12 +369,13 @@ def _compute_3pt_stats_survey_like(backend, kind, catalogue_data,
     data_v, rand_v = _view(catalogue_data), _view(catalogue_rand)
     params = _params_struct(paramset)
     alpha = catalogue_data.wstotal / catalogue_rand.wstotal
+    observer = -(catalogue_data._offset - offset0) if derive else None
 
     norm_part = backend.norm_particles(rand_v, alpha)
     # One upload of each catalogue serves the mesh normalisation and the
     # estimator.
-    with backend.resident(rand_v, los_rand) as hr, \
-            backend.resident(data_v, los_data) as hd:
+    with backend.resident(rand_v, los_rand, observer) as hr, \
+            backend.resident(data_v, los_data, observer) as hd:
         handles = (hd, hr) if hr is not None else None
         norm_mesh = backend.norm_mesh(rand_v, params, alpha, handle=hr)
         norm_factor = _select_norm(paramset, norm_part, norm_mesh)
