@@ -161,8 +161,8 @@ __host__ __device__ inline double sph_plm_dev(int l, int m, double x) {
   return pll;
 }
 
-__host__ __device__ inline double2 ylm_reduced_dev(int ell, int m, double x,
-                                                   double y, double z) {
+__host__ __device__ inline double2 ylm_reduced_generic(int ell, int m, double x,
+                                                       double y, double z) {
   if (ell == 0 && m == 0) return dmk(1., 0.);
   double r = sqrt(x * x + y * y + z * z);
   if (fabs(r) < 1.e-9) return dmk(0., 0.);
@@ -186,6 +186,100 @@ __host__ __device__ inline double2 ylm_reduced_dev(int ell, int m, double x,
   double norm = sqrt(4. * M_PI / (2. * ell + 1.));
   // parity * conj(e^{i m phi} P) * norm
   return dmk(parity * er * p * norm, -parity * ei * p * norm);
+}
+
+// Constants of the associated Legendre recurrence for l <= YLM_LMAX, as
+// sph_plm_dev forms them (sqrt evaluated at compile time by Newton's
+// iteration; within one unit in the last place of the run-time values).
+constexpr int YLM_LMAX = 8;
+
+constexpr double cx_sqrt(double x) {
+  double r = x > 1. ? x : 1.;
+  for (int i = 0; i < 200; i++) {
+    const double n = 0.5 * (r + x / r);
+    if (n == r) break;
+    r = n;
+  }
+  return r;
+}
+
+struct YlmCoef {
+  double pm0[YLM_LMAX + 1];                    // prod_i -sqrt((2i-1)/(2i)) sqrt((2m+1)/(4 pi))
+  double c1[YLM_LMAX + 1];                     // sqrt(2m + 3)
+  double a[YLM_LMAX + 1][YLM_LMAX + 1];        // [l][m] recurrence coefficients
+  double b[YLM_LMAX + 1][YLM_LMAX + 1];
+  double norm[YLM_LMAX + 1];                   // sqrt(4 pi / (2l + 1))
+};
+
+constexpr YlmCoef make_ylm_coef() {
+  YlmCoef c{};
+  const double pi = 3.14159265358979323846;
+  for (int m = 0; m <= YLM_LMAX; m++) {
+    double p = 1.;
+    for (int i = 1; i <= m; i++) p *= -cx_sqrt((2. * i - 1.) / (2. * i));
+    c.pm0[m] = p * cx_sqrt((2. * m + 1.) / (4. * pi));
+    c.c1[m] = cx_sqrt(2. * m + 3.);
+  }
+  for (int l = 0; l <= YLM_LMAX; l++) {
+    c.norm[l] = cx_sqrt(4. * pi / (2. * l + 1.));
+    for (int m = 0; m + 2 <= l; m++) {
+      const double d = double(l * l) - double(m * m);
+      c.a[l][m] = cx_sqrt((4. * l * l - 1.) / d);
+      c.b[l][m] = cx_sqrt(((2. * l + 1.) * (double((l - 1) * (l - 1)) - double(m * m))) /
+                          ((2. * l - 3.) * d));
+    }
+  }
+  return c;
+}
+
+constexpr YlmCoef YLM_COEF = make_ylm_coef();
+
+// The same function with one square root and one division: with
+// u = (x + i y) / r, sin^|m|(theta) e^{i |m| phi} = u^|m|, so the
+// P_l^|m| recurrence runs on P_l^|m| / sin^|m|(theta) in mu = z / r alone
+// and the azimuthal factor needs no r_xy (near the z axis, r_xy < 1e-9,
+// u = (sin(theta), 0) with the reference's azimuth 0).
+__host__ __device__ inline double2 ylm_reduced_dev(int ell, int m, double x,
+                                                   double y, double z) {
+  if (ell == 0 && m == 0) return dmk(1., 0.);
+  if (ell > YLM_LMAX) return ylm_reduced_generic(ell, m, x, y, z);
+  const double r = sqrt(x * x + y * y + z * z);
+  if (fabs(r) < 1.e-9) return dmk(0., 0.);
+  const double ir = 1. / r;
+  const double mu = z * ir;
+  const int am = m < 0 ? -m : m;
+  double er = 1., ei = 0.;
+  if (am > 0) {
+    const double rxy2 = x * x + y * y;
+    double ur = x * ir, ui = y * ir;
+    if (rxy2 < 1.e-18) {
+      // Azimuth 0 and sin(theta) from mu, as the reference forms them.
+      const double mr = z / r;
+      ur = sqrt((1. - mr) * (1. + mr));
+      ui = 0.;
+    }
+    er = ur;
+    ei = ui;
+    for (int q = 1; q < am; q++) {
+      const double t = er * ur - ei * ui;
+      ei = er * ui + ei * ur;
+      er = t;
+    }
+    if (m < 0) ei = -ei;
+  }
+  double p = YLM_COEF.pm0[am];
+  if (ell > am) {
+    double pmm = p, pmmp1 = mu * YLM_COEF.c1[am] * pmm;
+    for (int ll = am + 2; ll <= ell; ll++) {
+      const double pll = YLM_COEF.a[ll][am] * mu * pmmp1 - YLM_COEF.b[ll][am] * pmm;
+      pmm = pmmp1;
+      pmmp1 = pll;
+    }
+    p = pmmp1;
+  }
+  const double parity = (m < 0 && (am & 1)) ? -1. : 1.;
+  const double f = parity * p * YLM_COEF.norm[ell];
+  return dmk(f * er, -f * ei);
 }
 
 // y_lm (as ylm_reduced_dev) at (+-x, +-y, +-z) from its value y at

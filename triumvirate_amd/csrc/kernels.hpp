@@ -545,8 +545,10 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
     double c1 = alias_at(tab, i, j, k);
     double w = tab.interlace ? window_at(tab, i, j, k) * window_at(tab, i, j, k)
                              : c1;
-    double2 sn = dmk(sbar.x * c1 / w, sbar.y * c1 / w);
-    pk = dmk(pk.x / w, pk.y / w);
+    const double iw = 1. / w;
+    const double cw = c1 * iw;
+    double2 sn = dmk(sbar.x * cw, sbar.y * cw);
+    pk = dmk(pk.x * iw, pk.y * iw);
     double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
     pk = dmul(pk, y);
     sn = dmul(sn, y);
