@@ -542,6 +542,24 @@ class ParameterSet {
   }
 };
 
+// trv::set_ngrid_from_cutoff (src/parameters.cpp:1434-1463): mesh numbers
+// from the Nyquist cutoff, ceil(L k_cut / pi) in Fourier space and
+// ceil(2 L / r_cut) in configuration space, rounded up to even.
+inline void set_ngrid_from_cutoff(ParameterSet& params) {
+  for (int a = 0; a < 3; a++) {
+    int n;
+    if (params.space == "fourier")
+      n = static_cast<int>(std::ceil(params.boxsize[a] * params.cutoff_nyq / M_PI));
+    else if (params.space == "config")
+      n = static_cast<int>(std::ceil(2. * params.boxsize[a] / params.cutoff_nyq));
+    else
+      throw sys::InvalidParameterError("Space must be 'fourier' or 'config': `space` = '" +
+                                       params.space + "'.");
+    params.ngrid[a] = n + (n % 2);
+  }
+  params.validate();
+}
+
 // trv::Binning (include/dataobjs.hpp:58-150): linear, logarithmic and the
 // padded schemes (src/dataobjs.cpp:134-250 arithmetic), the grid-scale
 // default binning and custom edges.

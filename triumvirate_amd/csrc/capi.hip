@@ -847,6 +847,7 @@ int trvg_batch_plan_create(const trvg_params* multipoles, int nmultipoles,
       throw Error(TRVG_ERR_PARAM, "Batch size and capacity must be positive.");
     std::vector<Multipole> mps;
     trvg_params base = three_point_params(&multipoles[0], kbinning->num_bins);
+    require_pow2_mesh(base, "A batched plan");
     for (int q = 0; q < nmultipoles; q++) {
       trvg_params m = three_point_params(&multipoles[q], kbinning->num_bins);
       for (int a = 0; a < 3; a++)
@@ -902,6 +903,7 @@ int trvg_diff_assign(const trvg_params* params, const double* pos,
   return guarded([&] {
     require_device();
     trvg_params q = mesh_params(params);
+    require_pow2_mesh(q, "The differentiable path");
     if (n < 0 || (n > 0 && pos == nullptr) || mesh == nullptr)
       throw Error(TRVG_ERR_PARAM, "Invalid particle or mesh pointers.");
     Context ctx(q);
@@ -919,6 +921,7 @@ int trvg_diff_assign_backward(const trvg_params* params, const double* pos,
   return guarded([&] {
     require_device();
     trvg_params q = mesh_params(params);
+    require_pow2_mesh(q, "The differentiable path");
     if (n < 0 || (n > 0 && pos == nullptr) || grad_mesh == nullptr)
       throw Error(TRVG_ERR_PARAM, "Invalid particle or mesh pointers.");
     Context ctx(q);
@@ -938,6 +941,7 @@ int trvg_diff_bispec_diag(const trvg_params* params,
     require_device();
     check_binning(kbinning);
     trvg_params q = mesh_params(params);
+    require_pow2_mesh(q, "The differentiable path");
     if (mesh == nullptr || bk == nullptr)
       throw Error(TRVG_ERR_PARAM, "Null mesh or output.");
     Context ctx(q);
@@ -962,6 +966,7 @@ int trvg_diff_bispec_diag_noalloc(const trvg_params* params,
     require_device();
     check_binning(kbinning);
     trvg_params q = mesh_params(params);
+    require_pow2_mesh(q, "The differentiable path");
     if (mesh == nullptr || bk == nullptr)
       throw Error(TRVG_ERR_PARAM, "Null mesh or output.");
     if (q.device >= 0) TRVG_HIP(hipSetDevice(q.device));
@@ -981,6 +986,7 @@ int trvg_diff_bispec_diag_backward(const trvg_params* params,
     require_device();
     check_binning(kbinning);
     trvg_params q = mesh_params(params);
+    require_pow2_mesh(q, "The differentiable path");
     if (mesh == nullptr || grad_bk == nullptr || grad_mesh == nullptr)
       throw Error(TRVG_ERR_PARAM, "Null mesh, cotangent or gradient.");
     Context ctx(q);

@@ -360,6 +360,9 @@ int result_dim(int shape, int nb, int idx_bin);
 void pair_bins(int shape, int nb, int idx_bin, std::vector<int>& ra,
                std::vector<int>& cb);
 void validate_params(const trvg_params& p, int num_bins);
+// Batched plans, the differentiable path and the allocation-free entry
+// point keep power-of-two meshes only.
+void require_pow2_mesh(const trvg_params& p, const char* what);
 
 // FFT of a mesh: first pass (z) loads through `Loader`, then y and x passes.
 // read_bytes: compulsory bytes the loader reads (profiler accounting; a

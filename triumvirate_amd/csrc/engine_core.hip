@@ -456,19 +456,34 @@ void pair_bins(int shape, int nb, int idx_bin, std::vector<int>& ra,
   }
 }
 
-static bool pow2_in_range(int n) {
-  return n >= 8 && n <= 1024 && (n & (n - 1)) == 0;
+// Mesh axes: even lengths in [8, 2048].  Powers of two up to 1024 take the
+// compile-time FFT passes, other lengths the mixed-radix ones.  Odd lengths
+// are refused: the octant histogram of the shot-noise terms and the
+// Hermitian unpacking of packed transforms pair k with -k through a
+// Nyquist plane (the reference's cutoff-derived meshes are always even,
+// parameters.cpp:1434-1463).
+static bool mesh_axis_ok(int n) { return n >= 8 && n <= 2048 && n % 2 == 0; }
+
+void require_pow2_mesh(const trvg_params& p, const char* what) {
+  for (int a = 0; a < 3; a++) {
+    const int n = p.ngrid[a];
+    if (n < 8 || n > 1024 || (n & (n - 1)) != 0)
+      throw Error(TRVG_ERR_PARAM, std::string(what) +
+                                      " needs mesh numbers that are powers of two in "
+                                      "[8, 1024]: ngrid[" + std::to_string(a) + "] = " +
+                                      std::to_string(n) + ".");
+  }
 }
 
 void validate_params(const trvg_params& p, int num_bins) {
   for (int a = 0; a < 3; a++) {
     if (!(p.boxsize[a] > 0.))
       throw Error(TRVG_ERR_PARAM, "Box size must be positive.");
-    if (!pow2_in_range(p.ngrid[a]))
+    if (!mesh_axis_ok(p.ngrid[a]))
       throw Error(TRVG_ERR_PARAM,
-                  "Mesh number per dimension must be a power of two in "
-                  "[8, 1024]: ngrid[" + std::to_string(a) + "] = " +
-                      std::to_string(p.ngrid[a]) + ".");
+                  "Mesh number per dimension must be even and in [8, 2048]: "
+                  "ngrid[" + std::to_string(a) + "] = " + std::to_string(p.ngrid[a]) +
+                      ".");
   }
   long long nmesh = (long long)p.ngrid[0] * p.ngrid[1] * p.ngrid[2];
   if (nmesh < 2048)

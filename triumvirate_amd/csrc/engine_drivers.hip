@@ -37,7 +37,7 @@ template <typename T>
 static const c2<T>* ctx_twiddles(const Context& ctx, int n, int sign) {
   int lg = 0;
   while ((1 << lg) < n) lg++;
-  const void* b = lg < 11 ? ctx.tw[sign > 0 ? 1 : 0][lg] : nullptr;
+  const void* b = (lg < 11 && (1 << lg) == n) ? ctx.tw[sign > 0 ? 1 : 0][lg] : nullptr;
   return b ? static_cast<const c2<T>*>(b) : twiddles<T>(n, sign);
 }
 
@@ -152,6 +152,91 @@ static int ilog2(int n) {
     case 10: { constexpr int LG = 10; CALL; } break;             \
     default: throw Error(TRVG_ERR_PARAM, "Unsupported FFT size."); \
   }
+
+// Axes of power-of-two length up to 1024 take the compile-time passes of
+// fft.hpp; other (even) lengths the mixed-radix passes of fft_gen.hpp.  Both
+// give a workgroup lines_per_block(n) lines (2048 / n for the former).
+static bool pow2_axis(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+// The compile-time passes take whole blocks of 2048 elements: a power-of-two
+// axis of a mesh whose cell count is not a multiple of 2048 (other axes not
+// powers of two) goes through the mixed-radix passes too.
+static bool fast_axis(const Geom& g, int n) { return pow2_axis(n) && g.nmesh % 2048 == 0; }
+static int lines_per_block(int n) { return gen_lines_per_block(n); }
+static long long blocks_of(long long nlines, int nl) { return (nlines + nl - 1) / nl; }
+
+// Dynamic LDS above the default 64 KiB needs the kernel's attribute raised.
+template <class K>
+static void gen_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    TRVG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+}
+
+template <typename T, class Loader>
+static void launch_z_gen(Context& ctx, c2<T>* data, const Loader& load, int sign,
+                         double bytes, const Prune& pr) {
+  const int N = ctx.g.n[2];
+  const GenPlan P = make_gen_plan(N);
+  const long long nlines = (long long)ctx.g.n[0] * ctx.g.n[1];
+  const size_t smem = gen_smem<T>(N, false);
+  auto kern = fft_pass_z_gen<T, Loader>;
+  gen_lds(kern, smem);
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, bytes);
+  hipLaunchKernelGGL(kern, dim3(unsigned(blocks_of(nlines, P.nl))), dim3(FFT_THREADS), smem,
+                     ctx.stream, data, load, ctx_twiddles<T>(ctx, N, sign), ctx.g.n[1],
+                     nlines, sign, pr, P);
+  TRVG_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void launch_strided_gen(Context& ctx, c2<T>* data, int N, long long S,
+                               long long outer_stride, long long inner, int sign,
+                               const Prune& pr, int prune_mode, double bytes) {
+  const GenPlan P = make_gen_plan(N);
+  const long long nlines = ctx.g.nmesh / N;
+  const size_t smem = gen_smem<T>(N, true);
+  auto kern = fft_pass_strided_gen<T>;
+  gen_lds(kern, smem);
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
+  hipLaunchKernelGGL(kern, dim3(unsigned(blocks_of(nlines, P.nl))), dim3(FFT_THREADS), smem,
+                     ctx.stream, data, ctx_twiddles<T>(ctx, N, sign), S, outer_stride, inner,
+                     nlines, pr, prune_mode, P);
+  TRVG_LAUNCH_CHECK();
+}
+
+template <typename T, class BatchLoader>
+static void launch_z_batch_gen(Context& ctx, c2<T>* base, const BatchLoader& load, int sign,
+                               const Prune& pr, const double* kcut2, const int2* work,
+                               int nwork, double bytes) {
+  const int N = ctx.g.n[2];
+  const GenPlan P = make_gen_plan(N);
+  const long long nlines = (long long)ctx.g.n[0] * ctx.g.n[1];
+  const size_t smem = gen_smem<T>(N, false);
+  auto kern = fft_pass_z_batch_gen<T, BatchLoader>;
+  gen_lds(kern, smem);
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_Z64 : KC_FFT_Z, bytes);
+  hipLaunchKernelGGL(kern, dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream, base,
+                     ctx.g.nmesh, load, ctx_twiddles<T>(ctx, N, sign), ctx.g.n[1], nlines,
+                     sign, pr, kcut2, work, P);
+  TRVG_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void launch_strided_batch_gen(Context& ctx, c2<T>* base, int N, long long S,
+                                     long long outer_stride, long long inner, int sign,
+                                     const Prune& pr, const double* kcut2, const int2* work,
+                                     int nwork, int prune_mode, double bytes) {
+  const GenPlan P = make_gen_plan(N);
+  const long long nlines = ctx.g.nmesh / N;
+  const size_t smem = gen_smem<T>(N, true);
+  auto kern = fft_pass_strided_batch_gen<T>;
+  gen_lds(kern, smem);
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
+  hipLaunchKernelGGL(kern, dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream, base,
+                     ctx.g.nmesh, ctx_twiddles<T>(ctx, N, sign), S, outer_stride, inner,
+                     nlines, pr, kcut2, work, prune_mode, P);
+  TRVG_LAUNCH_CHECK();
+}
 
 Prune make_prune(const Geom& g, double k_hi) {
   Prune pr{};
@@ -294,17 +379,35 @@ void fft3d_batch(Context& ctx, c2<T>* base, const BatchLoader& load, int sign,
   pr.n0 = g.n[0];
   pr.n1 = g.n[1];
   const double blk = double(FFT_THREADS * FftEpt<T>::v) * sizeof(c2<T>);
-  if (nz > 0)
-    TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
-        ctx, base, load, sign, pr, kcut2, wz, nz, 2. * blk * nz)));
-  if (ny > 0)
-    TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
-        ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], sign, pr, kcut2, wy,
-        ny, 1, 2. * blk * ny)));
-  if (nx > 0)
-    TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_strided_batch<T, LG>(
-        ctx, base, (long long)g.n[1] * g.n[2], 0, (long long)g.n[1] * g.n[2], sign,
-        pr, kcut2, wx, nx, 2, 2. * blk * nx)));
+  if (nz > 0) {
+    if (fast_axis(g, g.n[2])) {
+      TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
+          ctx, base, load, sign, pr, kcut2, wz, nz, 2. * blk * nz)));
+    } else {
+      launch_z_batch_gen<T>(ctx, base, load, sign, pr, kcut2, wz, nz, 2. * blk * nz);
+    }
+  }
+  if (ny > 0) {
+    if (fast_axis(g, g.n[1])) {
+      TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
+          ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], sign, pr, kcut2, wy,
+          ny, 1, 2. * blk * ny)));
+    } else {
+      launch_strided_batch_gen<T>(ctx, base, g.n[1], g.n[2], (long long)g.n[1] * g.n[2],
+                                  g.n[2], sign, pr, kcut2, wy, ny, 1, 2. * blk * ny);
+    }
+  }
+  if (nx > 0) {
+    if (fast_axis(g, g.n[0])) {
+      TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_strided_batch<T, LG>(
+          ctx, base, (long long)g.n[1] * g.n[2], 0, (long long)g.n[1] * g.n[2], sign,
+          pr, kcut2, wx, nx, 2, 2. * blk * nx)));
+    } else {
+      launch_strided_batch_gen<T>(ctx, base, g.n[0], (long long)g.n[1] * g.n[2], 0,
+                                  (long long)g.n[1] * g.n[2], sign, pr, kcut2, wx, nx, 2,
+                                  2. * blk * nx);
+    }
+  }
 }
 
 // Work lists of a shell slab transform over `ngroups` groups of all
@@ -318,9 +421,8 @@ void shell_slab_work(const Geom& g, const trvg_binning& kb, int nbins,
                      int ngroups, bool packed, std::vector<ShellBand>& bands,
                      std::vector<double>& kcut2, std::vector<int2>& wz,
                      std::vector<int2>& wy, std::vector<int2>& wx) {
-  const int E = FFT_THREADS * FftEpt<T>::v;
-  const int nl_z = E / g.n[2], nl_y = E / g.n[1];
-  const long long nblk = g.nmesh / E;
+  const int nl_z = lines_per_block(g.n[2]), nl_y = lines_per_block(g.n[1]);
+  const long long nblk = blocks_of((long long)g.n[1] * g.n[2], lines_per_block(g.n[0]));
   const int per = packed ? 2 : 1;
   const int nt = (nbins + per - 1) / per;
   bands.assign((size_t)nt * per, ShellBand{HUGE_VAL, HUGE_VAL});
@@ -454,6 +556,22 @@ static void launch_x_load_batch(Context& ctx, c2<T>* base, const BatchLoader& lo
   TRVG_LAUNCH_CHECK();
 }
 
+template <typename T, class BatchLoader>
+static void launch_x_load_batch_gen(Context& ctx, c2<T>* base, const BatchLoader& load,
+                                    const Prune& pr, const double* kcut2, const int2* work,
+                                    int nwork, double bytes) {
+  const int N = ctx.g.n[0];
+  const GenPlan P = make_gen_plan(N);
+  const size_t smem = gen_smem<T>(N, false);
+  auto kern = fft_pass_x_load_batch_gen<T, BatchLoader>;
+  gen_lds(kern, smem);
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
+  hipLaunchKernelGGL(kern, dim3(unsigned(nwork)), dim3(FFT_THREADS), smem, ctx.stream, base,
+                     ctx.g.nmesh, load, ctx_twiddles<T>(ctx, N, +1), ctx.g.n[1], ctx.g.n[2],
+                     pr, kcut2, work, P);
+  TRVG_LAUNCH_CHECK();
+}
+
 template <typename T>
 static void xfirst_yz(Context& ctx, c2<T>* base, const double* kcut2, const int2* wy, int ny,
                       const int2* wz, int nz, double by, double bz);
@@ -466,9 +584,14 @@ static void xfirst_passes(Context& ctx, c2<T>* base, const BatchLoader& load,
                           const double* kcut2, const int2* wx, int nx, const int2* wy,
                           int ny, const int2* wz, int nz, double bx, double by, double bz) {
   const Geom& g = ctx.g;
-  if (nx > 0)
-    TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_load_batch<T, LG>(
-        ctx, base, load, prune_zy(g), kcut2, wx, nx, bx)));
+  if (nx > 0) {
+    if (fast_axis(g, g.n[0])) {
+      TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_load_batch<T, LG>(
+          ctx, base, load, prune_zy(g), kcut2, wx, nx, bx)));
+    } else {
+      launch_x_load_batch_gen<T>(ctx, base, load, prune_zy(g), kcut2, wx, nx, bx);
+    }
+  }
   xfirst_yz<T>(ctx, base, kcut2, wy, ny, wz, nz, by, bz);
 }
 
@@ -478,17 +601,28 @@ static void xfirst_yz(Context& ctx, c2<T>* base, const double* kcut2, const int2
                       const int2* wz, int nz, double by, double bz) {
   const Geom& g = ctx.g;
   const Prune pr = prune_zy(g);
-  if (ny > 0)
-    TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
-        ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], +1, pr, kcut2, wy, ny, 3,
-        by)));
+  if (ny > 0) {
+    if (fast_axis(g, g.n[1])) {
+      TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided_batch<T, LG>(
+          ctx, base, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], +1, pr, kcut2, wy, ny, 3,
+          by)));
+    } else {
+      launch_strided_batch_gen<T>(ctx, base, g.n[1], g.n[2], (long long)g.n[1] * g.n[2],
+                                  g.n[2], +1, pr, kcut2, wy, ny, 3, by);
+    }
+  }
   // z pass over every block, reading only the k_z rows the y pass wrote.
   Prune off{};
   off.on = 0;
   LoadMaskedZBatch<T> lz{base, g.nmesh, kcut2, g.dk[2], g.n[2]};
-  if (nz > 0)
-    TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
-        ctx, base, lz, +1, off, kcut2, wz, nz, bz)));
+  if (nz > 0) {
+    if (fast_axis(g, g.n[2])) {
+      TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z_batch<T, LG>(
+          ctx, base, lz, +1, off, kcut2, wz, nz, bz)));
+    } else {
+      launch_z_batch_gen<T>(ctx, base, lz, +1, off, kcut2, wz, nz, bz);
+    }
+  }
 }
 
 // Shell fields of (ell, m) for `bins`: the backward transform of
@@ -530,13 +664,12 @@ static std::shared_ptr<const ShellPlan> shell_plan(Context& ctx, const trvg_binn
   }
   const int nt = packed ? (nf + 1) / 2 : nf;
   const int per = packed ? 2 : 1;
-  constexpr int EPT = FftEpt<T>::v;
-  const int E = FFT_THREADS * EPT;
-  const int nl_x = E / g.n[0], nl_y = E / g.n[1];
+  const int nl_x = lines_per_block(g.n[0]), nl_y = lines_per_block(g.n[1]);
   const ShellBand none{HUGE_VAL, HUGE_VAL};
   std::vector<ShellBand> bands((size_t)nt * per, none);
   std::vector<double> kc(nt, 0.);
-  const long long nblk = g.nmesh / E;
+  // z blocks: every (i, j) line of the last pass.
+  const long long nblk = blocks_of((long long)g.n[0] * g.n[1], lines_per_block(g.n[2]));
   std::vector<int2> wx, wy, wz;
   const double cb = sizeof(c2<T>);
   auto pl = std::make_shared<ShellPlan>();
@@ -624,6 +757,32 @@ static void launch_x_sjl(Context& ctx, c2<T>* base, int nt, const double2* src,
   TRVG_LAUNCH_CHECK();
 }
 
+template <typename T>
+static void launch_x_sjl_gen(Context& ctx, c2<T>* base, int nt, const double2* src,
+                             const Spline& sp, const double* r, int ell, int m, bool packed,
+                             double bytes) {
+  const Geom& g = ctx.g;
+  const int N = g.n[0];
+  const GenPlan P = make_gen_plan(N);
+  const size_t smem = gen_smem<T>(N, false);
+  const unsigned grid = unsigned(blocks_of((long long)g.n[1] * g.n[2], P.nl));
+  ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
+  if (packed) {
+    auto kern = fft_x_sjl_multi_gen<T, true>;
+    gen_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(FFT_THREADS), smem, ctx.stream, base, g.nmesh,
+                       nt, src, ctx.tab, g, sp.dev(), r, ell, m,
+                       ctx_twiddles<T>(ctx, N, +1), P);
+  } else {
+    auto kern = fft_x_sjl_multi_gen<T, false>;
+    gen_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(FFT_THREADS), smem, ctx.stream, base, g.nmesh,
+                       nt, src, ctx.tab, g, sp.dev(), r, ell, m,
+                       ctx_twiddles<T>(ctx, N, +1), P);
+  }
+  TRVG_LAUNCH_CHECK();
+}
+
 // j_l-weighted shell fields of (ell, m) for `bins` (3PCF,
 // inv_fourier_transform_sjl_ylm_wgtd_field, field.cpp:1908-2010), r the
 // bins' effective radii: x-first batched transforms over every block (no
@@ -637,8 +796,6 @@ void sjl_slab_ifft(Context& ctx, c2<T>* base, const double2* src, const Spline& 
   if (nf == 0) return;
   const int nt = packed ? (nf + 1) / 2 : nf;
   const int per = packed ? 2 : 1;
-  constexpr int EPT = FftEpt<T>::v;
-  const int E = FFT_THREADS * EPT;
   std::vector<double> rr((size_t)nt * per, -1.);
   for (int q = 0; q < nf; q++) rr[q] = r[bins[q]];
   DevArray<double> dr(rr.size());
@@ -657,8 +814,9 @@ void sjl_slab_ifft(Context& ctx, c2<T>* base, const double2* src, const Spline& 
     if (it != cache.end()) plan = it->second;
   }
   if (!plan) {
-    const PruneListsXF& pl = prune_lists_xfirst(g, HUGE_VAL, E / g.n[0], E / g.n[1]);
-    const long long nblk = g.nmesh / E;
+    const PruneListsXF& pl = prune_lists_xfirst(g, HUGE_VAL, lines_per_block(g.n[0]),
+                                                lines_per_block(g.n[1]));
+    const long long nblk = blocks_of((long long)g.n[0] * g.n[1], lines_per_block(g.n[2]));
     std::vector<double> kc(nt, HUGE_VAL);
     std::vector<int2> w;   // y blocks, then z blocks
     for (int f = 0; f < nt; f++)
@@ -687,8 +845,12 @@ void sjl_slab_ifft(Context& ctx, c2<T>* base, const double2* src, const Spline& 
   // later ones read) every field.
   const double fb = double(g.nmesh) * sizeof(c2<T>) * nt;
   const double bx = double(g.nmesh) * sizeof(double2) + fb, by = 2. * fb, bz = 2. * fb;
-  TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_sjl<T, LG>(ctx, base, nt, src, sp, dr.get(), ell,
-                                                        m, packed, bx)));
+  if (fast_axis(g, g.n[0])) {
+    TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_x_sjl<T, LG>(ctx, base, nt, src, sp, dr.get(),
+                                                          ell, m, packed, bx)));
+  } else {
+    launch_x_sjl_gen<T>(ctx, base, nt, src, sp, dr.get(), ell, m, packed, bx);
+  }
   // The radii stay owned until the stream is synchronised: returned to the
   // shared pool now, another call could take and overwrite them while the
   // x pass still reads them.
@@ -721,14 +883,29 @@ void fft3d(Context& ctx, c2<T>* data, const Loader& load, int sign,
   if (read_bytes < 0.) read_bytes = double(g.nmesh) * cb;
   const double zw = d_lines * g.n[2] * cb;
   const double slab_vals = slabs * g.n[1] * g.n[2];
-  TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z<T, LG, Loader>(
-      ctx, data, load, sign, read_bytes, zw, pr)));
-  TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided<T, LG>(
-      ctx, data, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], sign, pr, 1,
-      zw + slab_vals * cb)));
-  TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_strided<T, LG>(
-      ctx, data, (long long)g.n[1] * g.n[2], 0, (long long)g.n[1] * g.n[2], sign,
-      pr, 2, slab_vals * cb + double(g.nmesh) * cb)));
+  if (fast_axis(g, g.n[2])) {
+    TRVG_LOGN_SWITCH(ilog2(g.n[2]), (launch_z<T, LG, Loader>(
+        ctx, data, load, sign, read_bytes, zw, pr)));
+  } else {
+    launch_z_gen<T>(ctx, data, load, sign, read_bytes + zw, pr);
+  }
+  if (fast_axis(g, g.n[1])) {
+    TRVG_LOGN_SWITCH(ilog2(g.n[1]), (launch_strided<T, LG>(
+        ctx, data, g.n[2], (long long)g.n[1] * g.n[2], g.n[2], sign, pr, 1,
+        zw + slab_vals * cb)));
+  } else {
+    launch_strided_gen<T>(ctx, data, g.n[1], g.n[2], (long long)g.n[1] * g.n[2], g.n[2],
+                          sign, pr, 1, zw + slab_vals * cb);
+  }
+  if (fast_axis(g, g.n[0])) {
+    TRVG_LOGN_SWITCH(ilog2(g.n[0]), (launch_strided<T, LG>(
+        ctx, data, (long long)g.n[1] * g.n[2], 0, (long long)g.n[1] * g.n[2], sign,
+        pr, 2, slab_vals * cb + double(g.nmesh) * cb)));
+  } else {
+    launch_strided_gen<T>(ctx, data, g.n[0], (long long)g.n[1] * g.n[2], 0,
+                          (long long)g.n[1] * g.n[2], sign, pr, 2,
+                          slab_vals * cb + double(g.nmesh) * cb);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1153,8 +1330,7 @@ static void pair_reduce(Context& ctx, const c2<T>* FA, int na, bool a_packed,
   // side, PT_MAXT tiles and PT_MAXROWS staged rows (4 per block a side).
   const int CB = std::min(cbn, 8);
   const int RB = std::max(1, std::min(std::min((PT_MAXROWS - 1 - 4 * CB) / 4, PT_MAXT / CB), 8));
-  if (nm % PairLds<T>::LDC != 0)
-    throw Error(TRVG_ERR_INTERNAL, "Pair reduction needs whole chunks of the mesh.");
+  // Whole chunks; the kernel reads the cells past the last one directly.
   const long long nchunks = nm / PairLds<T>::LDC;
   for (int c0b = 0; c0b < cbn; c0b += CB) {
     const int c1b = std::min(cbn, c0b + CB);

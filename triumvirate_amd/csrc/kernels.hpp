@@ -7,6 +7,7 @@
 
 #include "common.hpp"
 #include "fft.hpp"
+#include "fft_gen.hpp"
 
 namespace trvg {
 
@@ -361,6 +362,76 @@ fft_x_sjl_multi(c2<T>* __restrict__ base, long long fstride, int nt,
       const int idx = t + e * FFT_THREADS;
       const int n = idx / NLINES, l = idx % NLINES;
       out[id0 + l + (long long)n * S] = buf[l * LS + n];
+    }
+    __syncthreads();
+  }
+}
+
+// fft_x_sjl_multi for an x axis that is not a power of two (mixed-radix
+// transform of fft_gen.hpp; nl <= NL lines per workgroup, at most eight
+// elements per thread).
+template <typename T, bool PACKED>
+__global__ void __launch_bounds__(FFT_THREADS)
+fft_x_sjl_multi_gen(c2<T>* __restrict__ base, long long fstride, int nt,
+                    const double2* __restrict__ src, AxisTables tab, Geom geo,
+                    SplineDev sj, const double* __restrict__ r, int ell, int m,
+                    const c2<T>* __restrict__ tw, GenPlan P) {
+  constexpr int MAXE = GEN_ELEMS / FFT_THREADS;
+  const int N = P.n, LS = N + 1;
+  GenSmem<T> sm(P);
+  const int n1 = geo.n[1], n2 = geo.n[2];
+  const long long S = (long long)n1 * n2;
+  const long long id0 = (long long)blockIdx.x * P.nl;
+  if (id0 >= S) return;
+  const int nl = int(min((long long)P.nl, S - id0));
+  const int total = nl * N;
+  for (int q = threadIdx.x; q < N; q += FFT_THREADS) sm.W[q] = tw[q];
+  c2<T> fac[MAXE];
+  double km[MAXE];
+  const double inv_vol = 1. / geo.vol;
+#pragma unroll
+  for (int e = 0; e < MAXE; e++) {
+    const int idx = threadIdx.x + e * FFT_THREADS;
+    fac[e] = mk<T>(T(0), T(0));
+    km[e] = 0.;
+    if (idx >= total) continue;
+    const int n = idx / nl, l = idx % nl;
+    const long long id = id0 + l;
+    const int j = int(id / n2), k = int(id % n2);
+    const double kx = grid_coord(n, geo.n[0], geo.dk[0]);
+    const double ky = grid_coord(j, n1, geo.dk[1]);
+    const double kz = grid_coord(k, n2, geo.dk[2]);
+    km[e] = mag3_exact(kx, ky, kz);
+    const double2 v = src[id + (long long)n * S];
+    const double iw = inv_vol / window_at(tab, n, j, k);
+    const double2 fe = dmul(ylm_reduced_dev(ell, m, kx, ky, kz), dmk(v.x * iw, v.y * iw));
+    fac[e] = mk<T>(T(fe.x), T(fe.y));
+  }
+  for (int f = 0; f < nt; f++) {
+    const double r0 = PACKED ? r[2 * f] : r[f];
+    const double r1 = PACKED ? r[2 * f + 1] : -1.;
+#pragma unroll
+    for (int e = 0; e < MAXE; e++) {
+      const int idx = threadIdx.x + e * FFT_THREADS;
+      if (idx >= total) continue;
+      T j0, j1 = T(0);
+      if (sizeof(T) == 4) {
+        j0 = T(sj.eval32(km[e] * r0));
+        if (PACKED && r1 >= 0.) j1 = T(sj.eval32(km[e] * r1));
+      } else {
+        j0 = T(sj.eval(km[e] * r0));
+        if (PACKED && r1 >= 0.) j1 = T(sj.eval(km[e] * r1));
+      }
+      const c2<T> val = PACKED ? mk<T>(j0 * fac[e].x - j1 * fac[e].y, j0 * fac[e].y + j1 * fac[e].x)
+                               : mk<T>(j0 * fac[e].x, j0 * fac[e].y);
+      sm.a[(idx % nl) * LS + idx / nl] = val;
+    }
+    __syncthreads();
+    const c2<T>* R = lds_fft_gen<T>(sm.a, sm.b, sm.W, P, nl);
+    c2<T>* out = base + (long long)f * fstride;
+    for (int idx = threadIdx.x; idx < total; idx += FFT_THREADS) {
+      const int n = idx / nl, l = idx % nl;
+      st_mesh<T>(out + id0 + l + (long long)n * S, R[l * LS + n]);
     }
     __syncthreads();
   }
@@ -1475,7 +1546,7 @@ namespace trvg {
 // costs two real FMAs) and accumulates 16 pair sums in T within a chunk,
 // promoted to FP64 once per chunk.  MODE 0: complex sides; 1: B side real, two per mesh; 2: both sides
 // real, two per mesh, G real; 3: as 2 with both sides the same fields (read
-// once).  Requires nmesh % LDC == 0 (meshes are powers of two >= 8^3).
+// once).  Cells past the last whole chunk are read from global memory.
 // Output: per-block partials [gridDim.x][ntiles * 16].
 // ---------------------------------------------------------------------------
 
@@ -1660,6 +1731,72 @@ __device__ __forceinline__ void k_pair_lds_body(
   }
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
+  // Cells past the last whole chunk (a mesh whose cell count is not a
+  // multiple of LDC): read from global memory by the block next in turn.
+  const long long tail0 = nchunks * L::LDC;
+  if (tail0 < nmesh && blockIdx.x == nchunks % gridDim.x && active) {
+    auto row = [&](int r) -> const c2<T>* {
+      return r < MA ? FA + (long long)r * nmesh
+                    : (r < MA + MB ? FB + (long long)(r - MA) * nmesh : G);
+    };
+    T pr[16], pi[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) pr[q] = pi[q] = T(0);
+    for (long long c = tail0 + group; c < nmesh; c += ng) {
+      const c2<T> g = row(grow)[c];
+      if (MODE >= 2) {
+        T x[4], h[4];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+          const c2<T> xa = row(xr[q])[c], yb = row(yr[q])[c];
+          x[2 * q] = xa.x;
+          x[2 * q + 1] = xa.y;
+          h[2 * q] = yb.x * g.x;
+          h[2 * q + 1] = yb.y * g.x;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) pr[i * 4 + j] += x[i] * h[j];
+      } else if (MODE == 1) {
+        c2<T> ag[4];
+        T b[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) ag[i] = cmul<T>(row(xr[i])[c], g);
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+          const c2<T> yb = row(yr[q])[c];
+          b[2 * q] = yb.x;
+          b[2 * q + 1] = yb.y;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            pr[i * 4 + j] += ag[i].x * b[j];
+            pi[i * 4 + j] += ag[i].y * b[j];
+          }
+      } else {
+        c2<T> h[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) h[q] = cmul<T>(row(yr[q])[c], g);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const c2<T> a = row(xr[i])[c];
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            pr[i * 4 + j] += a.x * h[j].x - a.y * h[j].y;
+            pi[i * 4 + j] += a.x * h[j].y + a.y * h[j].x;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      ar[q] += double(pr[q]);
+      if (CPLX) ai[q] += double(pi[q]);
+    }
+  }
   double2* red = reinterpret_cast<double2*>(smp_raw);   // [256]
   for (int q = 0; q < 16; q++) {
     red[threadIdx.x] = active ? dmk(ar[q], CPLX ? ai[q] : 0.) : dmk(0., 0.);

@@ -20,6 +20,7 @@ Defaults follow the reference template
 
 import copy
 import logging
+import math
 from pathlib import Path
 from pprint import pformat, pprint
 
@@ -287,3 +288,27 @@ class ParameterSet:
             self._logger.info(
                 "Interlacing is unsupported for three-point measurements. "
                 "`interlace` is set to 'false'.")
+
+
+def set_ngrid_from_cutoff(paramset):
+    """Mesh numbers from the Nyquist cutoff (parameters.cpp:1434-1463):
+    ceil(L k_cut / pi) in Fourier space, ceil(2 L / r_cut) in configuration
+    space, rounded up to even."""
+    cut = paramset['cutoff_nyq']
+    if cut is None:
+        raise InvalidParameterError(
+            "`ngrid` is unset and `cutoff_nyq` is not given.")
+    ngrid = {}
+    for a in 'xyz':
+        L = float(paramset['boxsize'][a])
+        if paramset['space'] == 'fourier':
+            n = int(math.ceil(L * cut / math.pi))
+        elif paramset['space'] == 'config':
+            n = int(math.ceil(2. * L / cut))
+        else:
+            raise InvalidParameterError(
+                "Space must be 'fourier' or 'config': `space` = "
+                f"'{paramset['space']}'.")
+        ngrid[a] = n + n % 2
+    paramset['ngrid'] = ngrid
+    paramset._validate()

@@ -32,7 +32,7 @@ from . import _abi
 from . import io as _io
 from .dataobjs import Binning
 from .parameters import (InvalidParameterError, ParameterSet,
-                         fetch_paramset_template)
+                         fetch_paramset_template, set_ngrid_from_cutoff)
 
 
 class InvalidDataError(ValueError):
@@ -354,6 +354,9 @@ def _compute_3pt_stats_survey_like(backend, kind, catalogue_data,
             lo, hi = catalogue_rand.bounds[a]
             paramset['boxsize'][a] = (hi - lo) * paramset['expand']
         paramset._validate()
+    # threept.py:554-556: meshes from the Nyquist cutoff when unset.
+    if any(paramset['ngrid'][a] is None for a in 'xyz'):
+        set_ngrid_from_cutoff(paramset)
 
     boxsize = [paramset['boxsize'][a] for a in 'xyz']
     if paramset['alignment'] == 'centre':
@@ -413,6 +416,9 @@ def _compute_3pt_stats_sim_like(backend, kind, catalogue_data, paramset=None,
             lo, hi = catalogue_data.bounds[a]
             paramset['boxsize'][a] = (hi - lo) * paramset['expand']
         paramset._validate()
+    # threept.py:554-556: meshes from the Nyquist cutoff when unset.
+    if any(paramset['ngrid'][a] is None for a in 'xyz'):
+        set_ngrid_from_cutoff(paramset)
 
     boxsize = [paramset['boxsize'][a] for a in 'xyz']
     catalogue_data.periodise(boxsize)
@@ -533,6 +539,8 @@ def _compute_3pcf_window(backend, catalogue_rand, los_rand=None, paramset=None,
     if los_rand is None:
         los_rand = catalogue_rand.compute_los()
     los_rand = np.ascontiguousarray(los_rand, dtype=np.float64)
+    if any(paramset['ngrid'][a] is None for a in 'xyz'):
+        set_ngrid_from_cutoff(paramset)
     boxsize = [paramset['boxsize'][a] for a in 'xyz']
     catalogue_rand.centre(boxsize)
     catalogue_rand.periodise(boxsize)

@@ -32,7 +32,7 @@ from . import io as _io
 from .catalogue import ParticleCatalogue
 from .dataobjs import Binning
 from .parameters import (InvalidParameterError, ParameterSet,
-                         fetch_paramset_template)
+                         fetch_paramset_template, set_ngrid_from_cutoff)
 from .threept import CLibBackend, _engine_backend, _finish, _view
 
 # Mixed-mesh normalisation geometry (reference twopt.py:52-54).
@@ -112,30 +112,6 @@ def _amalgamate_parameters(paramset=None, params_sampling=None, degree=None,
         pdict['binning'] = binning.scheme
     pdict.update(type_kwargs)
     return ParameterSet(param_dict=pdict)
-
-
-def _set_ngrid_from_cutoff(paramset):
-    """Mesh numbers from the Nyquist cutoff (parameters.cpp:1434-1463):
-    ceil(L k_cut / pi) in Fourier space, ceil(2 L / r_cut) in configuration
-    space, rounded up to even."""
-    cut = paramset['cutoff_nyq']
-    if cut is None:
-        raise InvalidParameterError(
-            "`ngrid` is unset and `cutoff_nyq` is not given.")
-    ngrid = {}
-    for a in 'xyz':
-        L = float(paramset['boxsize'][a])
-        if paramset['space'] == 'fourier':
-            n = int(math.ceil(L * cut / math.pi))
-        elif paramset['space'] == 'config':
-            n = int(math.ceil(2. * L / cut))
-        else:
-            raise InvalidParameterError(
-                "Space must be 'fourier' or 'config': `space` = "
-                f"'{paramset['space']}'.")
-        ngrid[a] = n + n % 2
-    paramset['ngrid'] = ngrid
-    paramset._validate()
 
 
 def _params_struct(paramset):
@@ -222,7 +198,7 @@ def _prepare_box(paramset, catalogue):
             paramset['boxsize'][a] = (hi - lo) * paramset['expand']
         paramset._validate()
     if any(paramset['ngrid'][a] is None for a in 'xyz'):
-        _set_ngrid_from_cutoff(paramset)
+        set_ngrid_from_cutoff(paramset)
 
 
 # ---------------------------------------------------------------------------
