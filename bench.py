@@ -395,6 +395,10 @@ def cpu_baseline(args, kbv, pairs_per_step):
     # Pair-loop work per step: 210 pairs x 1 term (B_000) + 400 pairs x 3
     # terms (B_202); fixed per-term costs are not counted.
     work = 210 * 1 + 400 * 3
+    fft = fft_calibration(olib, args.ngrid, threads)
+    # The per-pair body holds three mesh transforms (two shell IFFTs and the
+    # shot-noise IFFT); with an FFTW-class transform in their place:
+    per_pair_fftw = per_pair - 3. * (fft['oracle_s'] - fft['torch_s'])
     return {
         'value': round(pairs_per_step / (per_pair * work), 5),
         'unit': 'bin-pair evals/s', 'cores': threads, 'kind': 'port',
@@ -405,7 +409,40 @@ def cpu_baseline(args, kbv, pairs_per_step):
                    f"s; per-term assignment/FFT costs excluded (favours the "
                    f"CPU)"),
         'seconds_per_pair': round(per_pair, 3),
+        'fft_calibration': fft,
+        'value_fftw_class': (round(pairs_per_step / (per_pair_fftw * work), 5)
+                             if per_pair_fftw > 0 else None),
     }
+
+
+def fft_calibration(olib, n, threads):
+    """One n^3 complex FP64 transform by the oracle's radix-2 FFT and by
+    torch's CPU FFT (an FFTW-class library) on the same threads."""
+    import torch
+    g = np.random.default_rng(0)
+    a = g.standard_normal((n, n, n)) + 1j * g.standard_normal((n, n, n))
+    buf = np.ascontiguousarray(a).view(np.float64).copy()
+    ng = (C.c_int * 3)(n, n, n)
+    olib.trvo_fft3d.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
+    olib.trvo_fft3d.restype = None
+    ptr = buf.ctypes.data_as(C.POINTER(C.c_double))
+    olib.trvo_fft3d(ptr, ng, +1)
+    t = time.perf_counter()
+    olib.trvo_fft3d(ptr, ng, +1)
+    t_oracle = time.perf_counter() - t
+    torch.set_num_threads(threads)
+    ta = torch.from_numpy(a)
+    torch.fft.ifftn(ta)
+    ts = []
+    for _ in range(3):
+        t = time.perf_counter()
+        torch.fft.ifftn(ta)
+        ts.append(time.perf_counter() - t)
+    t_torch = float(np.median(ts))
+    log(f"FFT {n}^3 FP64 on {threads} threads: oracle {t_oracle:.3f} s, "
+        f"torch {t_torch:.3f} s")
+    return {'n': n, 'threads': threads, 'oracle_s': round(t_oracle, 4),
+            'torch_s': round(t_torch, 4)}
 
 
 def _check(lib, st):
