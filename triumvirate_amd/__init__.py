@@ -4,7 +4,9 @@ statistics (power spectrum, correlation function, window).
 
 The estimators run in hand-written HIP kernels (``libtrvgpu.so``) behind a
 C ABI (``include/trv_gpu.h``); this package is the Python host layer with
-the reference's public call surface.
+the reference's public call surface.  The FFTLog Hankel and spherical
+Bessel transforms used downstream of the estimators are the submodules
+``fftlog`` and ``transforms``.
 """
 
 __version__ = '0.1.0'
