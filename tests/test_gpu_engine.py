@@ -337,3 +337,83 @@ def test_device_lines_of_sight_equal_host_arrays(gpu_backend, kind, deg):
     for key in (raw, shot):
         scale = np.max(np.abs(out[1][key]))
         assert np.max(np.abs(out[0][key] - out[1][key])) <= 1e-12 * scale, key
+
+
+def _multi_call(eng, dview, rview, observer, plist, kbv, handles=None):
+    from triumvirate_amd import _abi
+    res = [_abi.ResultBuffers(_abi.result_dim(p.shape, kbv.struct.num_bins, 0), 'bispec')
+           for p in plist]
+    arr = (_abi.Params * len(plist))(*plist)
+    outs = (_abi.BispecResult * len(plist))(*[r.struct for r in res])
+    norms = np.ones(len(plist))
+    if handles is None:
+        st = eng.trvg_compute_bispec_multi_observer(
+            C.byref(dview.struct), C.byref(rview.struct), _abi.dptr(observer), arr,
+            len(plist), C.byref(kbv.struct), _abi.dptr(norms), outs)
+    else:
+        st = eng.trvg_compute_bispec_multi_dev(handles[0], handles[1], arr, len(plist),
+                                               C.byref(kbv.struct), _abi.dptr(norms), outs)
+    assert st == 0, eng.trvg_last_error()
+    for r, o in zip(res, outs):
+        r.struct.dim = o.dim
+    return [r.as_dict() for r in res]
+
+
+@pytest.mark.parametrize('outlier', [False, True])
+def test_fp32_fixed_point_upload(eng, outlier):
+    """FP32 survey calls on host catalogues send positions as 32-bit fixed
+    point over [-L, 2L) (step 3L / 2^32): the estimates agree with the same
+    FP32 call on device-resident FP64 positions within 2e-6 of the column
+    maximum.  A chunk holding a position outside that range (here one far
+    particle in each catalogue) crosses in FP64, and the call then equals
+    the device-resident one up to the order of atomic sums (1e-12)."""
+    from triumvirate_amd import _abi, mocks
+    from triumvirate_amd.dataobjs import Binning
+    print(f"fixed-point upload outlier={outlier}", flush=True)
+    xd, yd, zd, _ = mocks.lognormal_survey(20000, ngrid=32, seed=7)
+    xr, yr, zr = mocks.uniform_shell_randoms(100000, seed=8)
+    L = 3200.
+    # Box frame: shift every catalogue by the same offset.
+    shift = L / 2.
+    cols_d = [xd + shift, yd + shift, zd + shift]
+    cols_r = [xr + shift, yr + shift, zr + shift]
+    if outlier:
+        for c in cols_d + cols_r:
+            c[5] = 3.5 * L
+    observer = np.full(3, shift)
+    dview = _abi.CatalogueView(*cols_d)
+    rview = _abi.CatalogueView(*cols_r)
+    kb = Binning('fourier', 'lin', bin_min=0.005, bin_max=0.105, num_bins=10)
+    kbv = _abi.BinningView(kb.bin_edges, kb.bin_centres)
+    plist = []
+    for deg, shape in (((0, 0, 0), 'diag'), ((2, 0, 2), 'full')):
+        p = _abi.Params()
+        for i in range(3):
+            p.ngrid[i] = 64
+            p.boxsize[i] = L
+        p.assignment_order = 4
+        p.ell1, p.ell2, p.ELL = deg
+        p.shape = _abi.SHAPE_CODES[shape]
+        p.precision = 32
+        p.device = 0
+        plist.append(p)
+    host = _multi_call(eng, dview, rview, observer, plist, kbv)
+    hs = []
+    try:
+        for view in (dview, rview):
+            h = C.c_void_p()
+            assert eng.trvg_catalogue_upload_observer(
+                C.byref(view.struct), _abi.dptr(observer), 0, C.byref(h)) == 0, \
+                eng.trvg_last_error()
+            hs.append(h)
+        dev = _multi_call(eng, dview, rview, observer, plist, kbv, handles=hs)
+    finally:
+        for h in hs:
+            eng.trvg_catalogue_free(h)
+    tol = 1e-12 if outlier else 2e-6
+    for a, b in zip(host, dev):
+        assert np.array_equal(a['nmodes_1'], b['nmodes_1'])
+        for key in ('bk_raw', 'bk_shot'):
+            scale = np.max(np.abs(b[key]))
+            err = np.max(np.abs(a[key] - b[key])) / scale
+            assert err < tol, (key, err)

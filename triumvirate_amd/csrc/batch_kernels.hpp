@@ -278,12 +278,12 @@ static __global__ void __launch_bounds__(RED_THREADS)
 k_twopt_fourier_batch(const double2* __restrict__ fa,
                       const double2* __restrict__ fb, long long stride, Geom g,
                       AxisTables tab, const double* __restrict__ nmock, int ell,
-                      int m, const double* __restrict__ edges, int nb,
+                      int m, TwoptMap map, int nb,
                       unsigned long long* __restrict__ counts,
                       double* __restrict__ part, int fb_dc) {
   const long long b = blockIdx.y;
   k_twopt_fourier_body<double>(fa + b * stride, fb + b * stride, g, tab,
-                               dmk(nmock[b], 0.), ell, m, edges, nb,
+                               dmk(nmock[b], 0.), ell, m, map, nb,
                                counts + b * nb, part + b * gridDim.x * nb * 4,
                                fb_dc ? nmock[b] : 0.);
 }

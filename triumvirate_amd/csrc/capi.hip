@@ -301,8 +301,8 @@ int trvg_compute_bispec(const trvg_catalogue* data, const trvg_catalogue* rand,
     require_los(los_rand, "random");
     Context ctx(q);
     DeviceCatalogue d, r;
-    upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx));
-    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx));
+    upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
+    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
     write_bispec(dispatch_bispec(ctx, d, &r, true, *kb, norm_factor), out);
   });
 }
@@ -319,8 +319,8 @@ int trvg_compute_3pcf(const trvg_catalogue* data, const trvg_catalogue* rand,
     require_los(los_rand, "random");
     Context ctx(q);
     DeviceCatalogue d, r;
-    upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx));
-    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx));
+    upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
+    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
     write_3pcf(dispatch_3pcf(ctx, d, &r, true, *rb, norm_factor), out);
   });
 }
@@ -335,7 +335,7 @@ int trvg_compute_bispec_in_gpp_box(const trvg_catalogue* data,
     trvg_params q = three_point_params(params, kb->num_bins);
     Context ctx(q);
     DeviceCatalogue d;
-    upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
+    upload_catalogue(data, nullptr, d, ctx.stream, &ctx, false, nullptr, fp32(ctx));
     write_bispec(dispatch_bispec(ctx, d, nullptr, false, *kb, norm_factor), out);
   });
 }
@@ -350,7 +350,7 @@ int trvg_compute_3pcf_in_gpp_box(const trvg_catalogue* data,
     trvg_params q = three_point_params(params, rb->num_bins);
     Context ctx(q);
     DeviceCatalogue d;
-    upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
+    upload_catalogue(data, nullptr, d, ctx.stream, &ctx, false, nullptr, fp32(ctx));
     write_3pcf(dispatch_3pcf(ctx, d, nullptr, false, *rb, norm_factor), out);
   });
 }
@@ -372,7 +372,7 @@ int trvg_compute_3pcf_window(const trvg_catalogue* rand, const double* los_rand,
       throw Error(TRVG_ERR_PARAM, "Wide-angle orders must be non-negative.");
     Context ctx(q);
     DeviceCatalogue r;
-    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx));
+    upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
     Window3pcf w;
     w.alpha = alpha;
     w.wa_power = wide_angle ? i_wa + j_wa : -1;
@@ -760,11 +760,11 @@ int trvg_compute_bispec_multi(const trvg_catalogue* data,
     if (rand) {
       require_los(los_data, "data");
       require_los(los_rand, "random");
-      upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx));
-      upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx));
+      upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
+      upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
       run_multi(ctx, d, &r, *kb, mps, outs);
     } else {
-      upload_catalogue(data, nullptr, d, ctx.stream, &ctx);
+      upload_catalogue(data, nullptr, d, ctx.stream, &ctx, false, nullptr, fp32(ctx));
       run_multi(ctx, d, nullptr, *kb, mps, outs);
     }
   });
@@ -786,8 +786,8 @@ int trvg_compute_bispec_multi_observer(const trvg_catalogue* data,
     std::vector<Multipole> mps = check_multipoles(params, n, kb, norms, &base);
     Context ctx(base);
     DeviceCatalogue d, r;
-    upload_catalogue(data, nullptr, d, ctx.stream, &ctx, false, observer);
-    upload_catalogue(rand, nullptr, r, ctx.stream, &ctx, false, observer);
+    upload_catalogue(data, nullptr, d, ctx.stream, &ctx, false, observer, fp32(ctx));
+    upload_catalogue(rand, nullptr, r, ctx.stream, &ctx, false, observer, fp32(ctx));
     run_multi(ctx, d, &r, *kb, mps, outs);
   });
 }

@@ -149,11 +149,15 @@ void release_stream(int dev, hipStream_t s);
 // (FP32-field calls) and are widened on the device.  Without `los` and
 // with `observer` (3 coordinates in the frame of the positions), the lines
 // of sight are formed on the device from the positions relative to the
-// observer, as the chunks arrive.
+// observer, as the chunks arrive.  With `fixed_pos` (FP32-field calls
+// with a context) the positions cross as 32-bit fixed-point offsets in
+// [-L, 2L) per axis (step 3L / 2^32, 2.2e-6 Mpc/h at L = 3200 Mpc/h) and are
+// decoded on the device to the nearest step centre; a chunk with a position
+// outside that range crosses in FP64.
 void upload_catalogue(const trvg_catalogue* c, const double* los,
                       DeviceCatalogue& out, hipStream_t s,
                       Context* sort_for = nullptr, bool los32 = false,
-                      const double* observer = nullptr);
+                      const double* observer = nullptr, bool fixed_pos = false);
 
 // ---------------------------------------------------------------------------
 // Spherical Bessel spline (host build, device tables).
@@ -352,6 +356,16 @@ struct ShellStats {
   std::vector<double> keff;
 };
 ShellStats shell_stats(Context& ctx, const trvg_binning& kb);
+
+// Coarse bin of each half-window cell of the z lines for the two-point
+// statistics (see TwoptMap); built once per device, geometry and edges.
+struct TwoptMapData {
+  DevArray<int> K;
+  DevArray<long long> off;
+  DevArray<unsigned short> bin;
+  TwoptMap view() const { return {K.get(), off.get(), bin.get()}; }
+};
+std::shared_ptr<const TwoptMapData> twopt_map(Context& ctx, const double* edges, int nb);
 
 // Maths shared with the drivers.
 double wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3);

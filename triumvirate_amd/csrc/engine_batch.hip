@@ -155,9 +155,7 @@ class BatchPlan : public BatchPlanBase {
     tfield_.reset((size_t)B_ * nm_);
     G_.reset((size_t)B_ * nm_);
     Q_.reset((size_t)B_ * 2 * nq_);
-    d_edges_.reset(nb_ + 1);
-    TRVG_HIP(hipMemcpyAsync(d_edges_.get(), edges_.data(), (nb_ + 1) * 8,
-                            hipMemcpyHostToDevice, ctx_.stream));
+    tmap_ = twopt_map(ctx_, edges_.data(), nb_);
 
     // Work lists: unpruned passes over B fields, and the shell slabs
     // (complex and packed real).
@@ -652,10 +650,9 @@ class BatchPlan : public BatchPlanBase {
     hipStream_t s = ctx_.stream;
     hipLaunchKernelGGL(k_fill_zero_u64, dim3(1), dim3(256), 0, s, counts_.get(),
                        (long long)B_ * nb_);
-    const size_t smem = nb_ * (4 * 8 + 4);
     hipLaunchKernelGGL(k_twopt_fourier_batch, dim3(tune_.stat_blocks, B_),
-                       dim3(RED_THREADS), smem, s, D_.get(), D_.get(), nm_, g,
-                       ctx_.tab, d_nmock_.get(), ell, m, d_edges_.get(), nb_,
+                       dim3(RED_THREADS), twopt_smem(nb_, 4, g), s, D_.get(), D_.get(),
+                       nm_, g, ctx_.tab, d_nmock_.get(), ell, m, tmap_->view(), nb_,
                        counts_.get(), tpart_.get(), 1);
     const int w = nb_ * 4;
     hipLaunchKernelGGL((k_reduce_cols_batch<double>), dim3((w + RC_COLS - 1) / RC_COLS, B_),
@@ -724,9 +721,9 @@ class BatchPlan : public BatchPlanBase {
       hipLaunchKernelGGL(k_fill_zero_u64, dim3(1), dim3(256), 0, ctx_.stream,
                          counts_.get(), (long long)B_ * nb_);
       hipLaunchKernelGGL(k_twopt_fourier_batch, dim3(tune_.stat_blocks, B_),
-                         dim3(RED_THREADS), nb_ * (4 * 8 + 4), ctx_.stream, D_.get(),
-                         D_.get(), nm_, ctx_.g, ctx_.tab, d_nmock_.get(), 0, 0,
-                         d_edges_.get(), nb_, counts_.get(), tpart_.get(), 1);
+                         dim3(RED_THREADS), twopt_smem(nb_, 4, ctx_.g), ctx_.stream,
+                         D_.get(), D_.get(), nm_, ctx_.g, ctx_.tab, d_nmock_.get(), 0, 0,
+                         tmap_->view(), nb_, counts_.get(), tpart_.get(), 1);
     });
     TRVG_HIP(hipStreamSynchronize(ctx_.stream));
   }
@@ -762,7 +759,8 @@ class BatchPlan : public BatchPlanBase {
   double* h_pos_ = nullptr;
   double* h_meta_ = nullptr;
   long long* h_offs_ = nullptr;
-  DevArray<double> d_x_, d_y_, d_z_, d_nmock_, d_norm_, d_edges_;
+  DevArray<double> d_x_, d_y_, d_z_, d_nmock_, d_norm_;
+  std::shared_ptr<const TwoptMapData> tmap_;
   DevArray<long long> d_offs_;
   TileGeom tg_{};
   long long nkeys_ = 0;

@@ -1281,6 +1281,58 @@ ShellStats shell_stats(Context& ctx, const trvg_binning& kb) {
   return s;
 }
 
+// The two-point binning map: z-line windows and offsets on the host (the
+// window only has to contain every cell with |k| <= twopt_kmax), the exact
+// bin rule per cell on the device.
+std::shared_ptr<const TwoptMapData> twopt_map(Context& ctx, const double* edges, int nb) {
+  if (nb < 1 || nb >= int(TWOPT_NONE)) throw std::invalid_argument("twopt_map: bin count");
+  std::vector<double> key = {double(ctx.dev)};
+  for (int a = 0; a < 3; a++) { key.push_back(ctx.g.n[a]); key.push_back(ctx.g.dk[a]); }
+  key.insert(key.end(), edges, edges + nb + 1);
+  static std::mutex mu;
+  static std::map<std::vector<double>, std::shared_ptr<const TwoptMapData>> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  const Geom& g = ctx.g;
+  const long long nlines = (long long)g.n[0] * g.n[1];
+  const int half = g.n[2] / 2;
+  const double kmax = twopt_kmax(edges[nb]);
+  std::vector<int> K(nlines);
+  std::vector<long long> off(nlines);
+  long long tot = 0;
+  for (long long l = 0; l < nlines; l++) {
+    const double kx = shifted(int(l / g.n[1]), g.n[0]) * g.dk[0];
+    const double ky = shifted(int(l % g.n[1]), g.n[1]) * g.dk[1];
+    const double rem = kmax * kmax - (kx * kx + ky * ky);
+    off[l] = tot;
+    if (rem < 0.) { K[l] = -1; continue; }
+    K[l] = int(std::sqrt(rem) / g.dk[2]) + 1;
+    tot += (K[l] >= half ? half : K[l]) + 1;
+  }
+  auto d = std::make_shared<TwoptMapData>();
+  d->K.reset(nlines);
+  d->off.reset(nlines);
+  d->bin.reset(tot > 0 ? tot : 1);
+  DevArray<double> de(nb + 1);
+  TRVG_HIP(hipMemcpyAsync(d->K.get(), K.data(), nlines * sizeof(int), hipMemcpyHostToDevice,
+                          ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(d->off.get(), off.data(), nlines * sizeof(long long),
+                          hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(de.get(), edges, (nb + 1) * 8, hipMemcpyHostToDevice, ctx.stream));
+  hipLaunchKernelGGL(k_twopt_binmap, dim3(grid_for(ctx, nlines * 64, 4)), dim3(RED_THREADS), 0,
+                     ctx.stream, g, d->view(), de.get(), nb, d->bin.get());
+  TRVG_LAUNCH_CHECK();
+  // The host vectors and `de` must outlive the copies and the build.
+  TRVG_HIP(hipStreamSynchronize(ctx.stream));
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache.size() >= 16) cache.clear();   // bounded; maps in use stay alive
+  cache[key] = d;
+  return d;
+}
+
 // Launch of the direct-to-LDS pair kernel (opting in to more than 64 KiB of
 // dynamic LDS where a launch needs it).
 template <typename T, int MODE>
@@ -1407,26 +1459,23 @@ static void twopt_fourier(Context& ctx, const c2<T>* fa, const c2<T>* fb, cplx s
                           int ell, int m, const trvg_binning& kb, double modes_in_range,
                           std::function<void(TwoPt&&)> sink) {
   const int nb = kb.num_bins;
-  DevArray<double> edges(nb + 1);
-  ctx.upload(edges.get(), kb.bin_edges, nb + 1);
-  // Latency-bound (dependent table and field loads per cell): eight
-  // workgroups per CU.
+  std::shared_ptr<const TwoptMapData> map = twopt_map(ctx, kb.bin_edges, nb);
+  // Latency-bound (dependent field loads per cell): eight workgroups per CU.
   unsigned grid = grid_for(ctx, ctx.g.nmesh, 8);
   DevArray<unsigned long long> cnt(nb);
   DevArray<double> part((size_t)grid * nb * 4), sum(nb * 4);
   TRVG_HIP(hipMemsetAsync(cnt.get(), 0, nb * 8, ctx.stream));
-  size_t smem = nb * (4 * 8 + 4);
   // Compulsory reads: both fields at the modes inside the binning range.
   ProfScope ps(ctx, KC_TWOPT, 2. * modes_in_range * sizeof(c2<T>));
-  hipLaunchKernelGGL((k_twopt_fourier<T>), dim3(grid), dim3(RED_THREADS), smem,
-                     ctx.stream, fa, fb, ctx.g, ctx.tab,
-                     dmk(sbar.real(), sbar.imag()), ell, m, edges.get(), nb,
+  hipLaunchKernelGGL((k_twopt_fourier<T>), dim3(grid), dim3(RED_THREADS),
+                     twopt_smem(nb, 4, ctx.g), ctx.stream, fa, fb, ctx.g, ctx.tab,
+                     dmk(sbar.real(), sbar.imag()), ell, m, map->view(), nb,
                      cnt.get(), part.get());
   TRVG_LAUNCH_CHECK();
   reduce_cols(ctx, part.get(), int(grid), nb * 4, sum.get());
   const unsigned long long* hc = ctx.download(cnt.get(), nb);
   const double* hs = ctx.download(sum.get(), (size_t)nb * 4);
-  ctx.keep_until_flush(std::move(edges));
+  ctx.defer([map]() {});   // keeps the map alive until the launch has run
   ctx.keep_until_flush(std::move(cnt));
   ctx.keep_until_flush(std::move(part));
   ctx.keep_until_flush(std::move(sum));
@@ -1531,6 +1580,58 @@ static PairBinLists pair_bin_lists(const std::vector<double>& keff,
   return L;
 }
 
+// Index of the radial histogram's staged entries by n = a^2 + b^2 + c^2
+// (see k_radial_stage / k_radial_gather), built once per device and mesh.
+struct RadialIndex {
+  DevArray<int> ofs, idx;
+  long long nq = 0, nentries = 0;
+};
+
+static std::shared_ptr<const RadialIndex> radial_index(Context& ctx) {
+  const Geom& g = ctx.g;
+  const std::vector<long long> key = {ctx.dev, g.n[0], g.n[1], g.n[2]};
+  static std::mutex mu;
+  static std::map<std::vector<long long>, std::shared_ptr<const RadialIndex>> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  const int h0 = g.n[0] / 2 + 1, h1 = g.n[1] / 2 + 1, h2 = g.n[2] / 2 + 1;
+  const bool swap = g.n[0] == g.n[1];
+  const long long nrows = swap ? (long long)h0 * (h0 + 1) / 2 : (long long)h0 * h1;
+  const long long ne = nrows * h2;
+  if (ne >= (1LL << 31))
+    throw Error(TRVG_ERR_PARAM, "Mesh too large for the radial histogram index.");
+  auto ri = std::make_shared<RadialIndex>();
+  ri->nq = (long long)(g.n[0] / 2) * (g.n[0] / 2) + (long long)(g.n[1] / 2) * (g.n[1] / 2) +
+           (long long)(g.n[2] / 2) * (g.n[2] / 2) + 1;
+  ri->nentries = ne;
+  std::vector<long long> rowsq(nrows);
+  for (int a = 0; a < h0; a++)
+    for (int b = 0; b < (swap ? a + 1 : h1); b++)
+      rowsq[swap ? (long long)a * (a + 1) / 2 + b : (long long)a * h1 + b] =
+          (long long)a * a + (long long)b * b;
+  std::vector<int> ofs(ri->nq + 1, 0), idx(ne);
+  for (long long r = 0; r < nrows; r++)
+    for (int c = 0; c < h2; c++) ofs[rowsq[r] + (long long)c * c + 1]++;
+  for (long long n = 0; n < ri->nq; n++) ofs[n + 1] += ofs[n];
+  std::vector<int> cur(ofs.begin(), ofs.end() - 1);
+  for (long long r = 0; r < nrows; r++)
+    for (int c = 0; c < h2; c++) idx[cur[rowsq[r] + (long long)c * c]++] = int(r * h2 + c);
+  ri->ofs.reset(ri->nq + 1);
+  ri->idx.reset(ne);
+  TRVG_HIP(hipMemcpyAsync(ri->ofs.get(), ofs.data(), (ri->nq + 1) * sizeof(int),
+                          hipMemcpyHostToDevice, ctx.stream));
+  TRVG_HIP(hipMemcpyAsync(ri->idx.get(), idx.data(), ne * sizeof(int), hipMemcpyHostToDevice,
+                          ctx.stream));
+  TRVG_HIP(hipStreamSynchronize(ctx.stream));   // the host vectors go out of scope
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache.size() >= 8) cache.clear();   // bounded; indexes in use stay alive
+  cache[key] = ri;
+  return ri;
+}
+
 template <typename T>
 static void shot_pairs_bispec(Context& ctx, const c2<T>* t, int la, int ma, int lb,
                               int mb, const Spline& sa, const Spline& sb,
@@ -1556,18 +1657,20 @@ static void shot_pairs_bispec(Context& ctx, const c2<T>* t, int la, int ma, int 
     ctx.upload(dkb.get(), L.kb.data(), nkb);
     ctx.upload(dp.get(), L.pidx.data(), P);
     const long long nchunks = (nq + nch - 1) / nch;
+    const std::shared_ptr<const RadialIndex> ri = radial_index(ctx);
     DevArray<double> Q(2 * nq);
+    DevArray<double2> S(ri->nentries);
     DevArray<double2> part((size_t)nchunks * P), res(P);
-    hipLaunchKernelGGL(k_zero_d, dim3(grid_for(ctx, 2 * nq)), dim3(RED_THREADS), 0,
-                       ctx.stream, Q.get(), 2 * nq);
-    TRVG_LAUNCH_CHECK();
     {
+      // Algorithmic traffic: t read once (the staging and the index are
+      // the price of summing without atomics).
       ProfScope ps(ctx, KC_RADIAL, double(g.nmesh) * sizeof(c2<T>));
-      const long long nrep = (long long)(g.n[0] / 2 + 1) * (g.n[1] / 2 + 1) *
-                             (g.n[2] / 2 + 1);
-      hipLaunchKernelGGL((k_radial_hist_oct<T>), dim3(grid_for(ctx, nrep, 8)),
-                         dim3(RED_THREADS), 0, ctx.stream, t, g, la, ma, lb, mb,
-                         Q.get());
+      hipLaunchKernelGGL((k_radial_stage<T>), dim3(grid_for(ctx, ri->nentries, 8)),
+                         dim3(RED_THREADS), 0, ctx.stream, t, g, la, ma, lb, mb, S.get());
+      TRVG_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_radial_gather, dim3(grid_for(ctx, nq * RG_LANES, 8)),
+                         dim3(RED_THREADS), 0,
+                         ctx.stream, S.get(), ri->ofs.get(), ri->idx.get(), nq, Q.get());
       TRVG_LAUNCH_CHECK();
     }
     {
@@ -1584,9 +1687,10 @@ static void shot_pairs_bispec(Context& ctx, const c2<T>* t, int la, int ma, int 
     ctx.keep_until_flush(std::move(dkb));
     ctx.keep_until_flush(std::move(dp));
     ctx.keep_until_flush(std::move(Q));
+    ctx.keep_until_flush(std::move(S));
     ctx.keep_until_flush(std::move(part));
     ctx.keep_until_flush(std::move(res));
-    ctx.defer([h, P, sink = std::move(sink)]() {
+    ctx.defer([h, P, ri, sink = std::move(sink)]() {
       std::vector<cplx> out(P);
       for (int p = 0; p < P; p++) out[p] = cplx(h[p].x, h[p].y);
       sink(std::move(out));

@@ -173,8 +173,7 @@ struct BinnedStat {
 BinnedStat stats_fourier(Context& ctx, const double2* fa, const double2* fb, cplx sbar,
                          int ell, int m, const trvg_binning& b) {
   const int nb = b.num_bins;
-  DevArray<double> edges(nb + 1);
-  ctx.upload(edges.get(), b.bin_edges, nb + 1);
+  std::shared_ptr<const TwoptMapData> map = twopt_map(ctx, b.bin_edges, nb);
   const unsigned grid = grid_for(ctx, ctx.g.nmesh, 8);
   DevArray<unsigned long long> cnt(nb);
   DevArray<double> part((size_t)grid * nb * 5), sum((size_t)nb * 5);
@@ -182,15 +181,14 @@ BinnedStat stats_fourier(Context& ctx, const double2* fa, const double2* fb, cpl
   {
     ProfScope ps(ctx, KC_TWOPT, 2. * double(ctx.g.nmesh) * 16.);
     hipLaunchKernelGGL((k_twopt_fourier<double, 5>), dim3(grid), dim3(RED_THREADS),
-                       size_t(nb) * (5 * 8 + 4), ctx.stream, fa, fb, ctx.g, ctx.tab,
-                       dmk(sbar.real(), sbar.imag()), ell, m, edges.get(), nb, cnt.get(),
+                       twopt_smem(nb, 5, ctx.g), ctx.stream, fa, fb, ctx.g, ctx.tab,
+                       dmk(sbar.real(), sbar.imag()), ell, m, map->view(), nb, cnt.get(),
                        part.get());
     TRVG_LAUNCH_CHECK();
   }
   reduce_cols(ctx, part.get(), int(grid), nb * 5, sum.get());
   const unsigned long long* hc = ctx.download(cnt.get(), nb);
   const double* hs = ctx.download(sum.get(), (size_t)nb * 5);
-  ctx.keep_until_flush(std::move(edges));
   ctx.keep_until_flush(std::move(cnt));
   ctx.keep_until_flush(std::move(part));
   ctx.keep_until_flush(std::move(sum));

@@ -553,95 +553,193 @@ k_reduce_cols(const V* __restrict__ part, int nrows, int width, V* __restrict__ 
 // Two-point statistics in Fourier space with the reference's two-level
 // binning (field.cpp:2511-2703): fine bin ik = int(|k| / 1e-5) (< 1e6);
 // the fine bin enters coarse bin b when edges[b] <= ik * 1e-5 < edges[b+1].
-// Accumulates n_modes, sum(f_a f_b^*/C1 * y) and sum(S C1/C1 * y) per bin.
-// Output per block: [nb] counts (global atomics) and [nb][4] doubles.
+// Accumulates n_modes, sum(f_a f_b^*/W * y), sum(S C1/W * y) (and sum |k|)
+// per bin.
+//
+// The coarse bin of every cell depends on the geometry and the edges only,
+// and is the same at k_z and -k_z.  A TwoptMap holds it per z line (i, j)
+// over the half window q = 0..K of k_z indices (all q <= n/2 when the window
+// covers the line), built once by k_twopt_binmap; the statistics kernel
+// visits each half-window cell q with its mirror n - q, sharing the bin,
+// the window factors and y_lm (y(-k_z) = (-1)^(l + |m|) y(k_z)).
 // ---------------------------------------------------------------------------
+
+constexpr unsigned short TWOPT_NONE = 0xFFFF;
+
+struct TwoptMap {
+  const int* K;               // per line: half-window end (-1: nothing in range)
+  const long long* off;       // per line: offset of its half window in `bin`
+  const unsigned short* bin;  // coarse bin per half-window cell, TWOPT_NONE outside
+};
+
+// Largest |k| that can enter a bin (top edge plus two fine-bin widths).
+__host__ __device__ inline double twopt_kmax(double top_edge) {
+  return (top_edge + 2. * 1.e-5) * (1. + 1.e-12);
+}
+
+// One wave per z line; the reference's exact rule per half-window cell.
+static __global__ void __launch_bounds__(RED_THREADS)
+k_twopt_binmap(Geom g, TwoptMap map, const double* __restrict__ edges, int nb,
+               unsigned short* __restrict__ bin) {
+  const int n_sample = 1000000;
+  const double dk_sample = 1.e-5;
+  const long long nlines = (long long)g.n[0] * g.n[1];
+  const int half = g.n[2] / 2;
+  const int nw = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  for (long long l = (long long)blockIdx.x * nw + wave; l < nlines;
+       l += (long long)gridDim.x * nw) {
+    const int K = map.K[l];
+    if (K < 0) continue;
+    const int nq = (K >= half ? half : K) + 1;
+    const int i = int(l / g.n[1]), j = int(l % g.n[1]);
+    const double kx = grid_coord(i, g.n[0], g.dk[0]);
+    const double ky = grid_coord(j, g.n[1], g.dk[1]);
+    unsigned short* out = bin + map.off[l];
+    for (int q = lane; q < nq; q += 64) {
+      const double kz = grid_coord(q, g.n[2], g.dk[2]);
+      const double km = mag3_exact(kx, ky, kz);
+      const int ik = int(__ddiv_rn(km, dk_sample));
+      int b = -1;
+      if (ik >= 0 && ik < n_sample) b = find_bin(edges, nb, __dmul_rn(double(ik), dk_sample));
+      out[q] = b < 0 ? TWOPT_NONE : (unsigned short)b;
+    }
+  }
+}
+
+// Dynamic LDS of the statistics kernel.
+__host__ __device__ inline size_t twopt_smem(int nb, int nv, const Geom& g) {
+  return size_t(nb) * (nv * 8 + 4) + size_t(2) * (g.n[2] / 2 + 1) * 8;
+}
 
 template <typename T, int NV = 4>
 __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
-                Geom g, AxisTables tab, double2 sbar, int ell, int m,
-                const double* __restrict__ edges, int nb,
+                Geom g, AxisTables tab, double2 sbar, int ell, int m, TwoptMap map, int nb,
                 unsigned long long* __restrict__ counts,
                 double* __restrict__ part, double fb_dc = 0.) {
   // NV = 4: sums of pk (re, im) and sn (re, im) per bin; NV = 5 adds the
   // sum of |k| (the power spectrum's effective wavenumber).
-  extern __shared__ double sh2[];   // [nb][NV] then counts
-  unsigned int* sh_cnt = reinterpret_cast<unsigned int*>(sh2 + NV * nb);
+  extern __shared__ double sh2[];   // [nb][NV], z tables [2][n/2 + 1], counts
+  const int half = g.n[2] / 2;
+  double* zc1 = sh2 + NV * nb;       // C1 factor of k_z (plain windows)
+  double* zrw = zc1 + half + 1;      // 1/W factor of k_z
+  unsigned int* sh_cnt = reinterpret_cast<unsigned int*>(zrw + half + 1);
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
     for (int q = 0; q < NV; q++) sh2[NV * b + q] = 0.;
     sh_cnt[b] = 0u;
   }
+  // W = C1 for plain meshes; W = (sinc_x sinc_y sinc_z)^(2p) with
+  // interlacing (C1 then from the isotropic form, alias_at).
+  for (int q = threadIdx.x; q <= half; q += blockDim.x) {
+    if (!tab.interlace) {
+      const double c = c1_axis(tab.sin2[2][q], tab.order);
+      zc1[q] = c;
+      zrw[q] = 1. / c;
+    } else {
+      double p = tab.sinc[2][q];
+      const double s = p;
+      for (int o = 1; o < tab.order; o++) p *= s;
+      zc1[q] = 0.;
+      zrw[q] = 1. / (p * p);
+    }
+  }
   __syncthreads();
-  const int n_sample = 1000000;
-  const double dk_sample = 1.e-5;
-  // Only cells with |k| below the top edge (plus a fine-bin width) can
-  // enter a bin: each wave takes whole z lines and visits the k_z window
-  // |k_z| <= K of its line (a conservative bound; the exact test below
-  // decides every cell).
-  const double kmax = (edges[nb] + 2. * dk_sample) * (1. + 1.e-12);
-  const double kmax2 = kmax * kmax;
+  const int am = m < 0 ? -m : m;
+  const double ysgn = ((ell + am) & 1) ? -1. : 1.;
   const long long nlines = (long long)g.n[0] * g.n[1];
-  const int half = g.n[2] / 2;
-  // Each lane takes a contiguous run of the line's window, so consecutive
-  // cells of a lane mostly share a bin: sums stay in registers and go to
-  // LDS (five atomics) only when the bin changes and at the end of the run.
+  // Each lane takes a contiguous run of the line's half window, so
+  // consecutive cells of a lane mostly share a bin: sums stay in registers
+  // and go to LDS only when the bin changes and at the end of the run.
   const int nw = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   for (long long l = (long long)blockIdx.x * nw + wave; l < nlines;
        l += (long long)gridDim.x * nw) {
+   const int K = map.K[l];
+   if (K < 0) continue;
+   const bool whole = K >= half;
+   const int nq = (whole ? half : K) + 1;
    const int i = int(l / g.n[1]), j = int(l % g.n[1]);
    const double kx = grid_coord(i, g.n[0], g.dk[0]);
    const double ky = grid_coord(j, g.n[1], g.dk[1]);
-   const double rem = kmax2 - (kx * kx + ky * ky);
-   if (rem < 0.) continue;
-   const int K = int(sqrt(rem) / g.dk[2]) + 1;
-   const bool whole = K >= half;
-   const int ncell = whole ? g.n[2] : 2 * K + 1;
-   const int run = (ncell + 63) / 64;
-   const int lo = lane * run, hi = lo + run < ncell ? lo + run : ncell;
+   double cxy = 0., rxy;
+   if (!tab.interlace) {
+     cxy = c1_axis(tab.sin2[0][i], tab.order) * c1_axis(tab.sin2[1][j], tab.order);
+     rxy = 1. / cxy;
+   } else {
+     double p = tab.sinc[0][i] * tab.sinc[1][j];
+     const double s = p;
+     for (int o = 1; o < tab.order; o++) p *= s;
+     rxy = 1. / (p * p);
+   }
+   const unsigned short* lb = map.bin + map.off[l];
+   const long long base = l * g.n[2];
+   const int run = (nq + 63) / 64;
+   const int lo = lane * run, hi = lo + run < nq ? lo + run : nq;
    int cur = -1;
    unsigned int acc_n = 0u;
    double a0 = 0., a1 = 0., a2 = 0., a3 = 0., a4 = 0.;
-   for (int kk = lo; kk < hi; kk++) {
-    const int k = whole ? kk : (kk - K + g.n[2]) % g.n[2];
-    const long long c = l * g.n[2] + k;
-    double kz = grid_coord(k, g.n[2], g.dk[2]);
-    double km = mag3_exact(kx, ky, kz);
-    int ik = int(__ddiv_rn(km, dk_sample));
-    if (ik < 0 || ik >= n_sample) continue;
-    int b = find_bin(edges, nb, __dmul_rn(double(ik), dk_sample));
-    if (b < 0) continue;
-    c2<T> a = fa[c], bb = fb[c];
-    if (c == 0) bb.x += T(fb_dc);   // k = 0 offset (as LoadTwopt::fb_dc)
-    double2 pk = dmul(dmk(a.x, a.y), dmk(bb.x, -double(bb.y)));
-    double c1 = alias_at(tab, i, j, k);
-    double w = tab.interlace ? window_at(tab, i, j, k) * window_at(tab, i, j, k)
-                             : c1;
-    const double iw = 1. / w;
-    const double cw = c1 * iw;
-    double2 sn = dmk(sbar.x * cw, sbar.y * cw);
-    pk = dmk(pk.x * iw, pk.y * iw);
-    double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
-    pk = dmul(pk, y);
-    sn = dmul(sn, y);
-    if (b != cur) {
-      if (acc_n) {
-        atomicAdd(&sh_cnt[cur], acc_n);
-        atomicAdd(&sh2[NV * cur], a0);
-        atomicAdd(&sh2[NV * cur + 1], a1);
-        atomicAdd(&sh2[NV * cur + 2], a2);
-        atomicAdd(&sh2[NV * cur + 3], a3);
-        if (NV == 5) atomicAdd(&sh2[NV * cur + 4], a4);
-      }
-      cur = b;
-      acc_n = 0u;
-      a0 = a1 = a2 = a3 = a4 = 0.;
+   // Two cells (and their mirrors) per step, all loads issued before either
+   // is used; the loads do not wait for the bin (cells outside the binning
+   // are rare inside the window).
+   constexpr int U = 2;
+   for (int q0 = lo; q0 < hi; q0 += U) {
+    int bq[U];
+    c2<T> A[U], B[U], AM[U], BM[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int qq = q0 + u < hi ? q0 + u : q0;
+      const bool pair = qq != 0 && !(whole && qq == half);
+      const long long c = base + qq, cm = pair ? base + g.n[2] - qq : c;
+      bq[u] = q0 + u < hi ? int(lb[qq]) : int(TWOPT_NONE);
+      A[u] = fa[c];
+      B[u] = fb[c];
+      AM[u] = fa[cm];
+      BM[u] = fb[cm];
     }
-    acc_n++;
-    a0 += pk.x;
-    a1 += pk.y;
-    a2 += sn.x;
-    a3 += sn.y;
-    a4 += km;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+     const int b = bq[u];
+     if (b == TWOPT_NONE) continue;
+     const int q = q0 + u;
+     const bool pair = q != 0 && !(whole && q == half);
+     const double kz = grid_coord(q, g.n[2], g.dk[2]);
+     c2<T> bb = B[u];
+     if (base + q == 0) bb.x += T(fb_dc);   // k = 0 offset (as LoadTwopt::fb_dc)
+     double2 pk = dmul(dmk(A[u].x, A[u].y), dmk(bb.x, -double(bb.y)));
+     if (pair) {
+       const double2 pm = dmul(dmk(AM[u].x, AM[u].y), dmk(BM[u].x, -double(BM[u].y)));
+       pk.x += ysgn * pm.x;
+       pk.y += ysgn * pm.y;
+     }
+     const double c1 = tab.interlace ? alias_at(tab, i, j, q) : cxy * zc1[q];
+     const double iw = rxy * zrw[q];
+     const double cw = c1 * iw * (pair ? 1. + ysgn : 1.);
+     double2 sn = dmk(sbar.x * cw, sbar.y * cw);
+     pk = dmk(pk.x * iw, pk.y * iw);
+     const double2 y = ylm_reduced_dev(ell, m, kx, ky, kz);
+     pk = dmul(pk, y);
+     sn = dmul(sn, y);
+     if (b != cur) {
+       if (acc_n) {
+         atomicAdd(&sh_cnt[cur], acc_n);
+         atomicAdd(&sh2[NV * cur], a0);
+         atomicAdd(&sh2[NV * cur + 1], a1);
+         atomicAdd(&sh2[NV * cur + 2], a2);
+         atomicAdd(&sh2[NV * cur + 3], a3);
+         if (NV == 5) atomicAdd(&sh2[NV * cur + 4], a4);
+       }
+       cur = b;
+       acc_n = 0u;
+       a0 = a1 = a2 = a3 = a4 = 0.;
+     }
+     acc_n += pair ? 2u : 1u;
+     a0 += pk.x;
+     a1 += pk.y;
+     a2 += sn.x;
+     a3 += sn.y;
+     if (NV == 5) {
+       const double km = mag3_exact(kx, ky, kz);
+       a4 += pair ? 2. * km : km;
+     }
+    }
    }
    if (acc_n) {
      atomicAdd(&sh_cnt[cur], acc_n);
@@ -663,11 +761,10 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
 template <typename T, int NV = 4>
 static __global__ void __launch_bounds__(RED_THREADS)
 k_twopt_fourier(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
-                Geom g, AxisTables tab, double2 sbar, int ell, int m,
-                const double* __restrict__ edges, int nb,
-                unsigned long long* __restrict__ counts,
+                Geom g, AxisTables tab, double2 sbar, int ell, int m, TwoptMap map,
+                int nb, unsigned long long* __restrict__ counts,
                 double* __restrict__ part) {
-  k_twopt_fourier_body<T, NV>(fa, fb, g, tab, sbar, ell, m, edges, nb, counts, part);
+  k_twopt_fourier_body<T, NV>(fa, fb, g, tab, sbar, ell, m, map, nb, counts, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -1527,6 +1624,80 @@ __global__ void __launch_bounds__(RED_THREADS)
 k_radial_hist_oct(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb,
                   int mb, double* __restrict__ Q) {
   k_radial_hist_oct_body<T>(t, g, la, ma, lb, mb, Q);
+}
+
+// Radial histogram without atomics, in two passes.  k_radial_stage writes,
+// per octant row (a, b) (rows with a >= b and the (b, a) row folded in when
+// n_x = n_y, as k_radial_hist_oct) and c, the mirror sum of t y_a y_b to
+// S[row * h_z + c] (lanes over c: coalesced reads of t and writes of S).
+// k_radial_gather then sums, for each n, the entries with
+// a^2 + b^2 + c^2 = n listed by a geometry-only index (ofs, idx) in a fixed
+// order, so the histogram is the same on every run.
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS)
+k_radial_stage(const c2<T>* __restrict__ t, Geom g, int la, int ma, int lb, int mb,
+               double2* __restrict__ S) {
+  const int h0 = g.n[0] / 2 + 1, h1 = g.n[1] / 2 + 1, h2 = g.n[2] / 2 + 1;
+  const bool trivial = (la == 0 && ma == 0 && lb == 0 && mb == 0);
+  const bool swap = g.n[0] == g.n[1];
+  const int nrows = swap ? h0 * (h0 + 1) / 2 : h0 * h1;
+  const int nw = blockDim.x / 64, lane = threadIdx.x % 64;
+  for (int r = blockIdx.x * nw + threadIdx.x / 64; r < nrows; r += gridDim.x * nw) {
+    int a, b;
+    if (swap) {
+      a = int((sqrt(8. * r + 1.) - 1.) / 2.);
+      while (a * (a + 1) / 2 > r) a--;
+      while ((a + 1) * (a + 2) / 2 <= r) a++;
+      b = r - a * (a + 1) / 2;
+    } else {
+      a = r / h1;
+      b = r % h1;
+    }
+    for (int c = lane; c < h2; c += 64) {
+      double2 s = oct_mirror_sum<T>(t, g, a, b, c, la, ma, lb, mb, trivial);
+      if (swap && a != b) {
+        const double2 s2 = oct_mirror_sum<T>(t, g, b, a, c, la, ma, lb, mb, trivial);
+        s.x += s2.x;
+        s.y += s2.y;
+      }
+      S[(long long)r * h2 + c] = s;
+    }
+  }
+}
+
+// Q[2n], Q[2n + 1] = sum of S[idx[e]] over e in [ofs[n], ofs[n + 1]): eight
+// lanes per n, each over every eighth entry, then a fixed butterfly.
+constexpr int RG_LANES = 8;
+
+static __global__ void __launch_bounds__(RED_THREADS)
+k_radial_gather(const double2* __restrict__ S, const int* __restrict__ ofs,
+                const int* __restrict__ idx, long long nq, double* __restrict__ Q) {
+  const int sub = threadIdx.x % RG_LANES;
+  const long long per_block = blockDim.x / RG_LANES;
+  // Every lane of a group runs the same trip count, so the butterfly below
+  // always has all eight lanes.
+  for (long long n0 = (long long)blockIdx.x * per_block; n0 < nq;
+       n0 += (long long)gridDim.x * per_block) {
+    const long long n = n0 + threadIdx.x / RG_LANES;
+    double sx = 0., sy = 0.;
+    if (n < nq) {
+      const int e1 = ofs[n + 1];
+      for (int e = ofs[n] + sub; e < e1; e += RG_LANES) {
+        const double2 v = S[idx[e]];
+        sx += v.x;
+        sy += v.y;
+      }
+    }
+#pragma unroll
+    for (int o = 1; o < RG_LANES; o <<= 1) {
+      sx += __shfl_xor(sx, o);
+      sy += __shfl_xor(sy, o);
+    }
+    if (n < nq && sub == 0) {
+      Q[2 * n] = sx;
+      Q[2 * n + 1] = sy;
+    }
+  }
 }
 
 }  // namespace trvg

@@ -178,6 +178,23 @@ void fill_device(double* f, long long n, double v, hipStream_t s) {
   TRVG_LAUNCH_CHECK();
 }
 
+// Fixed-point positions of one chunk ([x block, y block, z block] of m
+// words each) to FP64: x = lo + (q + 1/2) step per axis.
+struct FixedAxes {
+  double lo[3], step[3], inv[3];
+};
+
+__global__ void k_unfix(const unsigned int* __restrict__ q, long long m, FixedAxes f,
+                        double* __restrict__ x, double* __restrict__ y,
+                        double* __restrict__ z) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < m;
+       i += (long long)gridDim.x * blockDim.x) {
+    x[i] = f.lo[0] + (double(q[i]) + 0.5) * f.step[0];
+    y[i] = f.lo[1] + (double(q[m + i]) + 0.5) * f.step[1];
+    z[i] = f.lo[2] + (double(q[2 * m + i]) + 0.5) * f.step[2];
+  }
+}
+
 // Lines of sight of particles [lo, hi) from their positions relative to the
 // observer o: (r - o) / |r - o|, a zero norm taken as 1, in the operation
 // order of ParticleCatalogue.compute_los (catalogue.py:437-458:
@@ -221,7 +238,7 @@ struct alignas(64) Partial {
 
 void upload_catalogue(const trvg_catalogue* c, const double* los,
                       DeviceCatalogue& out, hipStream_t s, Context* sort_for,
-                      bool los32, const double* observer) {
+                      bool los32, const double* observer, bool fixed_pos) {
   if (c == nullptr || c->x == nullptr || c->y == nullptr || c->z == nullptr)
     throw Error(TRVG_ERR_DATA, "Particle data are uninitialised.");
   if (c->ntotal <= 0)
@@ -238,6 +255,16 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
   if (los || derive) out.los.reset(3 * n);
   else out.los.reset(0);
   los32 = los32 && los != nullptr && sort_for != nullptr;
+  fixed_pos = fixed_pos && sort_for != nullptr;
+  FixedAxes fx{};
+  if (fixed_pos) {
+    for (int a = 0; a < 3; a++) {
+      const double L = sort_for->g.L[a];
+      fx.lo[a] = -L;
+      fx.step[a] = 3. * L / 4294967296.;
+      fx.inv[a] = 4294967296. / (3. * L);
+    }
+  }
 
   // Copies on their own stream when the upload feeds a tile sort, so that
   // the counting of chunk k overlaps the copy of chunk k + 1.
@@ -250,6 +277,7 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
   std::unique_ptr<TileSorter> sorter;
   if (sort_for) sorter = std::make_unique<TileSorter>(*sort_for, out);
   DevArray<float> los_f(los32 ? 3 * n : 0);   // FP32 lines of sight (widened)
+  DevArray<unsigned int> pos_q(fixed_pos ? 3 * n : 0);   // fixed-point positions
 
   // w = ws * wc (particles.cpp:553); the reference weight of the first
   // particle decides uniformity.
@@ -266,21 +294,43 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
   try {
     for (long long base = 0; base < n; base += chunk, slot = (slot + 1) % RING_SLOTS) {
       const long long m = std::min(chunk, n - base);
-      double* bx = reinterpret_cast<double*>(ring.acquire(slot));
+      // Slot layout: FP64 x, y, z, w, then the lines of sight; with fixed-
+      // point positions w, then x, y, z as words, then the lines of sight.
+      char* slot_buf = ring.acquire(slot);
+      double* bx = reinterpret_cast<double*>(slot_buf);
       double* by = bx + m;
       double* bz = by + m;
       double* bw = bz + m;
       double* bl = bw + m;
+      unsigned int* bq = nullptr;
+      bool fixed_chunk = fixed_pos;
+      if (fixed_chunk) {
+        bw = reinterpret_cast<double*>(slot_buf);
+        bq = reinterpret_cast<unsigned int*>(bw + m);
+        bl = reinterpret_cast<double*>(slot_buf + size_t(m) * 20 + 7 - (size_t(m) * 20 + 7) % 8);
+      }
       float* bl32 = reinterpret_cast<float*>(bl);
+      std::vector<int> out_of_range(nt, 0);
       pool.run([&](int t) {
         const long long lo = m * t / nt, hi = m * (t + 1) / nt;
         double ps = 0., pw = 0.;
         int uni = 1;
+        int bad = 0;
         for (long long i = lo; i < hi; i++) {
           const long long g = (base + i) * st;
-          bx[i] = c->x[g];
-          by[i] = c->y[g];
-          bz[i] = c->z[g];
+          if (fixed_chunk) {
+            const double p3[3] = {c->x[g], c->y[g], c->z[g]};
+            for (int a = 0; a < 3; a++) {
+              const double u = (p3[a] - fx.lo[a]) * fx.inv[a];
+              const bool ok = u >= 0. && u < 4294967296.;
+              bad |= !ok;
+              bq[a * m + i] = ok ? (unsigned int)u : 0u;
+            }
+          } else {
+            bx[i] = c->x[g];
+            by[i] = c->y[g];
+            bz[i] = c->z[g];
+          }
           const double ws = c->ws ? c->ws[g] : 1.;
           const double wc = c->wc ? c->wc[g] : 1.;
           const double w = ws * wc;
@@ -289,6 +339,7 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
           pw += w;
           uni &= (w == w0);
         }
+        out_of_range[t] = bad;
         if (los && hi > lo) {
           if (los32) {
             const double* src = los + 3 * (base + lo);
@@ -300,6 +351,39 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
         }
         part[t] = Partial{ps, pw, uni};
       });
+      bool refetch = false;
+      for (int t = 0; t < nt; t++) refetch = refetch || out_of_range[t];
+      if (refetch) {
+        // A position outside [-L, 2L): this chunk crosses in FP64 (the
+        // slot holds 32 bytes per particle plus the lines of sight).
+        fixed_chunk = false;
+        bx = reinterpret_cast<double*>(slot_buf);
+        by = bx + m;
+        bz = by + m;
+        bw = bz + m;
+        double* bl64 = bw + m;
+        pool.run([&](int t) {
+          const long long lo = m * t / nt, hi = m * (t + 1) / nt;
+          for (long long i = lo; i < hi; i++) {
+            const long long g = (base + i) * st;
+            bx[i] = c->x[g];
+            by[i] = c->y[g];
+            bz[i] = c->z[g];
+            bw[i] = (c->ws ? c->ws[g] : 1.) * (c->wc ? c->wc[g] : 1.);
+          }
+          if (los && hi > lo) {
+            if (los32) {
+              const double* src = los + 3 * (base + lo);
+              float* d32 = reinterpret_cast<float*>(bl64);
+              for (long long q = 0; q < 3 * (hi - lo); q++) d32[3 * lo + q] = float(src[q]);
+            } else {
+              std::memcpy(bl64 + 3 * lo, los + 3 * (base + lo), size_t(hi - lo) * 24);
+            }
+          }
+        });
+        bl = bl64;
+        bl32 = reinterpret_cast<float*>(bl);
+      }
       bool chunk_uniform = true;
       for (int t = 0; t < nt; t++) {   // fixed order: deterministic sums
         wst += part[t].wst;
@@ -307,9 +391,19 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
         chunk_uniform = chunk_uniform && part[t].uniform;
       }
       const size_t cb = size_t(m) * 8;
-      TRVG_HIP(hipMemcpyAsync(out.x.get() + base, bx, cb, hipMemcpyHostToDevice, cs));
-      TRVG_HIP(hipMemcpyAsync(out.y.get() + base, by, cb, hipMemcpyHostToDevice, cs));
-      TRVG_HIP(hipMemcpyAsync(out.z.get() + base, bz, cb, hipMemcpyHostToDevice, cs));
+      if (fixed_chunk) {
+        TRVG_HIP(hipMemcpyAsync(pos_q.get() + 3 * base, bq, 3 * size_t(m) * 4,
+                                hipMemcpyHostToDevice, cs));
+        const long long blocks = std::min<long long>((m + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_unfix, dim3(unsigned(blocks)), dim3(256), 0, cs,
+                           pos_q.get() + 3 * base, m, fx, out.x.get() + base,
+                           out.y.get() + base, out.z.get() + base);
+        TRVG_LAUNCH_CHECK();
+      } else {
+        TRVG_HIP(hipMemcpyAsync(out.x.get() + base, bx, cb, hipMemcpyHostToDevice, cs));
+        TRVG_HIP(hipMemcpyAsync(out.y.get() + base, by, cb, hipMemcpyHostToDevice, cs));
+        TRVG_HIP(hipMemcpyAsync(out.z.get() + base, bz, cb, hipMemcpyHostToDevice, cs));
+      }
       if (!(uniform && chunk_uniform)) {
         if (uniform) fill_device(out.w.get(), base, w0, cs);   // the uniform prefix
         uniform = false;
@@ -338,6 +432,7 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
       TRVG_HIP(hipStreamWaitEvent(s, ring.tail_event(), 0));
       sorter->finish();
       sort_for->keep_until_flush(std::move(los_f));
+      sort_for->keep_until_flush(std::move(pos_q));
       release_stream(dev, cs);
     }
   } catch (...) {
