@@ -187,7 +187,7 @@ def test_invalid_inputs():
     with pytest.raises(ValueError):
         HankelTransform(0., 0., np.linspace(1., 2., 16), kr_c=1.)
     with pytest.raises(ValueError):
-        SphericalBesselTransform(0, 0, x[:-1], extrap=5)
+        SphericalBesselTransform(0, 0, x[:-1], extrap=6)
     with pytest.raises(ValueError):
         SphericalBesselTransform(0, 0, x[:-1], extrap=1, extrap_layer='inner')
 

@@ -33,10 +33,13 @@ is the Hankel transform of order l + 1/2 of r^{3/2} f, times
 (2 pi / k)^{3/2}.
 """
 
+import warnings
+
 import numpy as np
 from scipy.special import loggamma
 
-__all__ = ['HankelTransform', 'extrap_pad', 'extrap_lin', 'extrap_loglin']
+__all__ = ['HankelTransform', 'extrap_pad', 'extrap_lin', 'extrap_loglin',
+           'extrap_loglin_oscil', 'DivergenceWarning']
 
 
 # ---------------------------------------------------------------------------
@@ -91,6 +94,98 @@ def extrap_loglin(a, n_ext):
     return np.concatenate([left, a, right], axis=-1)
 
 
+class DivergenceWarning(UserWarning):
+    """A tail of the samples grows outwards; it is padded with its end
+    value instead."""
+
+
+def _lngrad(a):
+    with np.errstate(divide='ignore', invalid='ignore'):
+        return np.nan_to_num(np.gradient(np.log(np.abs(a))))
+
+
+def _sign_changes(v):
+    return np.nonzero(np.diff(np.sign(v)))[0]
+
+
+def _tail_rate(a, inner, quarter, side):
+    """Log-slope per sample that extends the samples beyond `end` on
+    `side` ('l' or 'r'), so that the extension decays away from the data:
+    the mean log-gradient of the tail when it oscillates, else the
+    end-point log-gradient, else the mean over the quarter of the samples
+    on that side; zero (constant padding) when each of these grows
+    outwards."""
+    tail = a[:inner] if side == 'l' else a[inner:]
+    g = _lngrad(tail)
+    out = 1. if side == 'l' else -1.     # sign of an outward-decaying slope
+    if _sign_changes(g).size:
+        m = np.mean(g)
+        if out * m >= 0.:
+            return -m if side == 'l' else m
+        warnings.warn(f"Divergent behaviour detected at the "
+                      f"{'left' if side == 'l' else 'right'} tail. "
+                      "Constant padding is used instead.", DivergenceWarning,
+                      stacklevel=4)
+        return 0.
+    with np.errstate(divide='ignore', invalid='ignore'):
+        if side == 'l':
+            gend = np.nan_to_num(np.log(np.abs(a[1])) - np.log(np.abs(a[0])))
+        else:
+            gend = np.nan_to_num(np.log(np.abs(a[-1])) - np.log(np.abs(a[-2])))
+    if out * gend >= 0.:
+        return -gend if side == 'l' else gend
+    m = np.nan_to_num(np.mean(_lngrad(a[:quarter] if side == 'l' else a[quarter:])))
+    if out * m >= 0.:
+        return -m if side == 'l' else m
+    warnings.warn(f"Divergent behaviour detected at the "
+                  f"{'left' if side == 'l' else 'right'} endpoints. "
+                  "Constant padding is used instead.", DivergenceWarning,
+                  stacklevel=4)
+    return 0.
+
+
+def extrap_loglin_oscil(a, n_ext):
+    """Extend a 1-d array of possibly oscillating samples log-linearly on
+    either side (_arrayops.py:332-487).
+
+    The slope of each side is taken from a tail chosen to avoid sign
+    changes and turning points: at most up to the first (last) sign change,
+    the last (first) turning point of the log-magnitude before (after) it,
+    and the quarter of the samples on that side of the peak, with at least
+    three samples kept; see `_tail_rate` for the slope itself.  Complex
+    samples take their sign changes from the real part and are extended
+    with the complex end values."""
+    a = _as_float(a)
+    if a.ndim != 1 or a.size < 2:
+        raise ValueError("Extrapolation needs a 1-d array of at least two "
+                         "entries.")
+    n = a.size
+    if n < 8:
+        warnings.warn("Array length is less than 8. Extrapolation may not "
+                      "be accurate.", UserWarning, stacklevel=2)
+    # Complex samples (a transform's output whose imaginary part is
+    # round-off): sign changes from the real part, slopes from |a|.
+    sc = _sign_changes(np.real(a))
+    lbank, rbank = ((max(sc[0] + 1, 3), min(sc[-1] + 1, n - 4)) if sc.size
+                    else (n - 1, 0))
+    tl = _sign_changes(_lngrad(a[:lbank]))
+    ldbank = max(tl[-1] + 1, 3) if tl.size else n - 1
+    tr = _sign_changes(_lngrad(a[rbank:]))
+    rdbank = min(rbank + tr[0], n - 4) if tr.size else 0
+    peak = int(np.argmax(np.abs(a)))
+    if peak in (0, n - 1):
+        peak = n // 2
+    lquarter = max((peak + 1) // 2 - 1, 3)
+    rquarter = min(peak + (n - 1 - peak) // 2, n - 4)
+    ltail = min(lbank, ldbank, lquarter)
+    rtail = max(rbank, rdbank, rquarter)
+    sl = _tail_rate(a, ltail, lquarter, 'l')
+    sr = _tail_rate(a, rtail, rquarter, 'r')
+    left = a[0] * np.exp(sl * np.arange(n_ext, 0, -1))
+    right = a[-1] * np.exp(sr * np.arange(1, n_ext + 1))
+    return np.concatenate([left, a, right])
+
+
 def _extrapolate(a, n_ext, option):
     if option == 1:
         return extrap_pad(a, n_ext, 0., 0.)
@@ -103,6 +198,13 @@ def _extrapolate(a, n_ext, option):
         return extrap_lin(a, n_ext)
     if option == 4:
         return extrap_loglin(a, n_ext)
+    if option == 5:
+        a = np.asarray(a)
+        if a.ndim == 1:
+            return extrap_loglin_oscil(a, n_ext)
+        rows = a.reshape(-1, a.shape[-1])
+        return np.array([extrap_loglin_oscil(r, n_ext) for r in rows]).reshape(
+            a.shape[:-1] + (a.shape[-1] + 2 * n_ext,))
     raise ValueError(f"Unsupported extrapolation option: {option}.")
 
 
@@ -142,7 +244,8 @@ class HankelTransform:
         Low-ringing pivot (default True).
     extrap : int, optional
         Extrapolation of the samples inside the transform: 0 none
-        (default), 1 zeros, 2 end values, 3 linear, 4 log-linear; the
+        (default), 1 zeros, 2 end values, 3 linear, 4 log-linear, 5
+        log-linear with oscillation-aware tails; the
         transform then takes the smallest power of two >= `extrap_exp`
         times the sample size, and the samples are taken as real.
     extrap_exp : float, optional

@@ -3,8 +3,8 @@
 :mod:`triumvirate_amd.fftlog`).
 
 Extrapolation options of the transforms: 0 none, 1 zeros, 2 end values,
-3 linear, 4 log-linear.  The reference's option 5 (log-linear with
-oscillation detection) is not provided and raises.
+3 linear, 4 log-linear, 5 log-linear with oscillation-aware tails (2-d
+samples: rows first, then columns).
 """
 
 from collections.abc import Sequence
@@ -19,8 +19,8 @@ __all__ = ['SphericalBesselTransform', 'DoubleSphericalBesselTransform',
 
 
 def _check_extrap(extrap):
-    if extrap not in {0, 1, 2, 3, 4}:
-        raise ValueError(f"Extrapolation option must be in 0 to 4: {extrap=}.")
+    if extrap not in {0, 1, 2, 3, 4, 5}:
+        raise ValueError(f"Extrapolation option must be in 0 to 5: {extrap=}.")
 
 
 class SphericalBesselTransform:
