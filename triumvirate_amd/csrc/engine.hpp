@@ -126,7 +126,10 @@ class TileSorter {
  public:
   TileSorter(Context& ctx, const DeviceCatalogue& c);
   void count(long long begin, long long end);
-  void finish();
+  // Permutes the columns into tile order.  A weight column whose values
+  // are all equal (move_w false) and lines of sight that the caller forms
+  // afterwards from the sorted positions (move_los false) are not moved.
+  void finish(bool move_w = true, bool move_los = true);
 
  private:
   Context& ctx_;

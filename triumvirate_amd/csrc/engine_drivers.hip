@@ -945,9 +945,10 @@ void TileSorter::count(long long begin, long long end) {
   TRVG_LAUNCH_CHECK();
 }
 
-void TileSorter::finish() {
+void TileSorter::finish(bool move_w, bool move_los) {
   Context& ctx = ctx_;
   const DeviceCatalogue& c = c_;
+  move_los = move_los && c.los.get() != nullptr;
   const long long nt = tv_->tg.ntiles;
   DevArray<unsigned int> counts(nt + 1), offs(nt + 2), nch(nt), woff(nt + 1);
   hipLaunchKernelGGL(k_tile_sum, dim3(grid_for(ctx, nt + 1)), dim3(256), 0, ctx.stream,
@@ -955,9 +956,10 @@ void TileSorter::finish() {
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx.stream,
                      counts.get(), nt + 1, offs.get());
   TRVG_LAUNCH_CHECK();
-  DevArray<double> sx(c.n), sy(c.n), sz(c.n), sw(c.n), slos(c.los.get() ? 3 * c.n : 0);
+  DevArray<double> sx(c.n), sy(c.n), sz(c.n), sw(move_w ? c.n : 0),
+      slos(move_los ? 3 * c.n : 0);
   {
-    ProfScope ps(ctx, KC_ASSIGN, c.n * (c.los.get() ? 116. : 68.));
+    ProfScope ps(ctx, KC_ASSIGN, c.n * (20. + 48. + (move_w ? 16. : 0.) + (move_los ? 48. : 0.)));
     hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for(ctx, c.n, 16)), dim3(256), 0,
                        ctx.stream, c.view(), tile_of_.get(), offs.get(), ccount_.get(),
                        key2_.get(), sx.get(), sy.get(), sz.get(), sw.get(), slos.get());
@@ -977,8 +979,9 @@ void TileSorter::finish() {
   TRVG_LAUNCH_CHECK();
   // The permuted columns replace the caller-order ones; the old columns and
   // the sort's temporaries stay allocated until the stream has passed them.
-  std::swap(c.x, sx); std::swap(c.y, sy); std::swap(c.z, sz); std::swap(c.w, sw);
-  if (c.los.get()) std::swap(c.los, slos);
+  std::swap(c.x, sx); std::swap(c.y, sy); std::swap(c.z, sz);
+  if (move_w) std::swap(c.w, sw);
+  if (move_los) std::swap(c.los, slos);
   ctx.keep_until_flush(std::move(sx)); ctx.keep_until_flush(std::move(sy));
   ctx.keep_until_flush(std::move(sz)); ctx.keep_until_flush(std::move(sw));
   ctx.keep_until_flush(std::move(slos));

@@ -1220,7 +1220,7 @@ k_tile_scatter(DevParticles p, const int* __restrict__ tile_of,
     sx[d] = p.x[i];
     sy[d] = p.y[i];
     sz[d] = p.z[i];
-    sw[d] = p.w[i];
+    if (sw) sw[d] = p.w[i];
     if (slos) {
       slos[3 * (long long)d] = p.los[3 * i];
       slos[3 * (long long)d + 1] = p.los[3 * i + 1];

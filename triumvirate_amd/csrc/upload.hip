@@ -417,7 +417,9 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
         TRVG_HIP(hipMemcpyAsync(out.los.get() + 3 * base, bl, 3 * cb,
                                 hipMemcpyHostToDevice, cs));
       }
-      if (derive) los_from_observer(out, observer, base, base + m, cs);
+      // With a tile sort, lines of sight are formed once from the sorted
+      // positions instead of being permuted with them.
+      if (derive && !sorter) los_from_observer(out, observer, base, base + m, cs);
       ring.release(slot, cs);
       if (sorter) {
         // The positions of this chunk are on the device once the slot's
@@ -430,7 +432,8 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
     if (sorter) {
       TRVG_HIP(hipEventRecord(ring.tail_event(), cs));
       TRVG_HIP(hipStreamWaitEvent(s, ring.tail_event(), 0));
-      sorter->finish();
+      sorter->finish(/*move_w=*/!uniform, /*move_los=*/los != nullptr);
+      if (derive) los_from_observer(out, observer, 0, n, s);
       sort_for->keep_until_flush(std::move(los_f));
       sort_for->keep_until_flush(std::move(pos_q));
       release_stream(dev, cs);
