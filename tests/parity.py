@@ -133,6 +133,13 @@ def survey_pair(nd, nr, L, seed, weights=False):
         # One weight per catalogue (not 1): the engine's uniform-weight path.
         kw_d = dict(ws=np.full(nd, 1.5), wc=np.full(nd, 0.8))
         kw_r = dict(ws=np.full(nr, 0.7), wc=np.full(nr, 1.1))
+    elif weights == 'one-off':
+        # Uniform but for the last particle of each catalogue: the upload's
+        # weight column is filled lazily for ranges that stay uniform.
+        kw_d = dict(ws=np.ones(nd), wc=np.ones(nd))
+        kw_r = dict(ws=np.ones(nr), wc=np.ones(nr))
+        kw_d['ws'][-1] = 2.
+        kw_r['wc'][-1] = 0.5
     elif weights:
         kw_d = dict(ws=g.uniform(0.5, 1.5, nd), wc=g.uniform(0.8, 1.2, nd))
         kw_r = dict(ws=g.uniform(0.5, 1.5, nr), wc=g.uniform(0.8, 1.2, nr))

@@ -417,3 +417,23 @@ def test_fp32_fixed_point_upload(eng, outlier):
             scale = np.max(np.abs(b[key]))
             err = np.max(np.abs(a[key] - b[key])) / scale
             assert err < tol, (key, err)
+
+
+@pytest.mark.parametrize('deg', [(0, 0, 0), (2, 0, 2)])
+def test_one_off_weights_match_oracle(gpu_backend, oracle_backend, deg):
+    """Catalogues whose weights are all 1 but for their last particle: the
+    upload writes the weight column only from the first differing weight of
+    each thread's range and fills the uniform ranges afterwards (GPU vs
+    oracle, 1e-10)."""
+    from triumvirate_amd import threept as T
+    from triumvirate_amd.dataobjs import Binning
+    print(f"one-off weights {deg}", flush=True)
+    ps = cf.make_test_paramset(ngrid={'x': 32, 'y': 32, 'z': 32},
+                               boxsize={'x': 800., 'y': 800., 'z': 800.})
+    b = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.11, num_bins=5)
+    out = []
+    for be in (gpu_backend, oracle_backend):
+        d, r = P.survey_pair(2000, 8000, 700., seed=9, weights='one-off')
+        out.append(T._compute_3pt_stats_survey_like(
+            be, 'bispec', d, r, paramset=ps, degrees=deg, binning=b, form='full'))
+    P.compare_results(out[0], out[1], 'bispec', rtol=1e-10)

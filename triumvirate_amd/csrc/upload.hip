@@ -324,20 +324,28 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
               const double u = (p3[a] - fx.lo[a]) * fx.inv[a];
               const bool ok = u >= 0. && u < 4294967296.;
               bad |= !ok;
-              bq[a * m + i] = ok ? (unsigned int)u : 0u;
+              // Streaming store: the staging buffer is not read back by
+              // the host, so its lines need not be fetched first.
+              __builtin_nontemporal_store(ok ? (unsigned int)u : 0u, &bq[a * m + i]);
             }
           } else {
-            bx[i] = c->x[g];
-            by[i] = c->y[g];
-            bz[i] = c->z[g];
+            __builtin_nontemporal_store(c->x[g], &bx[i]);
+            __builtin_nontemporal_store(c->y[g], &by[i]);
+            __builtin_nontemporal_store(c->z[g], &bz[i]);
           }
           const double ws = c->ws ? c->ws[g] : 1.;
           const double wc = c->wc ? c->wc[g] : 1.;
           const double w = ws * wc;
-          bw[i] = w;
+          // The weight column is written only once a weight differs from
+          // w0 (a uniform chunk is not copied); the range's uniform prefix
+          // is filled then.
+          if (uni && w != w0) {
+            for (long long k = lo; k < i; k++) bw[k] = w0;
+            uni = 0;
+          }
+          if (!uni) __builtin_nontemporal_store(w, &bw[i]);
           ps += ws;
           pw += w;
-          uni &= (w == w0);
         }
         out_of_range[t] = bad;
         if (los && hi > lo) {
@@ -350,6 +358,8 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
           }
         }
         part[t] = Partial{ps, pw, uni};
+        // Orders the streaming stores before the copy reads the buffer.
+        __builtin_ia32_sfence();
       });
       bool refetch = false;
       for (int t = 0; t < nt; t++) refetch = refetch || out_of_range[t];
@@ -389,6 +399,14 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
         wst += part[t].wst;
         wt += part[t].wt;
         chunk_uniform = chunk_uniform && part[t].uniform;
+      }
+      if (!chunk_uniform && !refetch) {
+        // Ranges that stayed uniform left their weights unwritten.
+        pool.run([&](int t) {
+          if (!part[t].uniform) return;
+          const long long lo = m * t / nt, hi = m * (t + 1) / nt;
+          for (long long i = lo; i < hi; i++) bw[i] = w0;
+        });
       }
       const size_t cb = size_t(m) * 8;
       if (fixed_chunk) {
