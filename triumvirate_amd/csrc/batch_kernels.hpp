@@ -273,7 +273,8 @@ k_reduce_cols_batch(const V* __restrict__ part, int nrows, int width,
 }
 
 // Two-point statistics of mock b (fa = dn00, fb = N00, Sbar = N_b); with
-// fb_dc, fb is dn00 and N_b is added at k = 0 to form N00.
+// fb_dc, fb is dn00 and N_b is added at k = 0 to form N00.  Plans are
+// three-point and so never interlaced.
 static __global__ void __launch_bounds__(RED_THREADS)
 k_twopt_fourier_batch(const double2* __restrict__ fa,
                       const double2* __restrict__ fb, long long stride, Geom g,

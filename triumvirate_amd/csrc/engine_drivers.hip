@@ -1470,7 +1470,8 @@ static void twopt_fourier(Context& ctx, const c2<T>* fa, const c2<T>* fb, cplx s
   TRVG_HIP(hipMemsetAsync(cnt.get(), 0, nb * 8, ctx.stream));
   // Compulsory reads: both fields at the modes inside the binning range.
   ProfScope ps(ctx, KC_TWOPT, 2. * modes_in_range * sizeof(c2<T>));
-  hipLaunchKernelGGL((k_twopt_fourier<T>), dim3(grid), dim3(RED_THREADS),
+  auto kern = ctx.tab.interlace ? k_twopt_fourier<T, 4, true> : k_twopt_fourier<T, 4, false>;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(RED_THREADS),
                      twopt_smem(nb, 4, ctx.g), ctx.stream, fa, fb, ctx.g, ctx.tab,
                      dmk(sbar.real(), sbar.imag()), ell, m, map->view(), nb,
                      cnt.get(), part.get());

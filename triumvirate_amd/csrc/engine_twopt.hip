@@ -180,7 +180,9 @@ BinnedStat stats_fourier(Context& ctx, const double2* fa, const double2* fb, cpl
   TRVG_HIP(hipMemsetAsync(cnt.get(), 0, nb * 8, ctx.stream));
   {
     ProfScope ps(ctx, KC_TWOPT, 2. * double(ctx.g.nmesh) * 16.);
-    hipLaunchKernelGGL((k_twopt_fourier<double, 5>), dim3(grid), dim3(RED_THREADS),
+    auto kern = ctx.tab.interlace ? k_twopt_fourier<double, 5, true>
+                                  : k_twopt_fourier<double, 5, false>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(RED_THREADS),
                        twopt_smem(nb, 5, ctx.g), ctx.stream, fa, fb, ctx.g, ctx.tab,
                        dmk(sbar.real(), sbar.imag()), ell, m, map->view(), nb, cnt.get(),
                        part.get());

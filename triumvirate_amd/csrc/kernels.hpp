@@ -611,7 +611,10 @@ __host__ __device__ inline size_t twopt_smem(int nb, int nv, const Geom& g) {
   return size_t(nb) * (nv * 8 + 4) + size_t(2) * (g.n[2] / 2 + 1) * 8;
 }
 
-template <typename T, int NV = 4>
+// ILV: interlaced meshes (W and C1 of the interlaced average), a template
+// parameter so that the plain kernel does not carry the isotropic C1 code
+// (which costs it occupancy).
+template <typename T, int NV = 4, bool ILV = false>
 __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
                 Geom g, AxisTables tab, double2 sbar, int ell, int m, TwoptMap map, int nb,
                 unsigned long long* __restrict__ counts,
@@ -630,7 +633,7 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
   // W = C1 for plain meshes; W = (sinc_x sinc_y sinc_z)^(2p) with
   // interlacing (C1 then from the isotropic form, alias_at).
   for (int q = threadIdx.x; q <= half; q += blockDim.x) {
-    if (!tab.interlace) {
+    if (!ILV) {
       const double c = c1_axis(tab.sin2[2][q], tab.order);
       zc1[q] = c;
       zrw[q] = 1. / c;
@@ -660,7 +663,7 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
    const double kx = grid_coord(i, g.n[0], g.dk[0]);
    const double ky = grid_coord(j, g.n[1], g.dk[1]);
    double cxy = 0., rxy;
-   if (!tab.interlace) {
+   if (!ILV) {
      cxy = c1_axis(tab.sin2[0][i], tab.order) * c1_axis(tab.sin2[1][j], tab.order);
      rxy = 1. / cxy;
    } else {
@@ -709,7 +712,7 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
        pk.x += ysgn * pm.x;
        pk.y += ysgn * pm.y;
      }
-     const double c1 = tab.interlace ? alias_at(tab, i, j, q) : cxy * zc1[q];
+     const double c1 = ILV ? alias_at(tab, i, j, q) : cxy * zc1[q];
      const double iw = rxy * zrw[q];
      const double cw = c1 * iw * (pair ? 1. + ysgn : 1.);
      double2 sn = dmk(sbar.x * cw, sbar.y * cw);
@@ -758,13 +761,13 @@ __device__ __forceinline__ void k_twopt_fourier_body(const c2<T>* __restrict__ f
   }
 }
 
-template <typename T, int NV = 4>
+template <typename T, int NV = 4, bool ILV = false>
 static __global__ void __launch_bounds__(RED_THREADS)
 k_twopt_fourier(const c2<T>* __restrict__ fa, const c2<T>* __restrict__ fb,
                 Geom g, AxisTables tab, double2 sbar, int ell, int m, TwoptMap map,
                 int nb, unsigned long long* __restrict__ counts,
                 double* __restrict__ part) {
-  k_twopt_fourier_body<T, NV>(fa, fb, g, tab, sbar, ell, m, map, nb, counts, part);
+  k_twopt_fourier_body<T, NV, ILV>(fa, fb, g, tab, sbar, ell, m, map, nb, counts, part);
 }
 
 // ---------------------------------------------------------------------------
