@@ -144,8 +144,9 @@ def bench_bispec(args, lib):
         if st != 0:
             raise RuntimeError(lib.trvg_last_error().decode())
 
-    dview = _abi.CatalogueView(*data.columns())
-    rview = _abi.CatalogueView(*rand.columns())
+    # Unit weights: the weight columns are not passed (absent = 1).
+    dview = _abi.CatalogueView(*data.engine_columns())
+    rview = _abi.CatalogueView(*rand.engine_columns())
 
     # Particle normalisation (reference default convention).
     alpha = data.wstotal / rand.wstotal

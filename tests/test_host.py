@@ -195,3 +195,24 @@ def test_differentiable_inputs_rejected_before_the_engine(monkeypatch):
         autodiff._check_weights(pos[:, 0], pos)
     with pytest.raises(ValueError):
         autodiff._check_weights(np.ones(10), pos)
+
+
+def test_unit_weight_columns_are_not_passed_to_the_engine():
+    """Default (unit) weight columns are read-only and go to the engine as
+    absent columns; weights set later are passed."""
+    import numpy as np
+    from triumvirate_amd.catalogue import ParticleCatalogue
+    x = np.linspace(0., 1., 8)
+    c = ParticleCatalogue(x, x, x, nz=1.)
+    cols = c.engine_columns()
+    assert cols[4] is None and cols[5] is None
+    np.testing.assert_array_equal(c['ws'], np.ones(8))
+    with pytest.raises(ValueError):
+        c['ws'][0] = 2.
+    c['ws'] = np.full(8, 2.)
+    assert c.engine_columns()[4] is not None and c.engine_columns()[5] is None
+    assert c.wstotal == 16.
+    c['ws'] = 1.
+    assert c.engine_columns()[4] is None
+    d = ParticleCatalogue(x, x, x, nz=1., wc=np.ones(8))
+    assert d.engine_columns()[5] is not None   # arrays are passed as given

@@ -69,6 +69,11 @@ class ParticleCatalogue:
             'x': x, 'y': col(y), 'z': col(z),
             'nz': col(nz), 'ws': col(ws), 'wc': col(wc),
         }
+        # Weight columns built from the default 1 are kept read-only, so
+        # that they stay identically 1 and need not be sent to the engine.
+        self._unit = {}
+        for key, v in (('ws', ws), ('wc', wc)):
+            self._set_unit(key, v)
         self._source = f'extdata:{id(self._pdata)}'
         self._offset = np.zeros(3)
         self._calc_bounds()
@@ -170,9 +175,16 @@ class ParticleCatalogue:
     def __getitem__(self, key):
         return self._pdata[key]
 
+    def _set_unit(self, key, value):
+        unit = np.ndim(value) == 0 and float(value) == 1.
+        self._unit[key] = unit
+        self._pdata[key].flags.writeable = not unit
+
     def __setitem__(self, key, value):
         self._pdata[key] = np.broadcast_to(
             np.asarray(value, dtype=np.float64), (self.ntotal,)).copy()
+        if key in ('ws', 'wc'):
+            self._set_unit(key, value)
         if key in ('x', 'y', 'z'):
             self._calc_bounds()
         if key in ('ws', 'wc'):
@@ -270,3 +282,12 @@ class ParticleCatalogue:
         """(x, y, z, nz, ws, wc) float64 arrays."""
         p = self._pdata
         return p['x'], p['y'], p['z'], p['nz'], p['ws'], p['wc']
+
+    def engine_columns(self):
+        """As `columns`, with None for a weight column that is identically 1
+        (the engine takes an absent column as unit weights and then neither
+        reads nor transfers it)."""
+        p = self._pdata
+        return (p['x'], p['y'], p['z'], p['nz'],
+                None if self._unit['ws'] else p['ws'],
+                None if self._unit['wc'] else p['wc'])

@@ -277,8 +277,7 @@ def _params_struct(paramset):
 
 
 def _view(catalogue):
-    x, y, z, nz, ws, wc = catalogue.columns()
-    return _abi.CatalogueView(x, y, z, nz, ws, wc)
+    return _abi.CatalogueView(*catalogue.engine_columns())
 
 
 def _binning_view(binning):
