@@ -112,30 +112,6 @@ static void launch_z(Context& ctx, c2<T>* data, const Loader& load, int sign,
   TRVG_LAUNCH_CHECK();
 }
 
-// Full-mesh strided passes of 256-point axes take 16 lines per workgroup
-// (512 threads): 128-byte FP32 row segments instead of 64 (measured 4.5 ->
-// 5.6 TB/s for one 256^3 FP32 pass, identical results).
-template <typename T, int LOGN, int NT>
-static void launch_strided_wide(Context& ctx, c2<T>* data, long long S,
-                                long long outer_stride, long long inner, int sign,
-                                const Prune& pr, int prune_mode) {
-  constexpr int EPT = FftEpt<T>::v;
-  constexpr int N = 1 << LOGN;
-  constexpr int E = NT * EPT;
-  constexpr int NL = E / N;
-  const long long nblocks = ctx.g.nmesh / E;
-  const size_t smem = size_t(NL * (N + 1) + N) * sizeof(c2<T>) + N;
-  const c2<T>* tw = ctx_twiddles<T>(ctx, N, sign);
-  auto kern = inner % NL == 0 ? fft_pass_strided<T, LOGN, EPT, true, NT>
-                              : fft_pass_strided<T, LOGN, EPT, false, NT>;
-  if (smem > 64 * 1024)
-    TRVG_HIP(hipFuncSetAttribute((const void*)kern,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(smem)));
-  hipLaunchKernelGGL(kern, dim3(unsigned(nblocks)), dim3(NT), smem, ctx.stream, data, tw, S,
-                     outer_stride, inner, sign, pr, prune_mode);
-  TRVG_LAUNCH_CHECK();
-}
-
 template <typename T, int LOGN>
 static void launch_strided(Context& ctx, c2<T>* data, long long S,
                            long long outer_stride, long long inner, int sign,
@@ -147,13 +123,6 @@ static void launch_strided(Context& ctx, c2<T>* data, long long S,
   const size_t smem = fft_smem<T, LOGN>(true);
   const c2<T>* tw = ctx_twiddles<T>(ctx, N, sign);
   ProfScope ps(ctx, sizeof(T) == 8 ? KC_FFT_STRIDED64 : KC_FFT_STRIDED, bytes);
-  if constexpr (LOGN == 8 && sizeof(T) == 4) {
-    if (ctx.g.nmesh % (2 * E) == 0) {
-      launch_strided_wide<T, LOGN, 2 * FFT_THREADS>(ctx, data, S, outer_stride, inner, sign,
-                                                    pr, prune_mode);
-      return;
-    }
-  }
   if (inner % (E / N) == 0)
     hipLaunchKernelGGL((fft_pass_strided<T, LOGN, EPT, true>), dim3(unsigned(nblocks)),
                        dim3(FFT_THREADS), smem, ctx.stream, data, tw, S, outer_stride,

@@ -71,12 +71,12 @@ struct Prune {
 // In-LDS Stockham transform of L lines of length N (N = 2^LOGN) stored as
 // buf[line * (N + 1) + n] (one pad element per line against bank
 // conflicts).  W[m] = e^{sign 2 pi i m / N} (m < N) in LDS.
-template <typename T, int LOGN, int EPT, int NT = FFT_THREADS>
+template <typename T, int LOGN, int EPT>
 __device__ __forceinline__ void lds_fft_lines(c2<T>* buf, const c2<T>* W,
                                               int sign) {
   constexpr int N = 1 << LOGN;
   constexpr int LS = N + 1;  // padded LDS line stride
-  constexpr int E = NT * EPT;
+  constexpr int E = FFT_THREADS * EPT;
   constexpr int NLINES = E / N;
   const int t = threadIdx.x;
   int Ns = 1;
@@ -86,12 +86,12 @@ __device__ __forceinline__ void lds_fft_lines(c2<T>* buf, const c2<T>* W,
     if (rem >= 2) {
       // radix-4
       constexpr int NB = E / 4;               // butterflies per workgroup
-      constexpr int PER = NB / NT;   // = EPT / 4
+      constexpr int PER = NB / FFT_THREADS;   // = EPT / 4
       c2<T> v[PER > 0 ? PER : 1][4];
       int jj[PER > 0 ? PER : 1], ll[PER > 0 ? PER : 1];
 #pragma unroll
       for (int u = 0; u < PER; u++) {
-        int b = t + u * NT;
+        int b = t + u * FFT_THREADS;
         int line = b / (N / 4);
         int j = b % (N / 4);
         jj[u] = j;
@@ -130,12 +130,12 @@ __device__ __forceinline__ void lds_fft_lines(c2<T>* buf, const c2<T>* W,
       rem -= 2;
     } else if (rem == 1) {
       constexpr int NB = E / 2;
-      constexpr int PER = NB / NT;   // = EPT / 2
+      constexpr int PER = NB / FFT_THREADS;   // = EPT / 2
       c2<T> v[PER][2];
       int jj[PER], ll[PER];
 #pragma unroll
       for (int u = 0; u < PER; u++) {
-        int b = t + u * NT;
+        int b = t + u * FFT_THREADS;
         int line = b / (N / 2);
         int j = b % (N / 2);
         jj[u] = j;
@@ -491,15 +491,15 @@ fft_pass_z_batch(c2<T>* __restrict__ base, long long fstride, BatchLoader load,
 // FAST (inner a multiple of L): the block's lines lie in one slice, so the
 // slice's pruning flags go to LDS once per block and each thread steps a
 // pointer by a constant per element (thread t always owns line t % L).
-template <typename T, int LOGN, int EPT, bool FAST, int NT = FFT_THREADS>
+template <typename T, int LOGN, int EPT, bool FAST>
 __device__ __forceinline__ void fft_strided_block(
     c2<T>* __restrict__ data, const c2<T>* __restrict__ tw, long long S,
     long long outer_stride, long long inner, int sign, const Prune& pr,
     int prune_mode, long long blk) {
   constexpr int N = 1 << LOGN;
-  constexpr int E = NT * EPT;
+  constexpr int E = FFT_THREADS * EPT;
   constexpr int NLINES = E / N;
-  constexpr int NSTEP = NT / NLINES;   // rows advanced per element
+  constexpr int NSTEP = FFT_THREADS / NLINES;   // rows advanced per element
   extern __shared__ char smem_raw[];
   constexpr int LS = N + 1;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
@@ -516,7 +516,7 @@ __device__ __forceinline__ void fft_strided_block(
       for (int l = 0; l < NLINES && !any; l++) any = pr.p2(int(id0 % inner) + l);
       if (!any) return;
     }
-    for (int m = t; m < N; m += NT) {
+    for (int m = t; m < N; m += FFT_THREADS) {
       W[m] = tw[m];
       bool need = true;
       if (pr.on) {
@@ -539,7 +539,7 @@ __device__ __forceinline__ void fft_strided_block(
       p += step;
     }
     __syncthreads();
-    lds_fft_lines<T, LOGN, EPT, NT>(buf, W, sign);
+    lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
     if (m3 && !pr.p2(kl)) return;
     p = line + (long long)n0 * S;
 #pragma unroll
@@ -559,11 +559,11 @@ __device__ __forceinline__ void fft_strided_block(
     for (int l = 0; l < NLINES && !any; l++) any = pr.p2(int((id0 + l) % inner));
     if (!any) return;
   }
-  for (int m = t; m < N; m += NT) W[m] = tw[m];
+  for (int m = t; m < N; m += FFT_THREADS) W[m] = tw[m];
   // Global element (n, l): line id0 + l, position n; LDS line-major [l][n].
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
-    const int idx = t + e * NT;
+    const int idx = t + e * FFT_THREADS;
     const int n = idx / NLINES, l = idx % NLINES;
     const long long id = id0 + l;
     const long long base = (id / inner) * outer_stride + id % inner;
@@ -581,10 +581,10 @@ __device__ __forceinline__ void fft_strided_block(
     buf[l * LS + n] = need ? ld_mesh<T>(data + base + (long long)n * S) : mk<T>(T(0), T(0));
   }
   __syncthreads();
-  lds_fft_lines<T, LOGN, EPT, NT>(buf, W, sign);
+  lds_fft_lines<T, LOGN, EPT>(buf, W, sign);
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
-    const int idx = t + e * NT;
+    const int idx = t + e * FFT_THREADS;
     const int n = idx / NLINES, l = idx % NLINES;
     const long long id = id0 + l;
     const long long base = (id / inner) * outer_stride + id % inner;
@@ -594,15 +594,13 @@ __device__ __forceinline__ void fft_strided_block(
   }
 }
 
-// NT threads per workgroup: 512 (16 lines of 256, 128-byte FP32 row
-// segments) for the full-mesh passes of 256-point axes, else FFT_THREADS.
-template <typename T, int LOGN, int EPT, bool FAST, int NT = FFT_THREADS>
-__global__ void __launch_bounds__(NT)
+template <typename T, int LOGN, int EPT, bool FAST>
+__global__ void __launch_bounds__(FFT_THREADS)
 fft_pass_strided(c2<T>* __restrict__ data, const c2<T>* __restrict__ tw,
                  long long S, long long outer_stride, long long inner,
                  int sign, Prune pr, int prune_mode) {
-  fft_strided_block<T, LOGN, EPT, FAST, NT>(data, tw, S, outer_stride, inner, sign, pr,
-                                            prune_mode, xcd_block(blockIdx.x, gridDim.x));
+  fft_strided_block<T, LOGN, EPT, FAST>(data, tw, S, outer_stride, inner, sign, pr,
+                                        prune_mode, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Batched strided pass over a slab of fields, work[b] = (field, block).
