@@ -3,8 +3,9 @@
 Loads ``oracle/liboracle.so`` (built from ``oracle/trv_oracle.cpp`` by
 ``oracle/Makefile``), a from-scratch FP64 restatement of the reference's
 three-point path.  Only the test suite, the smoke check and the benchmark's
-CPU-baseline leg may import this package, and only as the checker: it is
-never the thing measured as the engine, and the engine never calls it.
+CPU-baseline leg may import this package, and only to check or time the
+reference algorithm: it is never the thing measured as the engine, and the
+engine never calls it.
 """
 
 import ctypes as C

@@ -1,5 +1,5 @@
 """numpy restatement of the differentiable periodic-box B_000 diag chain and
-its adjoint, for small meshes (test-side checker of the engine's
+its adjoint, for small meshes (the test-side reference for the engine's
 trvg_diff_* entry points).
 
 Forward (compute_bispec_in_gpp_box, threept.cpp:1473-2188, for degrees
@@ -29,7 +29,7 @@ def _stencil(pos, n, L, order):
         w = np.stack([1. - s, s])
         dw = np.stack([-np.ones_like(s), np.ones_like(s)])
         return ix, w, dw
-    raise NotImplementedError("the checker covers CIC only")
+    raise NotImplementedError("this restatement covers CIC only")
 
 
 def assign(pos, w, n, L, order=2):

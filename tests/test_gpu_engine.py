@@ -1,8 +1,8 @@
 """GPU parity tests of the HIP engine (run on an MI355X).
 
 Every estimator call here goes through the engine's C ABI
-(``libtrvgpu.so``); the CPU FP64 oracle and the reference's own fixture
-files are the checkers.
+(``libtrvgpu.so``); results are compared with the CPU FP64 oracle and the
+reference's own fixture files.
 
 Tolerances (stated per test):
   * mode / pair counts: bit-exact;
