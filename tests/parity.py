@@ -140,6 +140,12 @@ def survey_pair(nd, nr, L, seed, weights=False):
         kw_r = dict(ws=np.ones(nr), wc=np.ones(nr))
         kw_d['ws'][-1] = 2.
         kw_r['wc'][-1] = 0.5
+    elif weights == 'random-one-off':
+        # Uniform data weights; randoms uniform but for their last particle,
+        # so only the data side of the engine's uniform-weight path applies.
+        kw_d = dict(ws=np.full(nd, 1.5), wc=np.full(nd, 0.8))
+        kw_r = dict(ws=np.ones(nr), wc=np.ones(nr))
+        kw_r['wc'][-1] = 0.5
     elif weights:
         kw_d = dict(ws=g.uniform(0.5, 1.5, nd), wc=g.uniform(0.8, 1.2, nd))
         kw_r = dict(ws=g.uniform(0.5, 1.5, nr), wc=g.uniform(0.8, 1.2, nr))

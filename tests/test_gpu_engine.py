@@ -303,6 +303,39 @@ def test_multipole_call_equals_single_calls(gpu_backend, oracle_backend, geo):
         P.compare_results(got, ref, 'bispec', rtol=1e-10)
 
 
+@pytest.mark.parametrize('weights', ['uniform', 'random-one-off'])
+def test_data_fields_assigned_ahead_of_randoms(gpu_backend, oracle_backend, weights):
+    """With uniform data weights, survey calls assign the data side of every
+    (L, M) field while the random catalogue is still being uploaded; the
+    estimator uses those fields when the random weights are uniform too
+    ('uniform') and assigns afresh when they are not ('random-one-off').
+    Multipole call (B_000 full, B_202 full, B_000 row) and single calls,
+    GPU vs oracle, 1e-10."""
+    from triumvirate_amd import threept as T
+    from triumvirate_amd.dataobjs import Binning
+    print(f"data fields ahead of randoms, weights={weights}", flush=True)
+    ps = cf.make_test_paramset(assignment='pcs',
+                               ngrid={'x': 32, 'y': 32, 'z': 32},
+                               boxsize={'x': 800., 'y': 800., 'z': 800.})
+    b = Binning('fourier', 'lin', bin_min=0.01, bin_max=0.11, num_bins=5)
+    mps = [((0, 0, 0), 'full'), ((2, 0, 2), 'full'), ((0, 0, 0), 'row', 2)]
+
+    def run(be, deg=None, form=None, idx=None, multipoles=None):
+        d, r = P.survey_pair(2000, 8000, 700., seed=9, weights=weights)
+        return T._compute_3pt_stats_survey_like(
+            be, 'bispec', d, r, paramset=ps, degrees=deg or (0, 0, 0),
+            binning=b, form=form or 'full', idx_bin=idx, multipoles=multipoles)
+
+    multi = run(gpu_backend, multipoles=mps)
+    for mp, got in zip(mps, multi):
+        args = (mp[0], mp[1], mp[2] if len(mp) > 2 else None)
+        ref = run(oracle_backend, *args)
+        P.compare_results(got, ref, 'bispec', rtol=1e-10)
+        single = run(gpu_backend, *args)
+        single.pop('_norm_factors', None)
+        P.compare_results(single, ref, 'bispec', rtol=1e-10)
+
+
 @pytest.mark.parametrize('kind,deg', [('bispec', (0, 0, 0)), ('bispec', (2, 0, 2)),
                                       ('3pcf', (1, 1, 0))])
 def test_device_lines_of_sight_equal_host_arrays(gpu_backend, kind, deg):

@@ -94,10 +94,11 @@ void write_3pcf(const Result& r, trvg_3pcf_result* out) {
 
 Result dispatch_bispec(Context& ctx, const DeviceCatalogue& d,
                        const DeviceCatalogue* r, bool survey,
-                       const trvg_binning& b, double norm) {
+                       const trvg_binning& b, double norm,
+                       SurveyPrefill* pre = nullptr) {
   if (ctx.p.precision == TRVG_PREC_FP32)
-    return run_bispec<float>(ctx, d, r, survey, b, norm);
-  return run_bispec<double>(ctx, d, r, survey, b, norm);
+    return run_bispec<float>(ctx, d, r, survey, b, norm, pre);
+  return run_bispec<double>(ctx, d, r, survey, b, norm, pre);
 }
 
 Result dispatch_3pcf(Context& ctx, const DeviceCatalogue& d,
@@ -302,8 +303,11 @@ int trvg_compute_bispec(const trvg_catalogue* data, const trvg_catalogue* rand,
     Context ctx(q);
     DeviceCatalogue d, r;
     upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
+    // The data side of the fields is assigned while the randoms cross.
+    SurveyPrefill pre = prefill_survey_data(
+        ctx, d, {Multipole{q.ell1, q.ell2, q.ELL, q.shape, q.idx_bin, norm_factor}});
     upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
-    write_bispec(dispatch_bispec(ctx, d, &r, true, *kb, norm_factor), out);
+    write_bispec(dispatch_bispec(ctx, d, &r, true, *kb, norm_factor, &pre), out);
   });
 }
 
@@ -683,11 +687,11 @@ std::vector<Multipole> check_multipoles(const trvg_params* params, int n,
 
 void run_multi(Context& ctx, const DeviceCatalogue& d, const DeviceCatalogue* r,
                const trvg_binning& kb, const std::vector<Multipole>& mps,
-               trvg_bispec_result* outs) {
+               trvg_bispec_result* outs, SurveyPrefill* pre = nullptr) {
   std::vector<Result> res =
       (ctx.p.precision == TRVG_PREC_FP32)
-          ? run_bispec_multi<float>(ctx, d, r, r != nullptr, kb, mps)
-          : run_bispec_multi<double>(ctx, d, r, r != nullptr, kb, mps);
+          ? run_bispec_multi<float>(ctx, d, r, r != nullptr, kb, mps, pre)
+          : run_bispec_multi<double>(ctx, d, r, r != nullptr, kb, mps, pre);
   for (size_t i = 0; i < res.size(); i++) write_bispec(res[i], &outs[i]);
 }
 
@@ -761,8 +765,10 @@ int trvg_compute_bispec_multi(const trvg_catalogue* data,
       require_los(los_data, "data");
       require_los(los_rand, "random");
       upload_catalogue(data, los_data, d, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
+      // The data side of the fields is assigned while the randoms cross.
+      SurveyPrefill pre = prefill_survey_data(ctx, d, mps);
       upload_catalogue(rand, los_rand, r, ctx.stream, &ctx, fp32(ctx), nullptr, fp32(ctx));
-      run_multi(ctx, d, &r, *kb, mps, outs);
+      run_multi(ctx, d, &r, *kb, mps, outs, &pre);
     } else {
       upload_catalogue(data, nullptr, d, ctx.stream, &ctx, false, nullptr, fp32(ctx));
       run_multi(ctx, d, nullptr, *kb, mps, outs);
@@ -787,8 +793,10 @@ int trvg_compute_bispec_multi_observer(const trvg_catalogue* data,
     Context ctx(base);
     DeviceCatalogue d, r;
     upload_catalogue(data, nullptr, d, ctx.stream, &ctx, false, observer, fp32(ctx));
+    // The data side of the fields is assigned while the randoms cross.
+    SurveyPrefill pre = prefill_survey_data(ctx, d, mps);
     upload_catalogue(rand, nullptr, r, ctx.stream, &ctx, false, observer, fp32(ctx));
-    run_multi(ctx, d, &r, *kb, mps, outs);
+    run_multi(ctx, d, &r, *kb, mps, outs, &pre);
   });
 }
 

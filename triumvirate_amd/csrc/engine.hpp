@@ -438,13 +438,29 @@ struct Multipole {
   double norm;
 };
 
+// Data-catalogue fields of a survey bispectrum call, assigned while the
+// random catalogue is still crossing from the host.  For catalogues of
+// uniform weights the survey pair of each (L, M) starts from
+// S_d = sum_d y_LM w W, which needs no random particle; the estimator takes
+// each mesh (moves it out) at its first use and, when the random catalogue
+// turns out not to have uniform weights, uses none of them.
+struct SurveyPrefill {
+  std::map<std::pair<int, int>, DevArray<double2>> sd;
+};
+// S_d of every (L, M) that `mps` use, in order of use, while their meshes
+// stay within 4 GiB; empty unless `d` has uniform weights.
+SurveyPrefill prefill_survey_data(Context& ctx, const DeviceCatalogue& d,
+                                  const std::vector<Multipole>& mps);
+
 // Several bispectrum multipoles on one catalogue pair, sharing the common
-// fields, shell statistics and shell slabs.
+// fields, shell statistics and shell slabs (`pre`: data fields assigned
+// ahead, or null).
 template <typename T>
 std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
                                      const DeviceCatalogue* r, bool survey,
                                      const trvg_binning& kb,
-                                     const std::vector<Multipole>& mps);
+                                     const std::vector<Multipole>& mps,
+                                     SurveyPrefill* pre = nullptr);
 
 // Batched periodic-box bispectrum plans (engine_batch.hip).
 struct BatchPlanBase {
@@ -491,7 +507,8 @@ size_t noalloc_scratch_bytes();
 template <typename T>
 Result run_bispec(Context& ctx, const DeviceCatalogue& d,
                   const DeviceCatalogue* r, bool survey,
-                  const trvg_binning& kb, double norm_factor);
+                  const trvg_binning& kb, double norm_factor,
+                  SurveyPrefill* pre = nullptr);
 
 // The window of one random catalogue (threept.cpp:2621-3077): fields of
 // the catalogue alone scaled by alpha, and the wide-angle kernel

@@ -960,10 +960,14 @@ void TileSorter::finish(bool move_w, bool move_los) {
       slos(move_los ? 3 * c.n : 0);
   {
     ProfScope ps(ctx, KC_ASSIGN, c.n * (20. + 48. + (move_w ? 16. : 0.) + (move_los ? 48. : 0.)));
+    DevArray<double2> rec(2 * size_t(c.n));
     hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for(ctx, c.n, 16)), dim3(256), 0,
                        ctx.stream, c.view(), tile_of_.get(), offs.get(), ccount_.get(),
-                       key2_.get(), sx.get(), sy.get(), sz.get(), sw.get(), slos.get());
+                       key2_.get(), rec.get(), move_w, slos.get());
+    hipLaunchKernelGGL(k_tile_columns, dim3(grid_for(ctx, c.n, 16)), dim3(256), 0,
+                       ctx.stream, rec.get(), c.n, sx.get(), sy.get(), sz.get(), sw.get());
     TRVG_LAUNCH_CHECK();
+    ctx.keep_until_flush(std::move(rec));
   }
   // Work list: chunks of at most TILE_CHUNK particles per tile.
   tv_->work_cap = nt + (c.n + TILE_CHUNK - 1) / TILE_CHUNK;
@@ -1179,21 +1183,29 @@ static void build_single_field_k(Context& ctx, double2* f, const DeviceCatalogue
 //   N = sum_d y^* w^2 + alpha^2 sum_r y^* w^2,
 // both assigned in one pass per catalogue and forward transformed.  For
 // m = 0 both fields are real: they are assigned as dn + i N into one mesh,
-// transformed once and separated by Hermitian symmetry.
+// transformed once and separated by Hermitian symmetry.  `data_assigned`
+// (catalogues of uniform weights only): dn already holds S_d, as
+// prefill_survey_data leaves it.
 static void build_survey_pair_k(Context& ctx, double2* dn, double2* N,
                                 const DeviceCatalogue& d, const DeviceCatalogue& r,
-                                double alpha, int ell, int m) {
+                                double alpha, int ell, int m,
+                                bool data_assigned = false) {
   double* rdn = reinterpret_cast<double*>(dn);
   // Catalogues of uniform weights: one accumulation per particle, S_c =
   // sum_c y w W, and the pair from the transforms by linearity (for a
   // uniform w, y^* w^2 = w (y w)^*).
   const bool uniform = std::isfinite(d.wuni) && std::isfinite(r.wuni);
+  if (data_assigned && !uniform)
+    throw Error(TRVG_ERR_PARAM, "Prefilled data fields need uniform weights.");
   if (m == 0) {
-    zero_field<double>(ctx, dn);
     if (uniform) {
-      assign<double>(ctx, dn, d, WM_YLM_W, ell, 0, 1.);
+      if (!data_assigned) {
+        zero_field<double>(ctx, dn);
+        assign<double>(ctx, dn, d, WM_YLM_W, ell, 0, 1.);
+      }
       assign<double>(ctx, dn, r, WM_YLM_W, ell, 0, 1., true);
     } else {
+      zero_field<double>(ctx, dn);
       assign_dual<double>(ctx, rdn, rdn + 1, d, ell, m, 1., 1.);
       assign_dual<double>(ctx, rdn, rdn + 1, r, ell, m, -alpha, alpha * alpha);
     }
@@ -1213,9 +1225,9 @@ static void build_survey_pair_k(Context& ctx, double2* dn, double2* N,
     // S_d in dn, S_r in a scratch mesh; N (out of place) first, then dn in
     // place over S_d, each formed by its transform's first-pass loader.
     DevArray<double2> sr(ctx.g.nmesh);
-    zero_field<double>(ctx, dn);
+    if (!data_assigned) zero_field<double>(ctx, dn);
     zero_field<double>(ctx, sr.get());
-    assign<double>(ctx, dn, d, WM_YLM_W, ell, m, 1.);
+    if (!data_assigned) assign<double>(ctx, dn, d, WM_YLM_W, ell, m, 1.);
     assign<double>(ctx, sr.get(), r, WM_YLM_W, ell, m, 1.);
     const double vc = ctx.g.vol_cell, rb2 = 2. * double(ctx.g.nmesh) * sizeof(double2);
     LoadLinear2<double> lb{dn, sr.get(), d.wuni * vc, alpha * alpha * r.wuni * vc, 1};
@@ -1902,21 +1914,68 @@ class SlabCache {
 // Bispectrum
 // ---------------------------------------------------------------------------
 
+SurveyPrefill prefill_survey_data(Context& ctx, const DeviceCatalogue& d,
+                                  const std::vector<Multipole>& mps) {
+  SurveyPrefill pre;
+  if (!std::isfinite(d.wuni)) return pre;
+  // Keys in the order the estimator uses them: the common (0, 0) pair, then
+  // the (L, M) of each term that is not (0, 0).
+  std::vector<std::pair<int, int>> keys{{0, 0}};
+  for (const Multipole& mp : mps) {
+    trvg_params p = ctx.p;
+    p.ell1 = mp.ell1; p.ell2 = mp.ell2; p.ELL = mp.ELL;
+    for (const Term& tm : enumerate_terms(p, true))
+      if (!(p.ELL == 0 && tm.M == 0)) keys.push_back({p.ELL, tm.M});
+  }
+  constexpr size_t cap = size_t(4) << 30;
+  const size_t bytes = size_t(ctx.g.nmesh) * sizeof(double2);
+  size_t held = 0;
+  for (const auto& k : keys) {
+    if (pre.sd.count(k)) continue;
+    if (held + bytes > cap) break;
+    held += bytes;
+    // As build_survey_pair_k assigns the data side of a uniform pair.
+    DevArray<double2> f(ctx.g.nmesh);
+    zero_field<double>(ctx, f.get());
+    assign<double>(ctx, f.get(), d, WM_YLM_W, k.first, k.second, 1.);
+    pre.sd.emplace(k, std::move(f));
+  }
+  return pre;
+}
+
 template <typename T>
 std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
                                      const DeviceCatalogue* r, bool survey,
                                      const trvg_binning& kb,
-                                     const std::vector<Multipole>& mps) {
+                                     const std::vector<Multipole>& mps,
+                                     SurveyPrefill* pre) {
   const Geom& g = ctx.g;
   const long long nm = g.nmesh;
   const double alpha = survey ? d.wstotal / r->wstotal : 1.;
   const int nb = kb.num_bins;
 
+  // S_d of (L, M) assigned ahead of the random catalogue (uniform weights
+  // on both sides only), taken once; an empty array when there is none.
+  const bool uniform = survey && std::isfinite(d.wuni) && std::isfinite(r->wuni);
+  auto take_prefill = [&](int L, int M) {
+    DevArray<double2> f;
+    if (pre && uniform) {
+      auto it = pre->sd.find({L, M});
+      if (it != pre->sd.end()) {
+        f = std::move(it->second);
+        pre->sd.erase(it);
+      }
+    }
+    return f;
+  };
+
   // Common fields in FP64 whatever the shell precision T: the shot-noise
   // path (two-point statistics, S_ij_k) subtracts nearly equal quantities.
-  DevArray<double2> dn00(nm), N00(nm);
+  DevArray<double2> dn00 = take_prefill(0, 0), N00(nm);
+  const bool dn00_pre = dn00.get() != nullptr;
+  if (!dn00_pre) dn00.reset(nm);
   if (survey) {
-    build_survey_pair_k(ctx, dn00.get(), N00.get(), d, *r, alpha, 0, 0);
+    build_survey_pair_k(ctx, dn00.get(), N00.get(), d, *r, alpha, 0, 0, dn00_pre);
   } else {
     build_field_k<double>(ctx, dn00.get(), d, r, false, alpha, 0, 0, false, true);
     build_field_k<double>(ctx, N00.get(), d, r, false, alpha, 0, 0, true, false);
@@ -1991,9 +2050,17 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
       const double2* dnLMp = dn00.get();
       const double2* NLMp = N00.get();
       if (!lm_zero) {
-        if (!dnLM.get()) { dnLM.reset(nm); NLM.reset(nm); }
-        build_survey_pair_k(ctx, dnLM.get(), NLM.get(), d, *r, alpha, p.ELL, tm.M);
-        dnLMp = dnLM.get();
+        if (!NLM.get()) NLM.reset(nm);
+        DevArray<double2> own = take_prefill(p.ELL, tm.M);
+        if (own.get()) {
+          build_survey_pair_k(ctx, own.get(), NLM.get(), d, *r, alpha, p.ELL, tm.M, true);
+          dnLMp = own.get();
+          ctx.keep_until_flush(std::move(own));   // read by this term's kernels
+        } else {
+          if (!dnLM.get()) dnLM.reset(nm);
+          build_survey_pair_k(ctx, dnLM.get(), NLM.get(), d, *r, alpha, p.ELL, tm.M);
+          dnLMp = dnLM.get();
+        }
         NLMp = NLM.get();
       }
 
@@ -2085,10 +2152,10 @@ std::vector<Result> run_bispec_multi(Context& ctx, const DeviceCatalogue& d,
 template <typename T>
 Result run_bispec(Context& ctx, const DeviceCatalogue& d,
                   const DeviceCatalogue* r, bool survey,
-                  const trvg_binning& kb, double norm_factor) {
+                  const trvg_binning& kb, double norm_factor, SurveyPrefill* pre) {
   Multipole mp{ctx.p.ell1, ctx.p.ell2, ctx.p.ELL, ctx.p.shape, ctx.p.idx_bin,
                norm_factor};
-  return std::move(run_bispec_multi<T>(ctx, d, r, survey, kb, {mp})[0]);
+  return std::move(run_bispec_multi<T>(ctx, d, r, survey, kb, {mp}, pre)[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2247,16 +2314,16 @@ double run_mesh_norm(Context& ctx, const DeviceCatalogue& c) {
 
 template std::vector<Result> run_bispec_multi<double>(
     Context&, const DeviceCatalogue&, const DeviceCatalogue*, bool,
-    const trvg_binning&, const std::vector<Multipole>&);
+    const trvg_binning&, const std::vector<Multipole>&, SurveyPrefill*);
 template std::vector<Result> run_bispec_multi<float>(
     Context&, const DeviceCatalogue&, const DeviceCatalogue*, bool,
-    const trvg_binning&, const std::vector<Multipole>&);
+    const trvg_binning&, const std::vector<Multipole>&, SurveyPrefill*);
 template Result run_bispec<double>(Context&, const DeviceCatalogue&,
                                    const DeviceCatalogue*, bool,
-                                   const trvg_binning&, double);
+                                   const trvg_binning&, double, SurveyPrefill*);
 template Result run_bispec<float>(Context&, const DeviceCatalogue&,
                                   const DeviceCatalogue*, bool,
-                                  const trvg_binning&, double);
+                                  const trvg_binning&, double, SurveyPrefill*);
 template Result run_3pcf<double>(Context&, const DeviceCatalogue&,
                                  const DeviceCatalogue*, bool,
                                  const trvg_binning&, double, const Window3pcf*);

@@ -1170,17 +1170,24 @@ k_tile_sum(const unsigned int* __restrict__ ccount, long long nkeys,
 }
 
 // Exclusive scan of n counts by one workgroup (offsets[n] = total): each
-// thread sums a contiguous segment, the 1024 segment sums are scanned in
-// LDS, and each thread writes its segment's offsets.
+// thread sums a contiguous segment (a multiple of four words, read and
+// written as 16-byte vectors; `counts` and `offsets` 16-byte aligned), the
+// 1024 segment sums are scanned in LDS, and each thread writes its
+// segment's offsets.
 static __global__ void __launch_bounds__(1024)
 k_scan_counts(const unsigned int* __restrict__ counts, long long n,
               unsigned int* __restrict__ offsets) {
   __shared__ unsigned int sh[1024];
-  const long long per = (n + 1023) / 1024;
-  const long long lo = threadIdx.x * per;
+  const long long per = ((n + 1023) / 1024 + 3) & ~3LL;
+  const long long lo = threadIdx.x * per < n ? threadIdx.x * per : n;
   const long long hi = lo + per < n ? lo + per : n;
   unsigned int own = 0u;
-  for (long long i = lo; i < hi; i++) own += counts[i];
+  long long i = lo;
+  for (; i + 4 <= hi; i += 4) {
+    const uint4 v = *reinterpret_cast<const uint4*>(counts + i);
+    own += v.x + v.y + v.z + v.w;
+  }
+  for (; i < hi; i++) own += counts[i];
   sh[threadIdx.x] = own;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -1190,7 +1197,17 @@ k_scan_counts(const unsigned int* __restrict__ counts, long long n,
     __syncthreads();
   }
   unsigned int run = sh[threadIdx.x] - own;
-  for (long long i = lo; i < hi; i++) {
+  for (i = lo; i + 4 <= hi; i += 4) {
+    const uint4 v = *reinterpret_cast<const uint4*>(counts + i);
+    uint4 o;
+    o.x = run;
+    o.y = o.x + v.x;
+    o.z = o.y + v.y;
+    o.w = o.z + v.z;
+    run = o.w + v.w;
+    *reinterpret_cast<uint4*>(offsets + i) = o;
+  }
+  for (; i < hi; i++) {
     const unsigned int v = counts[i];
     offsets[i] = run;
     run += v;
@@ -1198,37 +1215,59 @@ k_scan_counts(const unsigned int* __restrict__ counts, long long n,
   if (threadIdx.x == 1023) offsets[n] = sh[1023];
 }
 
-// Scatter particle columns into tile order, ordered within a tile by
-// (rank in class, class): position = sum over classes c of min(n_c, rank)
-// plus the classes before the particle's that have more than rank members.
+// Scatter particles into tile order, ordered within a tile by (rank in
+// class, class): position = sum over classes c of min(n_c, rank) plus the
+// classes before the particle's that have more than rank members.  Each
+// particle lands as one 32-byte record {x, y, z, w} (w = 0 unless `with_w`)
+// rather than one word per column: for input in random order every store
+// is a partial line, and one record is one partial line instead of three or
+// four.  k_tile_columns turns the records back into columns.
 static __global__ void __launch_bounds__(256)
 k_tile_scatter(DevParticles p, const int* __restrict__ tile_of,
                const unsigned int* __restrict__ offs, const unsigned int* __restrict__ ccount,
-               const unsigned int* __restrict__ key2, double* __restrict__ sx,
-               double* __restrict__ sy, double* __restrict__ sz,
-               double* __restrict__ sw, double* __restrict__ slos) {
+               const unsigned int* __restrict__ key2, double2* __restrict__ rec,
+               bool with_w, double* __restrict__ slos) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < p.n;
        i += (long long)gridDim.x * blockDim.x) {
     const long long t = tile_of[i];
     const unsigned int r = key2[i] / TILE_CLASSES, cls = key2[i] % TILE_CLASSES;
-    const unsigned int* cc = ccount + t * TILE_CLASSES;
+    // The tile's 32 class counts (one 128-byte line) as 16-byte vectors.
+    const uint4* cc = reinterpret_cast<const uint4*>(ccount + t * TILE_CLASSES);
     unsigned int pos = 0;
-#pragma unroll 8
-    for (unsigned int c = 0; c < TILE_CLASSES; c++) {
-      const unsigned int n = cc[c];
-      pos += n < r ? n : r;
-      if (c < cls && n > r) pos++;
+#pragma unroll
+    for (int c4 = 0; c4 < TILE_CLASSES / 4; c4++) {
+      const uint4 v = cc[c4];
+      const unsigned int nn[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const unsigned int c = unsigned(4 * c4 + k), n = nn[k];
+        pos += n < r ? n : r;
+        if (c < cls && n > r) pos++;
+      }
     }
     const unsigned int d = offs[t] + pos;
-    sx[d] = p.x[i];
-    sy[d] = p.y[i];
-    sz[d] = p.z[i];
-    if (sw) sw[d] = p.w[i];
+    rec[2 * (long long)d] = make_double2(p.x[i], p.y[i]);
+    rec[2 * (long long)d + 1] = make_double2(p.z[i], with_w ? p.w[i] : 0.);
     if (slos) {
       slos[3 * (long long)d] = p.los[3 * i];
       slos[3 * (long long)d + 1] = p.los[3 * i + 1];
       slos[3 * (long long)d + 2] = p.los[3 * i + 2];
     }
+  }
+}
+
+// Tile-ordered records {x, y, z, w} of k_tile_scatter back to columns
+// (sw null: the weights stay where they are).
+static __global__ void __launch_bounds__(256)
+k_tile_columns(const double2* __restrict__ rec, long long n, double* __restrict__ sx,
+               double* __restrict__ sy, double* __restrict__ sz, double* __restrict__ sw) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double2 a = rec[2 * i], b = rec[2 * i + 1];
+    sx[i] = a.x;
+    sy[i] = a.y;
+    sz[i] = b.x;
+    if (sw) sw[i] = b.y;
   }
 }
 

@@ -230,6 +230,92 @@ struct alignas(64) Partial {
   int uniform;
 };
 
+// Staging buffers of one chunk of m particles: FP64 columns x, y, z, w, or
+// the three fixed-point blocks q (x, y, z words) and w.
+struct SlotCols {
+  double *x, *y, *z, *w;
+  unsigned int* q;
+  long long m;
+};
+
+// Gather of particles base + [lo, hi) into a slot (positions only; the
+// weights are gathered by gather_weights).  FIXED: positions as 32-bit
+// fixed point, returning whether a position lay outside [-L, 2L) (its word
+// is then 0 and the chunk must cross in FP64).  CONTIG: unit stride, so
+// the loop reads the columns directly and vectorises.  Stores are
+// streaming: the host never reads the staging buffer back.
+template <bool FIXED, bool CONTIG>
+int gather_positions(const trvg_catalogue* c, long long st, long long base, long long lo,
+                     long long hi, const FixedAxes& fx, const SlotCols& s) {
+  const double* __restrict__ X = c->x;
+  const double* __restrict__ Y = c->y;
+  const double* __restrict__ Z = c->z;
+  if (CONTIG) {
+    X += base;
+    Y += base;
+    Z += base;
+  }
+  const long long m = s.m;
+  if constexpr (FIXED) {
+    unsigned int* __restrict__ qx = s.q;
+    unsigned int* __restrict__ qy = s.q + m;
+    unsigned int* __restrict__ qz = s.q + 2 * m;
+    const double l0 = fx.lo[0], l1 = fx.lo[1], l2 = fx.lo[2];
+    const double i0 = fx.inv[0], i1 = fx.inv[1], i2 = fx.inv[2];
+    int bad = 0;
+    for (long long i = lo; i < hi; i++) {
+      const long long g = CONTIG ? i : (base + i) * st;
+      const double u0 = (X[g] - l0) * i0, u1 = (Y[g] - l1) * i1, u2 = (Z[g] - l2) * i2;
+      const bool k0 = u0 >= 0. && u0 < 4294967296.;
+      const bool k1 = u1 >= 0. && u1 < 4294967296.;
+      const bool k2 = u2 >= 0. && u2 < 4294967296.;
+      bad |= int(!(k0 && k1 && k2));
+      __builtin_nontemporal_store(k0 ? (unsigned int)u0 : 0u, &qx[i]);
+      __builtin_nontemporal_store(k1 ? (unsigned int)u1 : 0u, &qy[i]);
+      __builtin_nontemporal_store(k2 ? (unsigned int)u2 : 0u, &qz[i]);
+    }
+    return bad;
+  } else {
+    double* __restrict__ bx = s.x;
+    double* __restrict__ by = s.y;
+    double* __restrict__ bz = s.z;
+    for (long long i = lo; i < hi; i++) {
+      const long long g = CONTIG ? i : (base + i) * st;
+      __builtin_nontemporal_store(X[g], &bx[i]);
+      __builtin_nontemporal_store(Y[g], &by[i]);
+      __builtin_nontemporal_store(Z[g], &bz[i]);
+    }
+    return 0;
+  }
+}
+
+// Weights w = ws * wc (particles.cpp:553) of particles base + [lo, hi):
+// the sums of ws and w, and whether every w equals w0.  The weight column
+// is written only once a weight differs from w0 (a uniform chunk is not
+// copied); the range's uniform prefix is filled then.  Absent ws and wc
+// columns are unit weights: nothing to read or write.
+Partial gather_weights(const trvg_catalogue* c, long long st, long long base, long long lo,
+                       long long hi, double w0, double* __restrict__ bw) {
+  if (c->ws == nullptr && c->wc == nullptr)
+    return Partial{double(hi - lo), double(hi - lo), w0 == 1.};
+  double ps = 0., pw = 0.;
+  int uni = 1;
+  for (long long i = lo; i < hi; i++) {
+    const long long g = (base + i) * st;
+    const double ws = c->ws ? c->ws[g] : 1.;
+    const double wc = c->wc ? c->wc[g] : 1.;
+    const double w = ws * wc;
+    if (uni && w != w0) {
+      for (long long k = lo; k < i; k++) bw[k] = w0;
+      uni = 0;
+    }
+    if (!uni) __builtin_nontemporal_store(w, &bw[i]);
+    ps += ws;
+    pw += w;
+  }
+  return Partial{ps, pw, uni};
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -311,43 +397,18 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
       }
       float* bl32 = reinterpret_cast<float*>(bl);
       std::vector<int> out_of_range(nt, 0);
+      const SlotCols cols{bx, by, bz, bw, bq, m};
       pool.run([&](int t) {
         const long long lo = m * t / nt, hi = m * (t + 1) / nt;
-        double ps = 0., pw = 0.;
-        int uni = 1;
-        int bad = 0;
-        for (long long i = lo; i < hi; i++) {
-          const long long g = (base + i) * st;
-          if (fixed_chunk) {
-            const double p3[3] = {c->x[g], c->y[g], c->z[g]};
-            for (int a = 0; a < 3; a++) {
-              const double u = (p3[a] - fx.lo[a]) * fx.inv[a];
-              const bool ok = u >= 0. && u < 4294967296.;
-              bad |= !ok;
-              // Streaming store: the staging buffer is not read back by
-              // the host, so its lines need not be fetched first.
-              __builtin_nontemporal_store(ok ? (unsigned int)u : 0u, &bq[a * m + i]);
-            }
-          } else {
-            __builtin_nontemporal_store(c->x[g], &bx[i]);
-            __builtin_nontemporal_store(c->y[g], &by[i]);
-            __builtin_nontemporal_store(c->z[g], &bz[i]);
-          }
-          const double ws = c->ws ? c->ws[g] : 1.;
-          const double wc = c->wc ? c->wc[g] : 1.;
-          const double w = ws * wc;
-          // The weight column is written only once a weight differs from
-          // w0 (a uniform chunk is not copied); the range's uniform prefix
-          // is filled then.
-          if (uni && w != w0) {
-            for (long long k = lo; k < i; k++) bw[k] = w0;
-            uni = 0;
-          }
-          if (!uni) __builtin_nontemporal_store(w, &bw[i]);
-          ps += ws;
-          pw += w;
-        }
+        int bad;
+        if (fixed_chunk)
+          bad = st == 1 ? gather_positions<true, true>(c, st, base, lo, hi, fx, cols)
+                        : gather_positions<true, false>(c, st, base, lo, hi, fx, cols);
+        else
+          bad = st == 1 ? gather_positions<false, true>(c, st, base, lo, hi, fx, cols)
+                        : gather_positions<false, false>(c, st, base, lo, hi, fx, cols);
         out_of_range[t] = bad;
+        part[t] = gather_weights(c, st, base, lo, hi, w0, bw);
         if (los && hi > lo) {
           if (los32) {
             const double* src = los + 3 * (base + lo);
@@ -357,7 +418,6 @@ void upload_catalogue(const trvg_catalogue* c, const double* los,
             std::memcpy(bl + 3 * lo, los + 3 * (base + lo), size_t(hi - lo) * 24);
           }
         }
-        part[t] = Partial{ps, pw, uni};
         // Orders the streaming stores before the copy reads the buffer.
         __builtin_ia32_sfence();
       });
